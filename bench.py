@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Headline benchmark: particle-steps/s of the fused PIC-MCC particle pass on
+one MI355X, for the RF 22-pole ion trap with buffer-gas MCC:
+
+  4 194 304 singly charged ions of 3 u on a 512 x 512 grid (2 cm x 2 cm),
+  uRF = Laplace basis of the 22 alternating +-1 rods (solved on the GPU),
+  E = -grad(u + V_rf cos(2 pi 17 MHz t) uRF), V_rf = 30 V, B = 0,
+  He buffer at 300 K, 1e20 m^-3, constant k = 1e-15 m^3/s elastic,
+  Maxwellian 0.1 eV on a 5 mm disk, dt = 1e-9 s.
+
+One step = push + E gather + MCC trial/scatter + boundary + CIC deposit
+(64-bit fixed point) of every live ion, inputs resident in HBM.  Prints one
+JSON line.  The CPU baseline is the oracle's serial FP64 loop (AoS, the
+reference's own loop order and SHR3 stream) on a bounded sample of the same
+workload, timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from mag2d_amd import abi  # noqa: E402
+
+METRIC = "particle-steps/s on 1 GPU (4M ions, 512² grid) vs CPU; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+M_ION = 3 * 1.66053906660e-27
+M_HE = 6.69e-27
+
+
+def build_scene(ctx, n_grid, n_particles):
+    L = 2e-2
+    dx = L / (n_grid - 1)
+    g = abi.grid2d(n_grid, n_grid, dx, dx)
+    mask, volt = abi.grid_build(g, abi.GEO_RF_22PT, L, L)
+    dV = (n_particles / 1e11) / ((n_grid - 1) ** 2)  # Param::dV with density_total 1e11
+    params = abi.FieldsParams(g, L, L, 0, 0, 0, abi.BC_FREE, 0.0, 1, 30.0, 2 * math.pi * 17e6, 0.0,
+                              0.0, 0.0, 0.0, dV)
+    fields = abi.Fields2D(ctx, params, mask, volt)
+    t0 = time.perf_counter()
+    info_rf = fields.solve(abi.SOLVE_RF, 1e-10)
+    t1 = time.perf_counter()
+    fields.reset()
+    info_st = fields.solve(abi.SOLVE_STATIC, 1e-10)
+    fields.reset()
+    poisson = {"grid": [n_grid, n_grid], "rf_basis_solve_ms": round((t1 - t0) * 1e3, 3),
+               "outer_iters": info_rf.outer_iters, "inner_pcg_iters": info_rf.inner_iters,
+               "rel_residual_fp64": info_rf.rel_residual, "static_rel_residual": info_st.rel_residual}
+    tgt = abi.Target(M_HE, 1e20, 300.0, 0.0, abi.Particles2D())
+    it = abi.Interaction(abi.ELASTIC, 0, 0.0, 1e-15, 0.0, 0, 0)
+    T_ion_eV = 0.1
+    E_max = 10 * T_ion_eV  # BaseSpecies default: 10 k T of the species (particles.hpp:160)
+    mcc = abi.Mcc(ctx, M_ION, [tgt], [it], E_max)
+    return g, mask, volt, params, fields, mcc, poisson
+
+
+def cpu_baseline(fields, params, mask, n_grid, budget_s):
+    """Oracle serial FP64 step (boris + MCC with the SHR3 stream, boundary +
+    deposit) on a sample of the same scene; returns particle-steps/s."""
+    from oracle import oracle as O
+    import ctypes as C
+    L = 2e-2
+    dx = L / (n_grid - 1)
+    og = O.grid2(n_grid, n_grid, dx, dx)
+    u = fields.download(abi.FIELD_U)
+    urf = fields.download(abi.FIELD_URF)
+    of = O.make_fields(og, mask, u, urf, x_max=L, z_max=L, selfconsistent=1, rf=1, rf_amplitude=30.0,
+                       rf_omega=2 * math.pi * 17e6)
+    mcc = O.make_mcc(M_ION, 1.0, [dict(mass=M_HE, density=1e20, temperature=300.0,
+                                        inters=[O.inter(O.ELASTIC, M_ION, M_HE, rate=1e-15)])])
+    r = O.Rng()
+    O.lib().o_rng_shr3_seed(C.byref(r), 1234)
+
+    def sample(n):
+        rng = np.random.default_rng(1)
+        p = O.particles(n)
+        ang = rng.uniform(0, 2 * np.pi, n)
+        rad = 5e-3 * np.sqrt(rng.uniform(0, 1, n))
+        p["x"] = 1e-2 + rad * np.cos(ang)
+        p["z"] = 1e-2 + rad * np.sin(ang)
+        vth = math.sqrt(0.1 * abi.QE / M_ION)
+        for k in ("vx", "vy", "vz"):
+            p[k] = rng.normal(size=n) * vth
+        return p
+
+    rho = np.zeros(n_grid * n_grid)
+    s = O.make_species(abi.QE, M_ION, 1e-9, lifetime=mcc.lifetime, rho=rho, mcc=mcc,
+                       rng_kind=O.RNG_SHR3, shr3=r)
+    n_cal = 200_000
+    p = sample(n_cal)
+    t_cal = O.lib().o_bench_steps(C.byref(of), C.byref(s), O.dptr(p), n_cal, 2)
+    per_ps = t_cal / (2 * n_cal)
+    n = 1_000_000
+    steps = max(2, int(budget_s / (per_ps * n)))
+    p = sample(n)
+    t = O.lib().o_bench_steps(C.byref(of), C.byref(s), O.dptr(p), n, steps)
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} ions x {steps} steps of the same RF 22-pole scene (uRF from the GPU solve), "
+                      f"FP64 AoS, serial reference loop order, SHR3 stream, deposit on; {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=4 * 1024 * 1024)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): the "
+              "particle pass would shard by particles with an int64 all-reduce of the deposit grid, "
+              "which this build does not implement", file=sys.stderr)
+        sys.exit(2)
+
+    ctx = abi.Context(0, seed=1234)
+    g, mask, volt, params, fields, mcc, poisson = build_scene(ctx, args.grid, args.particles)
+    N = args.particles
+    P = abi.ParticlesSoA(ctx, N)
+    vth = math.sqrt(0.1 * abi.QE / M_ION)
+    abi.add_particles_on_disk(ctx, fields, P, 1e-2, 1e-2, 5e-3, vth, stream=0)
+    sp = abi.Species2D(abi.QE, M_ION, 1e-9, mcc.lifetime, 0, 0, 0)
+    abi.advance_init(ctx, fields, sp, P)
+    rho = abi.DeviceArray(ctx, (args.grid * args.grid,), np.float64)
+    rho.zero()
+    flags = abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT
+
+    step = 0
+    for _ in range(args.warmup):
+        sp.niter = step
+        abi.advance(ctx, fields, sp, P, mcc=mcc, rho_ptr=rho.ptr, flags=flags)
+        step += 1
+    ctx.sync()
+    live0 = abi.count_live(ctx, P)
+    ctx.profile(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sp.niter = step
+        abi.advance(ctx, fields, sp, P, mcc=mcc, rho_ptr=rho.ptr, flags=flags)
+        step += 1
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    kern_ms, launches = ctx.profile_read()
+    ctx.profile(False)
+    live1 = abi.count_live(ctx, P)
+    flags_dev = ctx.status(clear=True)
+    live_mean = 0.5 * (live0 + live1)
+    value = live_mean * args.steps / elapsed
+    Nc = args.grid * args.grid
+    kern_avg_s = kern_ms / max(launches, 1) * 1e-3
+    bytes_per_launch = 40.0 * live_mean + 24.0 * Nc  # 2 x 20 B SoA state + 4 B x 2 potentials + 16 B deposit grid
+    achieved = bytes_per_launch / kern_avg_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "RF 22-pole ion trap with He buffer-gas MCC (push+gather+MCC+boundary+"
+                               "CIC deposit per step)",
+                   "particles": N, "live_particles_mean": live_mean, "grid": [args.grid, args.grid],
+                   "dt": 1e-9, "layout": "SoA FP32 x,z,vx,vy,vz (20 B); int64 fixed-point deposit",
+                   "device_flags": flags_dev},
+        "roofline": {"kernel": "k_advance2d", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_avg_us": kern_avg_s * 1e6, "bytes_per_launch": bytes_per_launch},
+        "poisson": poisson,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(fields, params, mask, args.grid, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

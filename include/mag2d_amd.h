@@ -1,0 +1,304 @@
+/*
+ * mag2d_amd.h -- C ABI of the MI355X-native PIC-MCC inner loop of mag2d.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No
+ * call throws.  Every entry point below names the reference interface it
+ * replaces (rouckas/mag2d, file:line).  The C++ host layer
+ * (mag2d_amd/host/mag2d.hpp) rebuilds the reference class API
+ * (Species / Field2D / Fields / t_grid) on top of these calls.
+ *
+ * Conventions
+ *  - Units are SI: positions in m, velocities in m/s, charge density in C per
+ *    node (Field2D::accumulate), potentials in V.
+ *  - Grids are row-major double arrays data[i*nz + j], z fastest, like
+ *    Array2D (Array.hpp:7-23).
+ *  - Particles are structure-of-arrays FP32 (x, z, vx, vy, vz); an empty slot
+ *    has x == M2D_DEAD (a NaN payload), the counterpart of t_particle::empty
+ *    (particles.hpp:32).
+ *  - All device work is queued on the context's stream; calls that return a
+ *    host value synchronise that stream.  One host thread per context.
+ *  - Device pointers in structs are device memory from m2d_malloc (or any
+ *    allocation of the same device, e.g. a torch tensor's data_ptr()).
+ */
+#ifndef MAG2D_AMD_H
+#define MAG2D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M2D_ABI_VERSION 1
+
+/* ---- status codes (the reference throws std::runtime_error instead) ---- */
+#define M2D_OK                 0
+#define M2D_ERR_BAD_ARG       -1
+#define M2D_ERR_OUT_OF_RANGE  -2   /* Field2D::accumulate "outside of range" (Field2D.hpp:55-56) */
+#define M2D_ERR_NAN           -3
+#define M2D_ERR_HIP           -4
+#define M2D_ERR_NOT_CONVERGED -5
+#define M2D_ERR_UNSUPPORTED   -6
+#define M2D_ERR_NO_MEMORY     -7
+
+/* device status flags accumulated by kernels (m2d_ctx_status) */
+#define M2D_FLAG_OUT_OF_RANGE  1u  /* a gather/deposit index left the grid (value clamped / skipped) */
+#define M2D_FLAG_NAN           2u  /* a live particle had a NaN position and was removed */
+#define M2D_FLAG_PARTNER       4u  /* MCC found no live partner particle in the retry budget */
+
+/* mask values (fields.hpp:20) */
+#define M2D_FIXED    0
+#define M2D_FIXED_RF 1
+#define M2D_FREE     2
+#define M2D_BOUNDARY 3
+
+/* empty-slot marker stored in x (bit pattern of a quiet NaN) */
+#define M2D_DEAD_BITS 0xFFC0DEADu
+
+typedef struct m2d_ctx m2d_ctx;
+typedef struct m2d_fields2d m2d_fields2d;
+typedef struct m2d_mcc m2d_mcc;
+
+/* ======================= context, errors, memory ======================== */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *m2d_last_error(void);
+int m2d_abi_version(void);
+
+/* Opens `device` and creates a stream of default priority; `seed` keys the
+ * counter-based RNG (replaces the shared t_random of pic.cpp:124). */
+int m2d_ctx_create(int device, uint64_t seed, m2d_ctx **out);
+int m2d_ctx_destroy(m2d_ctx *ctx);
+/* Queue all later work on `stream` (a hipStream_t of the same device, e.g.
+ * torch.cuda.current_stream().cuda_stream); NULL restores the own stream. */
+int m2d_ctx_set_stream(m2d_ctx *ctx, void *stream);
+void *m2d_ctx_stream(m2d_ctx *ctx);
+uint64_t m2d_ctx_seed(m2d_ctx *ctx);
+int m2d_ctx_synchronize(m2d_ctx *ctx);
+/* OR of M2D_FLAG_* raised since the last clear; synchronises. */
+int m2d_ctx_status(m2d_ctx *ctx, uint32_t *flags, int clear);
+/* Kernel timing (the counterpart of t_timer around Pic::advance, pic.cpp:332,355):
+ * while enabled, every launch of the particle pass is bracketed by HIP events
+ * on the context's stream.  _read synchronises, returns the summed kernel
+ * time in ms and the launch count, and clears them. */
+int m2d_ctx_profile(m2d_ctx *ctx, int enable);
+int m2d_ctx_profile_read(m2d_ctx *ctx, double *ms, int64_t *launches);
+
+int m2d_malloc(m2d_ctx *ctx, size_t bytes, void **out);
+int m2d_free(m2d_ctx *ctx, void *p);
+int m2d_memcpy_h2d(m2d_ctx *ctx, void *dst, const void *src, size_t bytes);   /* synchronous */
+int m2d_memcpy_d2h(m2d_ctx *ctx, void *dst, const void *src, size_t bytes);   /* synchronous */
+int m2d_memcpy_d2d(m2d_ctx *ctx, void *dst, const void *src, size_t bytes);   /* stream-ordered */
+/* Fill with a byte value by a kernel (no hipMemsetAsync: safe under capture). */
+int m2d_memset(m2d_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ======================= grid and geometry (host) ======================= */
+
+/* Field2D geometry (Field2D.hpp:16-19); dx = x_max/(x_sampl-1) (param.cpp:126-127) */
+typedef struct {
+    int32_t nx, nz;        /* x_sampl, z_sampl (Array2D jmax, lmax) */
+    double dx, dz;
+    double xmin, zmin;
+} m2d_grid2d;
+
+/* geometries of Param::geometry (param.hpp:21-22) */
+enum { M2D_GEO_EMPTY = 0, M2D_GEO_PROBE, M2D_GEO_RF_22PT, M2D_GEO_RF_8PT, M2D_GEO_RF_HAITRAP,
+       M2D_GEO_RF_QUAD, M2D_GEO_MAC, M2D_GEO_PENNING, M2D_GEO_PENNING_SIMPLE, M2D_GEO_TUBE };
+
+typedef struct {
+    int32_t geometry;
+    double x_max, z_max;     /* Param::x_max, z_max */
+    double probe_radius;     /* param.cpp:31 */
+    double u_probe;          /* param.cpp:33 */
+    double extern_field;     /* param.cpp:34 */
+} m2d_geom2d;
+
+/* t_grid::t_grid + builders (fields.cpp:374-833): fills mask[nx*nz] and
+ * voltage[nx*nz] (host arrays). */
+int m2d_grid_build_2d(const m2d_grid2d *g, const m2d_geom2d *p, uint8_t *mask, double *voltage);
+/* t_grid::square_electrode (fields.cpp:835-850) */
+int m2d_grid_square_electrode_2d(const m2d_grid2d *g, uint8_t *mask, double *voltage,
+                                 double rmin, double rmax, double zmin, double zmax,
+                                 double v, int mask_type);
+/* t_grid::circle_electrode (fields.cpp:852-868) */
+int m2d_grid_circle_electrode_2d(const m2d_grid2d *g, uint8_t *mask, double *voltage,
+                                 double xc, double zc, double radius, double v, int mask_type);
+/* BOUNDARY marking of the builders (e.g. fields.cpp:507-518) */
+int m2d_grid_mark_boundary_2d(const m2d_grid2d *g, uint8_t *mask);
+/* t_grid::is_free (fields.hpp:94-101), host; cells on the last row/column are not free */
+int m2d_grid_is_free_2d(const m2d_grid2d *g, const uint8_t *mask, double x, double z);
+
+/* ======================= particles ====================================== */
+
+/* Species particle store as SoA (replaces vector<t_particle>, particles.hpp:26-33,113) */
+typedef struct {
+    float *x, *z;          /* in-plane position (reference x, z) */
+    float *vx, *vy, *vz;   /* vy is the out-of-plane component */
+    int64_t n;             /* slots, live + empty */
+} m2d_particles2d;
+
+/* per-species constants of one step (BaseSpecies members, particles.hpp:118-131) */
+typedef struct {
+    double charge, mass, dt;
+    double lifetime;       /* 1/sum(rates); INFINITY disables MCC */
+    uint64_t niter;        /* Species::niter: E at niter*dt, RNG counter */
+    uint32_t stream;       /* RNG stream id of this species */
+    uint32_t reserved;
+} m2d_species2d;
+
+/* ======================= fields ========================================== */
+
+enum { M2D_BC_FREE = 0, M2D_BC_PERIODIC = 1 };
+
+/* Param members read by Fields and Species::advance_boundary (param.hpp) */
+typedef struct {
+    m2d_grid2d grid;
+    double x_max, z_max;             /* box is [xmin, x_max] x [zmin, z_max] (param.cpp:23) */
+    int32_t geometry_empty;          /* Param::geometry == EMPTY (fields.hpp:126) */
+    int32_t selfconsistent;
+    int32_t electric_field_from_file;
+    int32_t boundary;                /* M2D_BC_FREE / M2D_BC_PERIODIC */
+    double extern_field;
+    int32_t rf;
+    double rf_amplitude, rf_omega, rf_U0;
+    double Br, Bz, Bt;               /* magnetic_field_const (fields.hpp:154-169) */
+    double dV;                       /* Param::dV (param.cpp:134-136) */
+} m2d_fields2d_params;
+
+enum { M2D_FIELD_U = 0, M2D_FIELD_URF = 1, M2D_FIELD_RHO = 2, M2D_FIELD_VOLTAGE = 3 };
+
+/* Fields::Fields (fields.cpp:115-276): uploads mask and voltage, builds the
+ * multigrid hierarchy that replaces the UMFPACK factorisation. */
+int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_t *mask,
+                        const double *voltage, m2d_fields2d **out);
+int m2d_fields2d_destroy(m2d_fields2d *f);
+/* device pointer of Fields::u / uRF / rho / grid.voltage (double[nx*nz]) */
+double *m2d_fields2d_data(m2d_fields2d *f, int which);
+int m2d_fields2d_upload(m2d_fields2d *f, int which, const double *host);
+int m2d_fields2d_download(m2d_fields2d *f, int which, double *host);
+/* Rebuild the gather tables after u or uRF changed outside a solve
+ * (e.g. Field2D::load for electric_field_from_file, pic.cpp:154-158). */
+int m2d_fields2d_update(m2d_fields2d *f);
+/* Fields::reset (fields.cpp:355-360) */
+int m2d_fields2d_reset(m2d_fields2d *f);
+
+enum { M2D_SOLVE_STATIC = 0, M2D_SOLVE_RF = 1, M2D_SOLVE_RAW = 2 };
+typedef struct {
+    int32_t outer_iters;      /* FP64 refinement steps */
+    int32_t inner_iters;      /* FP32 MG-PCG iterations in total */
+    double rel_residual;      /* ||b - A u||_2 / ||b_eff||_2 in FP64 over free nodes */
+} m2d_solve_info;
+/* Fields::boundary_solve (STATIC -> u, fields.cpp:278-312), boundary_solve_rf
+ * (RF -> uRF, fields.cpp:314-348), solve (RAW -> u, fields.cpp:350-353).
+ * Builds the rhs in place on Fields::rho like the reference, then solves to
+ * FP64 relative residual `tol` (<= 0: 1e-10). */
+int m2d_fields2d_solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info);
+/* Fields::E at host-given points (fields.hpp:124-150), FP64 inputs, for tests */
+int m2d_fields2d_E(m2d_fields2d *f, const double *x, const double *z, int64_t n, double t,
+                   double *ex, double *ez);
+
+/* ======================= Field2D kernels ================================= */
+
+/* Field2D::accumulate for all live particles (Species::accumulate,
+ * particles.hpp:358-367): rho[nx*nz] += charge * CIC weights.  Deterministic:
+ * weights are summed as 64-bit fixed point (2^-32) and converted once. */
+int m2d_accumulate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_particles2d *p,
+                      double charge, double *rho);
+/* Field2D::interpolate / grad (Field2D.hpp:64-168) at n FP64 points, device arrays */
+int m2d_interpolate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const double *data, const double *x,
+                       const double *z, int64_t n, double *out);
+int m2d_grad_2d(m2d_ctx *ctx, const m2d_grid2d *g, const double *data, const double *x,
+                const double *z, int64_t n, double *gx, double *gz);
+
+/* ======================= MCC ============================================= */
+
+/* interaction types (parser.hpp:15) */
+enum { M2D_ELASTIC = 0, M2D_LANGEVIN = 1, M2D_CX = 2, M2D_COULOMB = 3, M2D_SUPERELASTIC = 4 };
+
+/* one INTERACTION block of the species conf (parser.cpp:97-143), values as
+ * written there: DE in eV, RATE raw (the library applies the LANGEVIN
+ * cutoff^2 factor of particles.hpp:81-82) */
+typedef struct {
+    int32_t type;
+    int32_t target;           /* index into m2d_mcc_desc.targets */
+    double DE;
+    double rate;
+    double cutoff;
+    int32_t tab_offset, tab_len;  /* CROSS_SECTION rows in tab_E / tab_sigma */
+} m2d_interaction;
+
+/* one secondary species (speclist entry, particles.hpp:113-131) */
+typedef struct {
+    double mass, density, temperature, charge;
+    /* live ensemble of a particle species (random_particle, particles.hpp:201-211);
+     * x == NULL for a continuum sampled by rndv (particles.hpp:188-193) */
+    m2d_particles2d partner;
+} m2d_target;
+
+typedef struct {
+    double mass;              /* primary species */
+    double charge, temperature, density;  /* primary (COULOMB, particles.cpp:19-26) */
+    double E_max;             /* eV; svmax is searched up to veV(E_max) (particles.cpp:163) */
+    int32_t n_targets;
+    const m2d_target *targets;
+    int32_t n_inter;
+    const m2d_interaction *inter;
+    int32_t n_tab;
+    const double *tab_E;      /* eV */
+    const double *tab_sigma;  /* m^2 */
+} m2d_mcc_desc;
+
+/* BaseSpecies::lifetime_init + svmax_find (particles.cpp:142-206) and the
+ * device tables for BaseSpecies::scatter (particles.cpp:208-365). */
+int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out);
+int m2d_mcc_destroy(m2d_mcc *m);
+double m2d_mcc_lifetime(const m2d_mcc *m);
+int m2d_mcc_rates(const m2d_mcc *m, double *rates_by_species);
+/* point a particle target at its current store (after insert/compact/sort) */
+int m2d_mcc_set_partner(m2d_mcc *m, int32_t target, const m2d_particles2d *p);
+
+/* ======================= species step ==================================== */
+
+#define M2D_STEP_PUSH      1u   /* advance_boris incl. the MCC trial (particles.cpp:924-995) */
+#define M2D_STEP_BOUNDARY  2u   /* advance_boundary (particles.hpp:370-406) */
+#define M2D_STEP_DEPOSIT   4u   /* rho.accumulate in advance_boundary (particles.hpp:408-409) */
+
+typedef struct {
+    int64_t removed;          /* particles removed by the boundary pass */
+    int64_t collisions;       /* MCC trials that called scatter */
+} m2d_step_stats;
+
+/* Species<CARTESIAN>::advance (particles.hpp:342-349) fused into one pass:
+ * E/B gather + Boris + drift + MCC trial/scatter, then box test, is_free and
+ * (flag DEPOSIT) CIC deposit of the survivors into rho (C/node, added).
+ * Does not increment niter (the caller owns Species::niter).  `mcc` may be
+ * NULL.  `stats` may be NULL (no synchronisation then). */
+int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
+                   const m2d_particles2d *p, const m2d_mcc *mcc, double *rho,
+                   uint32_t flags, m2d_step_stats *stats);
+/* Species<CARTESIAN>::advance_boris_init (particles.cpp:997-1050) */
+int m2d_advance_init_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
+                        const m2d_particles2d *p);
+/* energy_dist.mean_tot() after energy_dist_compute (particles.cpp:408-414,
+ * histogram.cpp:68-71): mean kinetic energy in eV over live particles */
+int m2d_mean_energy_2d(m2d_ctx *ctx, const m2d_particles2d *p, double mass, double *mean_eV,
+                       int64_t *n_live);
+/* n_particles() (particles.hpp:248) */
+int m2d_count_live_2d(m2d_ctx *ctx, const m2d_particles2d *p, int64_t *n_live);
+/* Stable stream compaction src -> dst (dst has room for src->n), replacing
+ * the free list of BaseSpecies::remove/insert (particles.hpp:223-247). */
+int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles2d *dst,
+                   int64_t *n_live);
+/* Species<CARTESIAN>::add_particles_on_disk (particles.cpp:728-749) into the
+ * n slots of p: uniform on the disk by rejection, Maxwellian velocities of
+ * thermal speed vth = v_max/sqrt(2) (rndv, particles.hpp:188-193).  Samples
+ * outside the box become empty slots, as the reference skips them. */
+int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p,
+                                 double cx, double cz, double radius, double vth,
+                                 uint32_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

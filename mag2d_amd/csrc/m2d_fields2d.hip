@@ -1,0 +1,862 @@
+// m2d_fields2d.hip -- Fields: device grids, gather tables and the Poisson
+// solve that replaces the UMFPACK factorisation of Fields::Fields
+// (fields.cpp:115-276) and the back-solves of boundary_solve /
+// boundary_solve_rf / solve (fields.cpp:278-353).
+//
+// Solver: FP64 iterative refinement around an FP32 preconditioned conjugate
+// gradient on the Dirichlet-eliminated SPD system 4u_k - sum_free u_n = ...,
+// preconditioned by one symmetric multigrid V-cycle (red-black Gauss-Seidel
+// on the finest 5-point level, weighted Jacobi on Galerkin 9-point coarse
+// levels, dense inverse on the coarsest).  Each outer step computes the FP64
+// residual r = b - A u with the reference's own rows, solves A'e = -r in
+// FP32, and adds e in FP64, until ||r|| <= tol ||b_eff||.
+#include "m2d_objects.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace m2d {
+
+// ------------------------------------------------------------------ tables
+// Edge gradients of u and uRF, padded so Field2D::grad's first/last-row
+// fallbacks (Field2D.hpp:106-124,148-166) are the bilinear formula itself.
+__global__ void k_build_gx(const double *u, const double *urf, int nx, int nz, double idx,
+                           float2 *gx)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int e = blockIdx.y;  // 0..nx
+    if (j >= nz || e > nx) return;
+    int ee = e < 1 ? 1 : (e > nx - 1 ? nx - 1 : e);
+    size_t k1 = (size_t)ee * nz + j, k0 = k1 - nz;
+    float2 v;
+    v.x = (float)((u[k1] - u[k0]) * idx);
+    v.y = (float)((urf[k1] - urf[k0]) * idx);
+    gx[(size_t)e * nz + j] = v;
+}
+
+__global__ void k_build_gz(const double *u, const double *urf, int nx, int nz, double idz,
+                           float2 *gz)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+    int i = blockIdx.y;
+    if (e > nz || i >= nx) return;
+    int ee = e < 1 ? 1 : (e > nz - 1 ? nz - 1 : e);
+    size_t k1 = (size_t)i * nz + ee, k0 = k1 - 1;
+    float2 v;
+    v.x = (float)((u[k1] - u[k0]) * idz);
+    v.y = (float)((urf[k1] - urf[k0]) * idz);
+    gz[(size_t)i * (nz + 1) + e] = v;
+}
+
+int fields_build_tables(m2d_fields2d *f)
+{
+    hipStream_t s = f->ctx->stream;
+    dim3 b(256);
+    k_build_gx<<<dim3((f->nz + 255) / 256, f->nx + 1), b, 0, s>>>(f->u, f->urf, f->nx, f->nz, f->idx, f->gx);
+    k_build_gz<<<dim3((f->nz + 1 + 255) / 256, f->nx), b, 0, s>>>(f->u, f->urf, f->nx, f->nz, f->idz, f->gz);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+// ------------------------------------------------------------------ FP64 kernels
+__device__ __forceinline__ bool is_dir(uint8_t m) { return m == M2D_FIXED || m == M2D_FIXED_RF; }
+
+// rhs in place (fields.cpp:297-310, 333-346), Dirichlet values into u
+__global__ void k_rhs(const uint8_t *mask, const double *voltage, double *rho, double *u, size_t n,
+                      int mode, double scale)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint8_t m = mask[k];
+    double b;
+    if (mode == M2D_SOLVE_RAW) b = rho[k];
+    else if (m == M2D_FIXED) b = mode == M2D_SOLVE_STATIC ? voltage[k] : 0.0;
+    else if (m == M2D_FIXED_RF) b = mode == M2D_SOLVE_RF ? voltage[k] : 0.0;
+    else b = rho[k] * scale;
+    rho[k] = b;
+    if (is_dir(m)) u[k] = b;
+}
+
+// r = b - A u on free nodes (rows of fields.cpp:229-237), 0 on Dirichlet nodes;
+// also d = -r in FP32 and per-block partial sums of r^2 and b_eff^2
+__global__ void __launch_bounds__(256) k_residual64(const uint8_t *mask, const double *b,
+                                                    const double *u, int nx, int nz, double *r,
+                                                    float *d, double *partial)
+{
+    __shared__ double s1[256], s2[256];
+    size_t n = (size_t)nx * nz;
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0, bb = 0;
+    if (k < n) {
+        double rv = 0;
+        if (!is_dir(mask[k])) {
+            int i = (int)(k / nz), j = (int)(k % nz);
+            // free rows never sit on the frame (checked at creation)
+            double au = u[k - nz] + u[k - 1] - 4.0 * u[k] + u[k + 1] + u[k + nz];
+            rv = b[k] - au;
+            double be = b[k];
+            if (is_dir(mask[k - nz])) be -= u[k - nz];
+            if (is_dir(mask[k - 1])) be -= u[k - 1];
+            if (is_dir(mask[k + 1])) be -= u[k + 1];
+            if (is_dir(mask[k + nz])) be -= u[k + nz];
+            bb = be * be;
+            (void)i; (void)j;
+        }
+        r[k] = rv;
+        d[k] = (float)(-rv);
+        rr = rv * rv;
+    }
+    s1[threadIdx.x] = rr;
+    s2[threadIdx.x] = bb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
+}
+
+// u += e on free nodes
+__global__ void k_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && !is_dir(mask[k])) u[k] += (double)e[k];
+}
+
+// fixed-order sum of `np` pairs of partials into out[0..1]
+__global__ void __launch_bounds__(1024) k_sum_pairs(const double *partial, int np, double *out)
+{
+    __shared__ double s1[1024], s2[1024];
+    double a = 0, b = 0;
+    for (int k = threadIdx.x; k < np; k += 1024) { a += partial[2 * k]; b += partial[2 * k + 1]; }
+    s1[threadIdx.x] = a; s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = s1[0]; out[1] = s2[0]; }
+}
+
+// ------------------------------------------------------------------ FP32 MG kernels
+// 9-point operator y = A x (Dirichlet rows give 0)
+__device__ __forceinline__ float apply9(const float *st, const float *x, int i, int j, int nx, int nz)
+{
+    size_t k = (size_t)i * nz + j;
+    const float *c = st + 9 * k;
+    float s = 0.f;
+#pragma unroll
+    for (int di = -1; di <= 1; di++) {
+        int ii = i + di;
+        if (ii < 0 || ii >= nx) continue;
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++) {
+            int jj = j + dj;
+            if (jj < 0 || jj >= nz) continue;
+            float a = c[(di + 1) * 3 + (dj + 1)];
+            if (a != 0.f) s = fmaf(a, x[(size_t)ii * nz + jj], s);
+        }
+    }
+    return s;
+}
+
+// finest level: 5-point operator from the mask, A' = 4I - adjacency(free)
+__device__ __forceinline__ float apply5(const uint8_t *dir, const float *x, size_t k, int nz)
+{
+    float s = 4.f * x[k];
+    if (!dir[k - nz]) s -= x[k - nz];
+    if (!dir[k - 1]) s -= x[k - 1];
+    if (!dir[k + 1]) s -= x[k + 1];
+    if (!dir[k + nz]) s -= x[k + nz];
+    return s;
+}
+
+// red-black Gauss-Seidel half sweep on the finest level (colour = (i+j)&1)
+__global__ void k_rbgs5(const uint8_t *dir, float *x, const float *b, int nx, int nz, int colour)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nz || ((i + j) & 1) != colour) return;
+    size_t k = (size_t)i * nz + j;
+    if (dir[k]) return;
+    float s = b[k];
+    if (!dir[k - nz]) s += x[k - nz];
+    if (!dir[k - 1]) s += x[k - 1];
+    if (!dir[k + 1]) s += x[k + 1];
+    if (!dir[k + nz]) s += x[k + nz];
+    x[k] = 0.25f * s;
+}
+
+__global__ void k_resid5(const uint8_t *dir, const float *x, const float *b, float *r, int nx, int nz)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nz) return;
+    size_t k = (size_t)i * nz + j;
+    r[k] = dir[k] ? 0.f : b[k] - apply5(dir, x, k, nz);
+}
+
+__global__ void k_apply5(const uint8_t *dir, const float *x, float *y, int nx, int nz)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nz) return;
+    size_t k = (size_t)i * nz + j;
+    y[k] = dir[k] ? 0.f : apply5(dir, x, k, nz);
+}
+
+// weighted Jacobi on a 9-point level: xo = xi + w D^-1 (b - A xi)
+__global__ void k_jacobi9(const float *st, const float *dinv, const float *xi, float *xo,
+                          const float *b, int nx, int nz, float w)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nz) return;
+    size_t k = (size_t)i * nz + j;
+    float dv = dinv[k];
+    xo[k] = dv == 0.f ? 0.f : xi[k] + w * dv * (b[k] - apply9(st, xi, i, j, nx, nz));
+}
+
+__global__ void k_resid9(const float *st, const float *dinv, const float *x, const float *b, float *r,
+                         int nx, int nz)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nz) return;
+    size_t k = (size_t)i * nz + j;
+    r[k] = dinv[k] == 0.f ? 0.f : b[k] - apply9(st, x, i, j, nx, nz);
+}
+
+// b_c = P^T r_f with P the bilinear interpolation masked on fine Dirichlet nodes
+__global__ void k_restrict(const float *rf, int nxf, int nzf, float *bc, const float *dinvc, int nxc,
+                           int nzc)
+{
+    int J = blockIdx.x * blockDim.x + threadIdx.x;
+    int I = blockIdx.y;
+    if (J >= nzc) return;
+    size_t K = (size_t)I * nzc + J;
+    if (dinvc[K] == 0.f) { bc[K] = 0.f; return; }
+    float s = 0.f;
+    for (int di = -1; di <= 1; di++) {
+        int i = 2 * I + di;
+        if (i < 0 || i >= nxf) continue;
+        float wi = di == 0 ? 1.f : 0.5f;
+        for (int dj = -1; dj <= 1; dj++) {
+            int j = 2 * J + dj;
+            if (j < 0 || j >= nzf) continue;
+            float wj = dj == 0 ? 1.f : 0.5f;
+            s = fmaf(wi * wj, rf[(size_t)i * nzf + j], s);  // r is 0 on Dirichlet nodes
+        }
+    }
+    bc[K] = s;
+}
+
+// x_f += P x_c on free fine nodes
+__global__ void k_prolong(float *xf, const uint8_t *dirf, int nxf, int nzf, const float *xc, int nxc,
+                          int nzc)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nzf) return;
+    size_t k = (size_t)i * nzf + j;
+    if (dirf[k]) return;
+    int I0 = i >> 1, J0 = j >> 1;
+    float s = 0.f;
+    if ((i & 1) == 0) {
+        if ((j & 1) == 0) s = xc[(size_t)I0 * nzc + J0];
+        else {
+            s = 0.5f * xc[(size_t)I0 * nzc + J0];
+            if (J0 + 1 < nzc) s += 0.5f * xc[(size_t)I0 * nzc + J0 + 1];
+        }
+    } else {
+        float a = 0.f, b = 0.f;
+        if ((j & 1) == 0) {
+            a = xc[(size_t)I0 * nzc + J0];
+            if (I0 + 1 < nxc) b = xc[(size_t)(I0 + 1) * nzc + J0];
+        } else {
+            a = 0.5f * xc[(size_t)I0 * nzc + J0];
+            if (J0 + 1 < nzc) a += 0.5f * xc[(size_t)I0 * nzc + J0 + 1];
+            if (I0 + 1 < nxc) {
+                b = 0.5f * xc[(size_t)(I0 + 1) * nzc + J0];
+                if (J0 + 1 < nzc) b += 0.5f * xc[(size_t)(I0 + 1) * nzc + J0 + 1];
+            }
+        }
+        s = 0.5f * (a + b);
+    }
+    xf[k] += s;
+}
+
+// dense coarsest solve x = A^-1 b, one block
+__global__ void __launch_bounds__(256) k_coarse_solve(const float *inv, const float *b, float *x, int n)
+{
+    extern __shared__ float sb[];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sb[k] = b[k];
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        const float *a = inv + (size_t)row * n;
+        float s = 0.f;
+        for (int c = 0; c < n; c++) s = fmaf(a[c], sb[c], s);
+        x[row] = s;
+    }
+}
+
+// ------------------------------------------------------------------ PCG helpers
+// per-block partial dot products (FP64 accumulation) of up to two pairs
+__global__ void __launch_bounds__(256) k_dot2(const float *a, const float *b, const float *c,
+                                              const float *d, size_t n, double *partial)
+{
+    __shared__ double s1[256], s2[256];
+    double x = 0, y = 0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        x += (double)a[k] * b[k];
+        if (c) y += (double)c[k] * d[k];
+    }
+    s1[threadIdx.x] = x; s2[threadIdx.x] = y;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
+}
+
+// e += alpha p ; r -= alpha q with alpha = dots[0] / dots[1] read on the device
+__global__ void k_cg_update(float *e, float *r, const float *p, const float *q, const double *sc,
+                            size_t n)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double den = sc[1];
+    float alpha = den != 0.0 ? (float)(sc[0] / den) : 0.f;
+    e[k] = fmaf(alpha, p[k], e[k]);
+    r[k] = fmaf(-alpha, q[k], r[k]);
+}
+
+// p = z + beta p, beta = rz_new / rz_old
+__global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old,
+                         size_t n, int first)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (first) { p[k] = z[k]; return; }
+    double o = *rz_old;
+    float beta = o != 0.0 ? (float)(*rz_new / o) : 0.f;
+    p[k] = fmaf(beta, p[k], z[k]);
+}
+
+__global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
+
+// ------------------------------------------------------------------ host: hierarchy
+static void p1d(int i, int nc, int I[2], float w[2], int &cnt)
+{
+    cnt = 0;
+    if ((i & 1) == 0) {
+        if (i / 2 < nc) { I[cnt] = i / 2; w[cnt++] = 1.f; }
+    } else {
+        if ((i - 1) / 2 < nc) { I[cnt] = (i - 1) / 2; w[cnt++] = 0.5f; }
+        if ((i + 1) / 2 < nc) { I[cnt] = (i + 1) / 2; w[cnt++] = 0.5f; }
+    }
+}
+
+// Galerkin A_c = P^T A_f P on the structured hierarchy, FP64 on the host
+static void galerkin(const std::vector<double> &Af, const std::vector<uint8_t> &dirf, int nxf, int nzf,
+                     std::vector<double> &Ac, std::vector<uint8_t> &dirc, int nxc, int nzc)
+{
+    Ac.assign((size_t)nxc * nzc * 9, 0.0);
+    for (int i = 0; i < nxf; i++)
+        for (int j = 0; j < nzf; j++) {
+            size_t k = (size_t)i * nzf + j;
+            if (dirf[k]) continue;
+            int Ii[2], Jj[2], ci, cj;
+            float wi[2], wj[2];
+            p1d(i, nxc, Ii, wi, ci);
+            p1d(j, nzc, Jj, wj, cj);
+            for (int di = -1; di <= 1; di++)
+                for (int dj = -1; dj <= 1; dj++) {
+                    double a = Af[9 * k + (di + 1) * 3 + (dj + 1)];
+                    if (a == 0.0) continue;
+                    int i2 = i + di, j2 = j + dj;
+                    if (i2 < 0 || i2 >= nxf || j2 < 0 || j2 >= nzf) continue;
+                    if (dirf[(size_t)i2 * nzf + j2]) continue;
+                    int Ii2[2], Jj2[2], ci2, cj2;
+                    float wi2[2], wj2[2];
+                    p1d(i2, nxc, Ii2, wi2, ci2);
+                    p1d(j2, nzc, Jj2, wj2, cj2);
+                    for (int u1 = 0; u1 < ci; u1++)
+                        for (int v1 = 0; v1 < cj; v1++) {
+                            double w1 = (double)wi[u1] * wj[v1];
+                            size_t K = (size_t)Ii[u1] * nzc + Jj[v1];
+                            for (int u2 = 0; u2 < ci2; u2++)
+                                for (int v2 = 0; v2 < cj2; v2++) {
+                                    int dI = Ii2[u2] - Ii[u1], dJ = Jj2[v2] - Jj[v1];
+                                    double w2 = (double)wi2[u2] * wj2[v2];
+                                    Ac[9 * K + (dI + 1) * 3 + (dJ + 1)] += w1 * a * w2;
+                                }
+                        }
+                }
+        }
+    dirc.assign((size_t)nxc * nzc, 0);
+    for (size_t K = 0; K < (size_t)nxc * nzc; K++)
+        if (!(Ac[9 * K + 4] > 0.0)) {
+            dirc[K] = 1;
+            for (int q = 0; q < 9; q++) Ac[9 * K + q] = 0.0;
+        }
+    // drop couplings to pinned coarse nodes
+    for (int I = 0; I < nxc; I++)
+        for (int J = 0; J < nzc; J++) {
+            size_t K = (size_t)I * nzc + J;
+            for (int dI = -1; dI <= 1; dI++)
+                for (int dJ = -1; dJ <= 1; dJ++) {
+                    int I2 = I + dI, J2 = J + dJ;
+                    if (I2 < 0 || I2 >= nxc || J2 < 0 || J2 >= nzc || dirc[(size_t)I2 * nzc + J2])
+                        Ac[9 * K + (dI + 1) * 3 + (dJ + 1)] = 0.0;
+                }
+        }
+}
+
+// Gauss-Jordan inverse of the SPD coarsest operator restricted to free nodes
+static bool dense_inverse(const std::vector<double> &A9, const std::vector<uint8_t> &dir, int nx, int nz,
+                          std::vector<float> &inv)
+{
+    int n = nx * nz;
+    std::vector<int> id;
+    for (int k = 0; k < n; k++) if (!dir[k]) id.push_back(k);
+    int m = (int)id.size();
+    std::vector<int> pos(n, -1);
+    for (int q = 0; q < m; q++) pos[id[q]] = q;
+    std::vector<double> a((size_t)m * 2 * m, 0.0);
+    for (int q = 0; q < m; q++) {
+        int k = id[q], i = k / nz, j = k % nz;
+        for (int di = -1; di <= 1; di++)
+            for (int dj = -1; dj <= 1; dj++) {
+                int i2 = i + di, j2 = j + dj;
+                if (i2 < 0 || i2 >= nx || j2 < 0 || j2 >= nz) continue;
+                int p = pos[i2 * nz + j2];
+                if (p < 0) continue;
+                a[(size_t)q * 2 * m + p] = A9[9 * (size_t)k + (di + 1) * 3 + (dj + 1)];
+            }
+        a[(size_t)q * 2 * m + m + q] = 1.0;
+    }
+    for (int c = 0; c < m; c++) {
+        int piv = c;
+        for (int r = c + 1; r < m; r++)
+            if (fabs(a[(size_t)r * 2 * m + c]) > fabs(a[(size_t)piv * 2 * m + c])) piv = r;
+        if (fabs(a[(size_t)piv * 2 * m + c]) < 1e-300) return false;
+        if (piv != c)
+            for (int t = 0; t < 2 * m; t++) std::swap(a[(size_t)c * 2 * m + t], a[(size_t)piv * 2 * m + t]);
+        double d = 1.0 / a[(size_t)c * 2 * m + c];
+        for (int t = 0; t < 2 * m; t++) a[(size_t)c * 2 * m + t] *= d;
+        for (int r = 0; r < m; r++) {
+            if (r == c) continue;
+            double f = a[(size_t)r * 2 * m + c];
+            if (f == 0.0) continue;
+            for (int t = 0; t < 2 * m; t++) a[(size_t)r * 2 * m + t] -= f * a[(size_t)c * 2 * m + t];
+        }
+    }
+    inv.assign((size_t)n * n, 0.f);
+    for (int q = 0; q < m; q++)
+        for (int p = 0; p < m; p++) inv[(size_t)id[q] * n + id[p]] = (float)a[(size_t)q * 2 * m + m + p];
+    return true;
+}
+
+static int dmalloc(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(M2D_ERR_NO_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return M2D_OK;
+}
+
+static int mg_setup(m2d_fields2d *f)
+{
+    Mg &mg = f->mg;
+    const int nx = f->nx, nz = f->nz;
+    // finest level operator from the mask
+    std::vector<uint8_t> dir((size_t)nx * nz);
+    for (size_t k = 0; k < dir.size(); k++) dir[k] = f->h_mask[k] == M2D_FIXED || f->h_mask[k] == M2D_FIXED_RF;
+    std::vector<double> A((size_t)nx * nz * 9, 0.0);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < nz; j++) {
+            size_t k = (size_t)i * nz + j;
+            if (dir[k]) continue;
+            A[9 * k + 4] = 4.0;
+            const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+            for (int q = 0; q < 4; q++) {
+                size_t k2 = (size_t)(i + di[q]) * nz + (j + dj[q]);
+                if (!dir[k2]) A[9 * k + (di[q] + 1) * 3 + (dj[q] + 1)] = -1.0;
+            }
+        }
+    std::vector<std::vector<double>> ops{A};
+    std::vector<std::vector<uint8_t>> dirs{dir};
+    std::vector<std::pair<int, int>> dims{{nx, nz}};
+    while (true) {
+        int cx = dims.back().first, cz = dims.back().second;
+        if ((size_t)cx * cz <= 256 || cx < 5 || cz < 5) break;
+        int nxc = (cx + 1) / 2, nzc = (cz + 1) / 2;
+        std::vector<double> Ac;
+        std::vector<uint8_t> dc;
+        galerkin(ops.back(), dirs.back(), cx, cz, Ac, dc, nxc, nzc);
+        ops.push_back(std::move(Ac));
+        dirs.push_back(std::move(dc));
+        dims.push_back({nxc, nzc});
+    }
+    mg.lv.resize(ops.size());
+    for (size_t l = 0; l < ops.size(); l++) {
+        MgLevel &L = mg.lv[l];
+        L.nx = dims[l].first;
+        L.nz = dims[l].second;
+        L.n = (size_t)L.nx * L.nz;
+        std::vector<float> st(9 * L.n), dinv(L.n);
+        for (size_t k = 0; k < L.n; k++) {
+            for (int q = 0; q < 9; q++) st[9 * k + q] = (float)ops[l][9 * k + q];
+            double d = ops[l][9 * k + 4];
+            dinv[k] = dirs[l][k] || !(d > 0) ? 0.f : (float)(1.0 / d);
+        }
+        M2D_TRY(dmalloc((void **)&L.st, sizeof(float) * 9 * L.n));
+        M2D_TRY(dmalloc((void **)&L.dinv, sizeof(float) * L.n));
+        M2D_TRY(dmalloc((void **)&L.dir, L.n));
+        M2D_TRY(dmalloc((void **)&L.x, sizeof(float) * L.n));
+        M2D_TRY(dmalloc((void **)&L.x2, sizeof(float) * L.n));
+        M2D_TRY(dmalloc((void **)&L.r, sizeof(float) * L.n));
+        M2D_TRY(dmalloc((void **)&L.b, sizeof(float) * L.n));
+        M2D_HIP(hipMemcpy(L.st, st.data(), sizeof(float) * 9 * L.n, hipMemcpyHostToDevice));
+        M2D_HIP(hipMemcpy(L.dinv, dinv.data(), sizeof(float) * L.n, hipMemcpyHostToDevice));
+        M2D_HIP(hipMemcpy(L.dir, dirs[l].data(), L.n, hipMemcpyHostToDevice));
+    }
+    std::vector<float> inv;
+    const MgLevel &C = mg.lv.back();
+    M2D_CHECK(dense_inverse(ops.back(), dirs.back(), C.nx, C.nz, inv), M2D_ERR_BAD_ARG,
+              "Fields: coarsest multigrid operator is singular");
+    mg.coarse_n = (int)C.n;
+    M2D_TRY(dmalloc((void **)&mg.coarse_inv, sizeof(float) * inv.size()));
+    M2D_HIP(hipMemcpy(mg.coarse_inv, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice));
+    const size_t n = f->n;
+    M2D_TRY(dmalloc((void **)&mg.pr, sizeof(float) * n));
+    M2D_TRY(dmalloc((void **)&mg.pz, sizeof(float) * n));
+    M2D_TRY(dmalloc((void **)&mg.pp, sizeof(float) * n));
+    M2D_TRY(dmalloc((void **)&mg.pq, sizeof(float) * n));
+    M2D_TRY(dmalloc((void **)&mg.pe, sizeof(float) * n));
+    M2D_TRY(dmalloc((void **)&mg.r64, sizeof(double) * n));
+    M2D_TRY(dmalloc((void **)&mg.dots, sizeof(double) * 16));
+    M2D_HIP(hipHostMalloc((void **)&mg.h_dots, sizeof(double) * 16));
+    mg.npartial = (int)std::min<size_t>((n + 255) / 256, 65536);
+    M2D_TRY(dmalloc((void **)&mg.partial, sizeof(double) * 2 * std::max<size_t>(mg.npartial, (n + 255) / 256)));
+    return M2D_OK;
+}
+
+static void mg_free(Mg &mg)
+{
+    for (MgLevel &L : mg.lv)
+        for (void *p : {(void *)L.st, (void *)L.dinv, (void *)L.dir, (void *)L.x, (void *)L.x2,
+                        (void *)L.r, (void *)L.b})
+            if (p) (void)hipFree(p);
+    mg.lv.clear();
+    for (void *p : {(void *)mg.coarse_inv, (void *)mg.pr, (void *)mg.pz, (void *)mg.pp, (void *)mg.pq,
+                    (void *)mg.pe, (void *)mg.r64, (void *)mg.dots, (void *)mg.partial})
+        if (p) (void)hipFree(p);
+    if (mg.h_dots) (void)hipHostFree(mg.h_dots);
+    mg = Mg();
+}
+
+// ------------------------------------------------------------------ host: V-cycle
+constexpr int kPre = 2, kPost = 2;
+constexpr float kOmega = 0.8f;
+
+static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
+{
+    Mg &mg = f->mg;
+    hipStream_t s = f->ctx->stream;
+    const int L = (int)mg.lv.size();
+    // level 0 works on (out, rhs); coarse levels on (x, b)
+    for (int l = 0; l < L - 1; l++) {
+        MgLevel &A = mg.lv[l];
+        float *x = l == 0 ? out : A.x;
+        const float *b = l == 0 ? rhs : A.b;
+        dim3 g((A.nz + 255) / 256, A.nx);
+        M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * A.n));
+        if (l == 0) {
+            for (int it = 0; it < kPre; it++) {
+                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 0);
+                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 1);
+            }
+            k_resid5<<<g, 256, 0, s>>>(A.dir, x, b, A.r, A.nx, A.nz);
+        } else {
+            float *xa = x, *xb = A.x2;
+            for (int it = 0; it < kPre; it++) {
+                k_jacobi9<<<g, 256, 0, s>>>(A.st, A.dinv, xa, xb, b, A.nx, A.nz, kOmega);
+                std::swap(xa, xb);
+            }
+            if (xa != x) M2D_HIP(hipMemcpyAsync(x, xa, sizeof(float) * A.n, hipMemcpyDeviceToDevice, s));
+            k_resid9<<<g, 256, 0, s>>>(A.st, A.dinv, x, b, A.r, A.nx, A.nz);
+        }
+        MgLevel &C = mg.lv[l + 1];
+        k_restrict<<<dim3((C.nz + 255) / 256, C.nx), 256, 0, s>>>(A.r, A.nx, A.nz, C.b, C.dinv, C.nx, C.nz);
+    }
+    MgLevel &C = mg.lv[L - 1];
+    {
+        float *x = L == 1 ? out : C.x;
+        const float *b = L == 1 ? rhs : C.b;
+        k_coarse_solve<<<1, 256, sizeof(float) * mg.coarse_n, s>>>(mg.coarse_inv, b, x, mg.coarse_n);
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        MgLevel &A = mg.lv[l];
+        MgLevel &Cc = mg.lv[l + 1];
+        float *x = l == 0 ? out : A.x;
+        const float *b = l == 0 ? rhs : A.b;
+        dim3 g((A.nz + 255) / 256, A.nx);
+        k_prolong<<<g, 256, 0, s>>>(x, A.dir, A.nx, A.nz, Cc.x, Cc.nx, Cc.nz);
+        if (l == 0) {
+            for (int it = 0; it < kPost; it++) {
+                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 1);
+                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 0);
+            }
+        } else {
+            float *xa = x, *xb = A.x2;
+            for (int it = 0; it < kPost; it++) {
+                k_jacobi9<<<g, 256, 0, s>>>(A.st, A.dinv, xa, xb, b, A.nx, A.nz, kOmega);
+                std::swap(xa, xb);
+            }
+            if (xa != x) M2D_HIP(hipMemcpyAsync(x, xa, sizeof(float) * A.n, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+// dots[slot] (and dots[slot+1]) <- a.b (and c.d)
+static int dot2(m2d_fields2d *f, const float *a, const float *b, const float *c, const float *d,
+                int slot)
+{
+    Mg &mg = f->mg;
+    hipStream_t s = f->ctx->stream;
+    int nb = std::min(mg.npartial, 1024);
+    k_dot2<<<nb, 256, 0, s>>>(a, b, c, d, f->n, mg.partial);
+    k_sum_pairs<<<1, 1024, 0, s>>>(mg.partial, nb, mg.dots + slot);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static int read_dots(m2d_fields2d *f)
+{
+    M2D_HIP(hipMemcpyAsync(f->mg.h_dots, f->mg.dots, sizeof(double) * 16, hipMemcpyDeviceToHost,
+                           f->ctx->stream));
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+// FP32 PCG on A' e = d (d in mg.pr on entry), e in mg.pe; stops at
+// ||r|| <= rtol ||d|| or max_it.  Returns iterations.
+static int pcg(m2d_fields2d *f, double rtol, int max_it, int *iters)
+{
+    Mg &mg = f->mg;
+    hipStream_t s = f->ctx->stream;
+    const size_t n = f->n;
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    const MgLevel &L0 = mg.lv[0];
+    dim3 g2((L0.nz + 255) / 256, L0.nx);
+    M2D_TRY(fill_bytes(f->ctx, mg.pe, 0, sizeof(float) * n));
+    // dots layout: [0] rz, [1] pq, [2] rr, [3] unused, [4] rz_old
+    M2D_TRY(dot2(f, mg.pr, mg.pr, nullptr, nullptr, 2));
+    M2D_TRY(read_dots(f));
+    double r0 = sqrt(mg.h_dots[2]);
+    *iters = 0;
+    if (r0 == 0.0) return M2D_OK;
+    M2D_TRY(vcycle(f, mg.pr, mg.pz));
+    M2D_TRY(dot2(f, mg.pr, mg.pz, nullptr, nullptr, 0));
+    k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1);
+    for (int it = 1; it <= max_it; it++) {
+        k_apply5<<<g2, 256, 0, s>>>(L0.dir, mg.pp, mg.pq, L0.nx, L0.nz);
+        // dots[0] = rz (kept), dots[1] = p.q
+        M2D_TRY(dot2(f, mg.pp, mg.pq, nullptr, nullptr, 1));
+        k_cg_update<<<gb, 256, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots, n);
+        M2D_TRY(dot2(f, mg.pr, mg.pr, nullptr, nullptr, 2));
+        *iters = it;
+        M2D_TRY(read_dots(f));
+        if (sqrt(mg.h_dots[2]) <= rtol * r0) break;
+        M2D_TRY(vcycle(f, mg.pr, mg.pz));
+        k_copy_scalar<<<1, 1, 0, s>>>(mg.dots, mg.dots + 4);
+        M2D_TRY(dot2(f, mg.pr, mg.pz, nullptr, nullptr, 0));
+        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 0);
+    }
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
+{
+    hipStream_t s = f->ctx->stream;
+    Mg &mg = f->mg;
+    const size_t n = f->n;
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    double *u = mode == M2D_SOLVE_RF ? f->urf : f->u;
+    double scale = -(f->p.grid.dx * f->p.grid.dx) / kEps0 / f->p.dV;  // fields.cpp:307
+    k_rhs<<<gb, 256, 0, s>>>(f->mask, f->voltage, f->rho, u, n, mode, scale);
+    M2D_HIP(hipGetLastError());
+    if (tol <= 0) tol = 1e-10;
+    int outer = 0, inner_total = 0;
+    double rel = 0;
+    const int max_outer = 12;
+    for (;;) {
+        k_residual64<<<gb, 256, 0, s>>>(f->mask, f->rho, u, f->nx, f->nz, mg.r64, mg.pr, mg.partial);
+        k_sum_pairs<<<1, 1024, 0, s>>>(mg.partial, (int)gb, mg.dots + 8);
+        M2D_HIP(hipGetLastError());
+        M2D_TRY(read_dots(f));
+        double rn = sqrt(mg.h_dots[8]), bn = sqrt(mg.h_dots[9]);
+        rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
+        if (!std::isfinite(rn)) {
+            if (info) { info->outer_iters = outer; info->inner_iters = inner_total; info->rel_residual = rel; }
+            return set_error(M2D_ERR_NAN, "Fields solve: residual is not finite");
+        }
+        if (rel <= tol || outer >= max_outer) break;
+        int it = 0;
+        M2D_TRY(pcg(f, 1e-5, 60, &it));
+        inner_total += it;
+        k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n);
+        M2D_HIP(hipGetLastError());
+        outer++;
+    }
+    if (info) { info->outer_iters = outer; info->inner_iters = inner_total; info->rel_residual = rel; }
+    M2D_TRY(fields_build_tables(f));
+    if (rel > tol)
+        return set_error(M2D_ERR_NOT_CONVERGED, "Fields solve: relative residual %.3e > %.3e after %d steps",
+                         rel, tol, outer);
+    return M2D_OK;
+}
+
+}  // namespace m2d
+
+using namespace m2d;
+
+extern "C" {
+
+int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_t *mask,
+                        const double *voltage, m2d_fields2d **out)
+{
+    M2D_CHECK(ctx && p && mask && voltage && out, M2D_ERR_BAD_ARG, "m2d_fields2d_create: NULL argument");
+    *out = nullptr;
+    const m2d_grid2d &g = p->grid;
+    M2D_CHECK(g.nx >= 3 && g.nz >= 3 && g.dx > 0 && g.dz > 0, M2D_ERR_BAD_ARG,
+              "m2d_fields2d_create: grid %dx%d spacing %g,%g", g.nx, g.nz, g.dx, g.dz);
+    M2D_CHECK(p->dV > 0, M2D_ERR_BAD_ARG, "m2d_fields2d_create: dV must be > 0");
+    const size_t n = (size_t)g.nx * g.nz;
+    // a free 5-point row on the frame would reach outside the grid (fields.cpp:229-237)
+    for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.nz; j++) {
+            uint8_t m = mask[(size_t)i * g.nz + j];
+            M2D_CHECK(m <= M2D_BOUNDARY, M2D_ERR_BAD_ARG, "mask value %d at (%d,%d)", (int)m, i, j);
+            bool edge = i == 0 || j == 0 || i == g.nx - 1 || j == g.nz - 1;
+            M2D_CHECK(!edge || m == M2D_FIXED || m == M2D_FIXED_RF, M2D_ERR_BAD_ARG,
+                      "m2d_fields2d_create: node (%d,%d) on the grid edge is not Dirichlet; its "
+                      "Cartesian 5-point row would leave the grid", i, j);
+        }
+    M2D_HIP(hipSetDevice(ctx->device));
+    m2d_fields2d *f = new (std::nothrow) m2d_fields2d();
+    M2D_CHECK(f, M2D_ERR_NO_MEMORY, "m2d_fields2d_create: host allocation");
+    f->ctx = ctx;
+    f->p = *p;
+    f->nx = g.nx;
+    f->nz = g.nz;
+    f->n = n;
+    f->idx = 1.0 / g.dx;
+    f->idz = 1.0 / g.dz;
+    f->h_mask.assign(mask, mask + n);
+    int r = M2D_OK;
+    auto fail = [&](int code) { m2d_fields2d_destroy(f); return code; };
+    if ((r = dmalloc((void **)&f->mask, n)) || (r = dmalloc((void **)&f->cellfree, n)) ||
+        (r = dmalloc((void **)&f->voltage, 8 * n)) || (r = dmalloc((void **)&f->u, 8 * n)) ||
+        (r = dmalloc((void **)&f->urf, 8 * n)) || (r = dmalloc((void **)&f->rho, 8 * n)) ||
+        (r = dmalloc((void **)&f->gx, sizeof(float2) * (size_t)(g.nx + 1) * g.nz)) ||
+        (r = dmalloc((void **)&f->gz, sizeof(float2) * (size_t)g.nx * (g.nz + 1))))
+        return fail(r);
+    std::vector<uint8_t> cf(n, 0);
+    for (int i = 0; i + 1 < g.nx; i++)
+        for (int j = 0; j + 1 < g.nz; j++) {
+            size_t k = (size_t)i * g.nz + j;
+            cf[k] = mask[k] == M2D_FREE || mask[k + g.nz] == M2D_FREE || mask[k + 1] == M2D_FREE ||
+                    mask[k + g.nz + 1] == M2D_FREE;
+        }
+    if (hipMemcpy(f->mask, mask, n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->cellfree, cf.data(), n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->voltage, voltage, 8 * n, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(M2D_ERR_HIP, "m2d_fields2d_create: upload failed"));
+    if ((r = fill_bytes(ctx, f->u, 0, 8 * n)) || (r = fill_bytes(ctx, f->urf, 0, 8 * n)) ||
+        (r = fill_bytes(ctx, f->rho, 0, 8 * n)))
+        return fail(r);
+    if ((r = mg_setup(f))) return fail(r);
+    if ((r = fields_build_tables(f))) return fail(r);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(set_error(M2D_ERR_HIP, "m2d_fields2d_create: synchronise failed"));
+    *out = f;
+    return M2D_OK;
+}
+
+int m2d_fields2d_destroy(m2d_fields2d *f)
+{
+    if (!f) return M2D_OK;
+    if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
+    for (void *p : {(void *)f->mask, (void *)f->cellfree, (void *)f->voltage, (void *)f->u,
+                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz})
+        if (p) (void)hipFree(p);
+    mg_free(f->mg);
+    delete f;
+    return M2D_OK;
+}
+
+static double *field_ptr(m2d_fields2d *f, int which)
+{
+    switch (which) {
+    case M2D_FIELD_U: return f->u;
+    case M2D_FIELD_URF: return f->urf;
+    case M2D_FIELD_RHO: return f->rho;
+    case M2D_FIELD_VOLTAGE: return f->voltage;
+    default: return nullptr;
+    }
+}
+
+double *m2d_fields2d_data(m2d_fields2d *f, int which) { return f ? field_ptr(f, which) : nullptr; }
+
+int m2d_fields2d_upload(m2d_fields2d *f, int which, const double *host)
+{
+    M2D_CHECK(f && host, M2D_ERR_BAD_ARG, "m2d_fields2d_upload: NULL argument");
+    double *d = field_ptr(f, which);
+    M2D_CHECK(d, M2D_ERR_BAD_ARG, "m2d_fields2d_upload: unknown field %d", which);
+    M2D_HIP(hipMemcpyAsync(d, host, 8 * f->n, hipMemcpyHostToDevice, f->ctx->stream));
+    if (which == M2D_FIELD_U || which == M2D_FIELD_URF) M2D_TRY(fields_build_tables(f));
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+int m2d_fields2d_download(m2d_fields2d *f, int which, double *host)
+{
+    M2D_CHECK(f && host, M2D_ERR_BAD_ARG, "m2d_fields2d_download: NULL argument");
+    double *d = field_ptr(f, which);
+    M2D_CHECK(d, M2D_ERR_BAD_ARG, "m2d_fields2d_download: unknown field %d", which);
+    M2D_HIP(hipMemcpyAsync(host, d, 8 * f->n, hipMemcpyDeviceToHost, f->ctx->stream));
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+int m2d_fields2d_update(m2d_fields2d *f)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields2d_update: NULL argument");
+    return fields_build_tables(f);
+}
+
+int m2d_fields2d_reset(m2d_fields2d *f)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields2d_reset: NULL argument");
+    return fill_bytes(f->ctx, f->rho, 0, 8 * f->n);
+}
+
+int m2d_fields2d_solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields2d_solve: NULL argument");
+    M2D_CHECK(mode == M2D_SOLVE_STATIC || mode == M2D_SOLVE_RF || mode == M2D_SOLVE_RAW, M2D_ERR_BAD_ARG,
+              "m2d_fields2d_solve: unknown mode %d", mode);
+    return solve(f, mode, tol, info);
+}
+
+}  // extern "C"

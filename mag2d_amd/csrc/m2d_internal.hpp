@@ -1,0 +1,169 @@
+// m2d_internal.hpp -- shared internals of libmag2d_amd (HIP, gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <string>
+#include <vector>
+#include "mag2d_amd.h"
+
+namespace m2d {
+
+// ---------------------------------------------------------------- constants
+// physical constants of the reference (param.cpp:7-11)
+constexpr double kEps0 = 8.854187817e-12;
+constexpr double kKB = 1.380662e-23;
+constexpr double kQE = 1.602189e-19;
+
+constexpr double kFix = 4294967296.0;        // 2^32: fixed-point scale of deposited weights
+constexpr double kInvFix = 1.0 / 4294967296.0;
+constexpr int kBlock = 256;                  // threads per block of the particle kernels
+
+// Philox counter word 3 carries (purpose << 24)
+constexpr uint32_t kPurposeMcc = 0;
+constexpr uint32_t kPurposeInit = 1;
+
+// ---------------------------------------------------------------- errors
+int set_error(int code, const char *fmt, ...);
+#define M2D_HIP(call)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::m2d::set_error(M2D_ERR_HIP, "%s: %s (%s:%d)", #call,             \
+                                    hipGetErrorString(e_), __FILE__, __LINE__);       \
+    } while (0)
+#define M2D_CHECK(cond, code, ...)                                                    \
+    do {                                                                              \
+        if (!(cond)) return ::m2d::set_error(code, __VA_ARGS__);                      \
+    } while (0)
+#define M2D_TRY(call)                                                                 \
+    do {                                                                              \
+        int r_ = (call);                                                              \
+        if (r_ != M2D_OK) return r_;                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------- device buffers
+// A growable device scratch buffer owned by a context.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace m2d
+
+// counters written by kernels
+enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_N = 8 };
+
+struct m2d_ctx {
+    int device = 0;
+    uint64_t seed = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    unsigned long long *counters = nullptr;  // CNT_N device words
+    unsigned long long *h_counters = nullptr; // pinned mirror
+    m2d::Scratch acc;       // int64 fixed-point deposit grid
+    m2d::Scratch red;       // reduction partials
+    m2d::Scratch tmp;       // general temporaries (compaction offsets)
+    // kernel timing: event pairs around particle-pass launches
+    bool profile = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+};
+
+namespace m2d {
+// bracket a launch with events when profiling is on
+hipEvent_t prof_begin(m2d_ctx *ctx);
+void prof_end(m2d_ctx *ctx, hipEvent_t start);
+}  // namespace m2d
+
+namespace m2d {
+
+int ensure(m2d_ctx *ctx, Scratch &s, size_t bytes);
+int fill_bytes(m2d_ctx *ctx, void *dst, int value, size_t bytes);
+int zero_words(m2d_ctx *ctx, unsigned long long *dst, size_t n);
+int read_counters(m2d_ctx *ctx);
+
+// ---------------------------------------------------------------- device helpers
+__device__ __forceinline__ bool is_dead(float x) { return __float_as_uint(x) == M2D_DEAD_BITS; }
+__device__ __forceinline__ float dead_value() { return __uint_as_float(M2D_DEAD_BITS); }
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2,
+                                              uint32_t &c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+
+// Draw stream of one (slot, step, purpose): words come in blocks of four.
+struct Rng {
+    uint32_t k0, k1, slot, step_lo, hi3, block;
+    uint32_t b0, b1, b2, b3;
+    int used;  // words consumed from the current block
+
+    __device__ __forceinline__ void begin(uint64_t seed, uint32_t stream, uint32_t s,
+                                          uint64_t step, uint32_t purpose)
+    {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32) ^ stream;
+        slot = s;
+        step_lo = (uint32_t)step;
+        hi3 = (purpose << 24) | ((uint32_t)(step >> 32) & 0xFFFFFFu);
+        block = 0;
+        used = 4;
+    }
+    __device__ __forceinline__ void refill()
+    {
+        b0 = slot; b1 = step_lo; b2 = block; b3 = hi3;
+        philox4x32_10(b0, b1, b2, b3, k0, k1);
+        block++;
+        used = 0;
+    }
+    __device__ __forceinline__ uint32_t u32()
+    {
+        if (used == 4) refill();
+        uint32_t w = used == 0 ? b0 : used == 1 ? b1 : used == 2 ? b2 : b3;
+        used++;
+        return w;
+    }
+    // uniform in (0,1), exact in FP64: (w + 0.5) 2^-32
+    __device__ __forceinline__ double uni() { return ((double)u32() + 0.5) * 2.3283064365386963e-10; }
+    // Box-Muller, cosine branch (two uniforms per variate)
+    __device__ __forceinline__ double normal()
+    {
+        double a = uni(), b = uni();
+        return sqrt(-2.0 * log(a)) * cos(2.0 * M_PI * b);
+    }
+};
+
+// wave-aggregated counter increment (one atomic per wavefront)
+__device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred)
+{
+    unsigned long long m = __ballot(pred);
+    if (m) {
+        int lane = __lane_id();
+        int first = __ffsll((long long)m) - 1;
+        if (lane == first) atomicAdd(ctr, (unsigned long long)__popcll(m));
+    }
+}
+
+__device__ __forceinline__ void raise_flag(unsigned long long *counters, unsigned long long f)
+{
+    atomicOr(&counters[CNT_FLAGS], f);
+}
+
+inline int grid_for(int64_t n, int per_thread, int block = kBlock)
+{
+    int64_t g = (n + (int64_t)block * per_thread - 1) / ((int64_t)block * per_thread);
+    return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace m2d

@@ -1,0 +1,96 @@
+// m2d_objects.hpp -- device-side objects shared by the kernels of
+// libmag2d_amd: the Fields object (gather tables, cell mask, multigrid
+// hierarchy) and the MCC tables.
+#pragma once
+#include "m2d_internal.hpp"
+
+namespace m2d {
+
+// ------------------------------------------------------------------ MCC
+constexpr int kMaxTargets = 8;
+constexpr int kMaxInter = 32;
+
+struct McInter {
+    int type;
+    int tab_off, tab_len;
+    int pad;
+    double DE;      // J
+    double rate;    // m^3/s, LANGEVIN already scaled by cutoff^2 (particles.hpp:81-82)
+    double cutoff;
+    double mu;      // reduced mass of primary and target
+};
+
+struct McTarget {
+    double rate;    // rates_by_species (particles.cpp:163)
+    double density;
+    double mass;
+    double vth;     // sqrt(k_B T / m) = v_max/sqrt(2) (particles.hpp:180,190-192)
+    int first, n;   // interactions [first, first+n)
+    const float *px, *pz, *pvx, *pvy, *pvz;  // partner ensemble (px == nullptr: continuum)
+    int64_t pn;
+};
+
+struct McDev {
+    int n_targets;
+    int pad;
+    double lifetime;
+    double mass;    // primary
+    McTarget t[kMaxTargets];
+    McInter in[kMaxInter];
+    const double *tabE;
+    const double *tabS;
+};
+
+// ------------------------------------------------------------------ multigrid
+struct MgLevel {
+    int nx, nz;
+    size_t n;
+    float *st;      // 9 coefficients per node, offsets (di,dj) in row-major (di+1)*3+(dj+1)
+    float *dinv;    // inverse diagonal, 0 on Dirichlet nodes
+    uint8_t *dir;   // 1 = correction pinned to 0
+    float *x, *x2, *r, *b;
+};
+
+struct Mg {
+    std::vector<MgLevel> lv;
+    float *coarse_inv = nullptr;  // dense inverse of the coarsest operator (n x n)
+    int coarse_n = 0;
+    // FP32 PCG vectors on the finest level
+    float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
+    // FP64 vectors
+    double *r64 = nullptr;
+    double *dots = nullptr;       // device scalars
+    double *h_dots = nullptr;     // pinned mirror
+    double *partial = nullptr;    // reduction partials
+    int npartial = 0;
+};
+
+}  // namespace m2d
+
+struct m2d_fields2d {
+    m2d_ctx *ctx = nullptr;
+    m2d_fields2d_params p{};
+    double idx = 0, idz = 0;
+    int nx = 0, nz = 0;
+    size_t n = 0;
+    uint8_t *mask = nullptr;      // device copy of t_grid::mask
+    uint8_t *cellfree = nullptr;  // nx*nz: 1 if any corner of cell (i,j) is FREE
+    double *voltage = nullptr, *u = nullptr, *urf = nullptr, *rho = nullptr;
+    float2 *gx = nullptr;         // (nx+1)*nz edge x-gradients {u, uRF}, padded rows 0 and nx
+    float2 *gz = nullptr;         // nx*(nz+1) edge z-gradients {u, uRF}, padded cols 0 and nz
+    m2d::Mg mg;
+    std::vector<uint8_t> h_mask;
+};
+
+struct m2d_mcc {
+    m2d_ctx *ctx = nullptr;
+    m2d::McDev h{};               // host copy
+    m2d::McDev *d = nullptr;      // device copy
+    double *tab = nullptr;        // device cross-section tables (E then sigma)
+    std::vector<double> rates;
+};
+
+namespace m2d {
+int fields_build_tables(m2d_fields2d *f);
+int mcc_sync(m2d_mcc *m);
+}  // namespace m2d

@@ -1,0 +1,970 @@
+// m2d_particles2d.hip -- the 2D PIC-MCC particle pass and its helpers.
+//
+// k_advance2d fuses Species<CARTESIAN>::advance (particles.hpp:342-349):
+//   advance_boris (particles.cpp:924-995): E gather (Fields::E, fields.hpp:124-150
+//   on Field2D::grad, Field2D.hpp:80-168), Boris kick-rotate-kick, drift, MCC
+//   trial and BaseSpecies::scatter (particles.cpp:208-365);
+//   advance_boundary (particles.hpp:370-411): box test, is_free, CIC deposit.
+// One read and one write of the 20-byte SoA state per live particle.
+#include "m2d_objects.hpp"
+#include <cmath>
+
+namespace m2d {
+
+// ------------------------------------------------------------------ args
+struct PushArgs {
+    float *x, *z, *vx, *vy, *vz;
+    int64_t n;
+    const float2 *gx, *gz;
+    const uint8_t *cellfree;
+    int nx, nz;
+    double idx, idz, xmin, zmin;
+    double bx0, bx1, bz0, bz1;     // box [x_min,x_max] x [z_min,z_max]
+    float qmdt2;                   // q dt / (2 m)
+    float tx, ty, tz, sx, sy, sz;  // Boris rotation vectors
+    float dt;
+    float phase;                   // E = -(grad u + phase grad uRF)
+    float ecx, ecz;                // constant field (EMPTY geometry, fields.hpp:126-131)
+    int econst;
+    int boundary, periodic, checkfree;
+    double prob;                   // 1 - exp(-dt/lifetime)
+    uint64_t seed;
+    uint32_t stream;
+    uint64_t step;
+    const McDev *mcc;
+    unsigned long long *acc;       // nx*nz fixed-point deposit grid
+    unsigned long long *counters;
+};
+
+// ------------------------------------------------------------------ gather
+// Fields::E through the padded edge-gradient tables.  Index arithmetic is
+// FP64 on the FP32 position (exactly the reference's x*idx), weights FP32.
+__device__ __forceinline__ void gather_E(const PushArgs &a, double xd, double zd, float &ex,
+                                         float &ez)
+{
+    if (a.econst) { ex = a.ecx; ez = a.ecz; return; }
+    const int nx = a.nx, nz = a.nz;
+    double X = (xd - a.xmin) * a.idx;
+    double Z = (zd - a.zmin) * a.idz;
+    // x component: i = (int)(x idx + .5), j = min((int)(z idz), lmax-2)
+    int ih = (int)(X + 0.5);
+    int jx = min((int)Z, nz - 2);
+    // z component: i = min((int)(x idx), jmax-2), j = (int)(z idz + .5)
+    int i0 = min((int)X, nx - 2);
+    int jh = (int)(Z + 0.5);
+    bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
+    if (bad) {
+        raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+        X = fmin(fmax(X, 0.0), (double)(nx - 1));
+        Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
+        ih = (int)(X + 0.5); jx = min((int)Z, nz - 2);
+        i0 = min((int)X, nx - 2); jh = (int)(Z + 0.5);
+    }
+    ih = max(0, min(ih, nx - 1));
+    jx = max(jx, 0);
+    i0 = max(i0, 0);
+    jh = max(0, min(jh, nz - 1));
+    const float fx = (float)(X - ih + 0.5), fyx = (float)(Z - jx);
+    const float fxz = (float)(X - i0), fz = (float)(Z - jh + 0.5);
+    const float ph = a.phase;
+
+    const float2 *r0 = a.gx + (size_t)ih * nz + jx;
+    const float2 *r1 = r0 + nz;
+    float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
+    const float2 *q0 = a.gz + (size_t)i0 * (nz + 1) + jh;
+    const float2 *q1 = q0 + (nz + 1);
+    float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
+
+    float g00 = fmaf(ph, p00.y, p00.x), g01 = fmaf(ph, p01.y, p01.x);
+    float g10 = fmaf(ph, p10.y, p10.x), g11 = fmaf(ph, p11.y, p11.x);
+    float gxv = (1.f - fx) * ((1.f - fyx) * g00 + fyx * g01) + fx * ((1.f - fyx) * g10 + fyx * g11);
+    float h00 = fmaf(ph, s00.y, s00.x), h01 = fmaf(ph, s01.y, s01.x);
+    float h10 = fmaf(ph, s10.y, s10.x), h11 = fmaf(ph, s11.y, s11.x);
+    float gzv = (1.f - fxz) * ((1.f - fz) * h00 + fz * h01) + fxz * ((1.f - fz) * h10 + fz * h11);
+    ex = -gxv;
+    ez = -gzv;
+}
+
+// ------------------------------------------------------------------ MCC
+__device__ __forceinline__ double vec_interp(const double *xs, const double *ys, int n, double x)
+{
+    // vec_interpolate::operator() (tabulate.cpp:124-141)
+    if (x >= xs[n - 1]) return ys[n - 1];
+    if (x <= xs[0]) return ys[0];
+    int j1 = 0, j2 = n - 1;
+    while (j2 - j1 > 1) {
+        int j3 = (j1 + j2) / 2;
+        if (x < xs[j3]) j2 = j3; else j1 = j3;
+    }
+    double w = (x - xs[j1]) / (xs[j2] - xs[j1]);
+    return ys[j1] * (1 - w) + ys[j2] * w;
+}
+
+__device__ __forceinline__ double sigma_v(const McDev &m, const McInter &it, double v)
+{
+    // Interaction::sigma_v (particles.hpp:61-70); COULOMB is rejected at creation
+    if (it.tab_len > 0)
+        return vec_interp(m.tabE + it.tab_off, m.tabS + it.tab_off, it.tab_len,
+                          0.5 * it.mu * v * v / kQE) * v;
+    return it.rate;
+}
+
+__device__ __forceinline__ void rot_len(Rng &r, double len, double &x, double &y, double &z)
+{
+    // t_random::rot(len, x, y, z) (random.cpp:103-115)
+    double cs = 1.0 - 2.0 * r.uni();
+    x = len * cs;
+    cs = sqrt(1.0 - cs * cs);
+    double sp, cp;
+    sincos(2.0 * M_PI * r.uni(), &sp, &cp);
+    y = len * cs * sp;
+    z = len * cs * cp;
+}
+
+__device__ __forceinline__ double ellint_k(double k)
+{
+    // complete elliptic integral K(modulus k) by the arithmetic-geometric mean
+    double a = 1.0, b = sqrt(1.0 - k * k);
+    for (int i = 0; i < 40 && fabs(a - b) > 1e-16 * a; i++) {
+        double an = 0.5 * (a + b);
+        b = sqrt(a * b);
+        a = an;
+    }
+    return M_PI / (a + b);
+}
+
+__device__ void deflect(Rng &r, double angle, double &x, double &y, double &z)
+{
+    // t_random::deflect (random.cpp:133-175)
+    double len = tan(angle / 2.0);
+    double t = 1.0 - 2.0 * r.uni();
+    double x1 = len * t;
+    t = sqrt(1.0 - t * t);
+    double sp, cp;
+    sincos(2.0 * M_PI * r.uni(), &sp, &cp);
+    double y1 = len * t * sp, z1 = len * t * cp;
+    double tx = y * z1 - z * y1, ty = z * x1 - x * z1, tz = x * y1 - y * x1;
+    t = len / sqrt(tx * tx + ty * ty + tz * tz);
+    tx *= t; ty *= t; tz *= t;
+    double xp = x - y * tz + z * ty, yp = y - z * tx + x * tz, zp = z - x * ty + y * tx;
+    t = 2.0 / (1.0 + len * len);
+    x1 = tx * t; y1 = ty * t; z1 = tz * t;
+    x += -yp * z1 + zp * y1;
+    y += -zp * x1 + xp * z1;
+    z += -xp * y1 + yp * x1;
+}
+
+// BaseSpecies::scatter (particles.cpp:208-365), FP64, same draw order as
+// the reference: species pick, partner, interaction pick, angles.
+__device__ void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, float &vzf,
+                        unsigned long long *counters)
+{
+    double vx = vxf, vy = vyf, vz = vzf;
+    double gamma = r.uni() / m.lifetime;
+    double tmp = 0.0;
+    int sid;
+    for (sid = 0; sid < m.n_targets - 1; sid++) {
+        tmp += m.t[sid].rate;
+        if (tmp > gamma) break;
+    }
+    const McTarget &T = m.t[sid];
+    double vr2, vz2, vt2;
+    if (T.px == nullptr || T.pn == 0) {
+        vr2 = r.normal() * T.vth;
+        vz2 = r.normal() * T.vth;
+        vt2 = r.normal() * T.vth;
+    } else {
+        int64_t i = (int64_t)(r.u32() % (uint32_t)T.pn);
+        int tries = 0;
+        while (is_dead(T.px[i]) && tries++ < 1000) i = (int64_t)(r.u32() % (uint32_t)T.pn);
+        if (is_dead(T.px[i])) { raise_flag(counters, M2D_FLAG_PARTNER); return; }
+        vr2 = T.pvx[i]; vz2 = T.pvz[i]; vt2 = T.pvy[i];
+    }
+    double dvx = vx - vr2, dvz = vz - vz2, dvy = vy - vt2;
+    double v_rel = sqrt(dvx * dvx + dvz * dvz + dvy * dvy);
+    double m2 = T.mass, mass = m.mass;
+
+    gamma = r.uni() * T.rate;
+    tmp = 0.0;
+    int k;
+    for (k = 0; k < T.n; k++) {
+        tmp += sigma_v(m, m.in[T.first + k], v_rel) * T.density;
+        if (tmp > gamma) break;
+    }
+    if (k == T.n) return;  // null collision
+    const McInter &it = m.in[T.first + k];
+    tmp = 1.0 / (mass + m2);
+    switch (it.type) {
+    case M2D_SUPERELASTIC: {
+        double E = 0.5 * it.mu * v_rel * v_rel + it.DE;
+        double vr = sqrt(2 * E / it.mu);
+        double rx, rz, ry;
+        rot_len(r, vr, rx, rz, ry);
+        vx = (rx * m2 + vx * mass + vr2 * m2) * tmp;
+        double nvz = (ry * m2 + vz * mass + vz2 * m2) * tmp;
+        double nvy = (rz * m2 + vy * mass + vt2 * m2) * tmp;
+        vz = nvz; vy = nvy;
+        break;
+    }
+    case M2D_ELASTIC: {
+        double cx = (vx * mass + vr2 * m2) * tmp;
+        double cz = (vz * mass + vz2 * m2) * tmp;
+        double cy = (vy * mass + vt2 * m2) * tmp;
+        double rx, ry, rz;
+        rot_len(r, v_rel, rx, ry, rz);
+        vx = rx * m2 * tmp + cx;
+        vz = rz * m2 * tmp + cz;
+        vy = ry * m2 * tmp + cy;
+        break;
+    }
+    case M2D_CX:
+        vx = vr2; vz = vz2; vy = vt2;
+        break;
+    case M2D_LANGEVIN: {
+        double ax = (vx - vr2) * m2 * tmp, ay = (vz - vz2) * m2 * tmp, az = (vy - vt2) * m2 * tmp;
+        double beta = sqrt(r.uni()) * it.cutoff;
+        if (beta > 1.0) {
+            // Nanbu & Kitatani, J. Phys. D 28 (1995) 324 (particles.cpp:337-345)
+            double t4 = sqrt(beta * beta * beta * beta - 1.0);
+            double xi0 = sqrt(beta * beta - t4), xi1 = sqrt(beta * beta + t4);
+            double theta = ellint_k(xi0 / xi1) * M_SQRT2 * beta / xi1;
+            deflect(r, M_PI - 2 * theta, ax, ay, az);
+        } else {
+            double len = sqrt(ax * ax + ay * ay + az * az);
+            rot_len(r, len, ax, ay, az);
+        }
+        double nvx = ax + (vx * mass + vr2 * m2) * tmp;
+        double nvz = ay + (vz * mass + vz2 * m2) * tmp;
+        double nvy = az + (vy * mass + vt2 * m2) * tmp;
+        vx = nvx; vz = nvz; vy = nvy;
+        break;
+    }
+    default:
+        return;
+    }
+    vxf = (float)vx; vyf = (float)vy; vzf = (float)vz;
+}
+
+// ------------------------------------------------------------------ deposit
+// CIC weights (Field2D.hpp:45-62) as 2^-32 fixed point.  The four rounded
+// weights are corrected on the largest so each particle deposits exactly
+// 2^32: charge is conserved bit for bit and sums are order independent.
+__device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
+                                            long long w[4])
+{
+    if (!(X >= 0.0) || !(Z >= 0.0)) return false;
+    i = (int)X;
+    j = (int)Z;
+    if (i > nx - 2 || j > nz - 2) return false;
+    double u = X - i, v = Z - j;
+    w[0] = __double2ll_rn((1 - u) * (1 - v) * kFix);
+    w[1] = __double2ll_rn(u * (1 - v) * kFix);
+    w[2] = __double2ll_rn((1 - u) * v * kFix);
+    w[3] = __double2ll_rn(u * v * kFix);
+    long long s = w[0] + w[1] + w[2] + w[3];
+    int km = 0;
+#pragma unroll
+    for (int q = 1; q < 4; q++) km = w[q] > w[km] ? q : km;
+    w[km] += (1ll << 32) - s;
+    return true;
+}
+
+__device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, int i, int j,
+                                               const long long w[4])
+{
+    size_t k = (size_t)i * nz + j;
+    if (w[0]) atomicAdd(acc + k, (unsigned long long)w[0]);
+    if (w[1]) atomicAdd(acc + k + nz, (unsigned long long)w[1]);
+    if (w[2]) atomicAdd(acc + k + 1, (unsigned long long)w[2]);
+    if (w[3]) atomicAdd(acc + k + nz + 1, (unsigned long long)w[3]);
+}
+
+// ------------------------------------------------------------------ the pass
+constexpr int kPPT = 4;  // particles per thread
+
+template <bool DEPOSIT, bool MCC>
+__global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
+{
+    const int64_t base = (int64_t)blockIdx.x * (kBlock * kPPT) + threadIdx.x;
+    bool removed_any = false, nan_any = false;
+    int n_removed = 0, n_coll = 0;
+#pragma unroll
+    for (int q = 0; q < kPPT; q++) {
+        const int64_t p = base + (int64_t)q * kBlock;
+        if (p >= a.n) break;
+        float x = a.x[p];
+        if (is_dead(x)) continue;
+        float z = a.z[p];
+        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+
+        // ---- advance_boris (particles.cpp:945-988)
+        float ex, ez;
+        gather_E(a, (double)x, (double)z, ex, ez);
+        vx = fmaf(ex, a.qmdt2, vx);
+        vz = fmaf(ez, a.qmdt2, vz);
+        float vpr = vx - vy * a.tz + vz * a.ty;
+        float vpz = vz - vx * a.ty + vy * a.tx;
+        float vpt = vy - vz * a.tx + vx * a.tz;
+        vx = vx - vpt * a.sz + vpz * a.sy;
+        vy = vy - vpz * a.sx + vpr * a.sz;
+        vz = vz - vpr * a.sy + vpt * a.sx;
+        vx = fmaf(ex, a.qmdt2, vx);
+        vz = fmaf(ez, a.qmdt2, vz);
+        x = fmaf(vx, a.dt, x);
+        z = fmaf(vz, a.dt, z);
+
+        // ---- MCC trial (particles.cpp:990-993)
+        if (MCC) {
+            Rng r;
+            r.begin(a.seed, a.stream, (uint32_t)p, a.step, kPurposeMcc);
+            if (r.uni() < a.prob) {
+                scatter(*a.mcc, r, vx, vy, vz, a.counters);
+                n_coll++;
+            }
+        }
+
+        // ---- advance_boundary (particles.hpp:376-406)
+        bool dead = false;
+        if (a.boundary) {
+            bool inside = x >= a.bx0 && x <= a.bx1 && z >= a.bz0 && z <= a.bz1;
+            if (!inside) {
+                if (x != x || z != z) { dead = true; nan_any = true; }
+                else if (!a.periodic) dead = true;
+                else {
+                    double xm = fmod((double)x, a.bx1), zm = fmod((double)z, a.bz1);
+                    if (xm < 0) xm += a.bx1;
+                    if (zm < 0) zm += a.bz1;
+                    x = (float)xm; z = (float)zm;
+                }
+            }
+            if (!dead && a.checkfree) {
+                double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
+                bool free_ = false;
+                if (X >= 0.0 && Z >= 0.0) {
+                    int ci = (int)X, cj = (int)Z;
+                    if (ci <= a.nx - 1 && cj <= a.nz - 1) free_ = a.cellfree[(size_t)ci * a.nz + cj] != 0;
+                }
+                dead = !free_;
+            }
+        }
+        if (dead) {
+            a.x[p] = dead_value();
+            removed_any = true;
+            n_removed++;
+            continue;
+        }
+        // ---- deposit (particles.hpp:408-409)
+        if (DEPOSIT) {
+            int ci, cj;
+            long long w[4];
+            if (cic_weights(((double)x - a.xmin) * a.idx, ((double)z - a.zmin) * a.idz, a.nx, a.nz,
+                            ci, cj, w))
+                deposit_global(a.acc, a.nz, ci, cj, w);
+            else
+                raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+        }
+        a.x[p] = x; a.z[p] = z;
+        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+    }
+    // ---- counters: one atomic per wavefront
+    unsigned long long m = __ballot(removed_any);
+    if (m) {
+        int sum = n_removed;
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (__lane_id() == 0) atomicAdd(&a.counters[CNT_REMOVED], (unsigned long long)sum);
+    }
+    if (MCC) {
+        int sum = n_coll;
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (__lane_id() == 0 && sum) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)sum);
+    }
+    if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+}
+
+// Species<CARTESIAN>::advance_boris_init (particles.cpp:997-1050)
+__global__ void __launch_bounds__(kBlock) k_advance_init2d(PushArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= a.n) return;
+    float x = a.x[p];
+    if (is_dead(x)) return;
+    float z = a.z[p];
+    float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+    float ex, ez;
+    gather_E(a, (double)x, (double)z, ex, ez);
+    float vpr = vx - vy * a.tz + vz * a.ty;
+    float vpz = vz - vx * a.ty + vy * a.tx;
+    float vpt = vy - vz * a.tx + vx * a.tz;
+    vx = vx - vpt * a.sz + vpz * a.sy;
+    vy = vy - vpz * a.sx + vpr * a.sz;
+    vz = vz - vpr * a.sy + vpt * a.sx;
+    vx = fmaf(ex, a.qmdt2, vx);
+    vz = fmaf(ez, a.qmdt2, vz);
+    a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+}
+
+// Fields::E at FP64 points through the same tables (test entry point)
+__global__ void k_fields_E(PushArgs a, const double *xs, const double *zs, int64_t n, double *ex,
+                           double *ez)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float fx, fz;
+    gather_E(a, xs[k], zs[k], fx, fz);
+    ex[k] = fx;
+    ez[k] = fz;
+}
+
+// Species::accumulate (particles.hpp:358-367) into the fixed-point grid
+__global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, int64_t n,
+                                                         int nx, int nz, double idx, double idz,
+                                                         double xmin, double zmin,
+                                                         unsigned long long *acc,
+                                                         unsigned long long *counters)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float x = xs[p];
+    if (is_dead(x)) return;
+    int i, j;
+    long long w[4];
+    if (cic_weights(((double)x - xmin) * idx, ((double)zs[p] - zmin) * idz, nx, nz, i, j, w))
+        deposit_global(acc, nz, i, j, w);
+    else
+        raise_flag(counters, M2D_FLAG_OUT_OF_RANGE);
+}
+
+// rho += charge * acc * 2^-32; acc = 0
+__global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    long long v = (long long)acc[k];
+    if (v) {
+        rho[k] += charge * ((double)v * kInvFix);
+        acc[k] = 0ull;
+    }
+}
+
+// Field2D::interpolate (Field2D.hpp:64-78) in FP64
+__global__ void k_interp2d(const double *d, int nx, int nz, double idx, double idz, double xmin,
+                           double zmin, const double *xs, const double *zs, int64_t n, double *out,
+                           unsigned long long *counters)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double x = xs[k] - xmin, z = zs[k] - zmin;
+    if (!(x >= 0) || !(z >= 0)) { out[k] = 0; raise_flag(counters, M2D_FLAG_OUT_OF_RANGE); return; }
+    int i = (int)(x * idx), j = (int)(z * idz);
+    if (i > nx - 2 || j > nz - 2) { out[k] = 0; raise_flag(counters, M2D_FLAG_OUT_OF_RANGE); return; }
+    double u = x * idx - i, v = z * idz - j;
+    const double *r = d + (size_t)i * nz + j;
+    out[k] = (1 - u) * (1 - v) * r[0] + u * (1 - v) * r[nz] + (1 - u) * v * r[1] + u * v * r[nz + 1];
+}
+
+// Field2D::grad (Field2D.hpp:80-168) in FP64, branch for branch
+__global__ void k_grad2d(const double *d, int nx, int nz, double idx, double idy, double xmin,
+                         double zmin, const double *xs, const double *zs, int64_t n, double *gx,
+                         double *gz)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int jmax = nx, lmax = nz;
+    auto D = [&](int a, int b) { return d[(size_t)a * nz + b]; };
+    double x = xs[k] - xmin, y = zs[k] - zmin;
+    double g1, g2, g3, g4, fx, fy;
+    int i = (int)(x * idx + 0.5), j = (int)(y * idy);
+    j = min(j, lmax - 2);
+    if (j >= 0) {
+        if (i > 0 && i < jmax - 1) {
+            g1 = (D(i, j) - D(i - 1, j)) * idx;
+            g2 = (D(i, j + 1) - D(i - 1, j + 1)) * idx;
+            g3 = (D(i + 1, j + 1) - D(i, j + 1)) * idx;
+            g4 = (D(i + 1, j) - D(i, j)) * idx;
+            fx = x * idx - i + .5;
+            fy = y * idy - j;
+            gx[k] = g1 * (1 - fx) * (1 - fy) + g2 * (1 - fx) * fy + g3 * fx * fy + g4 * fx * (1 - fy);
+        } else if (i == jmax - 1) {
+            g1 = (D(i, j) - D(i - 1, j)) * idx;
+            g2 = (D(i, j + 1) - D(i - 1, j + 1)) * idx;
+            fy = y * idy - j;
+            gx[k] = g1 * (1 - fy) + g2 * fy;
+        } else if (i == 0) {
+            g3 = (D(i + 1, j + 1) - D(i, j + 1)) * idx;
+            g4 = (D(i + 1, j) - D(i, j)) * idx;
+            fy = y * idy - j;
+            gx[k] = g3 * fy + g4 * (1 - fy);
+        }
+    }
+    i = (int)(x * idx);
+    j = (int)(y * idy + 0.5);
+    i = min(i, jmax - 2);
+    if (i >= 0) {
+        if (j > 0 && j < lmax - 1) {
+            g1 = (D(i, j) - D(i, j - 1)) * idy;
+            g2 = (D(i + 1, j) - D(i + 1, j - 1)) * idy;
+            g3 = (D(i + 1, j + 1) - D(i + 1, j)) * idy;
+            g4 = (D(i, j + 1) - D(i, j)) * idy;
+            fx = x * idx - i;
+            fy = y * idy - j + 0.5;
+            gz[k] = g1 * (1 - fx) * (1 - fy) + g2 * (1 - fy) * fx + g3 * fx * fy + g4 * fy * (1 - fx);
+        } else if (j == lmax - 1) {
+            g1 = (D(i, j) - D(i, j - 1)) * idy;
+            g2 = (D(i + 1, j) - D(i + 1, j - 1)) * idy;
+            fx = x * idx - i;
+            gz[k] = g1 * (1 - fx) + g2 * fx;
+        } else if (j == 0) {
+            g3 = (D(i + 1, j + 1) - D(i + 1, j)) * idy;
+            g4 = (D(i, j + 1) - D(i, j)) * idy;
+            fx = x * idx - i;
+            gz[k] = g3 * fx + g4 * (1 - fx);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ reductions
+// per-block sum of |v|^2 (FP64) and live count; fixed-order tree in LDS
+__global__ void __launch_bounds__(kBlock) k_energy_partial(const float *xs, const float *vx,
+                                                           const float *vy, const float *vz,
+                                                           int64_t n, double *psum,
+                                                           unsigned long long *pcnt)
+{
+    __shared__ double s[kBlock];
+    __shared__ unsigned long long c[kBlock];
+    double acc = 0;
+    unsigned long long cnt = 0;
+    const int64_t base = (int64_t)blockIdx.x * (kBlock * kPPT) + threadIdx.x;
+    for (int q = 0; q < kPPT; q++) {
+        int64_t p = base + (int64_t)q * kBlock;
+        if (p < n && !is_dead(xs[p])) {
+            double a = vx[p], b = vy[p], d = vz[p];
+            acc += a * a + d * d + b * b;
+            cnt++;
+        }
+    }
+    s[threadIdx.x] = acc;
+    c[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s[threadIdx.x] += s[threadIdx.x + o];
+            c[threadIdx.x] += c[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { psum[blockIdx.x] = s[0]; pcnt[blockIdx.x] = c[0]; }
+}
+
+__global__ void __launch_bounds__(1024) k_energy_final(const double *psum,
+                                                       const unsigned long long *pcnt, int nb,
+                                                       double *out_sum,
+                                                       unsigned long long *out_cnt)
+{
+    __shared__ double s[1024];
+    __shared__ unsigned long long c[1024];
+    double a = 0;
+    unsigned long long b = 0;
+    for (int k = threadIdx.x; k < nb; k += 1024) { a += psum[k]; b += pcnt[k]; }
+    s[threadIdx.x] = a;
+    c[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { s[threadIdx.x] += s[threadIdx.x + o]; c[threadIdx.x] += c[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { *out_sum = s[0]; *out_cnt = c[0]; }
+}
+
+// ------------------------------------------------------------------ compaction
+constexpr int kCompactItems = kBlock * kPPT;
+
+__global__ void __launch_bounds__(kBlock) k_count_blocks(const float *xs, int64_t n,
+                                                         unsigned int *counts)
+{
+    __shared__ unsigned int s[kBlock / 64];
+    unsigned int c = 0;
+    const int64_t base = (int64_t)blockIdx.x * kCompactItems + threadIdx.x;
+    for (int q = 0; q < kPPT; q++) {
+        int64_t p = base + (int64_t)q * kBlock;
+        c += (p < n && !is_dead(xs[p])) ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (__lane_id() == 0) s[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+        for (int w = 0; w < kBlock / 64; w++) t += s[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of nb block counts by one block; total into *total
+__global__ void __launch_bounds__(1024) k_scan_counts(const unsigned int *counts, int nb,
+                                                      unsigned long long *offs,
+                                                      unsigned long long *total)
+{
+    __shared__ unsigned long long s[1024];
+    const int per = (nb + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = min(nb, lo + per);
+    unsigned long long loc = 0;
+    for (int k = lo; k < hi; k++) loc += counts[k];
+    s[threadIdx.x] = loc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        unsigned long long v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s[threadIdx.x] - loc;
+    for (int k = lo; k < hi; k++) { offs[k] = run; run += counts[k]; }
+    if (threadIdx.x == 1023) *total = s[1023];
+}
+
+__global__ void __launch_bounds__(kBlock) k_compact_scatter(m2d_particles2d src, m2d_particles2d dst,
+                                                            const unsigned long long *offs)
+{
+    __shared__ unsigned int wsum[kBlock / 64];
+    unsigned long long run = offs[blockIdx.x];
+    const int lane = __lane_id(), wid = threadIdx.x / 64;
+    const int64_t base = (int64_t)blockIdx.x * kCompactItems + threadIdx.x;
+    for (int q = 0; q < kPPT; q++) {
+        int64_t p = base + (int64_t)q * kBlock;
+        bool live = p < src.n && !is_dead(src.x[p]);
+        unsigned long long m = __ballot(live);
+        unsigned int before = (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wid] = (unsigned int)__popcll(m);
+        __syncthreads();
+        unsigned int woff = 0, tot = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            woff += w < wid ? wsum[w] : 0u;
+            tot += wsum[w];
+        }
+        if (live) {
+            unsigned long long d = run + woff + before;
+            dst.x[d] = src.x[p]; dst.z[d] = src.z[p];
+            dst.vx[d] = src.vx[p]; dst.vy[d] = src.vy[p]; dst.vz[d] = src.vz[p];
+        }
+        run += tot;
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_dead(float *x, int64_t n)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = dead_value();
+}
+
+// ------------------------------------------------------------------ sampler
+// add_particles_on_disk (particles.cpp:728-749), one Philox stream per slot
+__global__ void __launch_bounds__(kBlock) k_disk2d(m2d_particles2d P, double cx, double cz,
+                                                   double R, double vth, double bx0, double bx1,
+                                                   double bz0, double bz1, uint64_t seed,
+                                                   uint32_t stream)
+{
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    Rng r;
+    r.begin(seed, stream, (uint32_t)p, 0, kPurposeInit);
+    double x = 1, y = 1;
+    for (int t = 0; t < 64 && x * x + y * y > 0.25; t++) {
+        x = r.uni() - 0.5;
+        y = r.uni() - 0.5;
+    }
+    x = x * 2 * R + cx;
+    y = y * 2 * R + cz;
+    if (x < bx0 || x > bx1 || y < bz0 || y > bz1) {
+        P.x[p] = dead_value();
+        return;
+    }
+    double a = r.normal() * vth, b = r.normal() * vth, c = r.normal() * vth;
+    P.x[p] = (float)x; P.z[p] = (float)y;
+    P.vx[p] = (float)a; P.vz[p] = (float)b; P.vy[p] = (float)c;
+}
+
+// ------------------------------------------------------------------ host helpers
+static int check_parts(const m2d_particles2d *p)
+{
+    M2D_CHECK(p, M2D_ERR_BAD_ARG, "particles is NULL");
+    M2D_CHECK(p->n >= 0 && p->n < (1ll << 32), M2D_ERR_BAD_ARG, "particle count %lld out of range",
+              (long long)p->n);
+    M2D_CHECK(p->n == 0 || (p->x && p->z && p->vx && p->vy && p->vz), M2D_ERR_BAD_ARG,
+              "particle arrays must be non-NULL");
+    return M2D_OK;
+}
+
+static int ensure_acc(m2d_ctx *ctx, size_t n)
+{
+    size_t need = n * sizeof(unsigned long long);
+    if (ctx->acc.bytes >= need) return M2D_OK;
+    M2D_TRY(ensure(ctx, ctx->acc, need));
+    return zero_words(ctx, (unsigned long long *)ctx->acc.ptr, ctx->acc.bytes / 8);
+}
+
+static int flush(m2d_ctx *ctx, size_t n, double *rho, double charge)
+{
+    k_flush<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>((unsigned long long *)ctx->acc.ptr,
+                                                                   rho, charge, n);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
+                           const m2d_particles2d *p, double rot_scale, double kick_scale)
+{
+    const m2d_fields2d_params &P = f->p;
+    a.x = p->x; a.z = p->z; a.vx = p->vx; a.vy = p->vy; a.vz = p->vz; a.n = p->n;
+    a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree;
+    a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
+    a.xmin = P.grid.xmin; a.zmin = P.grid.zmin;
+    a.bx0 = P.grid.xmin; a.bx1 = P.x_max; a.bz0 = P.grid.zmin; a.bz1 = P.z_max;
+    // rotation: t = B q dt/(2m) scaled (particles.cpp:959-975, init: -0.5, 1028)
+    double tmp = rot_scale * s->charge * s->dt / (2.0 * s->mass);
+    double tx = P.Br * tmp, ty = P.Bt * tmp, tz = P.Bz * tmp;
+    double sc = 2.0 / (1 + tx * tx + ty * ty + tz * tz);
+    a.tx = (float)tx; a.ty = (float)ty; a.tz = (float)tz;
+    a.sx = (float)(tx * sc); a.sy = (float)(ty * sc); a.sz = (float)(tz * sc);
+    a.qmdt2 = (float)(kick_scale * s->charge / s->mass * s->dt / 2.0);
+    a.dt = (float)s->dt;
+    // Fields::E (fields.hpp:124-150); phase hoisted: t = niter dt for every particle
+    a.econst = P.geometry_empty && !P.selfconsistent;
+    a.ecx = 0.f;
+    a.ecz = (float)P.extern_field;
+    double phase = 0.0;
+    if (P.rf) {
+        double t = (double)s->niter * s->dt;
+        double ph = t * P.rf_omega;
+        double y = 10000000 * M_PI;
+        if (!(ph >= 0.0 && ph <= y)) ph = ph - y * (int)(ph / y) + (ph < 0 ? y : 0);  // mymath.cpp:62-67
+        phase = P.rf_amplitude * cos(ph) + P.rf_U0;
+    }
+    a.phase = (float)phase;
+    a.boundary = 0; a.periodic = P.boundary == M2D_BC_PERIODIC;
+    a.checkfree = !P.electric_field_from_file;
+    a.prob = 0; a.seed = f->ctx->seed; a.stream = s->stream; a.step = s->niter;
+    a.mcc = nullptr;
+    a.acc = nullptr;
+    a.counters = f->ctx->counters;
+}
+
+}  // namespace m2d
+
+using namespace m2d;
+
+extern "C" {
+
+int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const m2d_particles2d *p,
+                   const m2d_mcc *mcc, double *rho, uint32_t flags, m2d_step_stats *stats)
+{
+    M2D_CHECK(ctx && f && s, M2D_ERR_BAD_ARG, "m2d_advance_2d: NULL argument");
+    M2D_CHECK(f->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_advance_2d: fields belong to another context");
+    M2D_TRY(check_parts(p));
+    M2D_CHECK(s->mass > 0 && s->dt > 0, M2D_ERR_BAD_ARG, "m2d_advance_2d: mass and dt must be > 0");
+    bool deposit = (flags & M2D_STEP_DEPOSIT) != 0;
+    M2D_CHECK(!deposit || rho, M2D_ERR_BAD_ARG, "m2d_advance_2d: DEPOSIT needs rho");
+    M2D_CHECK(flags & M2D_STEP_PUSH, M2D_ERR_UNSUPPORTED,
+              "m2d_advance_2d: PUSH is required (use m2d_boundary paths via flags)");
+    if (stats) {
+        M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
+    }
+    PushArgs a;
+    fill_push_args(a, f, s, p, 1.0, 1.0);
+    a.boundary = (flags & M2D_STEP_BOUNDARY) != 0;
+    bool use_mcc = mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
+    if (use_mcc) {
+        a.prob = 1.0 - exp(-s->dt / s->lifetime);  // particles.cpp:937
+        a.mcc = mcc->d;
+    }
+    if (deposit) {
+        M2D_TRY(ensure_acc(ctx, f->n));
+        a.acc = (unsigned long long *)ctx->acc.ptr;
+    }
+    if (p->n > 0) {
+        int g = grid_for(p->n, kPPT);
+        hipEvent_t t0 = prof_begin(ctx);
+        if (deposit && use_mcc) k_advance2d<true, true><<<g, kBlock, 0, ctx->stream>>>(a);
+        else if (deposit) k_advance2d<true, false><<<g, kBlock, 0, ctx->stream>>>(a);
+        else if (use_mcc) k_advance2d<false, true><<<g, kBlock, 0, ctx->stream>>>(a);
+        else k_advance2d<false, false><<<g, kBlock, 0, ctx->stream>>>(a);
+        M2D_HIP(hipGetLastError());
+        prof_end(ctx, t0);
+    }
+    if (deposit) M2D_TRY(flush(ctx, f->n, rho, s->charge));
+    if (stats) {
+        M2D_TRY(read_counters(ctx));
+        stats->removed = (int64_t)ctx->h_counters[CNT_REMOVED];
+        stats->collisions = (int64_t)ctx->h_counters[CNT_COLL];
+    }
+    return M2D_OK;
+}
+
+int m2d_advance_init_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
+                        const m2d_particles2d *p)
+{
+    M2D_CHECK(ctx && f && s, M2D_ERR_BAD_ARG, "m2d_advance_init_2d: NULL argument");
+    M2D_TRY(check_parts(p));
+    M2D_CHECK(s->mass > 0 && s->dt > 0, M2D_ERR_BAD_ARG, "m2d_advance_init_2d: mass and dt must be > 0");
+    PushArgs a;
+    fill_push_args(a, f, s, p, -0.5, -1.0);
+    if (p->n > 0) {
+        k_advance_init2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(a);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_fields2d_E(m2d_fields2d *f, const double *x, const double *z, int64_t n, double t,
+                   double *ex, double *ez)
+{
+    M2D_CHECK(f && (n == 0 || (x && z && ex && ez)), M2D_ERR_BAD_ARG, "m2d_fields2d_E: NULL argument");
+    m2d_species2d s{};
+    s.charge = 1; s.mass = 1; s.dt = 1;
+    // phase at time t: niter*dt == t with dt = t, niter = 1 (t > 0) or dt = 1, niter = 0
+    if (t > 0) { s.dt = t; s.niter = 1; }
+    PushArgs a;
+    m2d_particles2d none{};
+    fill_push_args(a, f, &s, &none, 1.0, 1.0);
+    if (n > 0) {
+        k_fields_E<<<(unsigned)((n + 255) / 256), 256, 0, f->ctx->stream>>>(a, x, z, n, ex, ez);
+        M2D_HIP(hipGetLastError());
+    }
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+int m2d_accumulate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_particles2d *p, double charge,
+                      double *rho)
+{
+    M2D_CHECK(ctx && g && rho, M2D_ERR_BAD_ARG, "m2d_accumulate_2d: NULL argument");
+    M2D_CHECK(g->nx >= 2 && g->nz >= 2 && g->dx > 0 && g->dz > 0, M2D_ERR_BAD_ARG,
+              "m2d_accumulate_2d: bad grid");
+    M2D_TRY(check_parts(p));
+    size_t n = (size_t)g->nx * g->nz;
+    M2D_TRY(ensure_acc(ctx, n));
+    if (p->n > 0) {
+        k_accumulate2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(
+            p->x, p->z, p->n, g->nx, g->nz, 1.0 / g->dx, 1.0 / g->dz, g->xmin, g->zmin,
+            (unsigned long long *)ctx->acc.ptr, ctx->counters);
+        M2D_HIP(hipGetLastError());
+    }
+    return flush(ctx, n, rho, charge);
+}
+
+int m2d_interpolate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const double *data, const double *x,
+                       const double *z, int64_t n, double *out)
+{
+    M2D_CHECK(ctx && g && data && (n == 0 || (x && z && out)), M2D_ERR_BAD_ARG,
+              "m2d_interpolate_2d: NULL argument");
+    if (n > 0) {
+        k_interp2d<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+            data, g->nx, g->nz, 1.0 / g->dx, 1.0 / g->dz, g->xmin, g->zmin, x, z, n, out,
+            ctx->counters);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_grad_2d(m2d_ctx *ctx, const m2d_grid2d *g, const double *data, const double *x,
+                const double *z, int64_t n, double *gx, double *gz)
+{
+    M2D_CHECK(ctx && g && data && (n == 0 || (x && z && gx && gz)), M2D_ERR_BAD_ARG,
+              "m2d_grad_2d: NULL argument");
+    if (n > 0) {
+        k_grad2d<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+            data, g->nx, g->nz, 1.0 / g->dx, 1.0 / g->dz, g->xmin, g->zmin, x, z, n, gx, gz);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_mean_energy_2d(m2d_ctx *ctx, const m2d_particles2d *p, double mass, double *mean_eV,
+                       int64_t *n_live)
+{
+    M2D_CHECK(ctx && mean_eV, M2D_ERR_BAD_ARG, "m2d_mean_energy_2d: NULL argument");
+    M2D_TRY(check_parts(p));
+    int nb = grid_for(p->n, kPPT);
+    M2D_TRY(ensure(ctx, ctx->red, (size_t)nb * 16 + 64));
+    double *psum = (double *)ctx->red.ptr;
+    unsigned long long *pcnt = (unsigned long long *)(psum + nb);
+    double *osum = (double *)(pcnt + nb);
+    unsigned long long *ocnt = (unsigned long long *)(osum + 1);
+    k_energy_partial<<<nb, kBlock, 0, ctx->stream>>>(p->x, p->vx, p->vy, p->vz, p->n, psum, pcnt);
+    k_energy_final<<<1, 1024, 0, ctx->stream>>>(psum, pcnt, nb, osum, ocnt);
+    M2D_HIP(hipGetLastError());
+    double hs = 0;
+    unsigned long long hc = 0;
+    M2D_HIP(hipMemcpyAsync(&hs, osum, 8, hipMemcpyDeviceToHost, ctx->stream));
+    M2D_HIP(hipMemcpyAsync(&hc, ocnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+    M2D_HIP(hipStreamSynchronize(ctx->stream));
+    // Histogram::mean_tot = sum_tot / n_val_tot (histogram.cpp:68-71)
+    *mean_eV = hc ? hs * mass * 0.5 / kQE / (double)hc : 0.0;
+    if (n_live) *n_live = (int64_t)hc;
+    return M2D_OK;
+}
+
+int m2d_count_live_2d(m2d_ctx *ctx, const m2d_particles2d *p, int64_t *n_live)
+{
+    M2D_CHECK(ctx && n_live, M2D_ERR_BAD_ARG, "m2d_count_live_2d: NULL argument");
+    M2D_TRY(check_parts(p));
+    int nb = grid_for(p->n, kPPT);
+    M2D_TRY(ensure(ctx, ctx->tmp, (size_t)nb * 12 + 64));
+    unsigned int *counts = (unsigned int *)ctx->tmp.ptr;
+    unsigned long long *offs = (unsigned long long *)(((uintptr_t)(counts + nb) + 15) & ~(uintptr_t)15);
+    unsigned long long *total = offs + nb;
+    k_count_blocks<<<nb, kBlock, 0, ctx->stream>>>(p->x, p->n, counts);
+    k_scan_counts<<<1, 1024, 0, ctx->stream>>>(counts, nb, offs, total);
+    M2D_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    M2D_HIP(hipMemcpyAsync(&h, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    M2D_HIP(hipStreamSynchronize(ctx->stream));
+    *n_live = (int64_t)h;
+    return M2D_OK;
+}
+
+int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles2d *dst,
+                   int64_t *n_live)
+{
+    M2D_CHECK(ctx && n_live, M2D_ERR_BAD_ARG, "m2d_compact_2d: NULL argument");
+    M2D_TRY(check_parts(src));
+    M2D_TRY(check_parts(dst));
+    M2D_CHECK(dst->n >= src->n, M2D_ERR_BAD_ARG, "m2d_compact_2d: dst holds %lld < %lld slots",
+              (long long)dst->n, (long long)src->n);
+    M2D_CHECK(dst->x != src->x, M2D_ERR_BAD_ARG, "m2d_compact_2d: src and dst must differ");
+    int nb = grid_for(src->n, kPPT);
+    M2D_TRY(ensure(ctx, ctx->tmp, (size_t)nb * 12 + 64));
+    unsigned int *counts = (unsigned int *)ctx->tmp.ptr;
+    unsigned long long *offs = (unsigned long long *)(((uintptr_t)(counts + nb) + 15) & ~(uintptr_t)15);
+    unsigned long long *total = offs + nb;
+    k_count_blocks<<<nb, kBlock, 0, ctx->stream>>>(src->x, src->n, counts);
+    k_scan_counts<<<1, 1024, 0, ctx->stream>>>(counts, nb, offs, total);
+    k_compact_scatter<<<nb, kBlock, 0, ctx->stream>>>(*src, *dst, offs);
+    M2D_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    M2D_HIP(hipMemcpyAsync(&h, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    M2D_HIP(hipStreamSynchronize(ctx->stream));
+    *n_live = (int64_t)h;
+    // slots past the live prefix become empty
+    if ((int64_t)h < dst->n) {
+        int64_t rest = dst->n - (int64_t)h;
+        k_fill_dead<<<(unsigned)((rest + 255) / 256), 256, 0, ctx->stream>>>(dst->x + h, rest);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p, double cx,
+                                 double cz, double radius, double vth, uint32_t stream)
+{
+    M2D_CHECK(ctx && f, M2D_ERR_BAD_ARG, "m2d_add_particles_on_disk_2d: NULL argument");
+    M2D_TRY(check_parts(p));
+    M2D_CHECK(radius > 0 && vth >= 0, M2D_ERR_BAD_ARG, "m2d_add_particles_on_disk_2d: radius/vth");
+    if (p->n == 0) return M2D_OK;
+    k_disk2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(*p, cx, cz, radius, vth, f->p.grid.xmin,
+                                                             f->p.x_max, f->p.grid.zmin, f->p.z_max,
+                                                             ctx->seed, stream);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+import os
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the C ABI on cuda:0)")
+
+
+def _built():
+    return (ROOT / "mag2d_amd" / "libmag2d_amd.so").exists() and (ROOT / "oracle" / "liboracle.so").exists()
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _libraries():
+    # load what is built; build only when nothing is
+    if not _built():
+        from mag2d_amd import build
+        build.build_all()
+    yield
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from mag2d_amd import abi
+    c = abi.Context(0, seed=1234)
+    yield c
+    c.close()

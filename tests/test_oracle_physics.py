@@ -1,0 +1,349 @@
+"""Pin the oracle where the reference's own code cannot run here (Boris push,
+boundary, MCC scatter, Poisson, geometry builders): closed forms, invariants
+and independent library routines, each chosen so that a plausible slip
+(dropped term, sign, swapped component) fails."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+from scipy import stats
+
+from oracle import oracle as O
+
+QE, KB = O.QE, O.KB
+M_ION = 3 * 1.66053906660e-27
+
+
+def empty_box(n=33, L=1e-2):
+    g = O.grid2(n, n, L / (n - 1), L / (n - 1))
+    mask = np.full(n * n, O.FREE, np.uint8)
+    m2 = mask.reshape(n, n)
+    m2[0, :] = m2[-1, :] = m2[:, 0] = m2[:, -1] = O.FIXED
+    return g, mask
+
+
+def one_particle(x, z, vx, vy, vz):
+    p = O.particles(1)
+    p["x"], p["z"], p["vx"], p["vy"], p["vz"] = x, z, vx, vy, vz
+    return p
+
+
+# ------------------------------------------------------------------ Boris
+
+def test_boris_pure_rotation_angle_and_handedness():
+    # E = 0 (EMPTY geometry, extern_field 0), uniform Bz (in-plane z axis)
+    g, mask = empty_box()
+    u = np.zeros(g.nx * g.nz)
+    q, m, dt, B = QE, M_ION, 1e-9, 0.5
+    f = O.make_fields(g, mask, u, x_max=1e-2, z_max=1e-2, geometry_empty=1, extern_field=0.0, Bz=B)
+    s = O.make_species(q, m, dt)
+    p = one_particle(5e-3, 5e-3, 1000.0, 0.0, 0.0)
+    O.lib().o_advance_boris(C.byref(f), C.byref(s), O.dptr(p), 1, None)
+    theta = 2 * np.arctan(q * B * dt / (2 * m))
+    # By hand from particles.cpp:966-979 with only tz = qB dt/2m set:
+    #   vprime_r = vx - vy tz, vprime_t = vy + vx tz,
+    #   vx' = vx - vprime_t sz, vy' = vy + vprime_r sz
+    # so (1000, 0) turns towards +vy: the rotation is right-handed in the
+    # (x, z, y) frame (left-handed in (x, y, z)).
+    assert p["vx"][0] == pytest.approx(1000.0 * np.cos(theta), rel=1e-12)
+    assert p["vy"][0] == pytest.approx(+1000.0 * np.sin(theta), rel=1e-12)
+    assert p["vz"][0] == 0.0
+    assert np.hypot(p["vx"][0], p["vy"][0]) == pytest.approx(1000.0, rel=1e-14)
+
+
+def test_boris_free_acceleration_exact():
+    g, mask = empty_box()
+    u = np.zeros(g.nx * g.nz)
+    q, m, dt, Ez = QE, M_ION, 1e-9, 250.0
+    f = O.make_fields(g, mask, u, x_max=1e-2, z_max=1e-2, geometry_empty=1, extern_field=Ez)
+    s = O.make_species(q, m, dt)
+    p = one_particle(2e-3, 3e-3, 100.0, 7.0, -50.0)
+    O.lib().o_advance_boris(C.byref(f), C.byref(s), O.dptr(p), 1, None)
+    # EMPTY & !selfconsistent: E = (0, +extern_field) un-negated (fields.hpp:126-131)
+    vz = -50.0 + q * Ez * dt / m
+    assert p["vz"][0] == pytest.approx(vz, rel=1e-14)
+    assert p["vx"][0] == 100.0 and p["vy"][0] == 7.0
+    assert p["z"][0] == pytest.approx(3e-3 + vz * dt, rel=1e-14)
+    assert p["x"][0] == pytest.approx(2e-3 + 100.0 * dt, rel=1e-14)
+
+
+def test_boris_exb_drift():
+    # E along z from extern_field, B along the out-of-plane y (Bt): drift E x B / B^2 along x
+    g, mask = empty_box()
+    u = np.zeros(g.nx * g.nz)
+    q, m, dt = QE, M_ION, 2e-10
+    E, B = 100.0, 2.0
+    f = O.make_fields(g, mask, u, x_max=1e-2, z_max=1e-2, geometry_empty=1, extern_field=E, Bt=B)
+    s = O.make_species(q, m, dt)
+    p = one_particle(5e-3, 5e-3, 0.0, 0.0, 0.0)
+    O.lib().o_advance_boris_init(C.byref(f), C.byref(s), O.dptr(p), 1)
+    x0 = p["x"][0]
+    n = 20000
+    for _ in range(n):
+        O.lib().o_advance_boris(C.byref(f), C.byref(s), O.dptr(p), 1, None)
+    vdrift = (p["x"][0] - x0) / (n * dt)
+    # in the right-handed (x, z, y) frame: z^ x y^ = x^, so E x B / B^2 = +E/B x^
+    wc = q * B / m
+    period = 2 * np.pi / wc
+    assert n * dt > 20 * period
+    assert abs(vdrift - E / B) < 0.02 * E / B
+
+
+def test_boris_init_then_step_keeps_speed_in_pure_B():
+    g, mask = empty_box()
+    u = np.zeros(g.nx * g.nz)
+    f = O.make_fields(g, mask, u, x_max=1e-2, z_max=1e-2, geometry_empty=1, extern_field=0.0,
+                      Br=0.1, Bz=0.2, Bt=0.3)
+    s = O.make_species(QE, M_ION, 1e-9)
+    p = one_particle(5e-3, 5e-3, 300.0, -200.0, 100.0)
+    sp0 = np.sqrt(300.0 ** 2 + 200.0 ** 2 + 100.0 ** 2)
+    O.lib().o_advance_boris_init(C.byref(f), C.byref(s), O.dptr(p), 1)
+    for _ in range(50):
+        O.lib().o_advance_boris(C.byref(f), C.byref(s), O.dptr(p), 1, None)
+    v = np.sqrt(p["vx"][0] ** 2 + p["vy"][0] ** 2 + p["vz"][0] ** 2)
+    assert v == pytest.approx(sp0, rel=1e-13)
+
+
+# ------------------------------------------------------------------ Field2D closed forms
+
+def test_grad_of_linear_potential_is_exact_everywhere():
+    n = 17
+    g = O.grid2(n, 13, 1e-3, 1e-3)
+    i, j = np.meshgrid(np.arange(n), np.arange(13), indexing="ij")
+    a, b = 3.5, -1.25
+    u = (a * i * g.dx + b * j * g.dz).ravel()
+    rng = np.random.default_rng(1)
+    pts = list(zip(rng.uniform(0, (n - 1) * g.dx, 300), rng.uniform(0, 12 * g.dz, 300)))
+    pts += [(0.0, 0.0), ((n - 1) * g.dx, 12 * g.dz), (0.2 * g.dx, 12 * g.dz), ((n - 1) * g.dx, 0.3e-3)]
+    for x, z in pts:
+        gx, gz = C.c_double(), C.c_double()
+        O.lib().o_f2_grad(C.byref(g), O.dptr(u), x, z, C.byref(gx), C.byref(gz))
+        assert gx.value == pytest.approx(a, rel=1e-9)
+        assert gz.value == pytest.approx(b, rel=1e-9)
+
+
+def test_cic_single_particle_on_node_and_bilinear_weights():
+    g = O.grid2(4, 4, 1.0, 1.0)
+    acc = np.zeros(16)
+    assert O.lib().o_f2_accumulate(C.byref(g), O.dptr(acc), 2.0, 1.0, 2.0) == 0
+    assert acc[1 * 4 + 2] == 2.0 and acc.sum() == 2.0
+    acc[:] = 0
+    O.lib().o_f2_accumulate(C.byref(g), O.dptr(acc), 1.0, 1.25, 0.5)
+    want = np.zeros((4, 4))
+    want[1, 0], want[2, 0], want[1, 1], want[2, 1] = 0.75 * 0.5, 0.25 * 0.5, 0.75 * 0.5, 0.25 * 0.5
+    np.testing.assert_allclose(acc.reshape(4, 4), want, atol=1e-15)
+    # a stencil that would leave the grid is refused (the reference writes past it)
+    assert O.lib().o_f2_accumulate(C.byref(g), O.dptr(acc), 1.0, 3.0, 1.0) == -2
+
+
+# ------------------------------------------------------------------ geometry
+
+def test_rf22pt_geometry_invariants():
+    n = 128
+    g = O.grid2(n, n, 2e-2 / (n - 1), 2e-2 / (n - 1))
+    mask = np.zeros(n * n, np.uint8)
+    volt = np.zeros(n * n)
+    geo = O.Geom(2, 2e-2, 2e-2, 1e-4, -10.0, 100.0)
+    O.lib().o_grid_build(C.byref(g), C.byref(geo), O.dptr(mask), O.dptr(volt))
+    m = mask.reshape(n, n)
+    assert (m[0, :] == O.FIXED).all() and (m[:, -1] == O.FIXED).all()
+    rods = m == O.FIXED_RF
+    assert rods.sum() > 0
+    v = volt.reshape(n, n)
+    assert set(np.unique(v[rods])) == {-1.0, 1.0}
+    # 22 rods alternate in sign: equal node counts within discretisation
+    assert abs((v[rods] > 0).sum() - (v[rods] < 0).sum()) <= 0.2 * rods.sum()
+    # only FIXED 4-neighbours inside [2, M-2) create BOUNDARY (fields.cpp:507-518):
+    # the frame is two nodes away and the rods are FIXED_RF, so there is none
+    assert (m == O.BOUNDARY).sum() == 0
+    # a FIXED probe does get a BOUNDARY ring: exactly its free 4-neighbours
+    O.lib().o_grid_build(C.byref(g), C.byref(O.Geom(1, 2e-2, 2e-2, 2e-3, -10.0, 0.0)), O.dptr(mask),
+                         O.dptr(volt))
+    m = mask.reshape(n, n)
+    fixed = m == O.FIXED
+    nb = np.zeros_like(fixed)
+    nb[1:, :] |= fixed[:-1, :]
+    nb[:-1, :] |= fixed[1:, :]
+    nb[:, 1:] |= fixed[:, :-1]
+    nb[:, :-1] |= fixed[:, 1:]
+    want = nb & ~fixed
+    want[:2, :] = want[-2:, :] = want[:, :2] = want[:, -2:] = False
+    assert ((m == O.BOUNDARY) == want).all() and want.sum() > 0
+
+
+# ------------------------------------------------------------------ Poisson
+
+def _assembled(g, mask):
+    n = g.nx * g.nz
+    rows, cols, vals = [], [], []
+    for k in range(n):
+        if mask[k] in (O.FIXED, O.FIXED_RF):
+            rows.append(k); cols.append(k); vals.append(1.0)
+        else:
+            for c, v in ((k - g.nz, 1.0), (k - 1, 1.0), (k, -4.0), (k + 1, 1.0), (k + g.nz, 1.0)):
+                rows.append(k); cols.append(c); vals.append(v)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def test_poisson_direct_solve_matches_scipy_and_has_tiny_residual():
+    n = 40
+    g = O.grid2(n, n, 2e-2 / (n - 1), 2e-2 / (n - 1))
+    mask = np.zeros(n * n, np.uint8)
+    volt = np.zeros(n * n)
+    O.lib().o_grid_build(C.byref(g), C.byref(O.Geom(5, 2e-2, 2e-2, 1e-4, -10, 100)), O.dptr(mask),
+                         O.dptr(volt))
+    rng = np.random.default_rng(3)
+    rho = rng.normal(size=n * n) * 1e-15
+    O.lib().o_poisson_rhs(C.byref(g), O.dptr(mask), O.dptr(volt), O.dptr(rho), O.SOLVE_RF, 1e-12)
+    u = np.zeros(n * n)
+    assert O.lib().o_poisson_solve(C.byref(g), O.dptr(mask), O.dptr(rho), O.dptr(u)) == 0
+    A = _assembled(g, mask)
+    ref = spla.spsolve(A.tocsc(), rho)
+    np.testing.assert_allclose(u, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    y = np.zeros(n * n)
+    assert O.lib().o_poisson_apply(C.byref(g), O.dptr(mask), O.dptr(u), O.dptr(y)) == 0
+    assert np.linalg.norm(y - rho) <= 1e-12 * np.linalg.norm(rho)
+
+
+def test_poisson_discrete_eigenfunction():
+    # u = sin(k pi i/(M-1)) sin(l pi j/(N-1)) has A u = (2cos(k pi/(M-1)) + 2cos(l pi/(N-1)) - 4) u
+    M, N, k, l = 33, 25, 2, 3
+    g = O.grid2(M, N, 1e-3, 1e-3)
+    mask = np.full(M * N, O.FREE, np.uint8)
+    mm = mask.reshape(M, N)
+    mm[0, :] = mm[-1, :] = mm[:, 0] = mm[:, -1] = O.FIXED
+    i, j = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    ue = (np.sin(k * np.pi * i / (M - 1)) * np.sin(l * np.pi * j / (N - 1))).ravel()
+    lam = 2 * np.cos(k * np.pi / (M - 1)) + 2 * np.cos(l * np.pi / (N - 1)) - 4
+    b = lam * ue
+    b[mask == O.FIXED] = 0.0
+    u = np.zeros(M * N)
+    assert O.lib().o_poisson_solve(C.byref(g), O.dptr(mask), O.dptr(b), O.dptr(u)) == 0
+    np.testing.assert_allclose(u, ue, atol=1e-12)
+
+
+def test_poisson_empty_geometry_linear_voltage_reproduces_itself():
+    n = 30
+    g = O.grid2(n, n, 1e-3, 1e-3)
+    mask = np.zeros(n * n, np.uint8)
+    volt = np.zeros(n * n)
+    O.lib().o_grid_build(C.byref(g), C.byref(O.Geom(0, 1e-2, 1e-2, 1e-4, -10, 500.0)), O.dptr(mask),
+                         O.dptr(volt))
+    rho = np.zeros(n * n)
+    O.lib().o_poisson_rhs(C.byref(g), O.dptr(mask), O.dptr(volt), O.dptr(rho), O.SOLVE_STATIC, 1.0)
+    u = np.zeros(n * n)
+    O.lib().o_poisson_solve(C.byref(g), O.dptr(mask), O.dptr(rho), O.dptr(u))
+    j = np.tile(np.arange(n), n)
+    np.testing.assert_allclose(u, -500.0 * g.dz * (j - n // 2), atol=1e-11)
+
+
+# ------------------------------------------------------------------ MCC
+
+def _mcc_one_target(mass, T_target, m_target, inters, density=1e20, E_max=1.0):
+    return O.make_mcc(mass, E_max, [dict(mass=m_target, density=density, temperature=T_target,
+                                          inters=inters)])
+
+
+def test_lifetime_constant_rate():
+    m = _mcc_one_target(M_ION, 300.0, 6.69e-27, [O.inter(O.ELASTIC, M_ION, 6.69e-27, rate=1e-15)])
+    assert m.lifetime == pytest.approx(1.0 / (1e-15 * 1e20), rel=1e-14)
+
+
+def test_elastic_with_target_at_rest_conserves_cm_speed():
+    m2 = 6.69e-27
+    mcc = _mcc_one_target(M_ION, 0.0, m2, [O.inter(O.ELASTIC, M_ION, m2, rate=1e-15)])
+    r = O.Rng()
+    O.lib().o_rng_shr3_seed(C.byref(r), 99)
+    for _ in range(50):
+        p = one_particle(0, 0, 1200.0, -300.0, 450.0)
+        v0 = np.array([1200.0, -300.0, 450.0])
+        assert O.lib().o_scatter(C.byref(mcc), O.dptr(p), C.byref(r)) == 0
+        v1 = np.array([p["vx"][0], p["vy"][0], p["vz"][0]])
+        vcm = v0 * M_ION / (M_ION + m2)
+        assert np.linalg.norm(v1 - vcm) == pytest.approx(np.linalg.norm(v0) * m2 / (M_ION + m2), rel=1e-12)
+
+
+def test_cx_with_target_at_rest_stops_particle():
+    m2 = 6.69e-27
+    mcc = _mcc_one_target(M_ION, 0.0, m2, [O.inter(O.CX, M_ION, m2, rate=1e-15)])
+    r = O.Rng()
+    O.lib().o_rng_shr3_seed(C.byref(r), 5)
+    p = one_particle(0, 0, 1200.0, -300.0, 450.0)
+    assert O.lib().o_scatter(C.byref(mcc), O.dptr(p), C.byref(r)) == 0
+    assert p["vx"][0] == 0 and p["vy"][0] == 0 and p["vz"][0] == 0
+
+
+def test_isotropic_scattering_cos_uniform_ks():
+    m2 = 1e-20  # heavy target at rest: v1' = v_rel rotated
+    mcc = _mcc_one_target(M_ION, 0.0, m2, [O.inter(O.ELASTIC, M_ION, m2, rate=1e-15)])
+    r = O.Rng()
+    O.lib().o_rng_shr3_seed(C.byref(r), 11)
+    cosines = []
+    for _ in range(4000):
+        p = one_particle(0, 0, 1000.0, 0.0, 0.0)
+        O.lib().o_scatter(C.byref(mcc), O.dptr(p), C.byref(r))
+        v = np.array([p["vx"][0], p["vy"][0], p["vz"][0]])
+        cosines.append(v[0] / np.linalg.norm(v))
+    assert stats.kstest(cosines, stats.uniform(-1, 2).cdf).pvalue > 0.01
+
+
+def test_langevin_deflection_keeps_relative_speed():
+    m2 = 6.69e-27
+    mcc = _mcc_one_target(M_ION, 0.0, m2, [O.inter(O.LANGEVIN, M_ION, m2, rate=1e-15, cutoff=3.0)])
+    r = O.Rng()
+    O.lib().o_rng_shr3_seed(C.byref(r), 17)
+    for _ in range(100):
+        p = one_particle(0, 0, 800.0, 100.0, -50.0)
+        v0 = np.array([800.0, 100.0, -50.0])
+        O.lib().o_scatter(C.byref(mcc), O.dptr(p), C.byref(r))
+        v1 = np.array([p["vx"][0], p["vy"][0], p["vz"][0]])
+        vcm = v0 * M_ION / (M_ION + m2)
+        assert np.linalg.norm(v1 - vcm) == pytest.approx(np.linalg.norm(v0 - vcm), rel=1e-11)
+
+
+def test_collision_count_is_binomial():
+    g, mask = empty_box()
+    u = np.zeros(g.nx * g.nz)
+    f = O.make_fields(g, mask, u, x_max=1e-2, z_max=1e-2, geometry_empty=1, extern_field=0.0)
+    m2 = 6.69e-27
+    mcc = _mcc_one_target(M_ION, 300.0, m2, [O.inter(O.ELASTIC, M_ION, m2, rate=1e-15)], density=1e22)
+    dt = 1e-9
+    s = O.make_species(QE, M_ION, dt, lifetime=mcc.lifetime, mcc=mcc)
+    n = 20000
+    p = O.particles(n)
+    p["x"] = 5e-3
+    p["z"] = 5e-3
+    nc = C.c_long()
+    O.lib().o_advance_boris(C.byref(f), C.byref(s), O.dptr(p), n, C.byref(nc))
+    prob = 1 - np.exp(-dt / mcc.lifetime)
+    assert stats.binomtest(nc.value, n, prob).pvalue > 0.001
+
+
+def test_philox_and_shr3_thermalise_to_the_same_maxwellian():
+    """No fields: ions relax in the buffer gas; both RNG sources give the
+    same speed distribution (KS), and it approaches 3/2 kT of the gas."""
+    g, mask = empty_box()
+    u = np.zeros(g.nx * g.nz)
+    f = O.make_fields(g, mask, u, x_max=1e-2, z_max=1e-2, geometry_empty=1, extern_field=0.0)
+    m2 = 6.69e-27
+    mcc = _mcc_one_target(M_ION, 300.0, m2, [O.inter(O.ELASTIC, M_ION, m2, rate=1e-15)], density=1e24)
+    dt = 1e-9
+    res = []
+    for kind in (O.RNG_SHR3, O.RNG_PHILOX):
+        r = O.Rng()
+        O.lib().o_rng_shr3_seed(C.byref(r), 1234)
+        s = O.make_species(QE, M_ION, dt, lifetime=mcc.lifetime, mcc=mcc, rng_kind=kind, shr3=r)
+        n = 3000
+        p = O.particles(n)
+        p["x"] = 5e-3
+        p["z"] = 5e-3
+        p["vx"] = 3000.0
+        for step in range(150):
+            s.niter = step
+            O.lib().o_advance_boris(C.byref(f), C.byref(s), O.dptr(p), n, None)
+        res.append(np.sqrt(p["vx"] ** 2 + p["vy"] ** 2 + p["vz"] ** 2))
+        E = O.lib().o_mean_energy_eV(O.dptr(p), n, M_ION)
+        assert E == pytest.approx(1.5 * KB * 300.0 / QE, rel=0.08)
+    assert stats.ks_2samp(res[0], res[1]).pvalue > 0.01
