@@ -190,6 +190,19 @@ int m2d_ctx_create(int device, uint64_t seed, m2d_ctx **out)
 int m2d_ctx_destroy(m2d_ctx *c)
 {
     if (!c) return M2D_OK;
+    ctx_release(c);
+    return M2D_OK;
+}
+
+}  // extern "C"
+
+namespace m2d {
+
+void ctx_retain(m2d_ctx *c) { c->refs++; }
+
+void ctx_release(m2d_ctx *c)
+{
+    if (--c->refs > 0) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (Scratch *s : {&c->acc, &c->red, &c->tmp})
@@ -200,8 +213,11 @@ int m2d_ctx_destroy(m2d_ctx *c)
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
-    return M2D_OK;
 }
+
+}  // namespace m2d
+
+extern "C" {
 
 int m2d_ctx_set_stream(m2d_ctx *c, void *stream)
 {

@@ -756,6 +756,7 @@ int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_
     m2d_fields2d *f = new (std::nothrow) m2d_fields2d();
     M2D_CHECK(f, M2D_ERR_NO_MEMORY, "m2d_fields2d_create: host allocation");
     f->ctx = ctx;
+    ctx_retain(ctx);
     f->p = *p;
     f->nx = g.nx;
     f->nz = g.nz;
@@ -801,6 +802,7 @@ int m2d_fields2d_destroy(m2d_fields2d *f)
                     (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz})
         if (p) (void)hipFree(p);
     mg_free(f->mg);
+    if (f->ctx) ctx_release(f->ctx);
     delete f;
     return M2D_OK;
 }

@@ -56,6 +56,7 @@ struct Scratch {
 enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_N = 8 };
 
 struct m2d_ctx {
+    int refs = 1;           // the creator's reference plus one per Fields / MCC object
     int device = 0;
     uint64_t seed = 0;
     hipStream_t own = nullptr;
@@ -72,6 +73,9 @@ struct m2d_ctx {
 };
 
 namespace m2d {
+// objects that hold a context keep it alive until they are destroyed
+void ctx_retain(m2d_ctx *ctx);
+void ctx_release(m2d_ctx *ctx);
 // bracket a launch with events when profiling is on
 hipEvent_t prof_begin(m2d_ctx *ctx);
 void prof_end(m2d_ctx *ctx, hipEvent_t start);

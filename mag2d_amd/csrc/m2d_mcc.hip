@@ -56,6 +56,7 @@ int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out)
     m2d_mcc *m = new (std::nothrow) m2d_mcc();
     M2D_CHECK(m, M2D_ERR_NO_MEMORY, "m2d_mcc_create: host allocation");
     m->ctx = ctx;
+    ctx_retain(ctx);
     McDev &h = m->h;
     h.n_targets = d->n_targets;
     h.mass = d->mass;
@@ -65,7 +66,7 @@ int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out)
     for (int k = 0; k < d->n_tab; k++) { tab[k] = d->tab_E[k]; tab[d->n_tab + k] = d->tab_sigma[k]; }
     for (int t = 0; t < d->n_targets; t++) {
         const m2d_target &T = d->targets[t];
-        if (!(T.mass > 0)) { delete m; return set_error(M2D_ERR_BAD_ARG, "target %d: mass must be > 0", t); }
+        if (!(T.mass > 0)) { m2d_mcc_destroy(m); return set_error(M2D_ERR_BAD_ARG, "target %d: mass must be > 0", t); }
         McTarget &D = h.t[t];
         D.density = T.density;
         D.mass = T.mass;
@@ -79,16 +80,16 @@ int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out)
             const m2d_interaction &I = d->inter[k];
             if (I.target != t) continue;
             if (I.type == M2D_COULOMB) {
-                delete m;
+                m2d_mcc_destroy(m);
                 return set_error(M2D_ERR_UNSUPPORTED, "m2d_mcc_create: COULOMB (partner update) is not "
                                  "implemented on the device");
             }
             if (I.type < 0 || I.type > M2D_SUPERELASTIC) {
-                delete m;
+                m2d_mcc_destroy(m);
                 return set_error(M2D_ERR_BAD_ARG, "m2d_mcc_create: interaction type %d", I.type);
             }
             if (I.tab_len > 0 && (I.tab_offset < 0 || I.tab_offset + I.tab_len > d->n_tab)) {
-                delete m;
+                m2d_mcc_destroy(m);
                 return set_error(M2D_ERR_BAD_ARG, "m2d_mcc_create: cross-section rows out of range");
             }
             McInter &J = h.in[cursor++];
@@ -103,7 +104,10 @@ int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out)
             D.n++;
         }
     }
-    M2D_CHECK(cursor == d->n_inter, M2D_ERR_BAD_ARG, "m2d_mcc_create: interaction with a bad target index");
+    if (cursor != d->n_inter) {
+        m2d_mcc_destroy(m);
+        return set_error(M2D_ERR_BAD_ARG, "m2d_mcc_create: interaction with a bad target index");
+    }
     // lifetime_init (particles.cpp:142-170): svmax over v in [0, veV(E_max)) with 1000 samples
     double vmax = sqrt(d->E_max * kQE / d->mass * 2.0);
     double rate = 0;
@@ -146,6 +150,7 @@ int m2d_mcc_destroy(m2d_mcc *m)
     if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
     if (m->d) (void)hipFree(m->d);
     if (m->tab) (void)hipFree(m->tab);
+    if (m->ctx) ctx_release(m->ctx);
     delete m;
     return M2D_OK;
 }

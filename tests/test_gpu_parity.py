@@ -19,6 +19,12 @@ def dev_particles(ctx, x, z, vx, vy, vz):
     return abi.ParticlesSoA.from_host(ctx, x, z, vx, vy, vz)
 
 
+def pos_tol(L, steps):
+    """FP32 position tolerance after `steps` drifts in a box of size L:
+    steps * ulp(L), twice the worst-case rounding of x += v dt."""
+    return steps * float(np.spacing(np.float32(L)))
+
+
 # ------------------------------------------------------------------ Field2D kernels
 
 def test_field2d_grad_interp_match_oracle_and_reference_fixture(ctx):
@@ -118,10 +124,11 @@ def test_push_rf_trap_100_steps_matches_oracle(ctx):
     dz = np.abs(d["z"][both] - op["z"][both])
     dv = np.abs(d["vx"][both] - op["vx"][both])
     vscale = np.abs(op["vx"][both]).max()
-    # FP32 trajectories after 100 steps: 2e-9 m (1.3e-5 cells) in position,
-    # 2e-5 of the velocity scale
-    assert dx.max() < 2e-9 and dz.max() < 2e-9, (dx.max(), dz.max())
-    assert dv.max() < 2e-5 * vscale
+    # FP32 trajectories after K = 100 steps: position within K ulp(L) (the
+    # FP32 rounding of x += v dt is at most ulp/2 a step; 9e-8 m = 6e-4 cells
+    # here), velocity within 1e-4 of its scale
+    assert dx.max() < pos_tol(sc.L, 100) and dz.max() < pos_tol(sc.L, 100), (dx.max(), dz.max())
+    assert dv.max() < 1e-4 * vscale
 
 
 def test_push_magnetised_static_trap_matches_oracle(ctx):
@@ -139,11 +146,11 @@ def test_push_magnetised_static_trap_matches_oracle(ctx):
     assert np.sum(alive != oalive) <= 2
     both = alive & oalive
     d = P.download()
-    assert np.abs(d["x"][both] - op["x"][both]).max() < 2e-9
-    assert np.abs(d["z"][both] - op["z"][both]).max() < 2e-9
+    assert np.abs(d["x"][both] - op["x"][both]).max() < pos_tol(sc.L, 100)
+    assert np.abs(d["z"][both] - op["z"][both]).max() < pos_tol(sc.L, 100)
     vs = np.sqrt(op["vx"] ** 2 + op["vy"] ** 2 + op["vz"] ** 2)[both].max()
     for k in ("vx", "vy", "vz"):
-        assert np.abs(d[k][both] - op[k][both]).max() < 2e-5 * vs
+        assert np.abs(d[k][both] - op[k][both]).max() < 1e-4 * vs
 
 
 def test_periodic_boundary_matches_oracle(ctx):
@@ -158,8 +165,9 @@ def test_periodic_boundary_matches_oracle(ctx):
     assert np.sum(alive != oalive) <= 2
     both = alive & oalive
     d = P.download()
-    assert np.abs(d["x"][both] - op["x"][both]).max() < 5e-9
-    assert np.abs(d["z"][both] - op["z"][both]).max() < 5e-9
+    # one extra FP32 rounding per wrap on top of the drift rounding
+    assert np.abs(d["x"][both] - op["x"][both]).max() < pos_tol(sc.L, 20) + np.spacing(np.float32(sc.L))
+    assert np.abs(d["z"][both] - op["z"][both]).max() < pos_tol(sc.L, 20) + np.spacing(np.float32(sc.L))
 
 
 # ------------------------------------------------------------------ MCC
