@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--sort-every", type=int, default=20,
+                    help="cell-sort the particle store every K steps (inside the timed region)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -137,25 +139,34 @@ def main():
     rho.zero()
     flags = abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT
 
+    bufs = [P, abi.ParticlesSoA(ctx, N)]
+    cur = 0
+
+    def one_step(step):
+        nonlocal cur
+        if args.sort_every and step % args.sort_every == 0:
+            abi.sort(ctx, fields, bufs[cur], bufs[1 - cur], want_count=False)
+            cur = 1 - cur
+        sp.niter = step
+        abi.advance(ctx, fields, sp, bufs[cur], mcc=mcc, rho_ptr=rho.ptr, flags=flags)
+
     step = 0
     for _ in range(args.warmup):
-        sp.niter = step
-        abi.advance(ctx, fields, sp, P, mcc=mcc, rho_ptr=rho.ptr, flags=flags)
+        one_step(step)
         step += 1
     ctx.sync()
-    live0 = abi.count_live(ctx, P)
+    live0 = abi.count_live(ctx, bufs[cur])
     ctx.profile(True)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sp.niter = step
-        abi.advance(ctx, fields, sp, P, mcc=mcc, rho_ptr=rho.ptr, flags=flags)
+        one_step(step)
         step += 1
     ctx.sync()
     elapsed = time.perf_counter() - t0
     kern_ms, launches = ctx.profile_read()
     ctx.profile(False)
-    live1 = abi.count_live(ctx, P)
+    live1 = abi.count_live(ctx, bufs[cur])
     flags_dev = ctx.status(clear=True)
     live_mean = 0.5 * (live0 + live1)
     value = live_mean * args.steps / elapsed
@@ -180,6 +191,7 @@ def main():
                                "CIC deposit per step)",
                    "particles": N, "live_particles_mean": live_mean, "grid": [args.grid, args.grid],
                    "dt": 1e-9, "layout": "SoA FP32 x,z,vx,vy,vz (20 B); int64 fixed-point deposit",
+                   "sort_every": args.sort_every,
                    "device_flags": flags_dev},
         "roofline": {"kernel": "k_advance2d", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -290,6 +290,15 @@ int m2d_count_live_2d(m2d_ctx *ctx, const m2d_particles2d *p, int64_t *n_live);
  * the free list of BaseSpecies::remove/insert (particles.hpp:223-247). */
 int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles2d *dst,
                    int64_t *n_live);
+/* Rebuild the store in tile-major cell order (16 x 16-cell tiles, row-major
+ * cells inside a tile; equal cells keep their slot order) into dst, empty
+ * slots dropped to the end; *n_live (may be NULL) gets the live count.
+ * Replaces the insertion-order vector<t_particle> with holes
+ * (particles.hpp:223-249); the fused pass is fastest on a recently sorted
+ * store.  Slot order keys the MCC random streams, so sorting changes which
+ * draws a particle gets, never the statistics. */
+int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src,
+                const m2d_particles2d *dst, int64_t *n_live);
 /* Species<CARTESIAN>::add_particles_on_disk (particles.cpp:728-749) into the
  * n slots of p: uniform on the disk by rejection, Maxwellian velocities of
  * thermal speed vth = v_max/sqrt(2) (rndv, particles.hpp:188-193).  Samples

@@ -149,6 +149,8 @@ _SIGS = {
     "m2d_count_live_2d": (C.c_int, [_P, C.POINTER(Particles2D), C.POINTER(C.c_int64)]),
     "m2d_compact_2d": (C.c_int, [_P, C.POINTER(Particles2D), C.POINTER(Particles2D),
                                  C.POINTER(C.c_int64)]),
+    "m2d_sort_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.POINTER(Particles2D),
+                              C.POINTER(C.c_int64)]),
     "m2d_add_particles_on_disk_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.c_uint32]),
 }
@@ -475,6 +477,24 @@ def compact(ctx, src: ParticlesSoA, dst: ParticlesSoA):
     a, b = src.struct(), dst.struct()
     check(lib().m2d_compact_2d(ctx.h, C.byref(a), C.byref(b), C.byref(n)))
     return n.value
+
+
+def sort(ctx, fields, src: ParticlesSoA, dst: ParticlesSoA, want_count=True):
+    """Cell-sort src into dst; returns the live count (None if not requested)."""
+    n = C.c_int64()
+    a, b = src.struct(), dst.struct()
+    check(lib().m2d_sort_2d(ctx.h, fields.h, C.byref(a), C.byref(b), C.byref(n) if want_count else None))
+    return n.value if want_count else None
+
+
+def cell_keys(x, z, nx, nz, dx, dz, xmin=0.0, zmin=0.0):
+    """Host copy of the sort key (tile-major cells, 16 x 16 tiles) for tests."""
+    X = (np.asarray(x, np.float64) - xmin) * (1.0 / dx)
+    Z = (np.asarray(z, np.float64) - zmin) * (1.0 / dz)
+    i = np.where(X >= 0, np.minimum(X, nx - 2), 0).astype(np.int64)
+    j = np.where(Z >= 0, np.minimum(Z, nz - 2), 0).astype(np.int64)
+    ntz = (nz - 2) // 16 + 1
+    return (((i // 16) * ntz + (j // 16)) << 8) | ((i % 16) << 4) | (j % 16)
 
 
 def add_particles_on_disk(ctx, fields, parts, cx, cz, radius, vth, stream=0):

@@ -21,8 +21,11 @@ constexpr double kInvFix = 1.0 / 4294967296.0;
 constexpr int kBlock = 256;                  // threads per block of the particle kernels
 
 // Philox counter word 3 carries (purpose << 24)
-constexpr uint32_t kPurposeMcc = 0;
-constexpr uint32_t kPurposeInit = 1;
+constexpr uint32_t kPurposeMcc = 0;    // scatter draws of one slot: counter (slot, step, block)
+constexpr uint32_t kPurposeInit = 1;   // initial sampling of one slot
+constexpr uint32_t kPurposeTrial = 2;  // MCC trial: counter (slot/4, step, 0), word slot%4
+
+constexpr int kTile = 16;              // cells per side of the sort tile
 
 // ---------------------------------------------------------------- errors
 int set_error(int code, const char *fmt, ...);
@@ -66,6 +69,7 @@ struct m2d_ctx {
     m2d::Scratch acc;       // int64 fixed-point deposit grid
     m2d::Scratch red;       // reduction partials
     m2d::Scratch tmp;       // general temporaries (compaction offsets)
+    m2d::Scratch sortbuf;   // keys, slots and radix-sort temporaries
     // kernel timing: event pairs around particle-pass launches
     bool profile = false;
     std::vector<hipEvent_t> ev_pool;

@@ -280,103 +280,211 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 }
 
 // ------------------------------------------------------------------ the pass
-constexpr int kPPT = 4;  // particles per thread
+// Each thread owns kPPT consecutive slots, read and written as float4; a
+// block owns kChunk consecutive slots.  After a cell sort (m2d_sort_2d) the
+// chunk lies in one or two kTile x kTile tiles, so deposits go to an LDS
+// window of kWinI x kWinJ nodes anchored on the tile of the chunk's first
+// particle; the window is flushed to the global fixed-point grid once.
+constexpr int kPPT = 16;
+constexpr int kChunk = kBlock * kPPT;
+constexpr int kWinI = 32, kWinJ = 48;  // tile (ti, tj) and (ti, tj+1) with an 8-cell halo
+static_assert(kTile == 16, "window halo assumes 16-cell tiles");
+
+// running sum of one thread's deposits while consecutive particles share a cell
+struct DepRun {
+    int ci, cj;
+    long long w[4];
+};
+
+template <bool INWIN>
+__device__ __forceinline__ void run_flush(const DepRun &d, unsigned long long *win, int wi0, int wj0,
+                                          unsigned long long *acc, int nz)
+{
+    if (d.ci < 0) return;
+    const int a = d.ci - wi0, b = d.cj - wj0;
+    if (INWIN && a >= 0 && a + 1 < kWinI && b >= 0 && b + 1 < kWinJ) {
+        unsigned long long *s = win + a * kWinJ + b;
+        atomicAdd(s, (unsigned long long)d.w[0]);
+        atomicAdd(s + kWinJ, (unsigned long long)d.w[1]);
+        atomicAdd(s + 1, (unsigned long long)d.w[2]);
+        atomicAdd(s + kWinJ + 1, (unsigned long long)d.w[3]);
+    } else {
+        deposit_global(acc, nz, d.ci, d.cj, d.w);
+    }
+}
 
 template <bool DEPOSIT, bool MCC>
 __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
 {
-    const int64_t base = (int64_t)blockIdx.x * (kBlock * kPPT) + threadIdx.x;
-    bool removed_any = false, nan_any = false;
+    __shared__ unsigned long long win[DEPOSIT ? kWinI * kWinJ : 1];
+    __shared__ int anchor[2];
+    const int64_t base = (int64_t)blockIdx.x * kChunk;
+    int wi0 = 0, wj0 = 0;
+    if (DEPOSIT) {
+        for (int k = threadIdx.x; k < kWinI * kWinJ; k += kBlock) win[k] = 0ull;
+        if (threadIdx.x == 0) {
+            int ai = 0, aj = 0;
+            if (base < a.n && !is_dead(a.x[base])) {
+                double X = ((double)a.x[base] - a.xmin) * a.idx, Z = ((double)a.z[base] - a.zmin) * a.idz;
+                if (X >= 0.0 && Z >= 0.0 && X < a.nx && Z < a.nz) { ai = (int)X; aj = (int)Z; }
+            }
+            anchor[0] = (ai / kTile) * kTile - 8;
+            anchor[1] = (aj / kTile) * kTile - 8;
+        }
+        __syncthreads();
+        wi0 = anchor[0];
+        wj0 = anchor[1];
+    }
     int n_removed = 0, n_coll = 0;
+    bool nan_any = false;
+    DepRun run;
+    run.ci = -1;
+    const int64_t p0 = base + (int64_t)threadIdx.x * kPPT;
+    for (int g4 = 0; g4 < kPPT / 4; g4++) {
+        const int64_t pb = p0 + 4 * g4;
+        if (pb >= a.n) break;
+        const bool vec = pb + 4 <= a.n;
+        float X4[4], Z4[4], VX[4], VY[4], VZ[4];
+        if (vec) {
+            float4 t;
+            t = *(const float4 *)(a.x + pb);  X4[0] = t.x; X4[1] = t.y; X4[2] = t.z; X4[3] = t.w;
+            t = *(const float4 *)(a.z + pb);  Z4[0] = t.x; Z4[1] = t.y; Z4[2] = t.z; Z4[3] = t.w;
+            t = *(const float4 *)(a.vx + pb); VX[0] = t.x; VX[1] = t.y; VX[2] = t.z; VX[3] = t.w;
+            t = *(const float4 *)(a.vy + pb); VY[0] = t.x; VY[1] = t.y; VY[2] = t.z; VY[3] = t.w;
+            t = *(const float4 *)(a.vz + pb); VZ[0] = t.x; VZ[1] = t.y; VZ[2] = t.z; VZ[3] = t.w;
+        } else {
 #pragma unroll
-    for (int q = 0; q < kPPT; q++) {
-        const int64_t p = base + (int64_t)q * kBlock;
-        if (p >= a.n) break;
-        float x = a.x[p];
-        if (is_dead(x)) continue;
-        float z = a.z[p];
-        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
-
-        // ---- advance_boris (particles.cpp:945-988)
-        float ex, ez;
-        gather_E(a, (double)x, (double)z, ex, ez);
-        vx = fmaf(ex, a.qmdt2, vx);
-        vz = fmaf(ez, a.qmdt2, vz);
-        float vpr = vx - vy * a.tz + vz * a.ty;
-        float vpz = vz - vx * a.ty + vy * a.tx;
-        float vpt = vy - vz * a.tx + vx * a.tz;
-        vx = vx - vpt * a.sz + vpz * a.sy;
-        vy = vy - vpz * a.sx + vpr * a.sz;
-        vz = vz - vpr * a.sy + vpt * a.sx;
-        vx = fmaf(ex, a.qmdt2, vx);
-        vz = fmaf(ez, a.qmdt2, vz);
-        x = fmaf(vx, a.dt, x);
-        z = fmaf(vz, a.dt, z);
-
-        // ---- MCC trial (particles.cpp:990-993)
+            for (int q = 0; q < 4; q++) {
+                bool in = pb + q < a.n;
+                X4[q] = in ? a.x[pb + q] : dead_value();
+                Z4[q] = in ? a.z[pb + q] : 0.f;
+                VX[q] = in ? a.vx[pb + q] : 0.f;
+                VY[q] = in ? a.vy[pb + q] : 0.f;
+                VZ[q] = in ? a.vz[pb + q] : 0.f;
+            }
+        }
+        // MCC trial uniforms of the four slots: one Philox block
+        uint32_t tw[4] = {0u, 0u, 0u, 0u};
         if (MCC) {
-            Rng r;
-            r.begin(a.seed, a.stream, (uint32_t)p, a.step, kPurposeMcc);
-            if (r.uni() < a.prob) {
-                scatter(*a.mcc, r, vx, vy, vz, a.counters);
-                n_coll++;
-            }
+            tw[0] = (uint32_t)(pb >> 2); tw[1] = (uint32_t)a.step; tw[2] = 0u;
+            tw[3] = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
+            philox4x32_10(tw[0], tw[1], tw[2], tw[3], (uint32_t)a.seed,
+                          (uint32_t)(a.seed >> 32) ^ a.stream);
         }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float x = X4[q];
+            if (is_dead(x)) continue;
+            float z = Z4[q], vx = VX[q], vy = VY[q], vz = VZ[q];
 
-        // ---- advance_boundary (particles.hpp:376-406)
-        bool dead = false;
-        if (a.boundary) {
-            bool inside = x >= a.bx0 && x <= a.bx1 && z >= a.bz0 && z <= a.bz1;
-            if (!inside) {
-                if (x != x || z != z) { dead = true; nan_any = true; }
-                else if (!a.periodic) dead = true;
-                else {
-                    double xm = fmod((double)x, a.bx1), zm = fmod((double)z, a.bz1);
-                    if (xm < 0) xm += a.bx1;
-                    if (zm < 0) zm += a.bz1;
-                    x = (float)xm; z = (float)zm;
+            // ---- advance_boris (particles.cpp:945-988)
+            float ex, ez;
+            gather_E(a, (double)x, (double)z, ex, ez);
+            vx = fmaf(ex, a.qmdt2, vx);
+            vz = fmaf(ez, a.qmdt2, vz);
+            float vpr = vx - vy * a.tz + vz * a.ty;
+            float vpz = vz - vx * a.ty + vy * a.tx;
+            float vpt = vy - vz * a.tx + vx * a.tz;
+            vx = vx - vpt * a.sz + vpz * a.sy;
+            vy = vy - vpz * a.sx + vpr * a.sz;
+            vz = vz - vpr * a.sy + vpt * a.sx;
+            vx = fmaf(ex, a.qmdt2, vx);
+            vz = fmaf(ez, a.qmdt2, vz);
+            x = fmaf(vx, a.dt, x);
+            z = fmaf(vz, a.dt, z);
+
+            // ---- MCC trial (particles.cpp:990-993)
+            if (MCC) {
+                if (((double)tw[q] + 0.5) * 2.3283064365386963e-10 < a.prob) {
+                    Rng r;
+                    r.begin(a.seed, a.stream, (uint32_t)(pb + q), a.step, kPurposeMcc);
+                    scatter(*a.mcc, r, vx, vy, vz, a.counters);
+                    n_coll++;
                 }
             }
-            if (!dead && a.checkfree) {
-                double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
-                bool free_ = false;
-                if (X >= 0.0 && Z >= 0.0) {
-                    int ci = (int)X, cj = (int)Z;
-                    if (ci <= a.nx - 1 && cj <= a.nz - 1) free_ = a.cellfree[(size_t)ci * a.nz + cj] != 0;
+
+            // ---- advance_boundary (particles.hpp:376-406)
+            bool dead = false;
+            if (a.boundary) {
+                bool inside = x >= a.bx0 && x <= a.bx1 && z >= a.bz0 && z <= a.bz1;
+                if (!inside) {
+                    if (x != x || z != z) { dead = true; nan_any = true; }
+                    else if (!a.periodic) dead = true;
+                    else {
+                        double xm = fmod((double)x, a.bx1), zm = fmod((double)z, a.bz1);
+                        if (xm < 0) xm += a.bx1;
+                        if (zm < 0) zm += a.bz1;
+                        x = (float)xm; z = (float)zm;
+                    }
                 }
-                dead = !free_;
+                if (!dead && a.checkfree) {
+                    double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
+                    bool free_ = false;
+                    if (X >= 0.0 && Z >= 0.0) {
+                        int ci = (int)X, cj = (int)Z;
+                        if (ci <= a.nx - 1 && cj <= a.nz - 1) free_ = a.cellfree[(size_t)ci * a.nz + cj] != 0;
+                    }
+                    dead = !free_;
+                }
+            }
+            if (dead) {
+                X4[q] = dead_value();
+                n_removed++;
+                continue;
+            }
+            // ---- deposit (particles.hpp:408-409), summed per run of equal cells
+            if (DEPOSIT) {
+                int ci, cj;
+                long long w[4];
+                if (cic_weights(((double)x - a.xmin) * a.idx, ((double)z - a.zmin) * a.idz, a.nx, a.nz,
+                                ci, cj, w)) {
+                    if (ci == run.ci && cj == run.cj) {
+                        run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
+                    } else {
+                        run_flush<true>(run, win, wi0, wj0, a.acc, a.nz);
+                        run.ci = ci; run.cj = cj;
+                        run.w[0] = w[0]; run.w[1] = w[1]; run.w[2] = w[2]; run.w[3] = w[3];
+                    }
+                } else {
+                    raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+                }
+            }
+            X4[q] = x; Z4[q] = z; VX[q] = vx; VY[q] = vy; VZ[q] = vz;
+        }
+        if (vec) {
+            *(float4 *)(a.x + pb) = make_float4(X4[0], X4[1], X4[2], X4[3]);
+            *(float4 *)(a.z + pb) = make_float4(Z4[0], Z4[1], Z4[2], Z4[3]);
+            *(float4 *)(a.vx + pb) = make_float4(VX[0], VX[1], VX[2], VX[3]);
+            *(float4 *)(a.vy + pb) = make_float4(VY[0], VY[1], VY[2], VY[3]);
+            *(float4 *)(a.vz + pb) = make_float4(VZ[0], VZ[1], VZ[2], VZ[3]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (pb + q < a.n) {
+                    a.x[pb + q] = X4[q]; a.z[pb + q] = Z4[q];
+                    a.vx[pb + q] = VX[q]; a.vy[pb + q] = VY[q]; a.vz[pb + q] = VZ[q];
+                }
+        }
+    }
+    if (DEPOSIT) {
+        run_flush<true>(run, win, wi0, wj0, a.acc, a.nz);
+        __syncthreads();
+        for (int k = threadIdx.x; k < kWinI * kWinJ; k += kBlock) {
+            unsigned long long v = win[k];
+            if (v) {
+                int gi = wi0 + k / kWinJ, gj = wj0 + k % kWinJ;
+                atomicAdd(a.acc + (size_t)gi * a.nz + gj, v);
             }
         }
-        if (dead) {
-            a.x[p] = dead_value();
-            removed_any = true;
-            n_removed++;
-            continue;
-        }
-        // ---- deposit (particles.hpp:408-409)
-        if (DEPOSIT) {
-            int ci, cj;
-            long long w[4];
-            if (cic_weights(((double)x - a.xmin) * a.idx, ((double)z - a.zmin) * a.idz, a.nx, a.nz,
-                            ci, cj, w))
-                deposit_global(a.acc, a.nz, ci, cj, w);
-            else
-                raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
-        }
-        a.x[p] = x; a.z[p] = z;
-        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
     }
     // ---- counters: one atomic per wavefront
-    unsigned long long m = __ballot(removed_any);
-    if (m) {
-        int sum = n_removed;
-        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-        if (__lane_id() == 0) atomicAdd(&a.counters[CNT_REMOVED], (unsigned long long)sum);
-    }
+    int sum = n_removed;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (__lane_id() == 0 && sum) atomicAdd(&a.counters[CNT_REMOVED], (unsigned long long)sum);
     if (MCC) {
-        int sum = n_coll;
-        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-        if (__lane_id() == 0 && sum) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)sum);
+        int c = n_coll;
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (__lane_id() == 0 && c) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)c);
     }
     if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
 }
@@ -691,6 +799,9 @@ static int check_parts(const m2d_particles2d *p)
               (long long)p->n);
     M2D_CHECK(p->n == 0 || (p->x && p->z && p->vx && p->vy && p->vz), M2D_ERR_BAD_ARG,
               "particle arrays must be non-NULL");
+    uintptr_t al = (uintptr_t)p->x | (uintptr_t)p->z | (uintptr_t)p->vx | (uintptr_t)p->vy |
+                   (uintptr_t)p->vz;
+    M2D_CHECK((al & 15) == 0, M2D_ERR_BAD_ARG, "particle arrays must be 16-byte aligned");
     return M2D_OK;
 }
 

@@ -754,6 +754,17 @@ void o_fields_E(const o_fields *f, double x, double z, double t, double *ex, dou
     *ex = -gx; *ez = -gz;
 }
 
+/* Trial uniform of the Philox source: slots 4m..4m+3 share one block,
+ * counter (m, step lo, 0, 2 << 24 | step hi), word slot & 3. */
+double o_trial_uniform(uint64_t seed, uint32_t stream, uint32_t slot, uint64_t step)
+{
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) ^ stream };
+    uint32_t ctr[4] = { slot >> 2, (uint32_t)step, 0, (2u << 24) | ((uint32_t)(step >> 32) & 0xFFFFFFu) };
+    uint32_t out[4];
+    o_philox4x32_10(ctr, key, out);
+    return ((double)out[slot & 3] + 0.5) * 2.3283064365386963e-10;
+}
+
 static o_rng *species_rng(o_species *s, o_rng *local, long slot)
 {
     if (s->rng_kind == O_RNG_SHR3) return s->shr3;
@@ -792,7 +803,10 @@ void o_advance_boris(const o_fields *f, o_species *s, o_particle *P, long n, lon
         if (s->mcc) {
             o_rng local;
             o_rng *r = species_rng(s, &local, k);
-            if (o_rng_uni(r) < prob) { o_scatter(s->mcc, I, r); nc++; }
+            double u = s->rng_kind == O_RNG_SHR3 ? o_rng_uni(r)
+                                                 : o_trial_uniform(s->seed, s->stream,
+                                                                   (uint32_t)(k + s->slot0), s->niter);
+            if (u < prob) { o_scatter(s->mcc, I, r); nc++; }
         }
     }
     if (n_coll) *n_coll = nc;

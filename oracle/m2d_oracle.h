@@ -87,6 +87,10 @@ void     o_rng_shr3_seed(o_rng *r, uint32_t seed);             /* random.cpp:192
 void     o_rng_philox_begin(o_rng *r, uint64_t seed, uint32_t stream,
                             uint32_t slot, uint64_t step, uint32_t purpose);
 uint32_t o_rng_u32(o_rng *r);
+/* MCC trial uniform of slot `slot` at `step` for the Philox source: four
+ * slots share one block (counter (slot/4, step, 0, 2<<24 | step hi), word
+ * slot%4); a scatter then draws from the slot's own purpose-0 stream. */
+double   o_trial_uniform(uint64_t seed, uint32_t stream, uint32_t slot, uint64_t step);
 double   o_rng_uni(o_rng *r);     /* SHR3: random.cpp:42 ; Philox: (w+0.5)*2^-32 */
 double   o_rng_normal(o_rng *r);  /* Box-Muller, both sources (statistical parity only) */
 void     o_rot_len(o_rng *r, double len, double *x, double *y, double *z);   /* random.cpp:103-115 */

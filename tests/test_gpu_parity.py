@@ -274,6 +274,51 @@ def test_fused_deposit_matches_oracle_and_is_reproducible(ctx):
     assert res[0].sum() == pytest.approx(alive.sum() * QE, rel=1e-12)
 
 
+def test_sort_is_stable_cell_order_and_drops_empty_slots(ctx):
+    sc = Scene(ctx, n=128, solve_oracle=False)
+    rng = np.random.default_rng(15)
+    n = 300_001
+    x, z, vx, vy, vz = ions_on_disk(n, rng, r=8e-3)
+    dead = rng.uniform(size=n) < 0.2
+    x[dead] = np.array([abi.DEAD_BITS], np.uint32).view(np.float32)[0]
+    src = dev_particles(ctx, x, z, vx, vy, vz)
+    dst = abi.ParticlesSoA(ctx, n)
+    live = abi.sort(ctx, sc.fields, src, dst)
+    assert live == (~dead).sum()
+    keys = abi.cell_keys(x[~dead], z[~dead], sc.n, sc.n, sc.g.dx, sc.g.dz)
+    order = np.argsort(keys, kind="stable")
+    d = dst.download()
+    for k, a in zip(abi.ParticlesSoA.NAMES, (x, z, vx, vy, vz)):
+        assert np.array_equal(d[k][:live], a[~dead][order]), k
+    assert (d["x"][live:].view(np.uint32) == abi.DEAD_BITS).all()
+
+
+def test_deposit_does_not_depend_on_particle_order(ctx):
+    """The LDS-window path (sorted store) and the global-atomic fallback
+    (shuffled store) give the same grid bit for bit; so does the standalone
+    Species::accumulate of the pushed positions."""
+    sc = Scene(ctx, n=128)
+    rng = np.random.default_rng(16)
+    n = 200_000
+    x, z, vx, vy, vz = ions_on_disk(n, rng)
+    flags = abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT
+    a = dev_particles(ctx, x, z, vx, vy, vz)
+    s = abi.ParticlesSoA(ctx, n)
+    live = abi.sort(ctx, sc.fields, a, s)
+    s.n = live
+    perm = rng.permutation(n)
+    b = dev_particles(ctx, x[perm], z[perm], vx[perm], vy[perm], vz[perm])
+    grids = []
+    for P in (s, b):
+        rho = ctx.to_device(np.zeros(128 * 128))
+        abi.advance(ctx, sc.fields, species(QE, M_ION, 1e-9, niter=3), P, rho_ptr=rho.ptr, flags=flags)
+        grids.append(rho.get())
+    assert np.array_equal(grids[0], grids[1])
+    rho = ctx.to_device(np.zeros(128 * 128))
+    abi.accumulate(ctx, sc.g, b, QE, rho.ptr)
+    assert np.array_equal(rho.get(), grids[1])
+
+
 # ------------------------------------------------------------------ Poisson
 
 def test_poisson_rf_basis_matches_fp64_oracle(ctx):
