@@ -56,7 +56,7 @@ struct Scratch {
 }  // namespace m2d
 
 // counters written by kernels
-enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_N = 8 };
+enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_QLEN = 4 /* and 5 */, CNT_N = 8 };
 
 struct m2d_ctx {
     int refs = 1;           // the creator's reference plus one per Fields / MCC object
@@ -70,6 +70,8 @@ struct m2d_ctx {
     m2d::Scratch red;       // reduction partials
     m2d::Scratch tmp;       // general temporaries (compaction offsets)
     m2d::Scratch sortbuf;   // keys, slots and radix-sort temporaries
+    m2d::Scratch cq;        // MCC collision queue (slot indices)
+    uint64_t launch_seq = 0;  // selects the collision-queue counter of a launch
     // kernel timing: event pairs around particle-pass launches
     bool profile = false;
     std::vector<hipEvent_t> ev_pool;

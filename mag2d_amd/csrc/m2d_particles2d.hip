@@ -32,6 +32,8 @@ struct PushArgs {
     uint32_t stream;
     uint64_t step;
     const McDev *mcc;
+    uint32_t *cq;                  // collision queue: slots whose MCC trial fired
+    unsigned long long *cq_len;    // its length (one of two counters, alternating per launch)
     unsigned long long *acc;       // nx*nz fixed-point deposit grid
     unsigned long long *counters;
 };
@@ -68,10 +70,10 @@ __device__ __forceinline__ void gather_E(const PushArgs &a, double xd, double zd
     const float fxz = (float)(X - i0), fz = (float)(Z - jh + 0.5);
     const float ph = a.phase;
 
-    const float2 *r0 = a.gx + (size_t)ih * nz + jx;
+    const float2 *r0 = a.gx + (ih * nz + jx);
     const float2 *r1 = r0 + nz;
     float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
-    const float2 *q0 = a.gz + (size_t)i0 * (nz + 1) + jh;
+    const float2 *q0 = a.gz + (i0 * (nz + 1) + jh);
     const float2 *q1 = q0 + (nz + 1);
     float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
 
@@ -249,6 +251,14 @@ __device__ void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, float &v
 // CIC weights (Field2D.hpp:45-62) as 2^-32 fixed point.  The four rounded
 // weights are corrected on the largest so each particle deposits exactly
 // 2^32: charge is conserved bit for bit and sums are order independent.
+// round(w 2^32) for 0 <= w <= 1, to nearest even like __double2ll_rn: adding
+// 2^52 moves the integer part into the low mantissa bits in one rounding
+__device__ __forceinline__ long long fix32(double w)
+{
+    double d = fma(w, 4294967296.0, 4503599627370496.0);
+    return __double_as_longlong(d) - 0x4330000000000000ll;
+}
+
 __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
                                             long long w[4])
 {
@@ -257,10 +267,10 @@ __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, 
     j = (int)Z;
     if (i > nx - 2 || j > nz - 2) return false;
     double u = X - i, v = Z - j;
-    w[0] = __double2ll_rn((1 - u) * (1 - v) * kFix);
-    w[1] = __double2ll_rn(u * (1 - v) * kFix);
-    w[2] = __double2ll_rn((1 - u) * v * kFix);
-    w[3] = __double2ll_rn(u * v * kFix);
+    w[0] = fix32((1 - u) * (1 - v));
+    w[1] = fix32(u * (1 - v));
+    w[2] = fix32((1 - u) * v);
+    w[3] = fix32(u * v);
     long long s = w[0] + w[1] + w[2] + w[3];
     int km = 0;
 #pragma unroll
@@ -280,14 +290,21 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 }
 
 // ------------------------------------------------------------------ the pass
-// Each thread owns kPPT consecutive slots, read and written as float4; a
-// block owns kChunk consecutive slots.  After a cell sort (m2d_sort_2d) the
-// chunk lies in one or two kTile x kTile tiles, so deposits go to an LDS
-// window of kWinI x kWinJ nodes anchored on the tile of the chunk's first
-// particle; the window is flushed to the global fixed-point grid once.
-constexpr int kPPT = 16;
-constexpr int kChunk = kBlock * kPPT;
-constexpr int kWinI = 32, kWinJ = 48;  // tile (ti, tj) and (ti, tj+1) with an 8-cell halo
+// A block owns kRounds x 1024 consecutive slots.  Each round is staged
+// through LDS: global loads and stores are fully coalesced (thread t moves
+// slots t, t+256, ...), while the compute phase gives each thread 4
+// consecutive slots, so consecutive particles of a sorted store share a cell
+// and their deposits are summed in registers before touching memory.
+// Deposits go to an LDS window of kWinI x kWinJ nodes anchored on the
+// 16-cell tile of the block's first particle (after a cell sort the block
+// lies in one or two tiles); the window is flushed to the global
+// fixed-point grid once per block, and deposits outside it go straight to
+// global atomics.
+constexpr int kRound = kBlock * 4;
+constexpr int kRounds = 2;
+constexpr int kChunk = kRound * kRounds;
+constexpr int kPPT = kChunk / kBlock;       // slots per thread (grid sizing)
+constexpr int kWinI = 32, kWinJ = 48;       // tile (ti, tj) and (ti, tj+1) with an 8-cell halo
 static_assert(kTile == 16, "window halo assumes 16-cell tiles");
 
 // running sum of one thread's deposits while consecutive particles share a cell
@@ -296,13 +313,12 @@ struct DepRun {
     long long w[4];
 };
 
-template <bool INWIN>
 __device__ __forceinline__ void run_flush(const DepRun &d, unsigned long long *win, int wi0, int wj0,
                                           unsigned long long *acc, int nz)
 {
     if (d.ci < 0) return;
     const int a = d.ci - wi0, b = d.cj - wj0;
-    if (INWIN && a >= 0 && a + 1 < kWinI && b >= 0 && b + 1 < kWinJ) {
+    if (a >= 0 && a + 1 < kWinI && b >= 0 && b + 1 < kWinJ) {
         unsigned long long *s = win + a * kWinJ + b;
         atomicAdd(s, (unsigned long long)d.w[0]);
         atomicAdd(s + kWinJ, (unsigned long long)d.w[1]);
@@ -316,13 +332,15 @@ __device__ __forceinline__ void run_flush(const DepRun &d, unsigned long long *w
 template <bool DEPOSIT, bool MCC>
 __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
 {
+    __shared__ float st[5][kRound];
     __shared__ unsigned long long win[DEPOSIT ? kWinI * kWinJ : 1];
     __shared__ int anchor[2];
+    const int tid = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * kChunk;
     int wi0 = 0, wj0 = 0;
     if (DEPOSIT) {
-        for (int k = threadIdx.x; k < kWinI * kWinJ; k += kBlock) win[k] = 0ull;
-        if (threadIdx.x == 0) {
+        for (int k = tid; k < kWinI * kWinJ; k += kBlock) win[k] = 0ull;
+        if (tid == 0) {
             int ai = 0, aj = 0;
             if (base < a.n && !is_dead(a.x[base])) {
                 double X = ((double)a.x[base] - a.xmin) * a.idx, Z = ((double)a.z[base] - a.zmin) * a.idz;
@@ -335,37 +353,40 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
         wi0 = anchor[0];
         wj0 = anchor[1];
     }
+    float *const arr[5] = {a.x, a.z, a.vx, a.vy, a.vz};
     int n_removed = 0, n_coll = 0;
     bool nan_any = false;
     DepRun run;
     run.ci = -1;
-    const int64_t p0 = base + (int64_t)threadIdx.x * kPPT;
-    for (int g4 = 0; g4 < kPPT / 4; g4++) {
-        const int64_t pb = p0 + 4 * g4;
-        if (pb >= a.n) break;
-        const bool vec = pb + 4 <= a.n;
-        float X4[4], Z4[4], VX[4], VY[4], VZ[4];
-        if (vec) {
-            float4 t;
-            t = *(const float4 *)(a.x + pb);  X4[0] = t.x; X4[1] = t.y; X4[2] = t.z; X4[3] = t.w;
-            t = *(const float4 *)(a.z + pb);  Z4[0] = t.x; Z4[1] = t.y; Z4[2] = t.z; Z4[3] = t.w;
-            t = *(const float4 *)(a.vx + pb); VX[0] = t.x; VX[1] = t.y; VX[2] = t.z; VX[3] = t.w;
-            t = *(const float4 *)(a.vy + pb); VY[0] = t.x; VY[1] = t.y; VY[2] = t.z; VY[3] = t.w;
-            t = *(const float4 *)(a.vz + pb); VZ[0] = t.x; VZ[1] = t.y; VZ[2] = t.z; VZ[3] = t.w;
-        } else {
+    for (int r = 0; r < kRounds; r++) {
+        const int64_t co = base + (int64_t)r * kRound;
+        if (co >= a.n) break;  // uniform over the block
+        // ---- coalesced load into LDS
+        float v[5][4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                bool in = pb + q < a.n;
-                X4[q] = in ? a.x[pb + q] : dead_value();
-                Z4[q] = in ? a.z[pb + q] : 0.f;
-                VX[q] = in ? a.vx[pb + q] : 0.f;
-                VY[q] = in ? a.vy[pb + q] : 0.f;
-                VZ[q] = in ? a.vz[pb + q] : 0.f;
-            }
+        for (int k = 0; k < 4; k++) {
+            const int64_t idx = co + k * kBlock + tid;
+            const bool in = idx < a.n;
+#pragma unroll
+            for (int c = 0; c < 5; c++) v[c][k] = in ? arr[c][idx] : (c == 0 ? dead_value() : 0.f);
         }
-        // MCC trial uniforms of the four slots: one Philox block
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int c = 0; c < 5; c++) st[c][k * kBlock + tid] = v[c][k];
+        __syncthreads();
+        // ---- compute on 4 consecutive slots
+        float4 tx = *(float4 *)&st[0][4 * tid], tz = *(float4 *)&st[1][4 * tid];
+        float4 tvx = *(float4 *)&st[2][4 * tid], tvy = *(float4 *)&st[3][4 * tid];
+        float4 tvz = *(float4 *)&st[4][4 * tid];
+        float X4[4] = {tx.x, tx.y, tx.z, tx.w}, Z4[4] = {tz.x, tz.y, tz.z, tz.w};
+        float VX[4] = {tvx.x, tvx.y, tvx.z, tvx.w}, VY[4] = {tvy.x, tvy.y, tvy.z, tvy.w};
+        float VZ[4] = {tvz.x, tvz.y, tvz.z, tvz.w};
+        const int64_t pb = co + 4 * tid;
         uint32_t tw[4] = {0u, 0u, 0u, 0u};
         if (MCC) {
+            // trial uniforms of slots pb..pb+3: one Philox block
             tw[0] = (uint32_t)(pb >> 2); tw[1] = (uint32_t)a.step; tw[2] = 0u;
             tw[3] = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
             philox4x32_10(tw[0], tw[1], tw[2], tw[3], (uint32_t)a.seed,
@@ -394,11 +415,13 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
             z = fmaf(vz, a.dt, z);
 
             // ---- MCC trial (particles.cpp:990-993)
+            // A hit only changes the velocity after the drift, so it cannot
+            // change the boundary test or the deposit below: the slot is
+            // queued and k_scatter2d applies BaseSpecies::scatter afterwards.
             if (MCC) {
                 if (((double)tw[q] + 0.5) * 2.3283064365386963e-10 < a.prob) {
-                    Rng r;
-                    r.begin(a.seed, a.stream, (uint32_t)(pb + q), a.step, kPurposeMcc);
-                    scatter(*a.mcc, r, vx, vy, vz, a.counters);
+                    unsigned long long at = atomicAdd(a.cq_len, 1ull);
+                    a.cq[at] = (uint32_t)(pb + q);
                     n_coll++;
                 }
             }
@@ -417,15 +440,15 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
                         x = (float)xm; z = (float)zm;
                     }
                 }
-                if (!dead && a.checkfree) {
-                    double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
-                    bool free_ = false;
-                    if (X >= 0.0 && Z >= 0.0) {
-                        int ci = (int)X, cj = (int)Z;
-                        if (ci <= a.nx - 1 && cj <= a.nz - 1) free_ = a.cellfree[(size_t)ci * a.nz + cj] != 0;
-                    }
-                    dead = !free_;
+            }
+            const double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
+            if (!dead && a.boundary && a.checkfree) {
+                bool free_ = false;
+                if (X >= 0.0 && Z >= 0.0) {
+                    int ci = (int)X, cj = (int)Z;
+                    if (ci <= a.nx - 1 && cj <= a.nz - 1) free_ = a.cellfree[ci * a.nz + cj] != 0;
                 }
+                dead = !free_;
             }
             if (dead) {
                 X4[q] = dead_value();
@@ -436,12 +459,11 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
             if (DEPOSIT) {
                 int ci, cj;
                 long long w[4];
-                if (cic_weights(((double)x - a.xmin) * a.idx, ((double)z - a.zmin) * a.idz, a.nx, a.nz,
-                                ci, cj, w)) {
+                if (cic_weights(X, Z, a.nx, a.nz, ci, cj, w)) {
                     if (ci == run.ci && cj == run.cj) {
                         run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
                     } else {
-                        run_flush<true>(run, win, wi0, wj0, a.acc, a.nz);
+                        run_flush(run, win, wi0, wj0, a.acc, a.nz);
                         run.ci = ci; run.cj = cj;
                         run.w[0] = w[0]; run.w[1] = w[1]; run.w[2] = w[2]; run.w[3] = w[3];
                     }
@@ -451,29 +473,30 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
             }
             X4[q] = x; Z4[q] = z; VX[q] = vx; VY[q] = vy; VZ[q] = vz;
         }
-        if (vec) {
-            *(float4 *)(a.x + pb) = make_float4(X4[0], X4[1], X4[2], X4[3]);
-            *(float4 *)(a.z + pb) = make_float4(Z4[0], Z4[1], Z4[2], Z4[3]);
-            *(float4 *)(a.vx + pb) = make_float4(VX[0], VX[1], VX[2], VX[3]);
-            *(float4 *)(a.vy + pb) = make_float4(VY[0], VY[1], VY[2], VY[3]);
-            *(float4 *)(a.vz + pb) = make_float4(VZ[0], VZ[1], VZ[2], VZ[3]);
-        } else {
+        *(float4 *)&st[0][4 * tid] = make_float4(X4[0], X4[1], X4[2], X4[3]);
+        *(float4 *)&st[1][4 * tid] = make_float4(Z4[0], Z4[1], Z4[2], Z4[3]);
+        *(float4 *)&st[2][4 * tid] = make_float4(VX[0], VX[1], VX[2], VX[3]);
+        *(float4 *)&st[3][4 * tid] = make_float4(VY[0], VY[1], VY[2], VY[3]);
+        *(float4 *)&st[4][4 * tid] = make_float4(VZ[0], VZ[1], VZ[2], VZ[3]);
+        __syncthreads();
+        // ---- coalesced store
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (pb + q < a.n) {
-                    a.x[pb + q] = X4[q]; a.z[pb + q] = Z4[q];
-                    a.vx[pb + q] = VX[q]; a.vy[pb + q] = VY[q]; a.vz[pb + q] = VZ[q];
-                }
+        for (int k = 0; k < 4; k++) {
+            const int64_t idx = co + k * kBlock + tid;
+            if (idx < a.n) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) arr[c][idx] = st[c][k * kBlock + tid];
+            }
         }
     }
     if (DEPOSIT) {
-        run_flush<true>(run, win, wi0, wj0, a.acc, a.nz);
+        run_flush(run, win, wi0, wj0, a.acc, a.nz);
         __syncthreads();
-        for (int k = threadIdx.x; k < kWinI * kWinJ; k += kBlock) {
-            unsigned long long v = win[k];
-            if (v) {
+        for (int k = tid; k < kWinI * kWinJ; k += kBlock) {
+            unsigned long long w = win[k];
+            if (w) {
                 int gi = wi0 + k / kWinJ, gj = wj0 + k % kWinJ;
-                atomicAdd(a.acc + (size_t)gi * a.nz + gj, v);
+                atomicAdd(a.acc + (size_t)gi * a.nz + gj, w);
             }
         }
     }
@@ -487,6 +510,26 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
         if (__lane_id() == 0 && c) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)c);
     }
     if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+}
+
+// BaseSpecies::scatter for the slots queued by k_advance2d; each slot draws
+// from its own Philox stream, so the queue order does not matter.  Clears the
+// other queue counter for the next launch.
+__global__ void __launch_bounds__(kBlock) k_scatter2d(PushArgs a, unsigned long long *next_len)
+{
+    const unsigned long long len = *a.cq_len;
+    const unsigned long long start = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    if (start == 0) *next_len = 0ull;
+    for (unsigned long long k = start; k < len; k += stride) {
+        const uint32_t p = a.cq[k];
+        if (is_dead(a.x[p])) continue;  // removed by the boundary after its trial
+        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+        Rng rg;
+        rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
+        scatter(*a.mcc, rg, vx, vy, vz, a.counters);
+        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+    }
 }
 
 // Species<CARTESIAN>::advance_boris_init (particles.cpp:997-1050)
@@ -855,6 +898,8 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.checkfree = !P.electric_field_from_file;
     a.prob = 0; a.seed = f->ctx->seed; a.stream = s->stream; a.step = s->niter;
     a.mcc = nullptr;
+    a.cq = nullptr;
+    a.cq_len = nullptr;
     a.acc = nullptr;
     a.counters = f->ctx->counters;
 }
@@ -883,9 +928,17 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     fill_push_args(a, f, s, p, 1.0, 1.0);
     a.boundary = (flags & M2D_STEP_BOUNDARY) != 0;
     bool use_mcc = mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
+    unsigned long long *next_len = nullptr;
     if (use_mcc) {
         a.prob = 1.0 - exp(-s->dt / s->lifetime);  // particles.cpp:937
         a.mcc = mcc->d;
+        M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
+        a.cq = (uint32_t *)ctx->cq.ptr;
+        // two queue counters alternate between launches: a launch's scatter
+        // kernel clears the one the next launch appends to
+        const int slot = (int)(ctx->launch_seq++ & 1);
+        a.cq_len = ctx->counters + CNT_QLEN + slot;
+        next_len = ctx->counters + CNT_QLEN + (slot ^ 1);
     }
     if (deposit) {
         M2D_TRY(ensure_acc(ctx, f->n));
@@ -900,6 +953,10 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         else k_advance2d<false, false><<<g, kBlock, 0, ctx->stream>>>(a);
         M2D_HIP(hipGetLastError());
         prof_end(ctx, t0);
+        if (use_mcc) {
+            k_scatter2d<<<64, kBlock, 0, ctx->stream>>>(a, next_len);
+            M2D_HIP(hipGetLastError());
+        }
     }
     if (deposit) M2D_TRY(flush(ctx, f->n, rho, s->charge));
     if (stats) {
