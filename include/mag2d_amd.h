@@ -202,7 +202,7 @@ int m2d_fields2d_E(m2d_fields2d *f, const double *x, const double *z, int64_t n,
 
 /* Field2D::accumulate for all live particles (Species::accumulate,
  * particles.hpp:358-367): rho[nx*nz] += charge * CIC weights.  Deterministic:
- * weights are summed as 64-bit fixed point (2^-32) and converted once. */
+ * weights are summed as 64-bit fixed point (2^-31) and converted once. */
 int m2d_accumulate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_particles2d *p,
                       double charge, double *rho);
 /* Field2D::interpolate / grad (Field2D.hpp:64-168) at n FP64 points, device arrays */
@@ -299,6 +299,12 @@ int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles
  * draws a particle gets, never the statistics. */
 int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src,
                 const m2d_particles2d *dst, int64_t *n_live);
+/* The sort also records the tile schedule of dst (same x pointer and slot
+ * count) that the deposit of m2d_advance_2d uses.  Call this after writing
+ * the store by any other means (uploads, Species::insert): the schedule is
+ * dropped and the pass falls back to its unsorted deposit.  m2d_compact_2d
+ * and the samplers drop it for their destination themselves. */
+int m2d_sort_forget_2d(m2d_ctx *ctx, const m2d_particles2d *p);
 /* Species<CARTESIAN>::add_particles_on_disk (particles.cpp:728-749) into the
  * n slots of p: uniform on the disk by rejection, Maxwellian velocities of
  * thermal speed vth = v_max/sqrt(2) (rndv, particles.hpp:188-193).  Samples

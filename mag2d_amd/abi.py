@@ -151,6 +151,7 @@ _SIGS = {
                                  C.POINTER(C.c_int64)]),
     "m2d_sort_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.POINTER(Particles2D),
                               C.POINTER(C.c_int64)]),
+    "m2d_sort_forget_2d": (C.c_int, [_P, C.POINTER(Particles2D)]),
     "m2d_add_particles_on_disk_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.c_uint32]),
 }
@@ -299,6 +300,7 @@ class ParticlesSoA:
             assert a.shape == (self.n,)
             if self.n:
                 self.arr[k].set(a)
+        sort_forget(self.ctx, self)
 
     def download(self):
         return {k: self.arr[k].get()[: self.n] for k in self.NAMES}
@@ -488,13 +490,18 @@ def sort(ctx, fields, src: ParticlesSoA, dst: ParticlesSoA, want_count=True):
 
 
 def cell_keys(x, z, nx, nz, dx, dz, xmin=0.0, zmin=0.0):
-    """Host copy of the sort key (tile-major cells, 16 x 16 tiles) for tests."""
+    """Host copy of the sort key (tile-major cells, 8 x 8 tiles) for tests."""
     X = (np.asarray(x, np.float64) - xmin) * (1.0 / dx)
     Z = (np.asarray(z, np.float64) - zmin) * (1.0 / dz)
     i = np.where(X >= 0, np.minimum(X, nx - 2), 0).astype(np.int64)
     j = np.where(Z >= 0, np.minimum(Z, nz - 2), 0).astype(np.int64)
-    ntz = (nz - 2) // 16 + 1
-    return (((i // 16) * ntz + (j // 16)) << 8) | ((i % 16) << 4) | (j % 16)
+    ntz = (nz - 2) // 8 + 1
+    return (((i // 8) * ntz + (j // 8)) << 6) | ((i % 8) << 3) | (j % 8)
+
+
+def sort_forget(ctx, parts):
+    s = parts.struct()
+    check(lib().m2d_sort_forget_2d(ctx.h, C.byref(s)))
 
 
 def add_particles_on_disk(ctx, fields, parts, cx, cz, radius, vth, stream=0):

@@ -16,8 +16,7 @@ constexpr double kEps0 = 8.854187817e-12;
 constexpr double kKB = 1.380662e-23;
 constexpr double kQE = 1.602189e-19;
 
-constexpr double kFix = 4294967296.0;        // 2^32: fixed-point scale of deposited weights
-constexpr double kInvFix = 1.0 / 4294967296.0;
+constexpr double kInvFix = 1.0 / 2147483648.0;  // 2^-31: unit of the fixed-point deposit grid
 constexpr int kBlock = 256;                  // threads per block of the particle kernels
 
 // Philox counter word 3 carries (purpose << 24)
@@ -25,7 +24,11 @@ constexpr uint32_t kPurposeMcc = 0;    // scatter draws of one slot: counter (sl
 constexpr uint32_t kPurposeInit = 1;   // initial sampling of one slot
 constexpr uint32_t kPurposeTrial = 2;  // MCC trial: counter (slot/4, step, 0), word slot%4
 
-constexpr int kTile = 16;              // cells per side of the sort tile
+constexpr int kTile = 8;               // cells per side of the sort tile
+constexpr int kTileShift = 6;          // key = tile << 6 | cell in tile
+constexpr int kHalo = 4;               // deposit window halo (cells) around a tile
+constexpr int kWin = kTile + 1 + 2 * kHalo;  // window side in nodes
+constexpr int kChunkCap = 2040;        // particles per block of the tiled pass (2 staged rounds)
 
 // ---------------------------------------------------------------- errors
 int set_error(int code, const char *fmt, ...);
@@ -55,6 +58,22 @@ struct Scratch {
 
 }  // namespace m2d
 
+// Tile schedule of a cell-sorted store (built by m2d_sort_2d): the live
+// particles of sort tile t are slots [tile_start[t], tile_end[t]); chunk c
+// of the tiled pass covers sched[c] = (tile, start, end), at most kChunkCap
+// particles of one tile.  Valid for the store at `x` with `n` slots until
+// the store is written by anything but the particle pass.
+struct m2d_tiling {
+    const float *x = nullptr;
+    int64_t n = 0;
+    int ntx = 0, ntz = 0;      // sort tiles along x and z
+    int max_chunks = 0;        // launch bound
+    m2d::Scratch buf;          // device: tile_start, tile_end, chunk_first, chunk_count, sched, n_chunks
+    int *tile_start = nullptr, *tile_end = nullptr, *chunk_first = nullptr, *chunk_count = nullptr;
+    int4 *sched = nullptr;
+    unsigned int *n_chunks = nullptr;
+};
+
 // counters written by kernels
 enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_QLEN = 4 /* and 5 */, CNT_N = 8 };
 
@@ -72,6 +91,8 @@ struct m2d_ctx {
     m2d::Scratch sortbuf;   // keys, slots and radix-sort temporaries
     m2d::Scratch cq;        // MCC collision queue (slot indices)
     uint64_t launch_seq = 0;  // selects the collision-queue counter of a launch
+    m2d::Scratch partials;  // per-block deposit windows of the tiled pass
+    std::vector<struct m2d_tiling *> tilings;  // schedules of recently sorted stores
     // kernel timing: event pairs around particle-pass launches
     bool profile = false;
     std::vector<hipEvent_t> ev_pool;
@@ -79,6 +100,10 @@ struct m2d_ctx {
 };
 
 namespace m2d {
+// tile schedules of sorted stores (m2d_sort.hip)
+m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n);
+void forget_tiling(m2d_ctx *ctx, const float *x);
+void free_tilings(m2d_ctx *ctx);
 // objects that hold a context keep it alive until they are destroyed
 void ctx_retain(m2d_ctx *ctx);
 void ctx_release(m2d_ctx *ctx);

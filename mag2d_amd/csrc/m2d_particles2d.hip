@@ -36,6 +36,11 @@ struct PushArgs {
     unsigned long long *cq_len;    // its length (one of two counters, alternating per launch)
     unsigned long long *acc;       // nx*nz fixed-point deposit grid
     unsigned long long *counters;
+    // tiled deposit (store sorted by m2d_sort_2d)
+    const int4 *sched;             // chunk -> (tile, first slot, end slot)
+    const unsigned int *n_chunks;
+    int ntz;                       // sort tiles along z
+    unsigned long long *partials;  // chunk -> kWin x kWin window
 };
 
 // ------------------------------------------------------------------ gather
@@ -248,17 +253,12 @@ __device__ void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, float &v
 }
 
 // ------------------------------------------------------------------ deposit
-// CIC weights (Field2D.hpp:45-62) as 2^-32 fixed point.  The four rounded
-// weights are corrected on the largest so each particle deposits exactly
-// 2^32: charge is conserved bit for bit and sums are order independent.
-// round(w 2^32) for 0 <= w <= 1, to nearest even like __double2ll_rn: adding
-// 2^52 moves the integer part into the low mantissa bits in one rounding
-__device__ __forceinline__ long long fix32(double w)
-{
-    double d = fma(w, 4294967296.0, 4503599627370496.0);
-    return __double_as_longlong(d) - 0x4330000000000000ll;
-}
-
+// CIC weights (Field2D.hpp:45-62) as 2^-31 fixed point.  The cell offsets
+// u, v come from the FP64 products x*idx (exact subtraction of the cell
+// index), the four bilinear products are FP32 (relative error ~1e-7), and
+// the rounded weights are corrected on the largest so each particle deposits
+// exactly 2^31: charge is conserved bit for bit and node sums do not depend
+// on the order of the particles.
 __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
                                             long long w[4])
 {
@@ -266,16 +266,17 @@ __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, 
     i = (int)X;
     j = (int)Z;
     if (i > nx - 2 || j > nz - 2) return false;
-    double u = X - i, v = Z - j;
-    w[0] = fix32((1 - u) * (1 - v));
-    w[1] = fix32(u * (1 - v));
-    w[2] = fix32((1 - u) * v);
-    w[3] = fix32(u * v);
+    const float u = (float)(X - i), v = (float)(Z - j);
+    const float S = 2147483648.0f;  // 2^31
+    w[0] = (long long)__float2uint_rn((1.f - u) * (1.f - v) * S);
+    w[1] = (long long)__float2uint_rn(u * (1.f - v) * S);
+    w[2] = (long long)__float2uint_rn((1.f - u) * v * S);
+    w[3] = (long long)__float2uint_rn(u * v * S);
     long long s = w[0] + w[1] + w[2] + w[3];
     int km = 0;
 #pragma unroll
     for (int q = 1; q < 4; q++) km = w[q] > w[km] ? q : km;
-    w[km] += (1ll << 32) - s;
+    w[km] += (1ll << 31) - s;
     return true;
 }
 
@@ -293,19 +294,21 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 // A block owns kRounds x 1024 consecutive slots.  Each round is staged
 // through LDS: global loads and stores are fully coalesced (thread t moves
 // slots t, t+256, ...), while the compute phase gives each thread 4
-// consecutive slots, so consecutive particles of a sorted store share a cell
-// and their deposits are summed in registers before touching memory.
-// Deposits go to an LDS window of kWinI x kWinJ nodes anchored on the
-// 16-cell tile of the block's first particle (after a cell sort the block
-// lies in one or two tiles); the window is flushed to the global
-// fixed-point grid once per block, and deposits outside it go straight to
-// global atomics.
+// consecutive slots, so consecutive particles of a sorted store share a cell.
+// Deposits are summed twice before they reach memory: in registers over a
+// thread's run of equal cells, then across the wavefront by a segmented scan
+// over the lanes' last runs (lanes hold consecutive slots, so a wavefront of
+// a sorted store spans a few cells); only the last lane of each segment
+// issues the four 64-bit atomics.
 constexpr int kRound = kBlock * 4;
 constexpr int kRounds = 2;
 constexpr int kChunk = kRound * kRounds;
 constexpr int kPPT = kChunk / kBlock;       // slots per thread (grid sizing)
-constexpr int kWinI = 32, kWinJ = 48;       // tile (ti, tj) and (ti, tj+1) with an 8-cell halo
-static_assert(kTile == 16, "window halo assumes 16-cell tiles");
+
+// LDS stage index of slot s of a round: one pad word per 32 keeps both the
+// coalesced (stride 1) and the per-thread (stride 4) accesses free of bank
+// conflicts
+__device__ __forceinline__ int lds_at(int s) { return s + (s >> 5); }
 
 // running sum of one thread's deposits while consecutive particles share a cell
 struct DepRun {
@@ -313,45 +316,71 @@ struct DepRun {
     long long w[4];
 };
 
-__device__ __forceinline__ void run_flush(const DepRun &d, unsigned long long *win, int wi0, int wj0,
-                                          unsigned long long *acc, int nz)
+// a finished run into the block's LDS window (tile + halo), or into the
+// global grid when it lies outside
+__device__ __forceinline__ void run_to_window(const DepRun &d, unsigned long long *win, int wi0, int wj0,
+                                              unsigned long long *acc, int nz)
 {
     if (d.ci < 0) return;
     const int a = d.ci - wi0, b = d.cj - wj0;
-    if (a >= 0 && a + 1 < kWinI && b >= 0 && b + 1 < kWinJ) {
-        unsigned long long *s = win + a * kWinJ + b;
+    if (a >= 0 && a + 1 < kWin && b >= 0 && b + 1 < kWin) {
+        unsigned long long *s = win + a * kWin + b;
         atomicAdd(s, (unsigned long long)d.w[0]);
-        atomicAdd(s + kWinJ, (unsigned long long)d.w[1]);
+        atomicAdd(s + kWin, (unsigned long long)d.w[1]);
         atomicAdd(s + 1, (unsigned long long)d.w[2]);
-        atomicAdd(s + kWinJ + 1, (unsigned long long)d.w[3]);
+        atomicAdd(s + kWin + 1, (unsigned long long)d.w[3]);
     } else {
         deposit_global(acc, nz, d.ci, d.cj, d.w);
     }
 }
 
-template <bool DEPOSIT, bool MCC>
-__global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
+// segmented inclusive scan of the lanes' runs keyed by cell; segment tails
+// deposit.  All 64 lanes must call it together.
+__device__ __forceinline__ void wave_deposit(DepRun &d, unsigned long long *acc, int nz)
 {
-    __shared__ float st[5][kRound];
-    __shared__ unsigned long long win[DEPOSIT ? kWinI * kWinJ : 1];
-    __shared__ int anchor[2];
+    const int lane = __lane_id();
+    const bool valid = d.ci >= 0;
+    const int key = valid ? d.ci * nz + d.cj : -1 - lane;
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    const int seg = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        long long u0 = __shfl_up(d.w[0], off), u1 = __shfl_up(d.w[1], off);
+        long long u2 = __shfl_up(d.w[2], off), u3 = __shfl_up(d.w[3], off);
+        if (lane - off >= seg) { d.w[0] += u0; d.w[1] += u1; d.w[2] += u2; d.w[3] += u3; }
+    }
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+    if (valid && tail) deposit_global(acc, nz, d.ci, d.cj, d.w);
+    d.ci = -1;
+}
+
+template <bool DEPOSIT, bool MCC, bool TILED>
+// six wavefronts per SIMD: the pass is bound by memory latency, and the LDS
+// stage allows six blocks per CU
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+k_advance2d(PushArgs a)
+{
+    __shared__ float st[5][kRound + kRound / 32];
+    __shared__ unsigned long long win[TILED ? kWin * kWin : 1];
     const int tid = threadIdx.x;
-    const int64_t base = (int64_t)blockIdx.x * kChunk;
+    // slots [lo, hi) of this block; rounds start at lo rounded down to 4 so
+    // each thread's 4 slots share one trial Philox block
+    int64_t lo, hi, co0;
     int wi0 = 0, wj0 = 0;
-    if (DEPOSIT) {
-        for (int k = tid; k < kWinI * kWinJ; k += kBlock) win[k] = 0ull;
-        if (tid == 0) {
-            int ai = 0, aj = 0;
-            if (base < a.n && !is_dead(a.x[base])) {
-                double X = ((double)a.x[base] - a.xmin) * a.idx, Z = ((double)a.z[base] - a.zmin) * a.idz;
-                if (X >= 0.0 && Z >= 0.0 && X < a.nx && Z < a.nz) { ai = (int)X; aj = (int)Z; }
-            }
-            anchor[0] = (ai / kTile) * kTile - 8;
-            anchor[1] = (aj / kTile) * kTile - 8;
-        }
-        __syncthreads();
-        wi0 = anchor[0];
-        wj0 = anchor[1];
+    if constexpr (TILED) {
+        if (blockIdx.x >= *a.n_chunks) return;  // uniform: before any barrier
+        const int4 s = a.sched[blockIdx.x];
+        lo = s.y;
+        hi = s.z;
+        co0 = lo & ~(int64_t)3;
+        wi0 = (s.x / a.ntz) * kTile - kHalo;
+        wj0 = (s.x % a.ntz) * kTile - kHalo;
+        for (int k = tid; k < kWin * kWin; k += kBlock) win[k] = 0ull;
+    } else {
+        lo = (int64_t)blockIdx.x * kChunk;
+        hi = min(lo + (int64_t)kChunk, a.n);
+        co0 = lo;
     }
     float *const arr[5] = {a.x, a.z, a.vx, a.vy, a.vz};
     int n_removed = 0, n_coll = 0;
@@ -359,14 +388,14 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
     DepRun run;
     run.ci = -1;
     for (int r = 0; r < kRounds; r++) {
-        const int64_t co = base + (int64_t)r * kRound;
-        if (co >= a.n) break;  // uniform over the block
+        const int64_t co = co0 + (int64_t)r * kRound;
+        if (co >= hi) break;  // uniform over the block
         // ---- coalesced load into LDS
         float v[5][4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int64_t idx = co + k * kBlock + tid;
-            const bool in = idx < a.n;
+            const bool in = idx >= lo && idx < hi;
 #pragma unroll
             for (int c = 0; c < 5; c++) v[c][k] = in ? arr[c][idx] : (c == 0 ? dead_value() : 0.f);
         }
@@ -374,29 +403,24 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
-            for (int c = 0; c < 5; c++) st[c][k * kBlock + tid] = v[c][k];
+            for (int c = 0; c < 5; c++) st[c][lds_at(k * kBlock + tid)] = v[c][k];
         __syncthreads();
-        // ---- compute on 4 consecutive slots
-        float4 tx = *(float4 *)&st[0][4 * tid], tz = *(float4 *)&st[1][4 * tid];
-        float4 tvx = *(float4 *)&st[2][4 * tid], tvy = *(float4 *)&st[3][4 * tid];
-        float4 tvz = *(float4 *)&st[4][4 * tid];
-        float X4[4] = {tx.x, tx.y, tx.z, tx.w}, Z4[4] = {tz.x, tz.y, tz.z, tz.w};
-        float VX[4] = {tvx.x, tvx.y, tvx.z, tvx.w}, VY[4] = {tvy.x, tvy.y, tvy.z, tvy.w};
-        float VZ[4] = {tvz.x, tvz.y, tvz.z, tvz.w};
+        // ---- compute on 4 consecutive slots, one particle live at a time
         const int64_t pb = co + 4 * tid;
-        uint32_t tw[4] = {0u, 0u, 0u, 0u};
+        uint32_t tw0 = 0u, tw1 = 0u, tw2 = 0u, tw3 = 0u;
         if (MCC) {
             // trial uniforms of slots pb..pb+3: one Philox block
-            tw[0] = (uint32_t)(pb >> 2); tw[1] = (uint32_t)a.step; tw[2] = 0u;
-            tw[3] = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
-            philox4x32_10(tw[0], tw[1], tw[2], tw[3], (uint32_t)a.seed,
-                          (uint32_t)(a.seed >> 32) ^ a.stream);
+            tw0 = (uint32_t)(pb >> 2); tw1 = (uint32_t)a.step; tw2 = 0u;
+            tw3 = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
+            philox4x32_10(tw0, tw1, tw2, tw3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
         }
-#pragma unroll
+#pragma unroll 1
         for (int q = 0; q < 4; q++) {
-            float x = X4[q];
+            const int li = lds_at(4 * tid + q);
+            float x = st[0][li];
             if (is_dead(x)) continue;
-            float z = Z4[q], vx = VX[q], vy = VY[q], vz = VZ[q];
+            float z = st[1][li], vx = st[2][li], vy = st[3][li], vz = st[4][li];
+            const uint32_t twq = q == 0 ? tw0 : q == 1 ? tw1 : q == 2 ? tw2 : tw3;
 
             // ---- advance_boris (particles.cpp:945-988)
             float ex, ez;
@@ -414,12 +438,12 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
             x = fmaf(vx, a.dt, x);
             z = fmaf(vz, a.dt, z);
 
-            // ---- MCC trial (particles.cpp:990-993)
-            // A hit only changes the velocity after the drift, so it cannot
-            // change the boundary test or the deposit below: the slot is
-            // queued and k_scatter2d applies BaseSpecies::scatter afterwards.
+            // ---- MCC trial (particles.cpp:990-993).  A hit only changes the
+            // velocity after the drift, so it cannot change the boundary test
+            // or the deposit below: the slot is queued and k_scatter2d applies
+            // BaseSpecies::scatter afterwards.
             if (MCC) {
-                if (((double)tw[q] + 0.5) * 2.3283064365386963e-10 < a.prob) {
+                if (((double)twq + 0.5) * 2.3283064365386963e-10 < a.prob) {
                     unsigned long long at = atomicAdd(a.cq_len, 1ull);
                     a.cq[at] = (uint32_t)(pb + q);
                     n_coll++;
@@ -451,7 +475,7 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
                 dead = !free_;
             }
             if (dead) {
-                X4[q] = dead_value();
+                st[0][li] = dead_value();
                 n_removed++;
                 continue;
             }
@@ -463,7 +487,8 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
                     if (ci == run.ci && cj == run.cj) {
                         run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
                     } else {
-                        run_flush(run, win, wi0, wj0, a.acc, a.nz);
+                        if constexpr (TILED) run_to_window(run, win, wi0, wj0, a.acc, a.nz);
+                        else if (run.ci >= 0) deposit_global(a.acc, a.nz, run.ci, run.cj, run.w);
                         run.ci = ci; run.cj = cj;
                         run.w[0] = w[0]; run.w[1] = w[1]; run.w[2] = w[2]; run.w[3] = w[3];
                     }
@@ -471,34 +496,27 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
                     raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
                 }
             }
-            X4[q] = x; Z4[q] = z; VX[q] = vx; VY[q] = vy; VZ[q] = vz;
+            st[0][li] = x; st[1][li] = z; st[2][li] = vx; st[3][li] = vy; st[4][li] = vz;
         }
-        *(float4 *)&st[0][4 * tid] = make_float4(X4[0], X4[1], X4[2], X4[3]);
-        *(float4 *)&st[1][4 * tid] = make_float4(Z4[0], Z4[1], Z4[2], Z4[3]);
-        *(float4 *)&st[2][4 * tid] = make_float4(VX[0], VX[1], VX[2], VX[3]);
-        *(float4 *)&st[3][4 * tid] = make_float4(VY[0], VY[1], VY[2], VY[3]);
-        *(float4 *)&st[4][4 * tid] = make_float4(VZ[0], VZ[1], VZ[2], VZ[3]);
+        if constexpr (DEPOSIT && !TILED) wave_deposit(run, a.acc, a.nz);
         __syncthreads();
         // ---- coalesced store
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int64_t idx = co + k * kBlock + tid;
-            if (idx < a.n) {
+            if (idx >= lo && idx < hi) {
 #pragma unroll
-                for (int c = 0; c < 5; c++) arr[c][idx] = st[c][k * kBlock + tid];
+                for (int c = 0; c < 5; c++) arr[c][idx] = st[c][lds_at(k * kBlock + tid)];
             }
         }
     }
-    if (DEPOSIT) {
-        run_flush(run, win, wi0, wj0, a.acc, a.nz);
+    if constexpr (TILED) {
+        // the window goes out with plain stores; k_reduce_tiles adds the
+        // overlapping windows of neighbouring chunks
+        run_to_window(run, win, wi0, wj0, a.acc, a.nz);
         __syncthreads();
-        for (int k = tid; k < kWinI * kWinJ; k += kBlock) {
-            unsigned long long w = win[k];
-            if (w) {
-                int gi = wi0 + k / kWinJ, gj = wj0 + k % kWinJ;
-                atomicAdd(a.acc + (size_t)gi * a.nz + gj, w);
-            }
-        }
+        unsigned long long *out = a.partials + (size_t)blockIdx.x * (kWin * kWin);
+        for (int k = tid; k < kWin * kWin; k += kBlock) out[k] = win[k];
     }
     // ---- counters: one atomic per wavefront
     int sum = n_removed;
@@ -510,6 +528,37 @@ __global__ void __launch_bounds__(kBlock) k_advance2d(PushArgs a)
         if (__lane_id() == 0 && c) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)c);
     }
     if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+}
+
+// Node sums of the tiled pass: every node of an 8 x 8 output tile adds the
+// windows of the chunks of the 3 x 3 neighbouring sort tiles that cover it,
+// plus the fallback atomics in acc; then rho += charge * sum * 2^-31 and acc
+// is cleared (the flush of the untiled pass).
+__global__ void __launch_bounds__(64) k_reduce_tiles(const unsigned long long *partials,
+                                                     const int *chunk_first, const int *chunk_count,
+                                                     int ntx, int ntz, unsigned long long *acc,
+                                                     double *rho, double charge, int nx, int nz)
+{
+    const int TI = blockIdx.y, TJ = blockIdx.x;
+    const int i = TI * kTile + threadIdx.x / kTile, j = TJ * kTile + threadIdx.x % kTile;
+    if (i >= nx || j >= nz) return;
+    const size_t k = (size_t)i * nz + j;
+    long long s = (long long)acc[k];
+    for (int ai = TI - 1; ai <= TI + 1; ai++) {
+        if (ai < 0 || ai >= ntx) continue;
+        for (int aj = TJ - 1; aj <= TJ + 1; aj++) {
+            if (aj < 0 || aj >= ntz) continue;
+            const int t = ai * ntz + aj;
+            const int a = i - (ai * kTile - kHalo), b = j - (aj * kTile - kHalo);
+            if (a < 0 || a >= kWin || b < 0 || b >= kWin) continue;
+            const int c0 = chunk_first[t], c1 = c0 + chunk_count[t];
+            for (int c = c0; c < c1; c++) s += (long long)partials[(size_t)c * (kWin * kWin) + a * kWin + b];
+        }
+    }
+    if (s) {
+        rho[k] += charge * ((double)s * kInvFix);
+        if (acc[k]) acc[k] = 0ull;
+    }
 }
 
 // BaseSpecies::scatter for the slots queued by k_advance2d; each slot draws
@@ -585,7 +634,7 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
         raise_flag(counters, M2D_FLAG_OUT_OF_RANGE);
 }
 
-// rho += charge * acc * 2^-32; acc = 0
+// rho += charge * acc * 2^-31; acc = 0
 __global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -900,6 +949,10 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.mcc = nullptr;
     a.cq = nullptr;
     a.cq_len = nullptr;
+    a.sched = nullptr;
+    a.n_chunks = nullptr;
+    a.ntz = 0;
+    a.partials = nullptr;
     a.acc = nullptr;
     a.counters = f->ctx->counters;
 }
@@ -940,17 +993,34 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         a.cq_len = ctx->counters + CNT_QLEN + slot;
         next_len = ctx->counters + CNT_QLEN + (slot ^ 1);
     }
+    // a store sorted by m2d_sort_2d against these fields deposits through
+    // per-chunk LDS windows reduced by k_reduce_tiles
+    const m2d_tiling *T = deposit ? find_tiling(ctx, p->x, p->n) : nullptr;
+    if (T && (T->ntx != (f->nx - 2) / kTile + 1 || T->ntz != (f->nz - 2) / kTile + 1)) T = nullptr;
     if (deposit) {
         M2D_TRY(ensure_acc(ctx, f->n));
         a.acc = (unsigned long long *)ctx->acc.ptr;
     }
+    if (T) {
+        M2D_TRY(ensure(ctx, ctx->partials, (size_t)T->max_chunks * kWin * kWin * sizeof(unsigned long long)));
+        a.sched = T->sched;
+        a.n_chunks = T->n_chunks;
+        a.ntz = T->ntz;
+        a.partials = (unsigned long long *)ctx->partials.ptr;
+    }
     if (p->n > 0) {
-        int g = grid_for(p->n, kPPT);
         hipEvent_t t0 = prof_begin(ctx);
-        if (deposit && use_mcc) k_advance2d<true, true><<<g, kBlock, 0, ctx->stream>>>(a);
-        else if (deposit) k_advance2d<true, false><<<g, kBlock, 0, ctx->stream>>>(a);
-        else if (use_mcc) k_advance2d<false, true><<<g, kBlock, 0, ctx->stream>>>(a);
-        else k_advance2d<false, false><<<g, kBlock, 0, ctx->stream>>>(a);
+        if (T) {
+            const int g = T->max_chunks;
+            if (use_mcc) k_advance2d<true, true, true><<<g, kBlock, 0, ctx->stream>>>(a);
+            else k_advance2d<true, false, true><<<g, kBlock, 0, ctx->stream>>>(a);
+        } else {
+            const int g = grid_for(p->n, kPPT);
+            if (deposit && use_mcc) k_advance2d<true, true, false><<<g, kBlock, 0, ctx->stream>>>(a);
+            else if (deposit) k_advance2d<true, false, false><<<g, kBlock, 0, ctx->stream>>>(a);
+            else if (use_mcc) k_advance2d<false, true, false><<<g, kBlock, 0, ctx->stream>>>(a);
+            else k_advance2d<false, false, false><<<g, kBlock, 0, ctx->stream>>>(a);
+        }
         M2D_HIP(hipGetLastError());
         prof_end(ctx, t0);
         if (use_mcc) {
@@ -958,7 +1028,15 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
             M2D_HIP(hipGetLastError());
         }
     }
-    if (deposit) M2D_TRY(flush(ctx, f->n, rho, s->charge));
+    if (T) {
+        dim3 g((f->nz + kTile - 1) / kTile, (f->nx + kTile - 1) / kTile);
+        k_reduce_tiles<<<g, kTile * kTile, 0, ctx->stream>>>(a.partials, T->chunk_first, T->chunk_count,
+                                                             T->ntx, T->ntz, a.acc, rho, s->charge,
+                                                             f->nx, f->nz);
+        M2D_HIP(hipGetLastError());
+    } else if (deposit) {
+        M2D_TRY(flush(ctx, f->n, rho, s->charge));
+    }
     if (stats) {
         M2D_TRY(read_counters(ctx));
         stats->removed = (int64_t)ctx->h_counters[CNT_REMOVED];
@@ -1106,6 +1184,7 @@ int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles
     unsigned long long *total = offs + nb;
     k_count_blocks<<<nb, kBlock, 0, ctx->stream>>>(src->x, src->n, counts);
     k_scan_counts<<<1, 1024, 0, ctx->stream>>>(counts, nb, offs, total);
+    forget_tiling(ctx, dst->x);
     k_compact_scatter<<<nb, kBlock, 0, ctx->stream>>>(*src, *dst, offs);
     M2D_HIP(hipGetLastError());
     unsigned long long h = 0;
@@ -1128,6 +1207,7 @@ int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_partic
     M2D_TRY(check_parts(p));
     M2D_CHECK(radius > 0 && vth >= 0, M2D_ERR_BAD_ARG, "m2d_add_particles_on_disk_2d: radius/vth");
     if (p->n == 0) return M2D_OK;
+    forget_tiling(ctx, p->x);
     k_disk2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(*p, cx, cz, radius, vth, f->p.grid.xmin,
                                                              f->p.x_max, f->p.grid.zmin, f->p.z_max,
                                                              ctx->seed, stream);

@@ -1,13 +1,16 @@
-// m2d_sort.hip -- stable reordering of a species' particles by cell.
+// m2d_sort.hip -- stable reordering of a species' particles by cell, and the
+// tile schedule of the sorted store.
 //
 // The reference keeps particles in insertion order with holes
 // (particles.hpp:223-249).  Here the store is periodically rebuilt in
-// tile-major cell order, dropping empty slots: the E gather of neighbouring
-// threads then reads neighbouring table entries, and a block's deposits fall
-// in one LDS window.  Keys are (tile, cell-in-tile); the order among equal
-// keys is the old slot order, so the result is deterministic.  The key/slot
-// pairs are sorted with rocPRIM's LSD radix sort.
+// tile-major cell order (8 x 8-cell tiles, row-major cells inside a tile),
+// dropping empty slots: the E gather of neighbouring threads then reads
+// neighbouring table entries, and each block of the particle pass deposits
+// into one small LDS window.  The order among equal keys is the old slot
+// order, so the result is deterministic.  The key/slot pairs are sorted with
+// rocPRIM's LSD radix sort.
 #include <cstring>
+#include <new>
 #include <rocprim/device/device_radix_sort.hpp>
 #include "m2d_objects.hpp"
 
@@ -25,7 +28,7 @@ __device__ __forceinline__ uint32_t cell_key(const SortGeom &g, float x, float z
     int i = X >= 0.0 ? (int)fmin(X, (double)(g.nx - 2)) : 0;
     int j = Z >= 0.0 ? (int)fmin(Z, (double)(g.nz - 2)) : 0;
     uint32_t tile = (uint32_t)((i / kTile) * g.ntz + (j / kTile));
-    return (tile << 8) | ((uint32_t)(i % kTile) << 4) | (uint32_t)(j % kTile);
+    return (tile << kTileShift) | ((uint32_t)(i % kTile) << 3) | (uint32_t)(j % kTile);
 }
 
 __global__ void k_sort_keys(const float *xs, const float *zs, int64_t n, SortGeom g, uint32_t *keys,
@@ -57,6 +60,126 @@ __global__ void k_sort_gather(m2d_particles2d src, m2d_particles2d dst, const ui
     dst.vz[i] = src.vz[s];
 }
 
+// [start, end) of every non-empty tile in the sorted keys
+__global__ void k_tile_bounds(const uint32_t *keys, int64_t n, uint32_t dead_key, int *tstart, int *tend)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = keys[i];
+    if (k == dead_key) return;
+    uint32_t t = k >> kTileShift;
+    if (i == 0 || (keys[i - 1] >> kTileShift) != t) tstart[t] = (int)i;
+    if (i == n - 1 || keys[i + 1] == dead_key || (keys[i + 1] >> kTileShift) != t) tend[t] = (int)(i + 1);
+}
+
+__global__ void k_tile_chunks(const int *tstart, const int *tend, int ntiles, int *count)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int c = tend[t] - tstart[t];
+    count[t] = c > 0 ? (c + kChunkCap - 1) / kChunkCap : 0;
+}
+
+// exclusive scan of the per-tile chunk counts (one block) and the schedule
+__global__ void __launch_bounds__(1024) k_tile_schedule(const int *tstart, const int *tend,
+                                                        const int *count, int ntiles, int *first,
+                                                        int4 *sched, unsigned int *n_chunks)
+{
+    __shared__ int s[1024];
+    const int per = (ntiles + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = min(ntiles, lo + per);
+    int loc = 0;
+    for (int t = lo; t < hi; t++) loc += count[t];
+    s[threadIdx.x] = loc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - loc;
+    for (int t = lo; t < hi; t++) {
+        first[t] = run;
+        for (int c = 0; c < count[t]; c++) {
+            int a = tstart[t] + c * kChunkCap;
+            sched[run + c] = make_int4(t, a, min(tend[t], a + kChunkCap), 0);
+        }
+        run += count[t];
+    }
+    if (threadIdx.x == 1023) *n_chunks = (unsigned int)s[1023];
+}
+
+m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n)
+{
+    for (m2d_tiling *t : ctx->tilings)
+        if (t->x == x && t->n == n) return t;
+    return nullptr;
+}
+
+void forget_tiling(m2d_ctx *ctx, const float *x)
+{
+    for (m2d_tiling *t : ctx->tilings)
+        if (t->x == x) t->x = nullptr;
+}
+
+void free_tilings(m2d_ctx *ctx)
+{
+    for (m2d_tiling *t : ctx->tilings) {
+        if (t->buf.ptr) (void)hipFree(t->buf.ptr);
+        delete t;
+    }
+    ctx->tilings.clear();
+}
+
+// build the schedule of the store `dst` from the sorted keys
+static int build_tiling(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *dst, const uint32_t *keys,
+                        uint32_t dead_key, int ntx, int ntz)
+{
+    forget_tiling(ctx, dst->x);
+    m2d_tiling *T = nullptr;
+    for (m2d_tiling *t : ctx->tilings)
+        if (t->x == nullptr) { T = t; break; }
+    if (!T) {
+        if (ctx->tilings.size() >= 8) {  // reuse the oldest record
+            T = ctx->tilings.front();
+            ctx->tilings.erase(ctx->tilings.begin());
+        } else {
+            T = new (std::nothrow) m2d_tiling();
+            M2D_CHECK(T, M2D_ERR_NO_MEMORY, "tiling: host allocation");
+        }
+        ctx->tilings.push_back(T);
+    }
+    T->x = nullptr;
+    const int ntiles = ntx * ntz;
+    const int max_chunks = ntiles + (int)(dst->n / kChunkCap) + 1;
+    const size_t words = 4 * (size_t)ntiles + 4 * (size_t)max_chunks + 4;
+    M2D_TRY(ensure(ctx, T->buf, words * sizeof(int)));
+    int *b = (int *)T->buf.ptr;
+    T->tile_start = b;
+    T->tile_end = b + ntiles;
+    T->chunk_first = b + 2 * ntiles;
+    T->chunk_count = b + 3 * ntiles;
+    T->sched = (int4 *)(b + 4 * ntiles);
+    T->n_chunks = (unsigned int *)(b + 4 * ntiles + 4 * max_chunks);
+    M2D_TRY(fill_bytes(ctx, T->tile_start, 0, sizeof(int) * 2 * (size_t)ntiles));
+    const int64_t n = dst->n;
+    k_tile_bounds<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(keys, n, dead_key, T->tile_start,
+                                                                        T->tile_end);
+    k_tile_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, ctx->stream>>>(T->tile_start, T->tile_end,
+                                                                            ntiles, T->chunk_count);
+    k_tile_schedule<<<1, 1024, 0, ctx->stream>>>(T->tile_start, T->tile_end, T->chunk_count, ntiles,
+                                                 T->chunk_first, T->sched, T->n_chunks);
+    M2D_HIP(hipGetLastError());
+    T->ntx = ntx;
+    T->ntz = ntz;
+    T->max_chunks = max_chunks;
+    T->n = n;
+    T->x = dst->x;
+    (void)f;
+    return M2D_OK;
+}
+
 }  // namespace m2d
 
 using namespace m2d;
@@ -67,9 +190,9 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
                 int64_t *n_live)
 {
     M2D_CHECK(ctx && f && src && dst, M2D_ERR_BAD_ARG, "m2d_sort_2d: NULL argument");
-    M2D_CHECK(src->n >= 0 && src->n < (1ll << 32), M2D_ERR_BAD_ARG, "m2d_sort_2d: %lld slots",
+    M2D_CHECK(src->n >= 0 && src->n < (1ll << 31), M2D_ERR_BAD_ARG, "m2d_sort_2d: %lld slots",
               (long long)src->n);
-    M2D_CHECK(dst->n >= src->n, M2D_ERR_BAD_ARG, "m2d_sort_2d: dst holds %lld < %lld slots",
+    M2D_CHECK(dst->n == src->n, M2D_ERR_BAD_ARG, "m2d_sort_2d: dst holds %lld slots, src %lld",
               (long long)dst->n, (long long)src->n);
     M2D_CHECK(src->n == 0 || (src->x && src->z && src->vx && src->vy && src->vz && dst->x && dst->z &&
                               dst->vx && dst->vy && dst->vz),
@@ -78,16 +201,18 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     const size_t n = (size_t)src->n;
     SortGeom g;
     g.nx = f->nx; g.nz = f->nz;
-    int ntx = (f->nx - 2) / kTile + 1;
+    const int ntx = (f->nx - 2) / kTile + 1;
     g.ntz = (f->nz - 2) / kTile + 1;
     g.idx = f->idx; g.idz = f->idz;
     g.xmin = f->p.grid.xmin; g.zmin = f->p.grid.zmin;
-    uint64_t keyspace = (uint64_t)ntx * g.ntz * 256;
+    uint64_t keyspace = ((uint64_t)ntx * g.ntz) << kTileShift;
     int bits = 0;
     while ((1ull << bits) <= keyspace) bits++;
     M2D_CHECK(bits <= 32, M2D_ERR_BAD_ARG, "m2d_sort_2d: grid too large for 32-bit keys");
     g.dead_key = bits == 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
+    forget_tiling(ctx, src->x);
     if (n == 0) {
+        forget_tiling(ctx, dst->x);
         if (n_live) *n_live = 0;
         return M2D_OK;
     }
@@ -110,10 +235,18 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key, live);
     M2D_HIP(hipGetLastError());
+    M2D_TRY(build_tiling(ctx, f, dst, k1, g.dead_key, ntx, g.ntz));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];
     }
+    return M2D_OK;
+}
+
+int m2d_sort_forget_2d(m2d_ctx *ctx, const m2d_particles2d *p)
+{
+    M2D_CHECK(ctx && p, M2D_ERR_BAD_ARG, "m2d_sort_forget_2d: NULL argument");
+    forget_tiling(ctx, p->x);
     return M2D_OK;
 }
 
