@@ -20,6 +20,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -108,6 +109,103 @@ def cpu_baseline(fields, params, mask, n_grid, budget_s):
                       f"FP64 AoS, serial reference loop order, SHR3 stream, deposit on; {t:.1f} s"}
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "k_advance2d_rf22pt_512x512_4M_traffic.md"
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def copy_calibration(ctx, mib, reps=5):
+    """Plain device-to-device copies of a known size (the runtime's blit
+    kernel): each moves `mib` MiB in and `mib` MiB out, which calibrates
+    the FETCH_SIZE / WRITE_SIZE counters of a --pmc run of this program."""
+    nbytes = mib << 20
+    a = abi.DeviceArray(ctx, (nbytes,), np.uint8)
+    b = abi.DeviceArray(ctx, (nbytes,), np.uint8)
+    a.zero()
+    for _ in range(reps):
+        abi.check(abi.lib().m2d_memcpy_d2d(ctx.h, abi.C.c_void_p(b.ptr), abi.C.c_void_p(a.ptr), nbytes))
+    ctx.sync()
+
+
+def _counter_means(csv_path):
+    """Mean counter value per (kernel, counter) of a rocprofv3 csv counter file."""
+    import csv
+    acc = {}
+    with open(csv_path) as fh:
+        for row in csv.DictReader(fh):
+            key = (row["Kernel_Name"], row["Counter_Name"])
+            acc.setdefault(key, []).append(float(row["Counter_Value"]))
+    return acc
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_mib, kernel="k_advance2d"):
+    """Markdown summary of a FETCH_SIZE run and a WRITE_SIZE run: every
+    counter is in KiB per dispatch; each is scaled by the factor that maps
+    the copy kernel's reading onto the bytes it is known to move."""
+    import glob
+    fetch = _counter_means(glob.glob(os.path.join(fetch_dir, "*counter_collection.csv"))[0])
+    write = _counter_means(glob.glob(os.path.join(write_dir, "*counter_collection.csv"))[0])
+    copy_kib = copy_mib * 1024.0
+
+    def copy_reading(table, counter):
+        vals = [v for (k, c), vs in table.items() if c == counter and COPY_KERNEL in k for v in vs]
+        big = [v for v in vals if v > 0.5 * copy_kib]
+        return sum(big) / len(big) if big else None
+
+    f_copy, w_copy = copy_reading(fetch, "FETCH_SIZE"), copy_reading(write, "WRITE_SIZE")
+    f_scale = copy_kib / f_copy if f_copy else None
+    w_scale = copy_kib / w_copy if w_copy else None
+    lines = ["| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | calibrated MB read | calibrated MB written |",
+             "|---|---|---|---|---|---|"]
+    names = sorted({k for (k, c) in fetch} | {k for (k, c) in write})
+    target = None
+    for k in names:
+        fv = fetch.get((k, "FETCH_SIZE"), [])
+        wv = write.get((k, "WRITE_SIZE"), [])
+        fm = sum(fv) / len(fv) if fv else float("nan")
+        wm = sum(wv) / len(wv) if wv else float("nan")
+        rd = fm * 1024 * f_scale / 1e6 if f_scale else float("nan")
+        wr = wm * 1024 * w_scale / 1e6 if w_scale else float("nan")
+        short = k.split("(")[0][:60]
+        lines.append(f"| `{short}` | {max(len(fv), len(wv))} | {fm:.1f} | {wm:.1f} | {rd:.2f} | {wr:.2f} |")
+        if kernel in k and "<" in k and fv and wv:
+            target = (rd + wr) * 1e6
+    out = ["copy calibration: %d MiB copies read %s KiB (FETCH_SIZE) and wrote %s KiB (WRITE_SIZE)"
+           % (copy_mib, f"{f_copy:.1f}" if f_copy else "n/a", f"{w_copy:.1f}" if w_copy else "n/a"),
+           f"fetch_scale: {f_scale}", f"write_scale: {w_scale}", ""] + lines + [""]
+    if target is not None:
+        out.append(f"traffic_bytes_per_launch: {target:.0f}")
+    return "\n".join(out)
+
+
+def summarize_trace(trace_dir):
+    """Markdown table of a rocprofv3 --kernel-trace --stats csv run."""
+    import csv
+    import glob
+    path = glob.glob(os.path.join(trace_dir, "*kernel_stats.csv"))[0]
+    lines = ["| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            lines.append("| `%s` | %s | %.3f | %.2f | %.2f | %.2f | %.1f |" % (
+                row["Name"].split("(")[0][:70], row["Calls"], float(row["TotalDurationNs"]) / 1e6,
+                float(row["AverageNs"]) / 1e3, float(row["MinNs"]) / 1e3, float(row["MaxNs"]) / 1e3,
+                float(row["Percentage"])))
+    return "\n".join(lines)
+
+
+def committed_traffic(particles, grid):
+    """HBM bytes per launch of the particle pass from the committed counter
+    summary, when it was measured on this same workload."""
+    try:
+        text = TRAFFIC_FILE.read_text()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+    w = re.search(r"^workload: particles=(\d+) grid=(\d+)", text, re.M)
+    if not m or not w or int(w.group(1)) != particles or int(w.group(2)) != grid:
+        return None
+    return int(m.group(1))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,7 +217,20 @@ def main():
                     help="cell-sort the particle store every K steps (inside the timed region)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the timed steps, run plain copies of this many MiB (counter calibration)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print a markdown table of a rocprofv3 --kernel-trace --stats csv output and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the calibrated FETCH_SIZE/WRITE_SIZE summary of two rocprofv3 --pmc csv "
+                         "outputs (with --copy-calibration-mib) and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
+        return
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc, args.copy_calibration_mib))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): the "
@@ -172,8 +283,15 @@ def main():
     value = live_mean * args.steps / elapsed
     Nc = args.grid * args.grid
     kern_avg_s = kern_ms / max(launches, 1) * 1e-3
-    bytes_per_launch = 40.0 * live_mean + 24.0 * Nc  # 2 x 20 B SoA state + 4 B x 2 potentials + 16 B deposit grid
+    # algorithmic bytes: the particle pass reads and writes the 20 B SoA state
+    # of every live ion and reads the two FP32 potentials (u, uRF) once; the
+    # deposit grid's zero + flush (16 B per node) belongs to the tile reduction
+    bytes_per_launch = 40.0 * live_mean + 8.0 * Nc
+    bytes_per_step = 40.0 * live_mean + 24.0 * Nc
     achieved = bytes_per_launch / kern_avg_s / 1e9
+    if args.copy_calibration_mib:
+        copy_calibration(ctx, args.copy_calibration_mib)
+    traffic = committed_traffic(N, args.grid)
     out = {
         "metric": METRIC,
         "value": value,
@@ -194,8 +312,11 @@ def main():
                    "sort_every": args.sort_every,
                    "device_flags": flags_dev},
         "roofline": {"kernel": "k_advance2d", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel_avg_us": kern_avg_s * 1e6, "bytes_per_launch": bytes_per_launch},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel_avg_us": kern_avg_s * 1e6, "bytes_per_launch": bytes_per_launch,
+                     "launches_timed": launches,
+                     "step_bytes": bytes_per_step,
+                     "step_achieved_GBs": bytes_per_step / (elapsed / args.steps) / 1e9},
         "poisson": poisson,
     }
     if not args.no_cpu:
