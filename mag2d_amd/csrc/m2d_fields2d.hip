@@ -772,6 +772,10 @@ int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_
         (r = dmalloc((void **)&f->gx, sizeof(float2) * (size_t)(g.nx + 1) * g.nz)) ||
         (r = dmalloc((void **)&f->gz, sizeof(float2) * (size_t)g.nx * (g.nz + 1))))
         return fail(r);
+    // sort tiles of 8 x 8 cells; the deposit window of tile (ti, tj) spans
+    // cells [8 ti - 4, 8 ti + 12) x [8 tj - 4, 8 tj + 12)
+    const int ntx = (g.nx - 2) / kTile + 1, ntz = (g.nz - 2) / kTile + 1;
+    if ((r = dmalloc((void **)&f->winfree, (size_t)ntx * ntz))) return fail(r);
     std::vector<uint8_t> cf(n, 0);
     for (int i = 0; i + 1 < g.nx; i++)
         for (int j = 0; j + 1 < g.nz; j++) {
@@ -779,7 +783,17 @@ int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_
             cf[k] = mask[k] == M2D_FREE || mask[k + g.nz] == M2D_FREE || mask[k + 1] == M2D_FREE ||
                     mask[k + g.nz + 1] == M2D_FREE;
         }
+    std::vector<uint8_t> wf((size_t)ntx * ntz, 0);
+    for (int ti = 0; ti < ntx; ti++)
+        for (int tj = 0; tj < ntz; tj++) {
+            const int i0 = ti * kTile - kHalo, j0 = tj * kTile - kHalo;
+            bool all = i0 >= 0 && j0 >= 0 && i0 + kWin - 2 <= g.nx - 2 && j0 + kWin - 2 <= g.nz - 2;
+            for (int i = i0; all && i < i0 + kWin - 1; i++)
+                for (int j = j0; all && j < j0 + kWin - 1; j++) all = cf[(size_t)i * g.nz + j] != 0;
+            wf[(size_t)ti * ntz + tj] = all;
+        }
     if (hipMemcpy(f->mask, mask, n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->winfree, wf.data(), wf.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(f->cellfree, cf.data(), n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(f->voltage, voltage, 8 * n, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_error(M2D_ERR_HIP, "m2d_fields2d_create: upload failed"));
@@ -798,7 +812,7 @@ int m2d_fields2d_destroy(m2d_fields2d *f)
 {
     if (!f) return M2D_OK;
     if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
-    for (void *p : {(void *)f->mask, (void *)f->cellfree, (void *)f->voltage, (void *)f->u,
+    for (void *p : {(void *)f->mask, (void *)f->cellfree, (void *)f->winfree, (void *)f->voltage, (void *)f->u,
                     (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz})
         if (p) (void)hipFree(p);
     mg_free(f->mg);

@@ -28,7 +28,8 @@ constexpr int kTile = 8;               // cells per side of the sort tile
 constexpr int kTileShift = 6;          // key = tile << 6 | cell in tile
 constexpr int kHalo = 4;               // deposit window halo (cells) around a tile
 constexpr int kWin = kTile + 1 + 2 * kHalo;  // window side in nodes
-constexpr int kChunkCap = 2040;        // particles per block of the tiled pass (2 staged rounds)
+constexpr int kChunkCap = 1984;        // particles per block of the tiled pass: with its start
+                                       // rounded down to 64 slots, two staged rounds of 1024
 
 // ---------------------------------------------------------------- errors
 int set_error(int code, const char *fmt, ...);

@@ -75,6 +75,7 @@ struct m2d_fields2d {
     size_t n = 0;
     uint8_t *mask = nullptr;      // device copy of t_grid::mask
     uint8_t *cellfree = nullptr;  // nx*nz: 1 if any corner of cell (i,j) is FREE
+    uint8_t *winfree = nullptr;   // per sort tile: 1 if every cell of its deposit window is free
     double *voltage = nullptr, *u = nullptr, *urf = nullptr, *rho = nullptr;
     float2 *gx = nullptr;         // (nx+1)*nz edge x-gradients {u, uRF}, padded rows 0 and nx
     float2 *gz = nullptr;         // nx*(nz+1) edge z-gradients {u, uRF}, padded cols 0 and nz

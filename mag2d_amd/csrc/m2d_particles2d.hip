@@ -17,9 +17,13 @@ struct PushArgs {
     int64_t n;
     const float2 *gx, *gz;
     const uint8_t *cellfree;
+    const uint8_t *winfree;        // per sort tile: whole deposit window free
     int nx, nz;
     double idx, idz, xmin, zmin;
-    double bx0, bx1, bz0, bz1;     // box [x_min,x_max] x [z_min,z_max]
+    // box [x_min,x_max] x [z_min,z_max] as the FP32 bounds that give the same
+    // answer for FP32 positions; x_max, z_max in FP64 for the periodic wrap
+    float fx0, fx1, fz0, fz1;
+    double xper, zper;
     float qmdt2;                   // q dt / (2 m)
     float tx, ty, tz, sx, sy, sz;  // Boris rotation vectors
     float dt;
@@ -27,7 +31,8 @@ struct PushArgs {
     float ecx, ecz;                // constant field (EMPTY geometry, fields.hpp:126-131)
     int econst;
     int boundary, periodic, checkfree;
-    double prob;                   // 1 - exp(-dt/lifetime)
+    uint64_t trial_thr;            // trial word w fires iff w < trial_thr (uni() < 1 - exp(-dt/lifetime))
+    int rotate;                    // B != 0
     uint64_t seed;
     uint32_t stream;
     uint64_t step;
@@ -260,7 +265,7 @@ __device__ void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, float &v
 // exactly 2^31: charge is conserved bit for bit and node sums do not depend
 // on the order of the particles.
 __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
-                                            long long w[4])
+                                            uint32_t w[4])
 {
     if (!(X >= 0.0) || !(Z >= 0.0)) return false;
     i = (int)X;
@@ -268,15 +273,18 @@ __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, 
     if (i > nx - 2 || j > nz - 2) return false;
     const float u = (float)(X - i), v = (float)(Z - j);
     const float S = 2147483648.0f;  // 2^31
-    w[0] = (long long)__float2uint_rn((1.f - u) * (1.f - v) * S);
-    w[1] = (long long)__float2uint_rn(u * (1.f - v) * S);
-    w[2] = (long long)__float2uint_rn((1.f - u) * v * S);
-    w[3] = (long long)__float2uint_rn(u * v * S);
-    long long s = w[0] + w[1] + w[2] + w[3];
+    w[0] = __float2uint_rn((1.f - u) * (1.f - v) * S);
+    w[1] = __float2uint_rn(u * (1.f - v) * S);
+    w[2] = __float2uint_rn((1.f - u) * v * S);
+    w[3] = __float2uint_rn(u * v * S);
+    // each weight is at most 2^31, so the sum stays below 2^32
+    const uint32_t d = 0x80000000u - (w[0] + w[1] + w[2] + w[3]);
     int km = 0;
+    uint32_t wm = w[0];
 #pragma unroll
-    for (int q = 1; q < 4; q++) km = w[q] > w[km] ? q : km;
-    w[km] += (1ll << 31) - s;
+    for (int q = 1; q < 4; q++)
+        if (w[q] > wm) { km = q; wm = w[q]; }
+    w[km] += d;  // modulo 2^32: the correction is a few units of either sign
     return true;
 }
 
@@ -364,18 +372,21 @@ k_advance2d(PushArgs a)
     __shared__ float st[5][kRound + kRound / 32];
     __shared__ unsigned long long win[TILED ? kWin * kWin : 1];
     const int tid = threadIdx.x;
-    // slots [lo, hi) of this block; rounds start at lo rounded down to 4 so
-    // each thread's 4 slots share one trial Philox block
+    // slots [lo, hi) of this block; rounds start at lo rounded down to 64,
+    // so every wavefront moves whole aligned 256-byte segments and each
+    // thread's 4 slots share one trial Philox block
     int64_t lo, hi, co0;
     int wi0 = 0, wj0 = 0;
+    bool wfree = false;
     if constexpr (TILED) {
         if (blockIdx.x >= *a.n_chunks) return;  // uniform: before any barrier
         const int4 s = a.sched[blockIdx.x];
         lo = s.y;
         hi = s.z;
-        co0 = lo & ~(int64_t)3;
+        co0 = lo & ~(int64_t)63;
         wi0 = (s.x / a.ntz) * kTile - kHalo;
         wj0 = (s.x % a.ntz) * kTile - kHalo;
+        wfree = a.winfree[s.x] != 0;
         for (int k = tid; k < kWin * kWin; k += kBlock) win[k] = 0ull;
     } else {
         lo = (int64_t)blockIdx.x * kChunk;
@@ -427,12 +438,14 @@ k_advance2d(PushArgs a)
             gather_E(a, (double)x, (double)z, ex, ez);
             vx = fmaf(ex, a.qmdt2, vx);
             vz = fmaf(ez, a.qmdt2, vz);
-            float vpr = vx - vy * a.tz + vz * a.ty;
-            float vpz = vz - vx * a.ty + vy * a.tx;
-            float vpt = vy - vz * a.tx + vx * a.tz;
-            vx = vx - vpt * a.sz + vpz * a.sy;
-            vy = vy - vpz * a.sx + vpr * a.sz;
-            vz = vz - vpr * a.sy + vpt * a.sx;
+            if (a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
+                float vpr = vx - vy * a.tz + vz * a.ty;
+                float vpz = vz - vx * a.ty + vy * a.tx;
+                float vpt = vy - vz * a.tx + vx * a.tz;
+                vx = vx - vpt * a.sz + vpz * a.sy;
+                vy = vy - vpz * a.sx + vpr * a.sz;
+                vz = vz - vpr * a.sy + vpt * a.sx;
+            }
             vx = fmaf(ex, a.qmdt2, vx);
             vz = fmaf(ez, a.qmdt2, vz);
             x = fmaf(vx, a.dt, x);
@@ -443,7 +456,7 @@ k_advance2d(PushArgs a)
             // or the deposit below: the slot is queued and k_scatter2d applies
             // BaseSpecies::scatter afterwards.
             if (MCC) {
-                if (((double)twq + 0.5) * 2.3283064365386963e-10 < a.prob) {
+                if ((uint64_t)twq < a.trial_thr) {
                     unsigned long long at = atomicAdd(a.cq_len, 1ull);
                     a.cq[at] = (uint32_t)(pb + q);
                     n_coll++;
@@ -453,14 +466,14 @@ k_advance2d(PushArgs a)
             // ---- advance_boundary (particles.hpp:376-406)
             bool dead = false;
             if (a.boundary) {
-                bool inside = x >= a.bx0 && x <= a.bx1 && z >= a.bz0 && z <= a.bz1;
+                bool inside = x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1;
                 if (!inside) {
                     if (x != x || z != z) { dead = true; nan_any = true; }
                     else if (!a.periodic) dead = true;
                     else {
-                        double xm = fmod((double)x, a.bx1), zm = fmod((double)z, a.bz1);
-                        if (xm < 0) xm += a.bx1;
-                        if (zm < 0) zm += a.bz1;
+                        double xm = fmod((double)x, a.xper), zm = fmod((double)z, a.zper);
+                        if (xm < 0) xm += a.xper;
+                        if (zm < 0) zm += a.zper;
                         x = (float)xm; z = (float)zm;
                     }
                 }
@@ -470,7 +483,12 @@ k_advance2d(PushArgs a)
                 bool free_ = false;
                 if (X >= 0.0 && Z >= 0.0) {
                     int ci = (int)X, cj = (int)Z;
-                    if (ci <= a.nx - 1 && cj <= a.nz - 1) free_ = a.cellfree[ci * a.nz + cj] != 0;
+                    // a cell of an all-free window needs no table lookup
+                    if (wfree && (unsigned)(ci - wi0) < (unsigned)(kWin - 1) &&
+                        (unsigned)(cj - wj0) < (unsigned)(kWin - 1))
+                        free_ = true;
+                    else if (ci <= a.nx - 1 && cj <= a.nz - 1)
+                        free_ = a.cellfree[ci * a.nz + cj] != 0;
                 }
                 dead = !free_;
             }
@@ -482,7 +500,7 @@ k_advance2d(PushArgs a)
             // ---- deposit (particles.hpp:408-409), summed per run of equal cells
             if (DEPOSIT) {
                 int ci, cj;
-                long long w[4];
+                uint32_t w[4];
                 if (cic_weights(X, Z, a.nx, a.nz, ci, cj, w)) {
                     if (ci == run.ci && cj == run.cj) {
                         run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
@@ -530,15 +548,53 @@ k_advance2d(PushArgs a)
     if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
 }
 
+// BaseSpecies::scatter for the slots queued by k_advance2d, threads
+// [start, start + stride, ...); each slot draws from its own Philox stream,
+// so the queue order does not matter.  Thread 0 clears the other queue
+// counter for the next launch.
+__device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long long start,
+                                              unsigned long long stride, unsigned long long *next_len)
+{
+    const unsigned long long len = *a.cq_len;
+    if (start == 0) *next_len = 0ull;
+    for (unsigned long long k = start; k < len; k += stride) {
+        const uint32_t p = a.cq[k];
+        if (is_dead(a.x[p])) continue;  // removed by the boundary after its trial
+        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+        Rng rg;
+        rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
+        scatter(*a.mcc, rg, vx, vy, vz, a.counters);
+        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+    }
+}
+
+struct ReduceArgs {
+    const unsigned long long *partials;
+    const int *chunk_first, *chunk_count;
+    int ntx, ntz;
+    unsigned long long *acc;
+    double *rho;
+    double charge;
+    int nx, nz;
+};
+
 // Node sums of the tiled pass: every node of an 8 x 8 output tile adds the
 // windows of the chunks of the 3 x 3 neighbouring sort tiles that cover it,
 // plus the fallback atomics in acc; then rho += charge * sum * 2^-31 and acc
-// is cleared (the flush of the untiled pass).
-__global__ void __launch_bounds__(64) k_reduce_tiles(const unsigned long long *partials,
-                                                     const int *chunk_first, const int *chunk_count,
-                                                     int ntx, int ntz, unsigned long long *acc,
-                                                     double *rho, double charge, int nx, int nz)
+// is cleared (the flush of the untiled pass).  With MCC, one extra row of
+// blocks applies the collision queue meanwhile.
+template <bool MCC>
+__global__ void __launch_bounds__(64) k_reduce_tiles(ReduceArgs r, PushArgs a, unsigned long long *next_len)
 {
+    if (MCC && blockIdx.y == gridDim.y - 1) {
+        scatter_queue(a, (unsigned long long)blockIdx.x * 64 + threadIdx.x,
+                      (unsigned long long)gridDim.x * 64, next_len);
+        return;
+    }
+    const unsigned long long *partials = r.partials;
+    const int *chunk_first = r.chunk_first, *chunk_count = r.chunk_count;
+    const int ntx = r.ntx, ntz = r.ntz, nx = r.nx, nz = r.nz;
+    unsigned long long *acc = r.acc;
     const int TI = blockIdx.y, TJ = blockIdx.x;
     const int i = TI * kTile + threadIdx.x / kTile, j = TJ * kTile + threadIdx.x % kTile;
     if (i >= nx || j >= nz) return;
@@ -556,29 +612,16 @@ __global__ void __launch_bounds__(64) k_reduce_tiles(const unsigned long long *p
         }
     }
     if (s) {
-        rho[k] += charge * ((double)s * kInvFix);
+        r.rho[k] += r.charge * ((double)s * kInvFix);
         if (acc[k]) acc[k] = 0ull;
     }
 }
 
-// BaseSpecies::scatter for the slots queued by k_advance2d; each slot draws
-// from its own Philox stream, so the queue order does not matter.  Clears the
-// other queue counter for the next launch.
+// the collision queue of an untiled launch
 __global__ void __launch_bounds__(kBlock) k_scatter2d(PushArgs a, unsigned long long *next_len)
 {
-    const unsigned long long len = *a.cq_len;
-    const unsigned long long start = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    if (start == 0) *next_len = 0ull;
-    for (unsigned long long k = start; k < len; k += stride) {
-        const uint32_t p = a.cq[k];
-        if (is_dead(a.x[p])) continue;  // removed by the boundary after its trial
-        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
-        Rng rg;
-        rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
-        scatter(*a.mcc, rg, vx, vy, vz, a.counters);
-        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
-    }
+    scatter_queue(a, (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x,
+                  (unsigned long long)gridDim.x * blockDim.x, next_len);
 }
 
 // Species<CARTESIAN>::advance_boris_init (particles.cpp:997-1050)
@@ -627,9 +670,11 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
     float x = xs[p];
     if (is_dead(x)) return;
     int i, j;
-    long long w[4];
-    if (cic_weights(((double)x - xmin) * idx, ((double)zs[p] - zmin) * idz, nx, nz, i, j, w))
-        deposit_global(acc, nz, i, j, w);
+    uint32_t w[4];
+    if (cic_weights(((double)x - xmin) * idx, ((double)zs[p] - zmin) * idz, nx, nz, i, j, w)) {
+        const long long ww[4] = {w[0], w[1], w[2], w[3]};
+        deposit_global(acc, nz, i, j, ww);
+    }
     else
         raise_flag(counters, M2D_FLAG_OUT_OF_RANGE);
 }
@@ -913,21 +958,48 @@ static int flush(m2d_ctx *ctx, size_t n, double *rho, double charge)
     return M2D_OK;
 }
 
+// the smallest float >= v and the largest float <= v: for a float x,
+// x >= v iff x >= f32_at_least(v), and x <= v iff x <= f32_at_most(v)
+static float f32_at_least(double v)
+{
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return f;
+}
+static float f32_at_most(double v)
+{
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// The trial draw is uni() = (w + 1/2) 2^-32 for a 32-bit word w; since
+// scaling by 2^32 is exact, uni() < prob iff w < ceil(prob 2^32 - 1/2).
+static uint64_t trial_threshold(double prob)
+{
+    if (!(prob > 0.0)) return 0;
+    if (prob >= 1.0) return 1ull << 32;
+    return (uint64_t)ceil(prob * 4294967296.0 - 0.5);
+}
+
 static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
                            const m2d_particles2d *p, double rot_scale, double kick_scale)
 {
     const m2d_fields2d_params &P = f->p;
     a.x = p->x; a.z = p->z; a.vx = p->vx; a.vy = p->vy; a.vz = p->vz; a.n = p->n;
-    a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree;
+    a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree; a.winfree = f->winfree;
     a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
     a.xmin = P.grid.xmin; a.zmin = P.grid.zmin;
-    a.bx0 = P.grid.xmin; a.bx1 = P.x_max; a.bz0 = P.grid.zmin; a.bz1 = P.z_max;
+    a.fx0 = f32_at_least(P.grid.xmin); a.fx1 = f32_at_most(P.x_max);
+    a.fz0 = f32_at_least(P.grid.zmin); a.fz1 = f32_at_most(P.z_max);
+    a.xper = P.x_max; a.zper = P.z_max;
     // rotation: t = B q dt/(2m) scaled (particles.cpp:959-975, init: -0.5, 1028)
     double tmp = rot_scale * s->charge * s->dt / (2.0 * s->mass);
     double tx = P.Br * tmp, ty = P.Bt * tmp, tz = P.Bz * tmp;
     double sc = 2.0 / (1 + tx * tx + ty * ty + tz * tz);
     a.tx = (float)tx; a.ty = (float)ty; a.tz = (float)tz;
     a.sx = (float)(tx * sc); a.sy = (float)(ty * sc); a.sz = (float)(tz * sc);
+    a.rotate = tx != 0.0 || ty != 0.0 || tz != 0.0;
     a.qmdt2 = (float)(kick_scale * s->charge / s->mass * s->dt / 2.0);
     a.dt = (float)s->dt;
     // Fields::E (fields.hpp:124-150); phase hoisted: t = niter dt for every particle
@@ -945,7 +1017,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.phase = (float)phase;
     a.boundary = 0; a.periodic = P.boundary == M2D_BC_PERIODIC;
     a.checkfree = !P.electric_field_from_file;
-    a.prob = 0; a.seed = f->ctx->seed; a.stream = s->stream; a.step = s->niter;
+    a.trial_thr = 0; a.seed = f->ctx->seed; a.stream = s->stream; a.step = s->niter;
     a.mcc = nullptr;
     a.cq = nullptr;
     a.cq_len = nullptr;
@@ -983,7 +1055,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     bool use_mcc = mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
     unsigned long long *next_len = nullptr;
     if (use_mcc) {
-        a.prob = 1.0 - exp(-s->dt / s->lifetime);  // particles.cpp:937
+        a.trial_thr = trial_threshold(1.0 - exp(-s->dt / s->lifetime));  // particles.cpp:937
         a.mcc = mcc->d;
         M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
         a.cq = (uint32_t *)ctx->cq.ptr;
@@ -1023,16 +1095,19 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         }
         M2D_HIP(hipGetLastError());
         prof_end(ctx, t0);
-        if (use_mcc) {
+        if (use_mcc && !T) {
             k_scatter2d<<<64, kBlock, 0, ctx->stream>>>(a, next_len);
             M2D_HIP(hipGetLastError());
         }
     }
     if (T) {
-        dim3 g((f->nz + kTile - 1) / kTile, (f->nx + kTile - 1) / kTile);
-        k_reduce_tiles<<<g, kTile * kTile, 0, ctx->stream>>>(a.partials, T->chunk_first, T->chunk_count,
-                                                             T->ntx, T->ntz, a.acc, rho, s->charge,
-                                                             f->nx, f->nz);
+        // the tile reduction; with MCC its last row of blocks runs the scatter
+        const bool sc = use_mcc && p->n > 0;
+        dim3 g((f->nz + kTile - 1) / kTile, (f->nx + kTile - 1) / kTile + (sc ? 1 : 0));
+        ReduceArgs ra{a.partials, T->chunk_first, T->chunk_count, T->ntx, T->ntz, a.acc, rho, s->charge,
+                      f->nx, f->nz};
+        if (sc) k_reduce_tiles<true><<<g, kTile * kTile, 0, ctx->stream>>>(ra, a, next_len);
+        else k_reduce_tiles<false><<<g, kTile * kTile, 0, ctx->stream>>>(ra, a, next_len);
         M2D_HIP(hipGetLastError());
     } else if (deposit) {
         M2D_TRY(flush(ctx, f->n, rho, s->charge));
