@@ -92,6 +92,9 @@ int m2d_memcpy_d2h(m2d_ctx *ctx, void *dst, const void *src, size_t bytes);   /*
 int m2d_memcpy_d2d(m2d_ctx *ctx, void *dst, const void *src, size_t bytes);   /* stream-ordered */
 /* Fill with a byte value by a kernel (no hipMemsetAsync: safe under capture). */
 int m2d_memset(m2d_ctx *ctx, void *dst, int value, size_t bytes);
+/* dst[k] += scale * src[k] for k < n, device arrays (Array2D::add with
+ * scale 1, Array.hpp:51-58; Pic::advance sums species rho this way, pic.cpp:351-355) */
+int m2d_field_add(m2d_ctx *ctx, double *dst, const double *src, double scale, size_t n);
 
 /* ======================= grid and geometry (host) ======================= */
 
@@ -260,7 +263,8 @@ int m2d_mcc_set_partner(m2d_mcc *m, int32_t target, const m2d_particles2d *p);
 
 /* ======================= species step ==================================== */
 
-#define M2D_STEP_PUSH      1u   /* advance_boris incl. the MCC trial (particles.cpp:924-995) */
+#define M2D_STEP_PUSH      1u   /* advance_boris incl. the MCC trial (particles.cpp:924-995);
+                                   without it the pass is advance_boundary alone */
 #define M2D_STEP_BOUNDARY  2u   /* advance_boundary (particles.hpp:370-406) */
 #define M2D_STEP_DEPOSIT   4u   /* rho.accumulate in advance_boundary (particles.hpp:408-409) */
 
@@ -290,7 +294,7 @@ int m2d_count_live_2d(m2d_ctx *ctx, const m2d_particles2d *p, int64_t *n_live);
  * the free list of BaseSpecies::remove/insert (particles.hpp:223-247). */
 int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles2d *dst,
                    int64_t *n_live);
-/* Rebuild the store in tile-major cell order (16 x 16-cell tiles, row-major
+/* Rebuild the store in tile-major cell order (8 x 8-cell tiles, row-major
  * cells inside a tile; equal cells keep their slot order) into dst, empty
  * slots dropped to the end; *n_live (may be NULL) gets the live count.
  * Replaces the insertion-order vector<t_particle> with holes

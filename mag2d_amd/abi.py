@@ -117,6 +117,7 @@ _SIGS = {
     "m2d_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "m2d_memcpy_d2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "m2d_memset": (C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    "m2d_field_add": (C.c_int, [_P, _P, _P, C.c_double, C.c_size_t]),
     "m2d_grid_build_2d": (C.c_int, [C.POINTER(Grid2D), C.POINTER(Geom2D), _PU8, _PD]),
     "m2d_grid_square_electrode_2d": (C.c_int, [C.POINTER(Grid2D), _PU8, _PD, C.c_double, C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.c_int]),

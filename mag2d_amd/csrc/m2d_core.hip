@@ -1,6 +1,7 @@
 // m2d_core.hip -- context, error reporting, device memory, host geometry
 // builders (t_grid) of libmag2d_amd.
 #include "m2d_internal.hpp"
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -119,6 +120,15 @@ void prof_end(m2d_ctx *ctx, hipEvent_t start)
 }  // namespace m2d
 
 using namespace m2d;
+
+namespace m2d {
+// dst += scale * src (Array2D::add and Array2D::multiply, Array.hpp:51-72)
+__global__ void k_axpy(double *dst, const double *src, double scale, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] += scale * src[k];
+}
+}  // namespace m2d
 
 extern "C" {
 
@@ -302,6 +312,16 @@ int m2d_memset(m2d_ctx *c, void *dst, int value, size_t bytes)
 {
     M2D_CHECK(c && (bytes == 0 || dst), M2D_ERR_BAD_ARG, "m2d_memset: NULL argument");
     return fill_bytes(c, dst, value, bytes);
+}
+
+int m2d_field_add(m2d_ctx *c, double *dst, const double *src, double scale, size_t n)
+{
+    M2D_CHECK(c && (n == 0 || (dst && src)), M2D_ERR_BAD_ARG, "m2d_field_add: NULL argument");
+    if (n == 0) return M2D_OK;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    k_axpy<<<g, 256, 0, c->stream>>>(dst, src, scale, n);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
 }
 
 // ------------------------------------------------------------------ t_grid

@@ -33,6 +33,7 @@ struct PushArgs {
     int boundary, periodic, checkfree;
     uint64_t trial_thr;            // trial word w fires iff w < trial_thr (uni() < 1 - exp(-dt/lifetime))
     int rotate;                    // B != 0
+    int push;                      // advance_boris; 0: boundary and deposit only
     uint64_t seed;
     uint32_t stream;
     uint64_t step;
@@ -433,33 +434,35 @@ k_advance2d(PushArgs a)
             float z = st[1][li], vx = st[2][li], vy = st[3][li], vz = st[4][li];
             const uint32_t twq = q == 0 ? tw0 : q == 1 ? tw1 : q == 2 ? tw2 : tw3;
 
-            // ---- advance_boris (particles.cpp:945-988)
-            float ex, ez;
-            gather_E(a, (double)x, (double)z, ex, ez);
-            vx = fmaf(ex, a.qmdt2, vx);
-            vz = fmaf(ez, a.qmdt2, vz);
-            if (a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
-                float vpr = vx - vy * a.tz + vz * a.ty;
-                float vpz = vz - vx * a.ty + vy * a.tx;
-                float vpt = vy - vz * a.tx + vx * a.tz;
-                vx = vx - vpt * a.sz + vpz * a.sy;
-                vy = vy - vpz * a.sx + vpr * a.sz;
-                vz = vz - vpr * a.sy + vpt * a.sx;
-            }
-            vx = fmaf(ex, a.qmdt2, vx);
-            vz = fmaf(ez, a.qmdt2, vz);
-            x = fmaf(vx, a.dt, x);
-            z = fmaf(vz, a.dt, z);
+            if (a.push) {
+                // ---- advance_boris (particles.cpp:945-988)
+                float ex, ez;
+                gather_E(a, (double)x, (double)z, ex, ez);
+                vx = fmaf(ex, a.qmdt2, vx);
+                vz = fmaf(ez, a.qmdt2, vz);
+                if (a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
+                    float vpr = vx - vy * a.tz + vz * a.ty;
+                    float vpz = vz - vx * a.ty + vy * a.tx;
+                    float vpt = vy - vz * a.tx + vx * a.tz;
+                    vx = vx - vpt * a.sz + vpz * a.sy;
+                    vy = vy - vpz * a.sx + vpr * a.sz;
+                    vz = vz - vpr * a.sy + vpt * a.sx;
+                }
+                vx = fmaf(ex, a.qmdt2, vx);
+                vz = fmaf(ez, a.qmdt2, vz);
+                x = fmaf(vx, a.dt, x);
+                z = fmaf(vz, a.dt, z);
 
-            // ---- MCC trial (particles.cpp:990-993).  A hit only changes the
-            // velocity after the drift, so it cannot change the boundary test
-            // or the deposit below: the slot is queued and k_scatter2d applies
-            // BaseSpecies::scatter afterwards.
-            if (MCC) {
-                if ((uint64_t)twq < a.trial_thr) {
-                    unsigned long long at = atomicAdd(a.cq_len, 1ull);
-                    a.cq[at] = (uint32_t)(pb + q);
-                    n_coll++;
+                // ---- MCC trial (particles.cpp:990-993).  A hit only changes the
+                // velocity after the drift, so it cannot change the boundary test
+                // or the deposit below: the slot is queued and k_scatter2d applies
+                // BaseSpecies::scatter afterwards.
+                if (MCC) {
+                    if ((uint64_t)twq < a.trial_thr) {
+                        unsigned long long at = atomicAdd(a.cq_len, 1ull);
+                        a.cq[at] = (uint32_t)(pb + q);
+                        n_coll++;
+                    }
                 }
             }
 
@@ -1000,6 +1003,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.tx = (float)tx; a.ty = (float)ty; a.tz = (float)tz;
     a.sx = (float)(tx * sc); a.sy = (float)(ty * sc); a.sz = (float)(tz * sc);
     a.rotate = tx != 0.0 || ty != 0.0 || tz != 0.0;
+    a.push = 1;
     a.qmdt2 = (float)(kick_scale * s->charge / s->mass * s->dt / 2.0);
     a.dt = (float)s->dt;
     // Fields::E (fields.hpp:124-150); phase hoisted: t = niter dt for every particle
@@ -1044,15 +1048,17 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     M2D_CHECK(s->mass > 0 && s->dt > 0, M2D_ERR_BAD_ARG, "m2d_advance_2d: mass and dt must be > 0");
     bool deposit = (flags & M2D_STEP_DEPOSIT) != 0;
     M2D_CHECK(!deposit || rho, M2D_ERR_BAD_ARG, "m2d_advance_2d: DEPOSIT needs rho");
-    M2D_CHECK(flags & M2D_STEP_PUSH, M2D_ERR_UNSUPPORTED,
-              "m2d_advance_2d: PUSH is required (use m2d_boundary paths via flags)");
+    M2D_CHECK((flags & ~(M2D_STEP_PUSH | M2D_STEP_BOUNDARY | M2D_STEP_DEPOSIT)) == 0, M2D_ERR_BAD_ARG,
+              "m2d_advance_2d: unknown flags 0x%x", flags);
     if (stats) {
         M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
     }
     PushArgs a;
     fill_push_args(a, f, s, p, 1.0, 1.0);
     a.boundary = (flags & M2D_STEP_BOUNDARY) != 0;
-    bool use_mcc = mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
+    a.push = (flags & M2D_STEP_PUSH) != 0;
+    // the MCC trial belongs to advance_boris
+    bool use_mcc = a.push && mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
     unsigned long long *next_len = nullptr;
     if (use_mcc) {
         a.trial_thr = trial_threshold(1.0 - exp(-s->dt / s->lifetime));  // particles.cpp:937

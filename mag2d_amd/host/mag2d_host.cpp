@@ -1,0 +1,1036 @@
+// mag2d_host.cpp -- the reference class API (mag2d.hpp) over the C ABI, and
+// two C entry points that exercise it end to end: a host-only check of the
+// slot container, and a small trap run through Pic.
+#include "mag2d.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+namespace mag2d {
+
+void check(int status)
+{
+    if (status != M2D_OK) throw std::runtime_error(m2d_last_error());
+}
+
+// nearest integer of x, which must lie within tol of it (Field2D.cpp:87,121)
+static int double2int(double x, double tol)
+{
+    double r = std::nearbyint(x);
+    if (!(std::fabs(x - r) <= tol)) throw std::runtime_error("double2int: value is not close to an integer");
+    return (int)r;
+}
+
+static inline float dead_float()
+{
+    uint32_t b = M2D_DEAD_BITS;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+static inline bool is_dead(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b == M2D_DEAD_BITS;
+}
+
+// ================================================================ Device
+Device::Device(int device, uint64_t seed) { check(m2d_ctx_create(device, seed, &ctx_)); }
+
+Device::~Device()
+{
+    if (staging_) m2d_free(ctx_, staging_);
+    m2d_ctx_destroy(ctx_);
+}
+
+void Device::synchronize() { check(m2d_ctx_synchronize(ctx_)); }
+
+uint32_t Device::status(bool clear)
+{
+    uint32_t f = 0;
+    check(m2d_ctx_status(ctx_, &f, clear ? 1 : 0));
+    return f;
+}
+
+double *Device::staging(size_t n)
+{
+    if (n > staging_n_) {
+        if (staging_) check(m2d_free(ctx_, staging_));
+        staging_ = nullptr;
+        staging_n_ = 0;
+        size_t want = std::max<size_t>(n, 4096);
+        check(m2d_malloc(ctx_, want * sizeof(double), &staging_));
+        staging_n_ = want;
+    }
+    return (double *)staging_;
+}
+
+DeviceBuffer::DeviceBuffer(std::shared_ptr<Device> dev, size_t bytes) : dev_(std::move(dev)), bytes_(bytes)
+{
+    check(m2d_malloc(dev_->ctx(), std::max<size_t>(bytes, 16), &p_));
+}
+
+DeviceBuffer::~DeviceBuffer()
+{
+    if (p_ && dev_) m2d_free(dev_->ctx(), p_);
+}
+
+DeviceBuffer::DeviceBuffer(DeviceBuffer &&o) noexcept : dev_(std::move(o.dev_)), p_(o.p_), bytes_(o.bytes_)
+{
+    o.p_ = nullptr;
+    o.bytes_ = 0;
+}
+
+DeviceBuffer &DeviceBuffer::operator=(DeviceBuffer &&o) noexcept
+{
+    if (this != &o) {
+        if (p_ && dev_) m2d_free(dev_->ctx(), p_);
+        dev_ = std::move(o.dev_);
+        p_ = o.p_;
+        bytes_ = o.bytes_;
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
+    return *this;
+}
+
+// ================================================================ ParticleList
+void ParticleList::remove(int n)
+{
+    if (particles.at(n).empty) throw std::runtime_error("removing empty particle");
+    particles[n].empty = true;
+    empty_.push_back(n);
+}
+
+int ParticleList::insert()
+{
+    if (empty_.empty()) resize((unsigned int)(particles.size() * 1.1) + 1);
+    int tmp = empty_.back();
+    if (!particles[tmp].empty) throw std::runtime_error("error inserting particle");
+    empty_.pop_back();
+    particles[tmp].empty = false;
+    return tmp;
+}
+
+void ParticleList::resize(unsigned int newsize)
+{
+    size_t old = particles.size();
+    if (newsize <= old) return;
+    particles.resize(newsize);
+    for (size_t i = old; i < newsize; i++) {
+        particles[i] = t_particle{0, 0, 0, 0, 0, 0, 0, true};
+        empty_.push_back((int)i);
+    }
+}
+
+void ParticleList::rebuild_free_list()
+{
+    empty_.clear();
+    for (size_t i = 0; i < particles.size(); i++)
+        if (particles[i].empty) empty_.push_back((int)i);
+}
+
+void ParticleList::save(const std::string &filename) const
+{
+    std::ofstream fw(filename.c_str(), std::ios::out | std::ios::binary);
+    if (!fw.is_open()) throw std::runtime_error("Species::save(): failed opening file");
+    int size = (int)particles.size();
+    int live = (int)(particles.size() - empty_.size());
+    fw.write((const char *)&size, sizeof(int));
+    fw.write((const char *)&live, sizeof(int));
+    int written = 0;
+    for (const t_particle &p : particles) {
+        if (p.empty) continue;
+        if (++written > live) break;
+        fw.write((const char *)&p, sizeof(t_particle));
+    }
+    if (!fw.good()) throw std::runtime_error("Species::save(): write error");
+}
+
+void ParticleList::load(const std::string &filename)
+{
+    std::ifstream fr(filename.c_str(), std::ios::in | std::ios::binary);
+    if (!fr.is_open()) throw std::runtime_error("Species::load(): failed opening file");
+    int size = 0, live = 0;
+    fr.read((char *)&size, sizeof(int));
+    fr.read((char *)&live, sizeof(int));
+    if (!fr.good() || size < 0 || live < 0 || live > size)
+        throw std::runtime_error("Species::load(): bad header");
+    particles.assign((size_t)size, t_particle{0, 0, 0, 0, 0, 0, 0, true});
+    for (int i = 0; i < live; i++) {
+        fr.read((char *)&particles[i], sizeof(t_particle));
+        if (!fr.good()) throw std::runtime_error("Species::load(): read error");
+        particles[i].empty = false;
+    }
+    empty_.clear();
+    for (int i = live; i < size; i++) empty_.push_back(i);
+}
+
+// ================================================================ Field2D
+Field2D::Field2D(std::shared_ptr<Device> dev, int x_sampl, int y_sampl, double dx, double dy, double xmin,
+                 double ymin)
+    : jmax(x_sampl), lmax(y_sampl), dev_(std::move(dev))
+{
+    if (x_sampl < 2 || y_sampl < 2 || !(dx > 0) || !(dy > 0)) throw std::runtime_error("Field2D: bad shape");
+    g_ = m2d_grid2d{x_sampl, y_sampl, dx, dy, xmin, ymin};
+    const size_t n = (size_t)x_sampl * y_sampl;
+    own_ = DeviceBuffer(dev_, n * sizeof(double));
+    d_ = (double *)own_.get();
+    h_.assign(n, 0.0);
+    check(m2d_memset(dev_->ctx(), d_, 0, n * sizeof(double)));
+    state_ = SYNCED;
+}
+
+Field2D::Field2D(std::shared_ptr<Device> dev, double *device_data, int x_sampl, int y_sampl, double dx,
+                 double dy, double xmin, double ymin)
+    : jmax(x_sampl), lmax(y_sampl), dev_(std::move(dev)), d_(device_data)
+{
+    g_ = m2d_grid2d{x_sampl, y_sampl, dx, dy, xmin, ymin};
+    h_.assign((size_t)x_sampl * y_sampl, 0.0);
+    state_ = DEVICE_NEWER;
+}
+
+void Field2D::pull() const
+{
+    if (state_ == DEVICE_NEWER) {
+        check(m2d_memcpy_d2h(dev_->ctx(), h_.data(), d_, h_.size() * sizeof(double)));
+        state_ = SYNCED;
+    }
+}
+
+double *Field2D::operator[](int i)
+{
+    pull();
+    state_ = HOST_NEWER;
+    return h_.data() + (size_t)i * lmax;
+}
+
+const double *Field2D::operator[](int i) const
+{
+    pull();
+    return h_.data() + (size_t)i * lmax;
+}
+
+double *Field2D::device()
+{
+    if (state_ == HOST_NEWER) upload();
+    return d_;
+}
+
+void Field2D::download() { pull(); }
+
+void Field2D::upload()
+{
+    check(m2d_memcpy_h2d(dev_->ctx(), d_, h_.data(), h_.size() * sizeof(double)));
+    state_ = SYNCED;
+}
+
+void Field2D::reset()
+{
+    check(m2d_memset(dev_->ctx(), d_, 0, h_.size() * sizeof(double)));
+    state_ = DEVICE_NEWER;
+}
+
+void Field2D::add(const Field2D &other)
+{
+    if (other.jmax != jmax || other.lmax != lmax)
+        throw std::runtime_error("Array2D.add(): matrix sizes don't match\n");
+    double *src = const_cast<Field2D &>(other).device();
+    check(m2d_field_add(dev_->ctx(), device(), src, 1.0, h_.size()));
+    state_ = DEVICE_NEWER;
+}
+
+void Field2D::accumulate(double q, double x, double y) { accumulate(q, &x, &y, 1); }
+
+void Field2D::accumulate(double q, const double *x, const double *y, size_t n)
+{
+    if (n == 0) return;
+    // a store of n particles in the staging buffer: x, z, then three zero
+    // velocity arrays (16-byte aligned)
+    const size_t stride = (n + 3) & ~(size_t)3;
+    float *s = (float *)dev_->staging((5 * stride + 1) / 2 + 2);
+    std::vector<float> hs(5 * stride, 0.0f);
+    for (size_t k = 0; k < n; k++) {
+        hs[k] = (float)x[k];
+        hs[stride + k] = (float)y[k];
+    }
+    check(m2d_memcpy_h2d(dev_->ctx(), s, hs.data(), hs.size() * sizeof(float)));
+    m2d_particles2d p{s, s + stride, s + 2 * stride, s + 3 * stride, s + 4 * stride, (int64_t)n};
+    (void)dev_->status(true);
+    check(m2d_accumulate_2d(dev_->ctx(), &g_, &p, q, device()));
+    state_ = DEVICE_NEWER;
+    if (dev_->status(true) & M2D_FLAG_OUT_OF_RANGE)
+        throw std::runtime_error("Field2D::accumulate: outside of range");
+}
+
+void Field2D::interpolate(const double *x, const double *y, size_t n, double *out)
+{
+    if (n == 0) return;
+    double *s = dev_->staging(3 * n);
+    check(m2d_memcpy_h2d(dev_->ctx(), s, x, n * sizeof(double)));
+    check(m2d_memcpy_h2d(dev_->ctx(), s + n, y, n * sizeof(double)));
+    check(m2d_interpolate_2d(dev_->ctx(), &g_, device(), s, s + n, (int64_t)n, s + 2 * n));
+    check(m2d_memcpy_d2h(dev_->ctx(), out, s + 2 * n, n * sizeof(double)));
+}
+
+double Field2D::interpolate(double x, double y)
+{
+    double v = 0;
+    interpolate(&x, &y, 1, &v);
+    return v;
+}
+
+void Field2D::grad(const double *x, const double *y, size_t n, double *gx, double *gy)
+{
+    if (n == 0) return;
+    double *s = dev_->staging(4 * n);
+    check(m2d_memcpy_h2d(dev_->ctx(), s, x, n * sizeof(double)));
+    check(m2d_memcpy_h2d(dev_->ctx(), s + n, y, n * sizeof(double)));
+    check(m2d_grad_2d(dev_->ctx(), &g_, device(), s, s + n, (int64_t)n, s + 2 * n, s + 3 * n));
+    check(m2d_memcpy_d2h(dev_->ctx(), gx, s + 2 * n, n * sizeof(double)));
+    check(m2d_memcpy_d2h(dev_->ctx(), gy, s + 3 * n, n * sizeof(double)));
+}
+
+void Field2D::grad(double x, double y, double &gx, double &gy) { grad(&x, &y, 1, &gx, &gy); }
+
+void Field2D::print(std::ostream &out, double factor)
+{
+    pull();
+    for (int j = 0; j < jmax; j++) {
+        for (int l = 0; l < lmax; l++)
+            out << j * g_.dx + g_.xmin << "\t" << l * g_.dz + g_.zmin << "\t" << h_[(size_t)j * lmax + l] * factor
+                << std::endl;
+        out << std::endl;
+    }
+}
+
+void Field2D::print(const char *filename, double factor)
+{
+    std::ofstream out(filename);
+    print(out, factor);
+}
+
+void Field2D::load(const char *filename)
+{
+    std::ifstream fr(filename);
+    if (fr.fail()) throw std::runtime_error("Field2D::load(): failed opening file\n");
+    std::vector<double> xv, yv, fv;
+    std::string line;
+    while (std::getline(fr, line)) {
+        std::istringstream s(line);
+        double a, b, c;
+        if (s >> a >> b >> c) {
+            xv.push_back(a);
+            yv.push_back(b);
+            fv.push_back(c);
+        }
+    }
+    if (xv.size() < 2) throw std::runtime_error("Field2D::load(): no data");
+    // grid spacing and extent from the first change of each coordinate
+    auto axis = [](const std::vector<double> &v, double &d, double &vmin, double &vmax) {
+        vmin = v.front();
+        vmax = v.back();
+        size_t i = 1;
+        while (i < v.size() && v[i] - v[i - 1] == 0.0) i++;
+        if (i == v.size()) throw std::runtime_error("Field2D::load(): constant coordinate");
+        d = v[i] - v[i - 1];
+        if (d < 0) {
+            d = -d;
+            std::swap(vmin, vmax);
+        }
+    };
+    double dx, xmin, xmax, dy, ymin, ymax;
+    axis(xv, dx, xmin, xmax);
+    axis(yv, dy, ymin, ymax);
+    const int xs = double2int((xmax - xmin) / dx + 1, 1e-1), ys = double2int((ymax - ymin) / dy + 1, 1e-1);
+    if ((size_t)xs * ys != xv.size())
+        throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
+    if (xs != jmax || ys != lmax) throw std::runtime_error("Field2D::load(): file shape differs from the field");
+    std::vector<double> d((size_t)xs * ys, std::nan(""));
+    for (size_t i = 0; i < xv.size(); i++) {
+        int xi = double2int((xv[i] - xmin) / dx, 1e-1), yi = double2int((yv[i] - ymin) / dy, 1e-1);
+        d[(size_t)xi * ys + yi] = fv[i];
+    }
+    for (double v : d)
+        if (std::isnan(v)) throw std::runtime_error("Fields::load_magnetic_field() garbage loaded");
+    h_ = std::move(d);
+    state_ = HOST_NEWER;
+}
+
+// ================================================================ Param
+void Param::derive()
+{
+    dx = x_max / (x_sampl - 1);
+    dz = z_max / (z_sampl - 1);
+    idx = 1.0 / dx;
+    idz = 1.0 / dz;
+    V = n_particles_total / density_total;
+    dV = V / ((double)(x_sampl - 1) * (z_sampl - 1));
+}
+
+void Param::check() const
+{
+    if (selfconsistent && rf) throw std::runtime_error("Param: selfconsistent RF simulation not implemented");
+    if (x_sampl < 3 || z_sampl < 3) throw std::runtime_error("Param: need at least 3 samples per axis");
+}
+
+// ================================================================ t_grid
+t_grid::t_grid(const Param &p) : jmax(p.x_sampl), lmax(p.z_sampl), dx(p.dx), dz(p.dz)
+{
+    if (!(p.dx > 0)) throw std::runtime_error("t_grid: Param::derive() was not called");
+    mask.assign((size_t)jmax * lmax, 0);
+    voltage.assign((size_t)jmax * lmax, 0.0);
+    m2d_geom2d geo{p.geometry, p.x_max, p.z_max, p.probe_radius, p.u_probe, p.extern_field};
+    m2d_grid2d g = grid();
+    check(m2d_grid_build_2d(&g, &geo, mask.data(), voltage.data()));
+}
+
+m2d_grid2d t_grid::grid() const { return m2d_grid2d{jmax, lmax, dx, dz, 0.0, 0.0}; }
+
+void t_grid::square_electrode(double rmin, double rmax, double zmin, double zmax, double v, mask_type type)
+{
+    m2d_grid2d g = grid();
+    check(m2d_grid_square_electrode_2d(&g, mask.data(), voltage.data(), rmin, rmax, zmin, zmax, v, type));
+}
+
+void t_grid::circle_electrode(double x, double z, double r, double v, mask_type type)
+{
+    m2d_grid2d g = grid();
+    check(m2d_grid_circle_electrode_2d(&g, mask.data(), voltage.data(), x, z, r, v, type));
+}
+
+bool t_grid::is_free(double x, double z) const
+{
+    m2d_grid2d g = grid();
+    int r = m2d_grid_is_free_2d(&g, mask.data(), x, z);
+    if (r < 0) check(r);
+    return r != 0;
+}
+
+// ================================================================ Fields
+static m2d_fields2d_params fields_params(const Param &p)
+{
+    m2d_fields2d_params fp{};
+    fp.grid = m2d_grid2d{p.x_sampl, p.z_sampl, p.dx, p.dz, p.x_min, p.z_min};
+    fp.x_max = p.x_max;
+    fp.z_max = p.z_max;
+    fp.geometry_empty = p.geometry == EMPTY;
+    fp.selfconsistent = p.selfconsistent;
+    fp.electric_field_from_file = p.electric_field_from_file;
+    fp.boundary = p.boundary;
+    fp.extern_field = p.extern_field;
+    fp.rf = p.rf;
+    fp.rf_amplitude = p.rf_amplitude;
+    fp.rf_omega = p.rf_omega;
+    fp.rf_U0 = p.rf_U0;
+    fp.Br = p.Br;
+    fp.Bz = p.Bz;
+    fp.Bt = p.Bt;
+    fp.dV = p.dV;
+    return fp;
+}
+
+static m2d_fields2d *create_fields(Param &p, Device &dev, const t_grid &grid)
+{
+    p.check();
+    m2d_fields2d_params fp = fields_params(p);
+    m2d_fields2d *f = nullptr;
+    check(m2d_fields2d_create(dev.ctx(), &fp, grid.mask.data(), grid.voltage.data(), &f));
+    return f;
+}
+
+Fields::Fields(Param &p, std::shared_ptr<Device> dev) : Fields(p, dev, t_grid(p)) {}
+
+Fields::Fields(Param &p, std::shared_ptr<Device> dev, const t_grid &g)
+    : p_(&p), dev_(dev), f_(create_fields(p, *dev, g)), grid(g),
+      u(dev, m2d_fields2d_data(f_, M2D_FIELD_U), p.x_sampl, p.z_sampl, p.dx, p.dz, p.x_min, p.z_min),
+      uRF(dev, m2d_fields2d_data(f_, M2D_FIELD_URF), p.x_sampl, p.z_sampl, p.dx, p.dz, p.x_min, p.z_min),
+      rho(dev, m2d_fields2d_data(f_, M2D_FIELD_RHO), p.x_sampl, p.z_sampl, p.dx, p.dz, p.x_min, p.z_min)
+{
+}
+
+Fields::~Fields()
+{
+    if (f_) m2d_fields2d_destroy(f_);
+}
+
+m2d_fields2d *Fields::handle()
+{
+    // host edits of u / uRF reach the device and the gather tables
+    if (u.host_newer() || uRF.host_newer()) update();
+    return f_;
+}
+
+void Fields::update()
+{
+    (void)u.device();
+    (void)uRF.device();
+    check(m2d_fields2d_update(f_));
+}
+
+void Fields::run_solve(int mode)
+{
+    (void)rho.device();
+    check(m2d_fields2d_solve(handle(), mode, p_->solve_tolerance, &info_));
+    // the rhs is built in place on rho, as the reference does
+    rho.mark_device_written();
+    (mode == M2D_SOLVE_RF ? uRF : u).mark_device_written();
+}
+
+void Fields::boundary_solve() { run_solve(M2D_SOLVE_STATIC); }
+void Fields::boundary_solve_rf() { run_solve(M2D_SOLVE_RF); }
+void Fields::solve() { run_solve(M2D_SOLVE_RAW); }
+void Fields::reset() { rho.reset(); }
+
+void Fields::E(double x, double y, double &gx, double &gy, double t)
+{
+    double *s = dev_->staging(4);
+    m2d_fields2d *f = handle();
+    double pos[2] = {x, y}, out[2];
+    check(m2d_memcpy_h2d(dev_->ctx(), s, pos, sizeof pos));
+    check(m2d_fields2d_E(f, s, s + 1, 1, t, s + 2, s + 3));
+    check(m2d_memcpy_d2h(dev_->ctx(), out, s + 2, sizeof out));
+    gx = out[0];
+    gy = out[1];
+}
+
+void Fields::B(double, double, double &Br, double &Bz, double &Bt) const
+{
+    // magnetic_field_const (fields.hpp:154-169)
+    Br = p_->Br;
+    Bz = p_->Bz;
+    Bt = p_->Bt;
+}
+
+// ================================================================ Species
+Species::Species(const SpeciesParams &sp, Param &param, Fields &f, uint32_t stream)
+    : name(sp.name), particle(sp.particle), mass(sp.mass), charge(sp.charge), temperature(sp.temperature),
+      density(sp.density), dt(sp.dt), t(0), lifetime(INFINITY),
+      E_max(sp.E_max > 0. ? sp.E_max : sp.temperature * physconst::k_B / physconst::q_e * 10.0),
+      v_max(std::sqrt(2.0 * physconst::k_B * sp.temperature / sp.mass)),
+      rho(f.device(), param.x_sampl, param.z_sampl, param.dx, param.dz), field(&f), p_(&param),
+      dev_(f.device()), stream_(stream), params_(sp)
+{
+    if (!(mass > 0) || !(dt > 0)) throw std::runtime_error("Species " + name + ": mass and dt must be > 0");
+}
+
+Species::~Species()
+{
+    if (mcc_) m2d_mcc_destroy(mcc_);
+}
+
+m2d_particles2d Species::store()
+{
+    return m2d_particles2d{(float *)soa_[0].get(), (float *)soa_[1].get(), (float *)soa_[2].get(),
+                           (float *)soa_[3].get(), (float *)soa_[4].get(), cap_};
+}
+
+void Species::reserve_device(int64_t n)
+{
+    if (n == cap_) return;
+    for (int c = 0; c < 5; c++) soa_[c] = DeviceBuffer(dev_, (size_t)std::max<int64_t>(n, 1) * sizeof(float));
+    cap_ = n;
+}
+
+void Species::upload()
+{
+    const int64_t n = (int64_t)particles.size();
+    reserve_device(n);
+    std::vector<float> h[5];
+    for (auto &v : h) v.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const t_particle &p = particles[(size_t)i];
+        h[0][i] = p.empty ? dead_float() : (float)p.x;
+        h[1][i] = (float)p.z;
+        h[2][i] = (float)p.vx;
+        h[3][i] = (float)p.vy;
+        h[4][i] = (float)p.vz;
+    }
+    for (int c = 0; c < 5; c++)
+        if (n) check(m2d_memcpy_h2d(dev_->ctx(), soa_[c].get(), h[c].data(), (size_t)n * sizeof(float)));
+    m2d_particles2d s = store();
+    check(m2d_sort_forget_2d(dev_->ctx(), &s));
+    host_newer_ = dev_newer_ = false;
+}
+
+void Species::download()
+{
+    if (!dev_newer_) return;
+    const int64_t n = cap_;
+    std::vector<float> h[5];
+    for (int c = 0; c < 5; c++) {
+        h[c].resize((size_t)n);
+        if (n) check(m2d_memcpy_d2h(dev_->ctx(), h[c].data(), soa_[c].get(), (size_t)n * sizeof(float)));
+    }
+    // y and time_to_death are not on the device; slots keep them where they persist
+    particles.resize((size_t)n, t_particle{0, 0, 0, 0, 0, 0, 0, true});
+    for (int64_t i = 0; i < n; i++) {
+        t_particle &p = particles[(size_t)i];
+        p.empty = is_dead(h[0][i]);
+        if (p.empty) continue;
+        p.x = h[0][i];
+        p.z = h[1][i];
+        p.vx = h[2][i];
+        p.vy = h[3][i];
+        p.vz = h[4][i];
+    }
+    rebuild_free_list();
+    dev_newer_ = false;
+}
+
+void Species::ensure_device()
+{
+    if (host_newer_) upload();
+}
+
+void Species::ensure_host() { download(); }
+
+int Species::insert()
+{
+    ensure_host();
+    host_newer_ = true;
+    return ParticleList::insert();
+}
+
+void Species::remove(int n)
+{
+    ensure_host();
+    host_newer_ = true;
+    ParticleList::remove(n);
+}
+
+void Species::resize(unsigned int newsize)
+{
+    ensure_host();
+    host_newer_ = true;
+    ParticleList::resize(newsize);
+}
+
+int Species::n_particles()
+{
+    if (!dev_newer_) return ParticleList::n_particles();
+    m2d_particles2d s = store();
+    int64_t live = 0;
+    check(m2d_count_live_2d(dev_->ctx(), &s, &live));
+    return (int)live;
+}
+
+m2d_species2d Species::desc() const
+{
+    m2d_species2d d{};
+    d.charge = charge;
+    d.mass = mass;
+    d.dt = dt;
+    d.lifetime = lifetime;
+    d.niter = niter;
+    d.stream = stream_;
+    return d;
+}
+
+void Species::run_pass(uint32_t flags)
+{
+    ensure_device();
+    if (cap_ == 0) return;
+    for (size_t k = 0; k < partners_.size(); k++)
+        if (partners_[k]) {
+            partners_[k]->ensure_device();
+            m2d_particles2d ps = partners_[k]->store();
+            check(m2d_mcc_set_partner(mcc_, (int32_t)k, &ps));
+        }
+    m2d_species2d d = desc();
+    m2d_particles2d s = store();
+    const bool dep = (flags & M2D_STEP_DEPOSIT) != 0;
+    check(m2d_advance_2d(dev_->ctx(), field->handle(), &d, &s, mcc_, dep ? rho.device() : nullptr, flags,
+                         nullptr));
+    if (dep) rho.mark_device_written();
+    dev_newer_ = true;
+}
+
+void Species::advance()
+{
+    // advance_position + advance_boundary (particles.hpp:342-349) in one pass
+    run_pass(M2D_STEP_PUSH | M2D_STEP_BOUNDARY | (p_->selfconsistent ? M2D_STEP_DEPOSIT : 0u));
+    niter++;
+    t += dt;
+}
+
+void Species::advance_boundary()
+{
+    run_pass(M2D_STEP_BOUNDARY | (p_->selfconsistent ? M2D_STEP_DEPOSIT : 0u));
+}
+
+void Species::advance_init()
+{
+    ensure_device();
+    if (cap_ == 0) return;
+    m2d_species2d d = desc();
+    m2d_particles2d s = store();
+    check(m2d_advance_init_2d(dev_->ctx(), field->handle(), &d, &s));
+    dev_newer_ = true;
+}
+
+void Species::accumulate()
+{
+    ensure_device();
+    m2d_particles2d s = store();
+    check(m2d_accumulate_2d(dev_->ctx(), &rho.grid(), &s, charge, rho.device()));
+    rho.mark_device_written();
+}
+
+void Species::add_particles_on_disk(int nparticles, double centerx, double centery, double radius)
+{
+    if (nparticles <= 0) return;
+    ensure_device();
+    // sample into a fresh store, then append it to this one; every call draws
+    // from its own stream
+    DeviceBuffer tmp[5];
+    for (auto &b : tmp) b = DeviceBuffer(dev_, (size_t)nparticles * sizeof(float));
+    m2d_particles2d t{(float *)tmp[0].get(), (float *)tmp[1].get(), (float *)tmp[2].get(),
+                      (float *)tmp[3].get(), (float *)tmp[4].get(), nparticles};
+    const double vth = v_max * M_SQRT1_2;  // rndv: rnor() v_max / sqrt(2)
+    const uint32_t stream = stream_ ^ (disk_calls_++ << 16);
+    check(m2d_add_particles_on_disk_2d(dev_->ctx(), field->handle(), &t, centerx, centery, radius, vth, stream));
+    const int64_t old = cap_, n = cap_ + nparticles;
+    DeviceBuffer grown[5];
+    for (int c = 0; c < 5; c++) {
+        grown[c] = DeviceBuffer(dev_, (size_t)n * sizeof(float));
+        if (old) check(m2d_memcpy_d2d(dev_->ctx(), grown[c].get(), soa_[c].get(), (size_t)old * sizeof(float)));
+        check(m2d_memcpy_d2d(dev_->ctx(), (float *)grown[c].get() + old, tmp[c].get(),
+                             (size_t)nparticles * sizeof(float)));
+    }
+    dev_->synchronize();
+    for (int c = 0; c < 5; c++) soa_[c] = std::move(grown[c]);
+    cap_ = n;
+    dev_newer_ = true;
+}
+
+void Species::lifetime_init(const std::vector<Species *> &speclist)
+{
+    if (mcc_) {
+        m2d_mcc_destroy(mcc_);
+        mcc_ = nullptr;
+    }
+    partners_.clear();
+    lifetime = INFINITY;
+    if (params_.interactions.empty()) return;
+    std::vector<m2d_target> targets;
+    std::vector<m2d_interaction> inter;
+    std::vector<double> tabE, tabS;
+    std::vector<std::string> names;
+    for (const InteractionParams &ip : params_.interactions) {
+        auto it = std::find(names.begin(), names.end(), ip.target);
+        int k = (int)(it - names.begin());
+        if (it == names.end()) {
+            Species *ts = nullptr;
+            for (Species *s : speclist)
+                if (s->name == ip.target) ts = s;
+            if (!ts) throw std::runtime_error("Species " + name + ": unknown interaction target " + ip.target);
+            m2d_target tg{};
+            tg.mass = ts->mass;
+            tg.density = ts->density;
+            tg.temperature = ts->temperature;
+            tg.charge = ts->charge;
+            targets.push_back(tg);
+            names.push_back(ip.target);
+            partners_.push_back(ts->particle ? ts : nullptr);
+        }
+        m2d_interaction in{};
+        in.type = ip.type;
+        in.target = k;
+        in.DE = ip.DE;
+        in.rate = ip.rate;
+        in.cutoff = ip.cutoff;
+        if (!ip.cs_E.empty()) {
+            if (ip.cs_E.size() != ip.cs_sigma.size())
+                throw std::runtime_error("Species " + name + ": cross-section table columns differ");
+            in.tab_offset = (int32_t)tabE.size();
+            in.tab_len = (int32_t)ip.cs_E.size();
+            tabE.insert(tabE.end(), ip.cs_E.begin(), ip.cs_E.end());
+            tabS.insert(tabS.end(), ip.cs_sigma.begin(), ip.cs_sigma.end());
+        }
+        inter.push_back(in);
+    }
+    m2d_mcc_desc d{};
+    d.mass = mass;
+    d.charge = charge;
+    d.temperature = temperature;
+    d.density = density;
+    d.E_max = E_max;
+    d.n_targets = (int32_t)targets.size();
+    d.targets = targets.data();
+    d.n_inter = (int32_t)inter.size();
+    d.inter = inter.data();
+    d.n_tab = (int32_t)tabE.size();
+    d.tab_E = tabE.empty() ? nullptr : tabE.data();
+    d.tab_sigma = tabS.empty() ? nullptr : tabS.data();
+    check(m2d_mcc_create(dev_->ctx(), &d, &mcc_));
+    lifetime = m2d_mcc_lifetime(mcc_);
+}
+
+double Species::mean_energy()
+{
+    ensure_device();
+    m2d_particles2d s = store();
+    double mean = 0.0;
+    int64_t live = 0;
+    check(m2d_mean_energy_2d(dev_->ctx(), &s, mass, &mean, &live));
+    return mean;
+}
+
+void Species::sort()
+{
+    ensure_device();
+    if (cap_ == 0) return;
+    for (int c = 0; c < 5; c++)
+        if (tmp_[c].bytes() != (size_t)cap_ * sizeof(float))
+            tmp_[c] = DeviceBuffer(dev_, (size_t)cap_ * sizeof(float));
+    m2d_particles2d src = store();
+    m2d_particles2d dst{(float *)tmp_[0].get(), (float *)tmp_[1].get(), (float *)tmp_[2].get(),
+                        (float *)tmp_[3].get(), (float *)tmp_[4].get(), cap_};
+    check(m2d_sort_2d(dev_->ctx(), field->handle(), &src, &dst, nullptr));
+    for (int c = 0; c < 5; c++) std::swap(soa_[c], tmp_[c]);
+    dev_newer_ = true;
+}
+
+// ================================================================ Pic
+Pic::Pic(Param &p, const std::vector<SpeciesParams> &species, const t_grid *grid, int device)
+    : param(p), dev(std::make_shared<Device>(device, p.seed)), field(p, dev, grid ? *grid : t_grid(p))
+{
+    for (size_t i = 0; i < species.size(); i++)
+        speclist.emplace_back(new Species(species[i], p, field, (uint32_t)i));
+    std::vector<Species *> all;
+    for (auto &s : speclist) all.push_back(s.get());
+    for (auto &s : speclist)
+        if (s->particle) s->lifetime_init(all);
+}
+
+Species *Pic::find(const std::string &name)
+{
+    for (auto &s : speclist)
+        if (s->name == name) return s.get();
+    return nullptr;
+}
+
+void Pic::run_initscript(const std::string &filename)
+{
+    std::ifstream fr(filename.c_str());
+    if (!fr.is_open()) throw std::runtime_error("Pic::run_initscript: cannot open " + filename);
+    std::string line;
+    while (std::getline(fr, line)) {
+        if (!line.empty() && line[0] == '#') continue;
+        std::istringstream s(line);
+        std::vector<std::string> toks;
+        std::string tok;
+        while (s >> tok) toks.push_back(tok);
+        if (toks.empty()) continue;
+        auto species = [&](size_t want) {
+            if (toks.size() != want)
+                throw std::runtime_error("Pic::run_initscript: wrong number of parameters (" +
+                                         std::to_string(toks.size()) + ") to " + toks[0] + "\n");
+            Species *sp = find(toks[1]);
+            if (!sp) throw std::runtime_error("Pic::run_initscript: unrecognized species type \"" + toks[1] + "\"\n");
+            return sp;
+        };
+        if (toks[0] == "add_particles_on_disk") {
+            Species *sp = species(6);
+            sp->add_particles_on_disk(std::stoi(toks[2]), std::stod(toks[3]), std::stod(toks[4]),
+                                      std::stod(toks[5]));
+        } else if (toks[0] == "add_tracked_particle") {
+            // one particle at a given state (tracking output is not part of this build)
+            Species *sp = species(7);
+            int ii = sp->insert();
+            t_particle &p = sp->particles[(size_t)ii];
+            p.x = std::stod(toks[2]);
+            p.z = std::stod(toks[3]);
+            p.y = 0;
+            p.vx = std::stod(toks[4]);
+            p.vy = std::stod(toks[5]);
+            p.vz = std::stod(toks[6]);
+        } else if (toks[0] == "add_particles_everywhere" || toks[0] == "add_particles_bessel") {
+            throw std::runtime_error("Pic::run_initscript: " + toks[0] + " is not provided by this build");
+        }
+    }
+}
+
+void Pic::advance()
+{
+    if (param.selfconsistent) {
+        field.boundary_solve();
+        field.reset();
+        for (auto &s : speclist)
+            if (s->particle && s->n_particles() > 0) s->rho.reset();
+    }
+    for (auto &s : speclist) s->advance();
+    for (auto &s : speclist)
+        if (s->particle && param.selfconsistent) field.rho.add(s->rho);
+    iter++;
+}
+
+void Pic::advance_init()
+{
+    if (param.selfconsistent) {
+        for (auto &s : speclist)
+            if (s->particle && s->n_particles() > 0) {
+                s->accumulate();
+                field.rho.add(s->rho);
+            }
+        field.boundary_solve();
+        // boundary_solve builds its rhs in place on rho: rebuild rho
+        field.reset();
+        for (auto &s : speclist)
+            if (s->particle && s->n_particles() > 0) field.rho.add(s->rho);
+    }
+    for (auto &s : speclist) s->advance_init();
+}
+
+}  // namespace mag2d
+
+// ================================================================ C entry points
+using namespace mag2d;
+
+extern "C" {
+
+// Host-only check of the slot container: insert/remove/resize semantics of
+// BaseSpecies and the checkpoint round trip.  0 on success, else writes the
+// failed check into msg.
+int m2dh_list_selftest(const char *tmpfile, char *msg, int len)
+{
+    auto fail = [&](const char *what) {
+        std::snprintf(msg, (size_t)len, "%s", what);
+        return 1;
+    };
+    try {
+        ParticleList L;
+        if (L.n_particles() != 0) return fail("empty list has particles");
+        int a = L.insert();  // grows to 0 * 1.1 + 1 = 1
+        if (a != 0 || L.particles.size() != 1) return fail("first insert");
+        int b = L.insert();  // grows to 1 * 1.1 + 1 = 2
+        if (b != 1 || L.particles.size() != 2) return fail("second insert");
+        for (int i = 0; i < 8; i++) L.insert();
+        if (L.n_particles() != 10) return fail("count after 10 inserts");
+        L.remove(3);
+        bool threw = false;
+        try {
+            L.remove(3);
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        if (!threw) return fail("double remove must throw");
+        if (L.insert() != 3) return fail("insert must reuse the last removed slot");
+        size_t before = L.particles.size();
+        L.resize((unsigned)before + 5);
+        if (L.particles.size() != before + 5 || L.n_particles() != 10) return fail("resize");
+        L.resize(2);
+        if (L.particles.size() != before + 5) return fail("resize must not shrink");
+        for (int i = 0; i < (int)L.particles.size(); i++)
+            if (!L.particles[i].empty) {
+                L.particles[i].x = i * 1e-3;
+                L.particles[i].vz = -i;
+            }
+        L.remove(5);
+        L.save(tmpfile);
+        ParticleList M;
+        M.load(tmpfile);
+        if (M.particles.size() != L.particles.size() || M.n_particles() != L.n_particles())
+            return fail("load: sizes");
+        int k = 0;
+        for (size_t i = 0; i < L.particles.size(); i++)
+            if (!L.particles[i].empty) {
+                if (M.particles[k].x != L.particles[i].x || M.particles[k].vz != L.particles[i].vz)
+                    return fail("load: particle data");
+                k++;
+            }
+        for (size_t i = (size_t)k; i < M.particles.size(); i++)
+            if (!M.particles[i].empty) return fail("load: tail must be empty");
+        if (M.insert() != (int)M.particles.size() - 1) return fail("load: free stack order");
+        if (sizeof(t_particle) != 64) return fail("t_particle must be 64 bytes");
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "exception: %s", e.what());
+        return 2;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+// A trap run through Pic: EMPTY geometry (grounded frame), end caps along z
+// at end_cap_V, uniform Bz, one species loaded on a disk, advance_init and
+// `steps` advances.  Returns 0 or a negative status; positions and velocities
+// of the n slots (live particles first, in store order) go to out_* when
+// given, and metrics[0..5] = mean energy (eV), live count, sum of Fields::rho
+// (C), u at the centre node (V), last solve's FP64 relative residual, its
+// outer iterations.
+typedef struct {
+    int32_t grid, n_particles, steps, selfconsistent;
+    double length, dt, Bz, end_cap_V, temperature_K, mass, charge, radius, density_total;
+    uint64_t seed;
+} m2dh_trap_cfg;
+
+int m2dh_trap_run(const m2dh_trap_cfg *c, float *out_x, float *out_z, float *out_vx, float *out_vy,
+                  float *out_vz, int64_t n_out, double *metrics, char *msg, int len)
+{
+    try {
+        Param p;
+        p.x_sampl = p.z_sampl = c->grid;
+        p.x_max = p.z_max = c->length;
+        p.geometry = EMPTY;
+        p.selfconsistent = c->selfconsistent;
+        p.Bz = c->Bz;
+        p.n_particles_total = c->n_particles;
+        p.density_total = c->density_total;
+        p.seed = c->seed;
+        p.derive();
+        SpeciesParams sp;
+        sp.name = "particles";
+        sp.mass = c->mass;
+        sp.charge = c->charge;
+        sp.temperature = c->temperature_K;
+        sp.dt = c->dt;
+        // grounded frame (EMPTY builder) with end caps on the first and last
+        // two node rows along z
+        t_grid g(p);
+        const double h = 1.5 * p.dz;
+        g.square_electrode(0, p.x_max, 0, h, c->end_cap_V);
+        g.square_electrode(0, p.x_max, p.z_max - h, p.z_max, c->end_cap_V);
+        Pic run(p, {sp}, &g);
+        Species &s = *run.speclist[0];
+        s.add_particles_on_disk(c->n_particles, 0.5 * p.x_max, 0.5 * p.z_max, c->radius);
+        run.advance_init();
+        for (int k = 0; k < c->steps; k++) run.advance();
+        s.download();
+        if (metrics) {
+            metrics[0] = s.mean_energy();
+            metrics[1] = s.n_particles();
+            const Field2D &rho = run.field.rho, &u = run.field.u;
+            double rs = 0;
+            for (int i = 0; i < p.x_sampl; i++)
+                for (int j = 0; j < p.z_sampl; j++) rs += rho[i][j];
+            metrics[2] = rs;
+            metrics[3] = u[p.x_sampl / 2][p.z_sampl / 2];
+            metrics[4] = run.field.last_solve().rel_residual;
+            metrics[5] = run.field.last_solve().outer_iters;
+        }
+        int64_t k = 0;
+        for (const t_particle &q : s.particles) {
+            if (q.empty || k >= n_out) continue;
+            if (out_x) out_x[k] = (float)q.x;
+            if (out_z) out_z[k] = (float)q.z;
+            if (out_vx) out_vx[k] = (float)q.vx;
+            if (out_vy) out_vy[k] = (float)q.vy;
+            if (out_vz) out_vz[k] = (float)q.vz;
+            k++;
+        }
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+}  // extern "C"

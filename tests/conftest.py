@@ -13,7 +13,8 @@ def pytest_configure(config):
 
 
 def _built():
-    return (ROOT / "mag2d_amd" / "libmag2d_amd.so").exists() and (ROOT / "oracle" / "liboracle.so").exists()
+    return all((ROOT / f).exists() for f in ("mag2d_amd/libmag2d_amd.so", "mag2d_amd/host/libmag2d_host.so",
+                                              "oracle/liboracle.so"))
 
 
 @pytest.fixture(scope="session", autouse=True)

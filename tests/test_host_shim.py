@@ -1,0 +1,145 @@
+"""The C++ host layer (mag2d_amd/host/mag2d.hpp): the reference's class API
+(BaseSpecies container, Species, Field2D, Fields, Pic) over the C ABI.
+
+CPU: the library loads, and the slot container keeps the reference's
+insert/remove/resize semantics and checkpoint format (particles.hpp:223-249,
+particles.cpp:31-107).  GPU: a self-consistent trap run through
+Pic::advance_init / Pic::advance matches the oracle's restatement of the same
+step ordering (pic.cpp:330-384)."""
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from mag2d_amd import abi
+from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parent.parent
+HOST_LIB = ROOT / "mag2d_amd" / "host" / "libmag2d_host.so"
+
+
+class TrapCfg(C.Structure):
+    _fields_ = [("grid", C.c_int32), ("n_particles", C.c_int32), ("steps", C.c_int32),
+                ("selfconsistent", C.c_int32), ("length", C.c_double), ("dt", C.c_double),
+                ("Bz", C.c_double), ("end_cap_V", C.c_double), ("temperature_K", C.c_double),
+                ("mass", C.c_double), ("charge", C.c_double), ("radius", C.c_double),
+                ("density_total", C.c_double), ("seed", C.c_uint64)]
+
+
+def host():
+    abi.lib()  # libmag2d_amd first: the host layer links against it
+    L = C.CDLL(str(HOST_LIB))
+    L.m2dh_list_selftest.restype = C.c_int
+    L.m2dh_list_selftest.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+    _f = C.POINTER(C.c_float)
+    L.m2dh_trap_run.restype = C.c_int
+    L.m2dh_trap_run.argtypes = [C.POINTER(TrapCfg), _f, _f, _f, _f, _f, C.c_int64, C.POINTER(C.c_double),
+                                C.c_char_p, C.c_int]
+    return L
+
+
+def test_host_library_loads_and_exports():
+    L = host()
+    for name in ("m2dh_list_selftest", "m2dh_trap_run"):
+        assert hasattr(L, name)
+
+
+def test_particle_container_semantics_and_checkpoint(tmp_path):
+    msg = C.create_string_buffer(256)
+    rc = host().m2dh_list_selftest(str(tmp_path / "species.bin").encode(), msg, 256)
+    assert rc == 0, msg.value.decode()
+    # the checkpoint is the reference's layout: int size, int n_live, 64-byte records
+    raw = (tmp_path / "species.bin").read_bytes()
+    size, live = np.frombuffer(raw[:8], np.int32)
+    assert len(raw) == 8 + 64 * live and live <= size
+
+
+def oracle_trap(cfg):
+    """Pic<CARTESIAN> advance_init + advance (pic.cpp:330-384) in the oracle."""
+    n, L = cfg.grid, cfg.length
+    dx = L / (n - 1)
+    og = O.grid2(n, n, dx, dx)
+    mask = np.zeros(n * n, np.uint8)
+    volt = np.zeros(n * n)
+    O.lib().o_grid_build(C.byref(og), C.byref(O.Geom(abi.GEO_EMPTY, L, L, 1e-3, 0.0, 0.0)), O.dptr(mask),
+                         O.dptr(volt))
+    h = 1.5 * dx
+    O.lib().o_square_electrode(C.byref(og), O.dptr(mask), O.dptr(volt), 0.0, L, 0.0, h, cfg.end_cap_V, 0)
+    O.lib().o_square_electrode(C.byref(og), O.dptr(mask), O.dptr(volt), 0.0, L, L - h, L, cfg.end_cap_V, 0)
+    dV = (cfg.n_particles / cfg.density_total) / ((n - 1) ** 2)
+    u = np.zeros(n * n)
+    of = O.make_fields(og, mask, u, x_max=L, z_max=L, geometry_empty=1, selfconsistent=cfg.selfconsistent,
+                       Bz=cfg.Bz)
+    u = of._keep[1]  # the array the oracle fields read
+
+    def solve(rho_f):
+        b = rho_f.copy()
+        O.lib().o_poisson_rhs(C.byref(og), O.dptr(mask), O.dptr(volt), O.dptr(b), O.SOLVE_STATIC, dV)
+        out = np.zeros_like(b)
+        assert O.lib().o_poisson_solve(C.byref(og), O.dptr(mask), O.dptr(b), O.dptr(out)) == 0
+        u[:] = out
+
+    vth = math.sqrt(abi.KB * cfg.temperature_K / cfg.mass)
+    p = O.particles(cfg.n_particles)
+    O.lib().o_add_particles_on_disk_philox(O.dptr(p), cfg.n_particles, 0.5 * L, 0.5 * L, cfg.radius, vth,
+                                           C.byref(of), cfg.seed, 0)
+    rho_s = np.zeros(n * n)
+    rho_f = np.zeros(n * n)
+    s = O.make_species(cfg.charge, cfg.mass, cfg.dt, rho=rho_s)
+    N = cfg.n_particles
+    if cfg.selfconsistent:
+        O.lib().o_accumulate(C.byref(of), C.byref(s), O.dptr(p), N)
+        rho_f += rho_s
+        solve(rho_f)
+        rho_f[:] = 0
+        rho_f += rho_s
+    O.lib().o_advance_boris_init(C.byref(of), C.byref(s), O.dptr(p), N)
+    for k in range(cfg.steps):
+        if cfg.selfconsistent:
+            solve(rho_f)
+            rho_f[:] = 0
+            rho_s[:] = 0
+        s.niter = k
+        O.lib().o_advance_boris(C.byref(of), C.byref(s), O.dptr(p), N, None)
+        O.lib().o_advance_boundary(C.byref(of), C.byref(s), O.dptr(p), N, None)
+        if cfg.selfconsistent:
+            rho_f += rho_s
+    return p, rho_f, u
+
+
+@pytest.mark.gpu
+def test_pic_selfconsistent_trap_matches_oracle():
+    cfg = TrapCfg(grid=64, n_particles=3000, steps=25, selfconsistent=1, length=2e-2, dt=1e-11, Bz=0.1,
+                  end_cap_V=-10.0, temperature_K=11604.5, mass=9.11e-31, charge=-abi.QE, radius=5e-3,
+                  density_total=1e13, seed=1234)
+    n = cfg.n_particles
+    out = {k: np.zeros(n, np.float32) for k in ("x", "z", "vx", "vy", "vz")}
+    m = (C.c_double * 6)()
+    msg = C.create_string_buffer(512)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    rc = host().m2dh_trap_run(C.byref(cfg), fp(out["x"]), fp(out["z"]), fp(out["vx"]), fp(out["vy"]),
+                              fp(out["vz"]), n, m, msg, 512)
+    assert rc == 0, msg.value.decode()
+    p, rho_f, u = oracle_trap(cfg)
+    alive = p["empty"] == 0
+    k = int(alive.sum())
+    assert int(m[1]) == k
+    assert k > 0.5 * n
+    steps, L = cfg.steps, cfg.length
+    pos_tol = 4 * steps * np.spacing(np.float32(L))
+    assert np.abs(out["x"][:k] - p["x"][alive]).max() <= pos_tol
+    assert np.abs(out["z"][:k] - p["z"][alive]).max() <= pos_tol
+    vscale = np.abs(p["vx"][alive]).max()
+    for c in ("vx", "vy", "vz"):
+        assert np.abs(out[c][:k] - p[c][alive]).max() <= 1e-4 * vscale, c
+    # charge on the grid is exactly the live particles' charge (fixed-point deposit)
+    assert math.isclose(m[2], cfg.charge * k, rel_tol=1e-12)
+    assert math.isclose(m[2], rho_f.sum(), rel_tol=1e-12)
+    # the last solve reached the FP64 tolerance; potential at the centre as the oracle's
+    assert m[4] <= 1e-10
+    c = (cfg.grid // 2) * cfg.grid + cfg.grid // 2
+    assert abs(m[3] - u[c]) <= 1e-6 * max(1.0, abs(u).max())
+    e_or = O.lib().o_mean_energy_eV(O.dptr(p), n, cfg.mass)
+    assert math.isclose(m[0], e_or, rel_tol=1e-4)
