@@ -347,3 +347,126 @@ def test_philox_and_shr3_thermalise_to_the_same_maxwellian():
         E = O.lib().o_mean_energy_eV(O.dptr(p), n, M_ION)
         assert E == pytest.approx(1.5 * KB * 300.0 / QE, rel=0.08)
     assert stats.ks_2samp(res[0], res[1]).pvalue > 0.01
+
+
+def test_superelastic_with_target_at_rest_adds_DE_in_the_cm_frame():
+    # particles.cpp:268-288: E_rel' = mu v_rel^2 / 2 + DE, isotropic in the CM frame
+    m2, DE_eV = 6.69e-27, 0.05
+    mcc = _mcc_one_target(M_ION, 0.0, m2, [O.inter(O.SUPERELASTIC, M_ION, m2, rate=1e-15, DE=DE_eV)])
+    r = O.Rng()
+    O.lib().o_rng_shr3_seed(C.byref(r), 21)
+    mu = M_ION * m2 / (M_ION + m2)
+    for _ in range(50):
+        p = one_particle(0, 0, 1200.0, -300.0, 450.0)
+        v0 = np.array([1200.0, -300.0, 450.0])
+        assert O.lib().o_scatter(C.byref(mcc), O.dptr(p), C.byref(r)) == 0
+        v1 = np.array([p["vx"][0], p["vy"][0], p["vz"][0]])
+        vcm = v0 * M_ION / (M_ION + m2)
+        vrel = np.sqrt(np.dot(v0, v0) + 2 * DE_eV * QE / mu)
+        assert np.linalg.norm(v1 - vcm) == pytest.approx(vrel * m2 / (M_ION + m2), rel=1e-12)
+
+
+def _boundary_scene(boundary):
+    n, L = 33, 1e-2
+    g, mask = empty_box(n, L)
+    m2 = mask.reshape(n, n)
+    m2[12:19, 14:20] = O.FIXED  # an electrode: cells with four FIXED corners are not free
+    u = np.zeros(n * n)
+    f = O.make_fields(g, mask, u, x_max=L, z_max=L, selfconsistent=1, boundary=boundary)
+    return g, mask, f, n, L
+
+
+def _free_cells(mask, n, x, z, dx):
+    i = np.floor(x / dx).astype(int)
+    j = np.floor(z / dx).astype(int)
+    m2 = mask.reshape(n, n)
+    ok = (i >= 0) & (j >= 0) & (i < n - 1) & (j < n - 1)
+    i, j = np.clip(i, 0, n - 2), np.clip(j, 0, n - 2)
+    anyfree = ((m2[i, j] == O.FREE) | (m2[i + 1, j] == O.FREE) | (m2[i, j + 1] == O.FREE) |
+               (m2[i + 1, j + 1] == O.FREE))
+    return ok & anyfree
+
+
+def test_boundary_free_removes_outside_box_and_non_free_cells_and_deposits_survivors():
+    g, mask, f, n, L = _boundary_scene(0)
+    rng = np.random.default_rng(3)
+    N = 4000
+    p = O.particles(N)
+    p["x"] = rng.uniform(-0.2 * L, 1.2 * L, N)
+    p["z"] = rng.uniform(-0.2 * L, 1.2 * L, N)
+    rho = np.zeros(n * n)
+    s = O.make_species(-QE, M_ION, 1e-9, rho=rho)
+    x0, z0 = p["x"].copy(), p["z"].copy()
+    removed = O.lib().o_advance_boundary(C.byref(f), C.byref(s), O.dptr(p), N, None)
+    inside = (x0 >= 0) & (x0 <= L) & (z0 >= 0) & (z0 <= L)
+    keep = inside & _free_cells(mask, n, x0, z0, g.dx)
+    np.testing.assert_array_equal(p["empty"] == 0, keep)
+    assert removed == N - keep.sum()
+    # survivors' CIC deposit: total charge, and node values from numpy bilinear weights
+    assert rho.sum() == pytest.approx(-QE * keep.sum(), rel=1e-13)
+    ref = np.zeros((n, n))
+    X, Z = x0[keep] / g.dx, z0[keep] / g.dx
+    i, j = np.floor(X).astype(int), np.floor(Z).astype(int)
+    u_, v_ = X - i, Z - j
+    for di, dj, w in ((0, 0, (1 - u_) * (1 - v_)), (1, 0, u_ * (1 - v_)), (0, 1, (1 - u_) * v_), (1, 1, u_ * v_)):
+        np.add.at(ref, (i + di, j + dj), -QE * w)
+    np.testing.assert_allclose(rho.reshape(n, n), ref, rtol=1e-12, atol=1e-12 * QE)
+
+
+def test_boundary_periodic_wraps_with_fmod():
+    g, mask, f, n, L = _boundary_scene(1)
+    rng = np.random.default_rng(4)
+    N = 2000
+    p = O.particles(N)
+    p["x"] = rng.uniform(-0.5 * L, 1.5 * L, N)
+    p["z"] = rng.uniform(-0.5 * L, 1.5 * L, N)
+    x0, z0 = p["x"].copy(), p["z"].copy()
+    O.lib().o_advance_boundary(C.byref(f), C.byref(O.make_species(QE, M_ION, 1e-9)), O.dptr(p), N, None)
+    out = (x0 < 0) | (x0 > L) | (z0 < 0) | (z0 > L)
+    xw = np.where(out, np.fmod(x0, L), x0)
+    zw = np.where(out, np.fmod(z0, L), z0)
+    xw = np.where(xw < 0, xw + L, xw)
+    zw = np.where(zw < 0, zw + L, zw)
+    keep = _free_cells(mask, n, xw, zw, g.dx)
+    np.testing.assert_array_equal(p["empty"] == 0, keep)
+    np.testing.assert_array_equal(p["x"][keep], xw[keep])
+    np.testing.assert_array_equal(p["z"][keep], zw[keep])
+
+
+def test_fields_E_composition_with_linear_potentials_and_rf_phase():
+    # E = -(grad u + (V cos(mod(w t, 1e7 pi)) + U0) grad uRF); linear potentials have exact gradients
+    n, L = 41, 1e-2
+    g, mask = empty_box(n, L)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    X, Z = ii * g.dx, jj * g.dz
+    u = (3.0 * X - 7.0 * Z).ravel()
+    urf = (-2.0 * X + 5.0 * Z).ravel()
+    V, w, U0 = 30.0, 2 * np.pi * 17e6, 0.25
+    f = O.make_fields(g, mask, u, urf, x_max=L, z_max=L, rf=1, rf_amplitude=V, rf_omega=w, rf_U0=U0)
+    rng = np.random.default_rng(5)
+    for t in (0.0, 3.3e-9, 1.7e-6, 2.0e-1):
+        ph = w * t
+        ph = ph if ph <= 1e7 * np.pi else ph - 1e7 * np.pi * int(ph / (1e7 * np.pi))
+        phase = V * np.cos(ph) + U0
+        for x, z in rng.uniform(0.05 * L, 0.95 * L, (20, 2)):
+            ex, ez = C.c_double(), C.c_double()
+            O.lib().o_fields_E(C.byref(f), x, z, t, C.byref(ex), C.byref(ez))
+            assert ex.value == pytest.approx(-(3.0 - 2.0 * phase), rel=1e-9, abs=1e-9)
+            assert ez.value == pytest.approx(-(-7.0 + 5.0 * phase), rel=1e-9, abs=1e-9)
+    # EMPTY geometry without space charge: the constant field (0, +extern_field), not negated
+    f0 = O.make_fields(g, mask, u, x_max=L, z_max=L, geometry_empty=1, extern_field=123.0)
+    ex, ez = C.c_double(), C.c_double()
+    O.lib().o_fields_E(C.byref(f0), 0.3 * L, 0.6 * L, 0.0, C.byref(ex), C.byref(ez))
+    assert (ex.value, ez.value) == (0.0, 123.0)
+
+
+def test_mean_energy_is_mean_kinetic_energy_in_eV_over_live_particles():
+    rng = np.random.default_rng(6)
+    N = 1000
+    p = O.particles(N)
+    for k in ("vx", "vy", "vz"):
+        p[k] = rng.normal(size=N) * 900.0
+    p["empty"][::7] = 1
+    live = p["empty"] == 0
+    e = 0.5 * M_ION * (p["vx"] ** 2 + p["vy"] ** 2 + p["vz"] ** 2) / QE
+    assert O.lib().o_mean_energy_eV(O.dptr(p), N, M_ION) == pytest.approx(e[live].mean(), rel=1e-12)
