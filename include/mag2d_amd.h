@@ -317,6 +317,73 @@ int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_partic
                                  double cx, double cz, double radius, double vth,
                                  uint32_t stream);
 
+/* ======================= 3D (Field3D, Species<CARTESIAN3D>) ============== */
+
+typedef struct m2d_fields3d m2d_fields3d;
+
+/* Field3D geometry (Field3D.hpp:14-18) with the origin at 0; data[(i*ny + j)*nz + k]
+ * like Array3D (Array.hpp:148-151) */
+typedef struct {
+    int32_t nx, ny, nz;
+    double dx, dy, dz;
+} m2d_grid3d;
+
+/* 3D particle store: SoA FP32, x == M2D_DEAD bits marks an empty slot */
+typedef struct {
+    float *x, *y, *z;
+    float *vx, *vy, *vz;
+    int64_t n;
+} m2d_particles3d;
+
+typedef struct {
+    m2d_grid3d grid;
+    double x_max, y_max, z_max;      /* box [0, max] per axis (species3d.cpp:63-66) */
+    int32_t boundary;                /* M2D_BC_FREE removes; M2D_BC_PERIODIC wraps with mod */
+    double Bx, By, Bz;               /* uniform B (the reference's 3D mover sets B = 0) */
+    double rhs_scale;                /* free rows: rhs = -rho * rhs_scale; the reference uses
+                                        macroparticle_factor / eps_0 (fields3d.cpp:92) */
+} m2d_fields3d_params;
+
+/* ElMag3D / Geometry / Solver (fields3d.hpp:38-115, fields3d.cpp:12-101):
+ * mask (M2D_FIXED / M2D_FIXED_RF / M2D_FREE per node, Dirichlet on the
+ * grid faces) and voltage on the host; builds the multigrid hierarchy that
+ * replaces the UMFPACK factorisation of Solver::matrix_init / umfpack_init. */
+int m2d_fields3d_create(m2d_ctx *ctx, const m2d_fields3d_params *p, const uint8_t *mask,
+                        const double *voltage, m2d_fields3d **out);
+int m2d_fields3d_destroy(m2d_fields3d *f);
+/* device pointer of u / rho / voltage (M2D_FIELD_U, M2D_FIELD_RHO, M2D_FIELD_VOLTAGE) */
+double *m2d_fields3d_data(m2d_fields3d *f, int which);
+int m2d_fields3d_upload(m2d_fields3d *f, int which, const double *host);
+int m2d_fields3d_download(m2d_fields3d *f, int which, double *host);
+int m2d_fields3d_reset(m2d_fields3d *f);
+/* Solver::solve (fields3d.cpp:84-96): rhs built in place on rho, 7-point
+ * system solved into u to FP64 relative residual tol (<= 0: 1e-10) */
+int m2d_fields3d_solve(m2d_fields3d *f, double tol, m2d_solve_info *info);
+
+/* Field3D::accumulate over the live particles (Field3D.hpp:40-66), 2^-31 fixed point */
+int m2d_accumulate_3d(m2d_ctx *ctx, const m2d_grid3d *g, const m2d_particles3d *p, double charge,
+                      double *rho);
+/* Field3D::interpolate / grad (Field3D.hpp:68-163) at n FP64 points, device arrays */
+int m2d_interpolate_3d(m2d_ctx *ctx, const m2d_grid3d *g, const double *data, const double *x,
+                       const double *y, const double *z, int64_t n, double *out);
+int m2d_grad_3d(m2d_ctx *ctx, const m2d_grid3d *g, const double *data, const double *x,
+                const double *y, const double *z, int64_t n, double *gx, double *gy, double *gz);
+/* Species<CARTESIAN3D>::advance (species3d.cpp:3-94) in one pass: E = -grad u,
+ * Boris in (x, y, z), drift, MCC trial (scatter applied after the pass),
+ * box / mod wrap, Geometry::is_free, CIC deposit (flag DEPOSIT) into rho. */
+int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const m2d_particles3d *p,
+                   const m2d_mcc *mcc, double *rho, uint32_t flags, m2d_step_stats *stats);
+/* leapfrog start of the 3D mover (the 2D advance_boris_init, particles.cpp:997-1050) */
+int m2d_advance_init_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s,
+                        const m2d_particles3d *p);
+/* stable re-order by cell into dst (same slot count), empty slots dropped to the end */
+int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const m2d_particles3d *dst,
+                int64_t *n_live);
+/* n_particles() and energy_dist.mean_tot() of a 3D store */
+int m2d_count_live_3d(m2d_ctx *ctx, const m2d_particles3d *p, int64_t *n_live);
+int m2d_mean_energy_3d(m2d_ctx *ctx, const m2d_particles3d *p, double mass, double *mean_eV,
+                       int64_t *n_live);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,21 @@ class StepStats(C.Structure):
     _fields_ = [("removed", C.c_int64), ("collisions", C.c_int64)]
 
 
+class Grid3D(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class Particles3D(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("x", "y", "z", "vx", "vy", "vz")] + [("n", C.c_int64)]
+
+
+class Fields3DParams(C.Structure):
+    _fields_ = [("grid", Grid3D), ("x_max", C.c_double), ("y_max", C.c_double), ("z_max", C.c_double),
+                ("boundary", C.c_int32), ("Bx", C.c_double), ("By", C.c_double), ("Bz", C.c_double),
+                ("rhs_scale", C.c_double)]
+
+
 _P = C.c_void_p
 _PD = C.POINTER(C.c_double)
 _PU8 = C.POINTER(C.c_uint8)
@@ -155,6 +170,24 @@ _SIGS = {
     "m2d_sort_forget_2d": (C.c_int, [_P, C.POINTER(Particles2D)]),
     "m2d_add_particles_on_disk_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.c_uint32]),
+    "m2d_fields3d_create": (C.c_int, [_P, C.POINTER(Fields3DParams), _PU8, _PD, C.POINTER(_P)]),
+    "m2d_fields3d_destroy": (C.c_int, [_P]),
+    "m2d_fields3d_data": (_P, [_P, C.c_int]),
+    "m2d_fields3d_upload": (C.c_int, [_P, C.c_int, _PD]),
+    "m2d_fields3d_download": (C.c_int, [_P, C.c_int, _PD]),
+    "m2d_fields3d_reset": (C.c_int, [_P]),
+    "m2d_fields3d_solve": (C.c_int, [_P, C.c_double, C.POINTER(SolveInfo)]),
+    "m2d_accumulate_3d": (C.c_int, [_P, C.POINTER(Grid3D), C.POINTER(Particles3D), C.c_double, _P]),
+    "m2d_interpolate_3d": (C.c_int, [_P, C.POINTER(Grid3D), _P, _P, _P, _P, C.c_int64, _P]),
+    "m2d_grad_3d": (C.c_int, [_P, C.POINTER(Grid3D), _P, _P, _P, _P, C.c_int64, _P, _P, _P]),
+    "m2d_advance_3d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles3D), _P, _P,
+                                 C.c_uint32, C.POINTER(StepStats)]),
+    "m2d_advance_init_3d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles3D)]),
+    "m2d_sort_3d": (C.c_int, [_P, _P, C.POINTER(Particles3D), C.POINTER(Particles3D),
+                              C.POINTER(C.c_int64)]),
+    "m2d_count_live_3d": (C.c_int, [_P, C.POINTER(Particles3D), C.POINTER(C.c_int64)]),
+    "m2d_mean_energy_3d": (C.c_int, [_P, C.POINTER(Particles3D), C.c_double, _PD,
+                                     C.POINTER(C.c_int64)]),
 }
 
 _lib = None
@@ -537,3 +570,149 @@ def rf_phase(niter, dt, rf_omega, rf_amplitude, rf_U0):
     if not (0.0 <= ph <= y):
         ph = ph - y * int(ph / y) + (y if ph < 0 else 0)
     return rf_amplitude * math.cos(ph) + rf_U0
+
+
+# --------------------------------------------------------------------- 3D
+
+def grid3d(nx, ny, nz, dx, dy, dz) -> Grid3D:
+    return Grid3D(nx, ny, nz, dx, dy, dz)
+
+
+class ParticlesSoA3D:
+    """Device SoA x, y, z, vx, vy, vz (FP32) of n slots."""
+
+    NAMES = ("x", "y", "z", "vx", "vy", "vz")
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx, self.n = ctx, int(n)
+        self.arr = {k: DeviceArray(ctx, (max(self.n, 1),), np.float32) for k in self.NAMES}
+
+    @classmethod
+    def from_host(cls, ctx, *arrays):
+        p = cls(ctx, len(arrays[0]))
+        p.upload(*arrays)
+        return p
+
+    def upload(self, *arrays):
+        for k, a in zip(self.NAMES, arrays):
+            a = np.asarray(a, dtype=np.float32)
+            assert a.shape == (self.n,)
+            if self.n:
+                self.arr[k].set(a)
+
+    def download(self):
+        return {k: self.arr[k].get()[: self.n] for k in self.NAMES}
+
+    def struct(self) -> Particles3D:
+        return Particles3D(*(self.arr[k].ptr for k in self.NAMES), self.n)
+
+    def alive(self):
+        return self.arr["x"].get()[: self.n].view(np.uint32) != DEAD_BITS
+
+
+class Fields3D:
+    """m2d_fields3d: ElMag3D / Solver (u, rho, mask, voltage) on the device."""
+
+    def __init__(self, ctx: Context, params: Fields3DParams, mask: np.ndarray, voltage: np.ndarray):
+        self.ctx, self.params = ctx, params
+        g = params.grid
+        self.shape = (g.nx, g.ny, g.nz)
+        self.n = g.nx * g.ny * g.nz
+        mask = np.ascontiguousarray(mask, np.uint8).ravel()
+        voltage = np.ascontiguousarray(voltage, np.float64).ravel()
+        h = C.c_void_p()
+        check(lib().m2d_fields3d_create(ctx.h, C.byref(params), _ptr(mask, np.uint8),
+                                        _ptr(voltage, np.float64), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().m2d_fields3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ptr(self, which) -> int:
+        return lib().m2d_fields3d_data(self.h, which)
+
+    def upload(self, which, a):
+        a = np.ascontiguousarray(a, np.float64).ravel()
+        assert a.size == self.n
+        check(lib().m2d_fields3d_upload(self.h, which, _ptr(a, np.float64)))
+
+    def download(self, which) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        check(lib().m2d_fields3d_download(self.h, which, _ptr(out, np.float64)))
+        return out
+
+    def reset(self):
+        check(lib().m2d_fields3d_reset(self.h))
+
+    def solve(self, tol=1e-10) -> SolveInfo:
+        info = SolveInfo()
+        check(lib().m2d_fields3d_solve(self.h, tol, C.byref(info)))
+        return info
+
+
+def advance_3d(ctx, fields: Fields3D, species: Species2D, parts: ParticlesSoA3D, mcc=None, rho_ptr=None,
+               flags=STEP_PUSH | STEP_BOUNDARY, want_stats=False):
+    st = StepStats()
+    s = parts.struct()
+    check(lib().m2d_advance_3d(ctx.h, fields.h, C.byref(species), C.byref(s),
+                               mcc.h if mcc is not None else None,
+                               C.c_void_p(rho_ptr) if rho_ptr else None, flags,
+                               C.byref(st) if want_stats else None))
+    return st
+
+
+def advance_init_3d(ctx, fields, species, parts):
+    s = parts.struct()
+    check(lib().m2d_advance_init_3d(ctx.h, fields.h, C.byref(species), C.byref(s)))
+
+
+def accumulate_3d(ctx, g: Grid3D, parts: ParticlesSoA3D, charge, rho_ptr):
+    s = parts.struct()
+    check(lib().m2d_accumulate_3d(ctx.h, C.byref(g), C.byref(s), charge, C.c_void_p(rho_ptr)))
+
+
+def sort_3d(ctx, fields, src: ParticlesSoA3D, dst: ParticlesSoA3D):
+    n = C.c_int64()
+    a, b = src.struct(), dst.struct()
+    check(lib().m2d_sort_3d(ctx.h, fields.h, C.byref(a), C.byref(b), C.byref(n)))
+    return n.value
+
+
+def count_live_3d(ctx, parts):
+    n = C.c_int64()
+    s = parts.struct()
+    check(lib().m2d_count_live_3d(ctx.h, C.byref(s), C.byref(n)))
+    return n.value
+
+
+def mean_energy_3d(ctx, parts, mass):
+    m, n = C.c_double(), C.c_int64()
+    s = parts.struct()
+    check(lib().m2d_mean_energy_3d(ctx.h, C.byref(s), mass, C.byref(m), C.byref(n)))
+    return m.value, n.value
+
+
+def interpolate_3d(ctx, g, data_ptr, x, y, z):
+    n = len(x)
+    d = [ctx.to_device(np.asarray(a, np.float64)) for a in (x, y, z)]
+    out = DeviceArray(ctx, (n,), np.float64)
+    check(lib().m2d_interpolate_3d(ctx.h, C.byref(g), C.c_void_p(data_ptr), *(C.c_void_p(a.ptr) for a in d),
+                                   n, C.c_void_p(out.ptr)))
+    return out.get()
+
+
+def grad_3d(ctx, g, data_ptr, x, y, z):
+    n = len(x)
+    d = [ctx.to_device(np.asarray(a, np.float64)) for a in (x, y, z)]
+    out = [DeviceArray(ctx, (n,), np.float64) for _ in range(3)]
+    check(lib().m2d_grad_3d(ctx.h, C.byref(g), C.c_void_p(data_ptr), *(C.c_void_p(a.ptr) for a in d), n,
+                            *(C.c_void_p(o.ptr) for o in out)))
+    return tuple(o.get() for o in out)

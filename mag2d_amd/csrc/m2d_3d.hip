@@ -1,0 +1,1256 @@
+// m2d_3d.hip -- the 3D path: Field3D gather/deposit (Field3D.hpp:40-163),
+// the Species<CARTESIAN3D> step (species3d.cpp:3-94) as one pass, its cell
+// sort, and the 7-point Poisson solve that replaces Solver (fields3d.cpp:35-101).
+//
+// Solver: FP64 iterative refinement around FP32 conjugate gradients on the
+// Dirichlet-eliminated SPD system 6u - sum_free u_nb = ..., preconditioned by
+// one symmetric geometric multigrid V-cycle: coarse node I sits on fine node
+// 2I, every level carries the 7-point operator of its own mask, restriction
+// is P^T / 2 (full weighting with the h^2 of the coarser spacing), red-black
+// Gauss-Seidel smooths, and the coarsest level is solved by a dense inverse.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <rocprim/device/device_radix_sort.hpp>
+#include "m2d_objects.hpp"
+#include "m2d_scatter.hpp"
+
+namespace m2d {
+
+struct Mg3Level {
+    int nx = 0, ny = 0, nz = 0;
+    size_t n = 0;
+    uint8_t *dir = nullptr;   // 1 on Dirichlet nodes (and the coarse faces)
+    float *x = nullptr, *b = nullptr, *r = nullptr;
+};
+
+}  // namespace m2d
+
+struct m2d_fields3d {
+    m2d_ctx *ctx = nullptr;
+    m2d_fields3d_params p{};
+    int nx = 0, ny = 0, nz = 0;
+    size_t n = 0;
+    double idx = 0, idy = 0, idz = 0;
+    uint8_t *mask = nullptr;
+    uint8_t *cellfree = nullptr;   // cell (i,j,k) at its lower corner: any of 8 corners FREE
+    double *voltage = nullptr, *u = nullptr, *rho = nullptr;
+    float *tx = nullptr, *ty = nullptr, *tz = nullptr;  // (u_m - u_{m-e}) * 1/h per axis, FP32
+    std::vector<m2d::Mg3Level> lv;
+    float *coarse_inv = nullptr;
+    int coarse_n = 0;
+    bool coarse_dense = true;
+    float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
+    double *r64 = nullptr, *dots = nullptr, *h_dots = nullptr, *partial = nullptr;
+    int npartial = 0;
+};
+
+namespace m2d {
+
+__device__ __forceinline__ bool dir3(uint8_t m) { return m == M2D_FIXED || m == M2D_FIXED_RF; }
+
+__device__ __forceinline__ void node3(size_t m, int ny, int nz, int &i, int &j, int &k)
+{
+    k = (int)(m % nz);
+    size_t t = m / nz;
+    j = (int)(t % ny);
+    i = (int)(t / ny);
+}
+
+// ------------------------------------------------------------------ tables and solve kernels
+__global__ void k3_tables(const double *u, int nx, int ny, int nz, double idx, double idy, double idz,
+                          float *tx, float *ty, float *tz)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    int i, j, k;
+    node3(m, ny, nz, i, j, k);
+    const size_t sx = (size_t)ny * nz;
+    tx[m] = i >= 1 ? (float)((u[m] - u[m - sx]) * idx) : 0.f;
+    ty[m] = j >= 1 ? (float)((u[m] - u[m - nz]) * idy) : 0.f;
+    tz[m] = k >= 1 ? (float)((u[m] - u[m - 1]) * idz) : 0.f;
+}
+
+// Solver::solve rows (fields3d.cpp:86-93); Dirichlet values into u
+__global__ void k3_rhs(const uint8_t *mask, const double *voltage, double *rho, double *u, size_t n,
+                       double scale)
+{
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const bool d = dir3(mask[m]);
+    const double b = d ? voltage[m] : rho[m] * -scale;
+    rho[m] = b;
+    if (d) u[m] = b;
+}
+
+// r = b - A u on free rows (matrix_init, fields3d.cpp:35-76); d = -r in FP32;
+// per-block partial sums of r^2 and of the rhs with Dirichlet neighbours moved over
+__global__ void __launch_bounds__(256) k3_residual64(const uint8_t *mask, const double *b, const double *u,
+                                                     int nx, int ny, int nz, double *r, float *d,
+                                                     double *partial)
+{
+    __shared__ double s1[256], s2[256];
+    const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0, bb = 0;
+    if (m < n) {
+        double rv = 0;
+        if (!dir3(mask[m])) {
+            // free rows never touch the grid faces (checked at creation)
+            const size_t nb[6] = {m - sx, m - nz, m - 1, m + 1, m + nz, m + sx};
+            double au = -6.0 * u[m], be = b[m];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                au += u[nb[q]];
+                if (dir3(mask[nb[q]])) be -= u[nb[q]];
+            }
+            rv = b[m] - au;
+            bb = be * be;
+        }
+        r[m] = rv;
+        d[m] = (float)(-rv);
+        rr = rv * rv;
+    }
+    s1[threadIdx.x] = rr;
+    s2[threadIdx.x] = bb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
+}
+
+__global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
+{
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n && !dir3(mask[m])) u[m] += (double)e[m];
+}
+
+// A' x = 6 x - sum over free neighbours (Dirichlet rows: 0)
+__device__ __forceinline__ float apply7(const uint8_t *dir, const float *x, size_t m, size_t sx, int nz)
+{
+    float s = 6.f * x[m];
+    if (!dir[m - sx]) s -= x[m - sx];
+    if (!dir[m - nz]) s -= x[m - nz];
+    if (!dir[m - 1]) s -= x[m - 1];
+    if (!dir[m + 1]) s -= x[m + 1];
+    if (!dir[m + nz]) s -= x[m + nz];
+    if (!dir[m + sx]) s -= x[m + sx];
+    return s;
+}
+
+// red-black Gauss-Seidel half sweep, colour = (i+j+k) & 1
+__global__ void k3_rbgs(const uint8_t *dir, float *x, const float *b, int nx, int ny, int nz, int colour)
+{
+    const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n || dir[m]) return;
+    int i, j, k;
+    node3(m, ny, nz, i, j, k);
+    if (((i + j + k) & 1) != colour) return;
+    float s = b[m];
+    if (!dir[m - sx]) s += x[m - sx];
+    if (!dir[m - nz]) s += x[m - nz];
+    if (!dir[m - 1]) s += x[m - 1];
+    if (!dir[m + 1]) s += x[m + 1];
+    if (!dir[m + nz]) s += x[m + nz];
+    if (!dir[m + sx]) s += x[m + sx];
+    x[m] = s * (1.f / 6.f);
+}
+
+__global__ void k3_resid(const uint8_t *dir, const float *x, const float *b, float *r, int nx, int ny, int nz)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    r[m] = dir[m] ? 0.f : b[m] - apply7(dir, x, m, (size_t)ny * nz, nz);
+}
+
+__global__ void k3_apply(const uint8_t *dir, const float *x, float *y, int nx, int ny, int nz)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    y[m] = dir[m] ? 0.f : apply7(dir, x, m, (size_t)ny * nz, nz);
+}
+
+// 1D weights of the trilinear prolongation: fine i from coarse I
+__device__ __forceinline__ float w1(int i, int I) { int d = i - 2 * I; return d == 0 ? 1.f : (d == 1 || d == -1 ? 0.5f : 0.f); }
+
+// b_c = P^T r_f / 2 on free coarse nodes
+__global__ void k3_restrict(const float *rf, int fx, int fy, int fz, float *bc, const uint8_t *dirc, int cx,
+                            int cy, int cz)
+{
+    const size_t N = (size_t)cx * cy * cz;
+    size_t M = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (M >= N) return;
+    if (dirc[M]) { bc[M] = 0.f; return; }
+    int I, J, K;
+    node3(M, cy, cz, I, J, K);
+    float s = 0.f;
+    for (int i = 2 * I - 1; i <= 2 * I + 1; i++) {
+        if (i < 0 || i >= fx) continue;
+        for (int j = 2 * J - 1; j <= 2 * J + 1; j++) {
+            if (j < 0 || j >= fy) continue;
+            float wij = w1(i, I) * w1(j, J);
+            for (int k = 2 * K - 1; k <= 2 * K + 1; k++) {
+                if (k < 0 || k >= fz) continue;
+                s = fmaf(wij * w1(k, K), rf[((size_t)i * fy + j) * fz + k], s);  // r is 0 on Dirichlet nodes
+            }
+        }
+    }
+    bc[M] = 0.5f * s;
+}
+
+// x_f += P x_c on free fine nodes
+__global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int fz, const float *xc, int cx,
+                           int cy, int cz)
+{
+    const size_t n = (size_t)fx * fy * fz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n || dirf[m]) return;
+    int i, j, k;
+    node3(m, fy, fz, i, j, k);
+    int Ia[2], Ja[2], Ka[2];
+    float wa[2], wb[2], wc[2];
+    int ni = 0, nj = 0, nk = 0;
+    auto split = [](int f, int nc, int *I, float *w, int &c) {
+        c = 0;
+        if ((f & 1) == 0) { if (f / 2 < nc) { I[c] = f / 2; w[c++] = 1.f; } }
+        else {
+            if ((f - 1) / 2 < nc) { I[c] = (f - 1) / 2; w[c++] = 0.5f; }
+            if ((f + 1) / 2 < nc) { I[c] = (f + 1) / 2; w[c++] = 0.5f; }
+        }
+    };
+    split(i, cx, Ia, wa, ni);
+    split(j, cy, Ja, wb, nj);
+    split(k, cz, Ka, wc, nk);
+    float s = 0.f;
+    for (int a = 0; a < ni; a++)
+        for (int b = 0; b < nj; b++)
+            for (int c = 0; c < nk; c++)
+                s = fmaf(wa[a] * wb[b] * wc[c], xc[((size_t)Ia[a] * cy + Ja[b]) * cz + Ka[c]], s);
+    xf[m] += s;
+}
+
+__global__ void __launch_bounds__(256) k3_coarse_solve(const float *inv, const float *b, float *x, int n)
+{
+    extern __shared__ float sb[];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sb[k] = b[k];
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        const float *a = inv + (size_t)row * n;
+        float s = 0.f;
+        for (int c = 0; c < n; c++) s = fmaf(a[c], sb[c], s);
+        x[row] = s;
+    }
+}
+
+// ------------------------------------------------------------------ point kernels (FP64)
+struct Grid3 {
+    int nx, ny, nz;
+    double idx, idy, idz, hx, hy, hz;  // h = (n-1) d idx: the clamp limits of grad
+};
+
+__device__ __forceinline__ double d3at(const double *d, const Grid3 &g, int i, int j, int k)
+{
+    return d[((size_t)i * g.ny + j) * g.nz + k];
+}
+
+// Field3D::interpolate (Field3D.hpp:68-96); out of range gives NaN
+__global__ void k_interp3d(const double *d, Grid3 g, const double *xs, const double *ys, const double *zs,
+                           int64_t n, double *out)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double x = xs[p], y = ys[p], z = zs[p];
+    const double X = x * g.idx, Y = y * g.idy, Z = z * g.idz;
+    if (!(X >= 0 && X < g.nx - 1.0 && Y >= 0 && Y < g.ny - 1.0 && Z >= 0 && Z < g.nz - 1.0)) {
+        out[p] = NAN;
+        return;
+    }
+    const int i = (int)X, j = (int)Y, k = (int)Z;
+    double u = x * g.idx - i, v = y * g.idy - j, w = z * g.idz - k;
+    double r = 0;
+    r += (1 - u) * (1 - v) * (1 - w) * d3at(d, g, i, j, k);
+    r += u * (1 - v) * (1 - w) * d3at(d, g, i + 1, j, k);
+    r += (1 - u) * v * (1 - w) * d3at(d, g, i, j + 1, k);
+    r += u * v * (1 - w) * d3at(d, g, i + 1, j + 1, k);
+    r += (1 - u) * (1 - v) * w * d3at(d, g, i, j, k + 1);
+    r += u * (1 - v) * w * d3at(d, g, i + 1, j, k + 1);
+    r += (1 - u) * v * w * d3at(d, g, i, j + 1, k + 1);
+    r += u * v * w * d3at(d, g, i + 1, j + 1, k + 1);
+    out[p] = r;
+}
+
+// Field3D::grad_component (Field3D.hpp:112-141), FP64: clamped half a cell
+// inside along the differenced axis, staggered differences, trilinear weights
+__device__ double grad3_component(const double *d, const Grid3 &g, double x, double y, double z, int ex, int ey,
+                                  int ez)
+{
+    double X = fmin(fmax(x * g.idx, 0.5 * ex), g.hx - 0.5 * ex) + 0.5 * ex;
+    double Y = fmin(fmax(y * g.idy, 0.5 * ey), g.hy - 0.5 * ey) + 0.5 * ey;
+    double Z = fmin(fmax(z * g.idz, 0.5 * ez), g.hz - 0.5 * ez) + 0.5 * ez;
+    int i = min((int)X, g.nx - 2), j = min((int)Y, g.ny - 2), k = min((int)Z, g.nz - 2);
+    double u = X - i, v = Y - j, w = Z - k;
+    double gg[8];
+    for (int c = 0; c < 8; c++) {
+        int a = i + (c & 1), b = j + ((c >> 1) & 1), e = k + ((c >> 2) & 1);
+        gg[c] = d3at(d, g, a, b, e) - d3at(d, g, a - ex, b - ey, e - ez);
+    }
+    double r = (1 - u) * (1 - v) * (1 - w) * gg[0] + u * (1 - v) * (1 - w) * gg[1] +
+               (1 - u) * v * (1 - w) * gg[2] + u * v * (1 - w) * gg[3] + (1 - u) * (1 - v) * w * gg[4] +
+               u * (1 - v) * w * gg[5] + (1 - u) * v * w * gg[6] + u * v * w * gg[7];
+    return r * (g.idx * ex + g.idy * ey + g.idz * ez);
+}
+
+__global__ void k_grad3d(const double *d, Grid3 g, const double *xs, const double *ys, const double *zs,
+                         int64_t n, double *gx, double *gy, double *gz)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    gx[p] = grad3_component(d, g, xs[p], ys[p], zs[p], 1, 0, 0);
+    gy[p] = grad3_component(d, g, xs[p], ys[p], zs[p], 0, 1, 0);
+    gz[p] = grad3_component(d, g, xs[p], ys[p], zs[p], 0, 0, 1);
+}
+
+// ------------------------------------------------------------------ the 3D pass
+struct PushArgs3 {
+    float *x, *y, *z, *vx, *vy, *vz;
+    int64_t n;
+    const float *tx, *ty, *tz;
+    const uint8_t *cellfree;
+    Grid3 g;
+    float bx1, by1, bz1;           // largest floats <= x_max, y_max, z_max
+    double xper, yper, zper;
+    float qmdt2, tx_, ty_, tz_, sx, sy, sz, dt;
+    int rotate, push, boundary, periodic;
+    uint64_t trial_thr;
+    uint64_t seed;
+    uint32_t stream;
+    uint64_t step;
+    const McDev *mcc;
+    uint32_t *cq;
+    unsigned long long *cq_len;
+    unsigned long long *acc;
+    unsigned long long *counters;
+};
+
+// one E component through the FP32 difference table: the clamp and the
+// cell index in FP64 on the FP32 position, weights FP32
+__device__ __forceinline__ float gather3_component(const float *t, const Grid3 &g, double x, double y, double z,
+                                                   int ex, int ey, int ez)
+{
+    double X = fmin(fmax(x * g.idx, 0.5 * ex), g.hx - 0.5 * ex) + 0.5 * ex;
+    double Y = fmin(fmax(y * g.idy, 0.5 * ey), g.hy - 0.5 * ey) + 0.5 * ey;
+    double Z = fmin(fmax(z * g.idz, 0.5 * ez), g.hz - 0.5 * ez) + 0.5 * ez;
+    int i = min(max((int)X, 0), g.nx - 2), j = min(max((int)Y, 0), g.ny - 2), k = min(max((int)Z, 0), g.nz - 2);
+    const float u = (float)(X - i), v = (float)(Y - j), w = (float)(Z - k);
+    const size_t sx = (size_t)g.ny * g.nz;
+    const float *c = t + ((size_t)i * g.ny + j) * g.nz + k;
+    float a0 = c[0], a1 = c[sx], a2 = c[g.nz], a3 = c[sx + g.nz];
+    float a4 = c[1], a5 = c[sx + 1], a6 = c[g.nz + 1], a7 = c[sx + g.nz + 1];
+    float lo = (1.f - v) * ((1.f - u) * a0 + u * a1) + v * ((1.f - u) * a2 + u * a3);
+    float hi = (1.f - v) * ((1.f - u) * a4 + u * a5) + v * ((1.f - u) * a6 + u * a7);
+    return (1.f - w) * lo + w * hi;
+}
+
+// eight CIC weights in 2^-31 fixed point, corrected on the largest so a
+// particle deposits exactly 2^31 (order-independent integer sums)
+__device__ __forceinline__ bool cic3(double X, double Y, double Z, const Grid3 &g, int &cell, uint32_t w[8])
+{
+    // range in FP64 before any conversion: cells [0, n-2] on every axis
+    if (!(X >= 0.0 && X < g.nx - 1.0 && Y >= 0.0 && Y < g.ny - 1.0 && Z >= 0.0 && Z < g.nz - 1.0)) return false;
+    const int i = (int)X, j = (int)Y, k = (int)Z;
+    const float u = (float)(X - i), v = (float)(Y - j), ww = (float)(Z - k);
+    const float S = 2147483648.0f;
+    const float a[2] = {1.f - u, u}, b[2] = {1.f - v, v}, c[2] = {1.f - ww, ww};
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        w[q] = __float2uint_rn(a[q & 1] * b[(q >> 1) & 1] * c[q >> 2] * S);
+        s += w[q];
+    }
+    int km = 0;
+    uint32_t wm = w[0];
+#pragma unroll
+    for (int q = 1; q < 8; q++)
+        if (w[q] > wm) { km = q; wm = w[q]; }
+    w[km] += 0x80000000u - s;
+    cell = (int)(((size_t)i * g.ny + j) * g.nz + k);
+    return true;
+}
+
+// Segmented sum across the wavefront of the lanes' weights keyed by their
+// cell (a sorted store puts a cell's particles on neighbouring lanes); the
+// last lane of each segment adds its sums to the 64-bit grid.  All 64
+// lanes must call it together.
+__device__ __forceinline__ void wave_deposit3(int cell, const uint32_t w8[8], const Grid3 &g,
+                                              unsigned long long *acc)
+{
+    const int lane = __lane_id();
+    const bool valid = cell >= 0;
+    const int key = valid ? cell : -1 - lane;
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    const int seg = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+    unsigned long long s[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[q] = valid ? w8[q] : 0ull;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            unsigned long long t = __shfl_up(s[q], off);
+            if (lane - off >= seg) s[q] += t;
+        }
+    }
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+    if (valid && tail) {
+        const size_t sx = (size_t)g.ny * g.nz;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const size_t m = (size_t)cell + ((q & 1) ? sx : 0) + (((q >> 1) & 1) ? (size_t)g.nz : 0) + (q >> 2);
+            if (s[q]) atomicAdd(acc + m, s[q]);
+        }
+    }
+}
+
+// mod (mymath.cpp:62-67)
+__device__ __forceinline__ double mod3(double x, double y)
+{
+    if (x >= 0.0 && x <= y) return x;
+    return x - y * (int)(x / y) + (x < 0 ? y : 0);
+}
+
+template <bool DEPOSIT, bool MCC>
+__global__ void __launch_bounds__(256) k_advance3d(PushArgs3 a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = p < a.n;
+    float x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
+    if (live) {
+        x = a.x[p];
+        live = !is_dead(x);
+    }
+    bool removed = false, coll = false, nan_pos = false, bad = false;
+    int cell = -1;
+    uint32_t w8[8];
+    if (live) {
+        y = a.y[p]; z = a.z[p];
+        vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
+        if (a.push) {
+            // E = -grad u (ElMag3D::E, fields3d.hpp:91-97)
+            const double xd = x, yd = y, zd = z;
+            const float ex = -gather3_component(a.tx, a.g, xd, yd, zd, 1, 0, 0);
+            const float ey = -gather3_component(a.ty, a.g, xd, yd, zd, 0, 1, 0);
+            const float ez = -gather3_component(a.tz, a.g, xd, yd, zd, 0, 0, 1);
+            // species3d.cpp:24-61
+            vx = fmaf(ex, a.qmdt2, vx);
+            vy = fmaf(ey, a.qmdt2, vy);
+            vz = fmaf(ez, a.qmdt2, vz);
+            if (a.rotate) {
+                float vpx = vx - vy * a.tz_ + vz * a.ty_;
+                float vpy = vy - vz * a.tx_ + vx * a.tz_;
+                float vpz = vz - vx * a.ty_ + vy * a.tx_;
+                vx = vx - vpy * a.sz + vpz * a.sy;
+                vy = vy - vpz * a.sx + vpx * a.sz;
+                vz = vz - vpx * a.sy + vpy * a.sx;
+            }
+            vx = fmaf(ex, a.qmdt2, vx);
+            vy = fmaf(ey, a.qmdt2, vy);
+            vz = fmaf(ez, a.qmdt2, vz);
+            x = fmaf(vx, a.dt, x);
+            y = fmaf(vy, a.dt, y);
+            z = fmaf(vz, a.dt, z);
+            // MCC trial (species3d.cpp:64-67); the velocity change is applied after the pass
+            if (MCC) {
+                uint32_t c0 = (uint32_t)(p >> 2), c1 = (uint32_t)a.step, c2 = 0u;
+                uint32_t c3 = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
+                philox4x32_10(c0, c1, c2, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
+                const int q = (int)(p & 3);
+                const uint32_t tw = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : c3;
+                if ((uint64_t)tw < a.trial_thr) {
+                    unsigned long long at = atomicAdd(a.cq_len, 1ull);
+                    a.cq[at] = (uint32_t)p;
+                    coll = true;
+                }
+            }
+        }
+        if (a.boundary) {
+            // species3d.cpp:69-85
+            if (x > a.bx1 || x < 0.f || y > a.by1 || y < 0.f || z > a.bz1 || z < 0.f || x != x || y != y ||
+                z != z) {
+                if (x != x || y != y || z != z) { removed = true; nan_pos = true; }
+                else if (!a.periodic) removed = true;
+                else {
+                    x = (float)mod3(x, a.xper);
+                    y = (float)mod3(y, a.yper);
+                    z = (float)mod3(z, a.zper);
+                }
+            }
+            if (!removed) {
+                // Geometry::is_free (fields3d.hpp:48-55)
+                const double X = (double)x * a.g.idx, Y = (double)y * a.g.idy, Z = (double)z * a.g.idz;
+                bool fr = false;
+                if (X >= 0.0 && X < a.g.nx - 1.0 && Y >= 0.0 && Y < a.g.ny - 1.0 && Z >= 0.0 &&
+                    Z < a.g.nz - 1.0)
+                    fr = a.cellfree[((size_t)(int)X * a.g.ny + (int)Y) * a.g.nz + (int)Z] != 0;
+                removed = !fr;
+            }
+        }
+        if (DEPOSIT && !removed) {
+            if (!cic3((double)x * a.g.idx, (double)y * a.g.idy, (double)z * a.g.idz, a.g, cell, w8)) {
+                cell = -1;
+                bad = true;
+            }
+        }
+        if (removed) {
+            a.x[p] = dead_value();
+        } else {
+            a.x[p] = x; a.y[p] = y; a.z[p] = z;
+            a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+        }
+    }
+    if (DEPOSIT) wave_deposit3(cell, w8, a.g, a.acc);
+    wave_count(&a.counters[CNT_REMOVED], removed);
+    if (MCC) wave_count(&a.counters[CNT_COLL], coll);
+    if (__ballot(nan_pos) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+    if (__ballot(bad) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+}
+
+// BaseSpecies::scatter for the queued slots (velocity only, after the pass)
+__global__ void __launch_bounds__(256) k_scatter3d(PushArgs3 a, unsigned long long *next_len)
+{
+    const unsigned long long len = *a.cq_len;
+    const unsigned long long start = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    if (start == 0) *next_len = 0ull;
+    for (unsigned long long k = start; k < len; k += stride) {
+        const uint32_t p = a.cq[k];
+        if (is_dead(a.x[p])) continue;
+        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+        Rng rg;
+        rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
+        scatter(*a.mcc, rg, vx, vy, vz, a.counters);
+        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+    }
+}
+
+// leapfrog start: rotation by -dt/2, then a backward half kick
+__global__ void __launch_bounds__(256) k_advance_init3d(PushArgs3 a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const float x = a.x[p];
+    if (is_dead(x)) return;
+    const float y = a.y[p], z = a.z[p];
+    float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+    const double xd = x, yd = y, zd = z;
+    const float ex = -gather3_component(a.tx, a.g, xd, yd, zd, 1, 0, 0);
+    const float ey = -gather3_component(a.ty, a.g, xd, yd, zd, 0, 1, 0);
+    const float ez = -gather3_component(a.tz, a.g, xd, yd, zd, 0, 0, 1);
+    if (a.rotate) {
+        float vpx = vx - vy * a.tz_ + vz * a.ty_;
+        float vpy = vy - vz * a.tx_ + vx * a.tz_;
+        float vpz = vz - vx * a.ty_ + vy * a.tx_;
+        vx = vx - vpy * a.sz + vpz * a.sy;
+        vy = vy - vpz * a.sx + vpx * a.sz;
+        vz = vz - vpx * a.sy + vpy * a.sx;
+    }
+    a.vx[p] = fmaf(ex, a.qmdt2, vx);
+    a.vy[p] = fmaf(ey, a.qmdt2, vy);
+    a.vz[p] = fmaf(ez, a.qmdt2, vz);
+}
+
+// Field3D::accumulate for every live particle
+__global__ void __launch_bounds__(256) k_accumulate3d(const float *xs, const float *ys, const float *zs,
+                                                      int64_t n, Grid3 g, unsigned long long *acc,
+                                                      unsigned long long *counters)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int cell = -1;
+    uint32_t w8[8];
+    bool bad = false;
+    if (p < n) {
+        const float x = xs[p];
+        if (!is_dead(x) &&
+            !cic3((double)x * g.idx, (double)ys[p] * g.idy, (double)zs[p] * g.idz, g, cell, w8)) {
+            cell = -1;
+            bad = true;
+        }
+    }
+    wave_deposit3(cell, w8, g, acc);
+    if (__ballot(bad) && __lane_id() == 0) raise_flag(counters, M2D_FLAG_OUT_OF_RANGE);
+}
+
+// ------------------------------------------------------------------ sort
+__global__ void k_sort_keys3d(const float *xs, const float *ys, const float *zs, int64_t n, Grid3 g,
+                              uint32_t dead_key, uint32_t *keys, uint32_t *vals)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float x = xs[p];
+    uint32_t key = dead_key;
+    if (!is_dead(x)) {
+        auto idx = [](double X, int nmax) { return X >= 0.0 ? (int)fmin(X, (double)nmax) : 0; };
+        const int i = idx((double)x * g.idx, g.nx - 2), j = idx((double)ys[p] * g.idy, g.ny - 2),
+                  k = idx((double)zs[p] * g.idz, g.nz - 2);
+        key = (uint32_t)(((size_t)i * (g.ny - 1) + j) * (g.nz - 1) + k);
+    }
+    keys[p] = key;
+    vals[p] = (uint32_t)p;
+}
+
+__global__ void k_sort_gather3d(m2d_particles3d src, m2d_particles3d dst, const uint32_t *keys,
+                                const uint32_t *vals, uint32_t dead_key, unsigned long long *n_live)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= src.n) return;
+    const uint32_t k = keys[i];
+    if (k == dead_key) {
+        dst.x[i] = dead_value();
+        return;
+    }
+    if (i == src.n - 1 || keys[i + 1] == dead_key) *n_live = (unsigned long long)(i + 1);
+    const int64_t s = vals[i];
+    dst.x[i] = src.x[s];
+    dst.y[i] = src.y[s];
+    dst.z[i] = src.z[s];
+    dst.vx[i] = src.vx[s];
+    dst.vy[i] = src.vy[s];
+    dst.vz[i] = src.vz[s];
+}
+
+// ------------------------------------------------------------------ host helpers
+static int dmalloc3(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(M2D_ERR_NO_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return M2D_OK;
+}
+
+static Grid3 grid3_of(const m2d_grid3d &g)
+{
+    Grid3 r;
+    r.nx = g.nx; r.ny = g.ny; r.nz = g.nz;
+    r.idx = 1.0 / g.dx; r.idy = 1.0 / g.dy; r.idz = 1.0 / g.dz;
+    r.hx = (g.nx - 1) * g.dx * r.idx;
+    r.hy = (g.ny - 1) * g.dy * r.idy;
+    r.hz = (g.nz - 1) * g.dz * r.idz;
+    return r;
+}
+
+static int check_grid3(const m2d_grid3d *g)
+{
+    M2D_CHECK(g, M2D_ERR_BAD_ARG, "3D grid is NULL");
+    M2D_CHECK(g->nx >= 3 && g->ny >= 3 && g->nz >= 3 && g->dx > 0 && g->dy > 0 && g->dz > 0, M2D_ERR_BAD_ARG,
+              "3D grid %dx%dx%d spacing %g,%g,%g", g->nx, g->ny, g->nz, g->dx, g->dy, g->dz);
+    M2D_CHECK((size_t)g->nx * g->ny * g->nz < (1ull << 31), M2D_ERR_BAD_ARG, "3D grid too large");
+    return M2D_OK;
+}
+
+static int check_parts3(const m2d_particles3d *p)
+{
+    M2D_CHECK(p, M2D_ERR_BAD_ARG, "3D particle store is NULL");
+    M2D_CHECK(p->n >= 0 && p->n < (1ll << 31), M2D_ERR_BAD_ARG, "3D store of %lld slots", (long long)p->n);
+    M2D_CHECK(p->n == 0 || (p->x && p->y && p->z && p->vx && p->vy && p->vz), M2D_ERR_BAD_ARG,
+              "3D particle arrays must be non-NULL");
+    return M2D_OK;
+}
+
+// Gauss-Jordan inverse of the coarsest 7-point operator on its free nodes
+static bool dense_inverse7(const std::vector<uint8_t> &dir, int nx, int ny, int nz, std::vector<float> &inv)
+{
+    const int n = nx * ny * nz;
+    std::vector<int> id, pos(n, -1);
+    for (int m = 0; m < n; m++)
+        if (!dir[m]) { pos[m] = (int)id.size(); id.push_back(m); }
+    const int q = (int)id.size();
+    std::vector<double> a((size_t)q * 2 * q, 0.0);
+    const int sx = ny * nz;
+    for (int r = 0; r < q; r++) {
+        const int m = id[r];
+        a[(size_t)r * 2 * q + r] = 6.0;
+        for (int nb : {m - sx, m - nz, m - 1, m + 1, m + nz, m + sx}) {
+            if (nb < 0 || nb >= n || pos[nb] < 0) continue;
+            a[(size_t)r * 2 * q + pos[nb]] = -1.0;
+        }
+        a[(size_t)r * 2 * q + q + r] = 1.0;
+    }
+    for (int c = 0; c < q; c++) {
+        int piv = c;
+        for (int r = c + 1; r < q; r++)
+            if (fabs(a[(size_t)r * 2 * q + c]) > fabs(a[(size_t)piv * 2 * q + c])) piv = r;
+        if (fabs(a[(size_t)piv * 2 * q + c]) < 1e-300) return false;
+        if (piv != c)
+            for (int t = 0; t < 2 * q; t++) std::swap(a[(size_t)c * 2 * q + t], a[(size_t)piv * 2 * q + t]);
+        const double d = 1.0 / a[(size_t)c * 2 * q + c];
+        for (int t = 0; t < 2 * q; t++) a[(size_t)c * 2 * q + t] *= d;
+        for (int r = 0; r < q; r++) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * 2 * q + c];
+            if (f == 0.0) continue;
+            for (int t = 0; t < 2 * q; t++) a[(size_t)r * 2 * q + t] -= f * a[(size_t)c * 2 * q + t];
+        }
+    }
+    inv.assign((size_t)n * n, 0.f);
+    for (int r = 0; r < q; r++)
+        for (int c = 0; c < q; c++) inv[(size_t)id[r] * n + id[c]] = (float)a[(size_t)r * 2 * q + q + c];
+    return true;
+}
+
+constexpr int kCoarseDenseMax = 1024;   // free nodes for the dense coarsest solve
+constexpr int kCoarseSweeps = 40;       // symmetric RB-GS sweeps otherwise
+
+static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
+{
+    std::vector<std::vector<uint8_t>> dirs;
+    std::vector<int> dims;  // nx, ny, nz per level
+    std::vector<uint8_t> d0(f->n);
+    for (size_t m = 0; m < f->n; m++) d0[m] = mask[m] == M2D_FIXED || mask[m] == M2D_FIXED_RF;
+    dirs.push_back(std::move(d0));
+    dims.insert(dims.end(), {f->nx, f->ny, f->nz});
+    for (;;) {
+        const int fx = dims[dims.size() - 3], fy = dims[dims.size() - 2], fz = dims[dims.size() - 1];
+        if ((size_t)fx * fy * fz <= 512 || fx < 5 || fy < 5 || fz < 5) break;
+        const int cx = (fx - 1) / 2 + 1, cy = (fy - 1) / 2 + 1, cz = (fz - 1) / 2 + 1;
+        const std::vector<uint8_t> &df = dirs.back();
+        std::vector<uint8_t> dc((size_t)cx * cy * cz);
+        for (int I = 0; I < cx; I++)
+            for (int J = 0; J < cy; J++)
+                for (int K = 0; K < cz; K++) {
+                    const bool face = I == 0 || J == 0 || K == 0 || I == cx - 1 || J == cy - 1 || K == cz - 1;
+                    dc[((size_t)I * cy + J) * cz + K] =
+                        face || df[((size_t)(2 * I) * fy + 2 * J) * fz + 2 * K];
+                }
+        dirs.push_back(std::move(dc));
+        dims.insert(dims.end(), {cx, cy, cz});
+    }
+    const size_t L = dirs.size();
+    f->lv.resize(L);
+    for (size_t l = 0; l < L; l++) {
+        Mg3Level &V = f->lv[l];
+        V.nx = dims[3 * l]; V.ny = dims[3 * l + 1]; V.nz = dims[3 * l + 2];
+        V.n = (size_t)V.nx * V.ny * V.nz;
+        M2D_TRY(dmalloc3((void **)&V.dir, V.n));
+        M2D_TRY(dmalloc3((void **)&V.x, sizeof(float) * V.n));
+        M2D_TRY(dmalloc3((void **)&V.b, sizeof(float) * V.n));
+        M2D_TRY(dmalloc3((void **)&V.r, sizeof(float) * V.n));
+        M2D_HIP(hipMemcpy(V.dir, dirs[l].data(), V.n, hipMemcpyHostToDevice));
+    }
+    const Mg3Level &C = f->lv.back();
+    size_t nfree = 0;
+    for (uint8_t d : dirs.back()) nfree += !d;
+    f->coarse_dense = nfree <= (size_t)kCoarseDenseMax && C.n <= 1024;
+    if (f->coarse_dense) {
+        std::vector<float> inv;
+        M2D_CHECK(dense_inverse7(dirs.back(), C.nx, C.ny, C.nz, inv), M2D_ERR_BAD_ARG,
+                  "Fields3D: coarsest multigrid operator is singular");
+        f->coarse_n = (int)C.n;
+        M2D_TRY(dmalloc3((void **)&f->coarse_inv, sizeof(float) * inv.size()));
+        M2D_HIP(hipMemcpy(f->coarse_inv, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice));
+    }
+    const size_t n = f->n;
+    for (float **q : {&f->pr, &f->pz, &f->pp, &f->pq, &f->pe}) M2D_TRY(dmalloc3((void **)q, sizeof(float) * n));
+    M2D_TRY(dmalloc3((void **)&f->r64, sizeof(double) * n));
+    M2D_TRY(dmalloc3((void **)&f->dots, sizeof(double) * 16));
+    M2D_HIP(hipHostMalloc((void **)&f->h_dots, sizeof(double) * 16));
+    f->npartial = (int)std::min<size_t>((n + 255) / 256, 65536);
+    M2D_TRY(dmalloc3((void **)&f->partial, sizeof(double) * 2 * std::max<size_t>(f->npartial, (n + 255) / 256)));
+    return M2D_OK;
+}
+
+constexpr int kPre3 = 2, kPost3 = 2;
+
+static unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
+
+static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
+{
+    hipStream_t s = f->ctx->stream;
+    const int L = (int)f->lv.size();
+    auto smooth = [&](Mg3Level &V, float *x, const float *b, int sweeps, bool forward) {
+        const unsigned g = blocks_for(V.n);
+        for (int it = 0; it < sweeps; it++) {
+            k3_rbgs<<<g, 256, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1);
+            k3_rbgs<<<g, 256, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
+        }
+    };
+    for (int l = 0; l < L - 1; l++) {
+        Mg3Level &V = f->lv[l];
+        float *x = l == 0 ? out : V.x;
+        const float *b = l == 0 ? rhs : V.b;
+        M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * V.n));
+        smooth(V, x, b, kPre3, true);
+        k3_resid<<<blocks_for(V.n), 256, 0, s>>>(V.dir, x, b, V.r, V.nx, V.ny, V.nz);
+        Mg3Level &C = f->lv[l + 1];
+        k3_restrict<<<blocks_for(C.n), 256, 0, s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz);
+    }
+    {
+        Mg3Level &C = f->lv[L - 1];
+        float *x = L == 1 ? out : C.x;
+        const float *b = L == 1 ? rhs : C.b;
+        if (f->coarse_dense) {
+            k3_coarse_solve<<<1, 256, sizeof(float) * f->coarse_n, s>>>(f->coarse_inv, b, x, f->coarse_n);
+        } else {
+            M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * C.n));
+            smooth(C, x, b, kCoarseSweeps / 2, true);
+            smooth(C, x, b, kCoarseSweeps / 2, false);
+        }
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        Mg3Level &V = f->lv[l];
+        Mg3Level &C = f->lv[l + 1];
+        float *x = l == 0 ? out : V.x;
+        const float *b = l == 0 ? rhs : V.b;
+        k3_prolong<<<blocks_for(V.n), 256, 0, s>>>(x, V.dir, V.nx, V.ny, V.nz, C.x, C.nx, C.ny, C.nz);
+        smooth(V, x, b, kPost3, false);
+    }
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static int dot3(m2d_fields3d *f, const float *a, const float *b, int slot)
+{
+    hipStream_t s = f->ctx->stream;
+    const int nb = std::min(f->npartial, 1024);
+    k_dot2<<<nb, 256, 0, s>>>(a, b, nullptr, nullptr, f->n, f->partial);
+    k_sum_pairs<<<1, 1024, 0, s>>>(f->partial, nb, f->dots + slot);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static int read_dots3(m2d_fields3d *f)
+{
+    M2D_HIP(hipMemcpyAsync(f->h_dots, f->dots, sizeof(double) * 16, hipMemcpyDeviceToHost, f->ctx->stream));
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+// FP32 PCG on A' e = d (d in pr), e in pe
+static int pcg3(m2d_fields3d *f, double rtol, int max_it, int *iters)
+{
+    hipStream_t s = f->ctx->stream;
+    const size_t n = f->n;
+    const unsigned gb = blocks_for(n);
+    const Mg3Level &V = f->lv[0];
+    M2D_TRY(fill_bytes(f->ctx, f->pe, 0, sizeof(float) * n));
+    M2D_TRY(dot3(f, f->pr, f->pr, 2));
+    M2D_TRY(read_dots3(f));
+    const double r0 = sqrt(f->h_dots[2]);
+    *iters = 0;
+    if (r0 == 0.0) return M2D_OK;
+    M2D_TRY(vcycle3(f, f->pr, f->pz));
+    M2D_TRY(dot3(f, f->pr, f->pz, 0));
+    k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1);
+    for (int it = 1; it <= max_it; it++) {
+        k3_apply<<<gb, 256, 0, s>>>(V.dir, f->pp, f->pq, V.nx, V.ny, V.nz);
+        M2D_TRY(dot3(f, f->pp, f->pq, 1));
+        k_cg_update<<<gb, 256, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots, n);
+        M2D_TRY(dot3(f, f->pr, f->pr, 2));
+        *iters = it;
+        M2D_TRY(read_dots3(f));
+        if (sqrt(f->h_dots[2]) <= rtol * r0) break;
+        M2D_TRY(vcycle3(f, f->pr, f->pz));
+        k_copy_scalar<<<1, 1, 0, s>>>(f->dots, f->dots + 4);
+        M2D_TRY(dot3(f, f->pr, f->pz, 0));
+        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 0);
+    }
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static int build_tables3(m2d_fields3d *f)
+{
+    k3_tables<<<blocks_for(f->n), 256, 0, f->ctx->stream>>>(f->u, f->nx, f->ny, f->nz, f->idx, f->idy, f->idz,
+                                                            f->tx, f->ty, f->tz);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+static double *field3_ptr(m2d_fields3d *f, int which)
+{
+    switch (which) {
+    case M2D_FIELD_U: return f->u;
+    case M2D_FIELD_RHO: return f->rho;
+    case M2D_FIELD_VOLTAGE: return f->voltage;
+    default: return nullptr;
+    }
+}
+
+static float f32_below(double v)  // largest float <= v
+{
+    float r = (float)v;
+    if ((double)r > v) r = nextafterf(r, -INFINITY);
+    return r;
+}
+
+static uint64_t trial_thr3(double prob)
+{
+    if (!(prob > 0.0)) return 0;
+    if (prob >= 1.0) return 1ull << 32;
+    return (uint64_t)ceil(prob * 4294967296.0 - 0.5);
+}
+
+static void fill_args3(PushArgs3 &a, m2d_fields3d *f, const m2d_species2d *s, const m2d_particles3d *p,
+                       double rot_scale, double kick_scale)
+{
+    a.x = p->x; a.y = p->y; a.z = p->z; a.vx = p->vx; a.vy = p->vy; a.vz = p->vz; a.n = p->n;
+    a.tx = f->tx; a.ty = f->ty; a.tz = f->tz;
+    a.cellfree = f->cellfree;
+    a.g = grid3_of(f->p.grid);
+    a.bx1 = f32_below(f->p.x_max); a.by1 = f32_below(f->p.y_max); a.bz1 = f32_below(f->p.z_max);
+    a.xper = f->p.x_max; a.yper = f->p.y_max; a.zper = f->p.z_max;
+    const double tmp = rot_scale * s->charge * s->dt / (2.0 * s->mass);
+    const double tx = f->p.Bx * tmp, ty = f->p.By * tmp, tz = f->p.Bz * tmp;
+    const double sc = 2.0 / (1 + tx * tx + ty * ty + tz * tz);
+    a.tx_ = (float)tx; a.ty_ = (float)ty; a.tz_ = (float)tz;
+    a.sx = (float)(tx * sc); a.sy = (float)(ty * sc); a.sz = (float)(tz * sc);
+    a.rotate = tx != 0.0 || ty != 0.0 || tz != 0.0;
+    a.qmdt2 = (float)(kick_scale * s->charge / s->mass * s->dt / 2.0);
+    a.dt = (float)s->dt;
+    a.push = 1;
+    a.boundary = 0;
+    a.periodic = f->p.boundary == M2D_BC_PERIODIC;
+    a.trial_thr = 0;
+    a.seed = f->ctx->seed;
+    a.stream = s->stream;
+    a.step = s->niter;
+    a.mcc = nullptr; a.cq = nullptr; a.cq_len = nullptr;
+    a.acc = nullptr;
+    a.counters = f->ctx->counters;
+}
+
+static void free3(m2d_fields3d *f)
+{
+    for (void *q : {(void *)f->mask, (void *)f->cellfree, (void *)f->voltage, (void *)f->u, (void *)f->rho,
+                    (void *)f->tx, (void *)f->ty, (void *)f->tz, (void *)f->coarse_inv, (void *)f->pr,
+                    (void *)f->pz, (void *)f->pp, (void *)f->pq, (void *)f->pe, (void *)f->r64,
+                    (void *)f->dots, (void *)f->partial})
+        if (q) (void)hipFree(q);
+    for (Mg3Level &V : f->lv)
+        for (void *q : {(void *)V.dir, (void *)V.x, (void *)V.b, (void *)V.r})
+            if (q) (void)hipFree(q);
+    if (f->h_dots) (void)hipHostFree(f->h_dots);
+}
+
+}  // namespace m2d
+
+using namespace m2d;
+
+extern "C" {
+
+int m2d_fields3d_create(m2d_ctx *ctx, const m2d_fields3d_params *p, const uint8_t *mask, const double *voltage,
+                        m2d_fields3d **out)
+{
+    M2D_CHECK(ctx && p && mask && voltage && out, M2D_ERR_BAD_ARG, "m2d_fields3d_create: NULL argument");
+    *out = nullptr;
+    M2D_TRY(check_grid3(&p->grid));
+    M2D_CHECK(p->x_max > 0 && p->y_max > 0 && p->z_max > 0, M2D_ERR_BAD_ARG, "m2d_fields3d_create: box");
+    const m2d_grid3d &g = p->grid;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++)
+            for (int k = 0; k < g.nz; k++) {
+                const uint8_t m = mask[((size_t)i * g.ny + j) * g.nz + k];
+                M2D_CHECK(m <= M2D_BOUNDARY, M2D_ERR_BAD_ARG, "mask value %d at (%d,%d,%d)", (int)m, i, j, k);
+                const bool face = i == 0 || j == 0 || k == 0 || i == g.nx - 1 || j == g.ny - 1 || k == g.nz - 1;
+                M2D_CHECK(!face || m == M2D_FIXED || m == M2D_FIXED_RF, M2D_ERR_BAD_ARG,
+                          "m2d_fields3d_create: node (%d,%d,%d) on a grid face is not Dirichlet", i, j, k);
+            }
+    M2D_HIP(hipSetDevice(ctx->device));
+    m2d_fields3d *f = new (std::nothrow) m2d_fields3d();
+    M2D_CHECK(f, M2D_ERR_NO_MEMORY, "m2d_fields3d_create: host allocation");
+    f->ctx = ctx;
+    ctx_retain(ctx);
+    f->p = *p;
+    f->nx = g.nx; f->ny = g.ny; f->nz = g.nz;
+    f->n = n;
+    f->idx = 1.0 / g.dx; f->idy = 1.0 / g.dy; f->idz = 1.0 / g.dz;
+    int r = M2D_OK;
+    auto fail = [&](int code) { m2d_fields3d_destroy(f); return code; };
+    if ((r = dmalloc3((void **)&f->mask, n)) || (r = dmalloc3((void **)&f->cellfree, n)) ||
+        (r = dmalloc3((void **)&f->voltage, 8 * n)) || (r = dmalloc3((void **)&f->u, 8 * n)) ||
+        (r = dmalloc3((void **)&f->rho, 8 * n)) || (r = dmalloc3((void **)&f->tx, 4 * n)) ||
+        (r = dmalloc3((void **)&f->ty, 4 * n)) || (r = dmalloc3((void **)&f->tz, 4 * n)))
+        return fail(r);
+    std::vector<uint8_t> hm(mask, mask + n), cf(n, 0);
+    for (int i = 0; i + 1 < g.nx; i++)
+        for (int j = 0; j + 1 < g.ny; j++)
+            for (int k = 0; k + 1 < g.nz; k++) {
+                bool any = false;
+                for (int c = 0; c < 8; c++)
+                    any = any || hm[((size_t)(i + (c & 1)) * g.ny + j + ((c >> 1) & 1)) * g.nz + k + (c >> 2)] ==
+                                     M2D_FREE;
+                cf[((size_t)i * g.ny + j) * g.nz + k] = any;
+            }
+    if (hipMemcpy(f->mask, mask, n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->cellfree, cf.data(), n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->voltage, voltage, 8 * n, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(M2D_ERR_HIP, "m2d_fields3d_create: upload failed"));
+    if ((r = fill_bytes(ctx, f->u, 0, 8 * n)) || (r = fill_bytes(ctx, f->rho, 0, 8 * n))) return fail(r);
+    if ((r = mg3_setup(f, hm))) return fail(r);
+    if ((r = build_tables3(f))) return fail(r);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(set_error(M2D_ERR_HIP, "m2d_fields3d_create: synchronise failed"));
+    *out = f;
+    return M2D_OK;
+}
+
+int m2d_fields3d_destroy(m2d_fields3d *f)
+{
+    if (!f) return M2D_OK;
+    if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
+    free3(f);
+    if (f->ctx) ctx_release(f->ctx);
+    delete f;
+    return M2D_OK;
+}
+
+double *m2d_fields3d_data(m2d_fields3d *f, int which) { return f ? field3_ptr(f, which) : nullptr; }
+
+int m2d_fields3d_upload(m2d_fields3d *f, int which, const double *host)
+{
+    M2D_CHECK(f && host, M2D_ERR_BAD_ARG, "m2d_fields3d_upload: NULL argument");
+    double *d = field3_ptr(f, which);
+    M2D_CHECK(d, M2D_ERR_BAD_ARG, "m2d_fields3d_upload: unknown field %d", which);
+    M2D_HIP(hipMemcpyAsync(d, host, 8 * f->n, hipMemcpyHostToDevice, f->ctx->stream));
+    if (which == M2D_FIELD_U) M2D_TRY(build_tables3(f));
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+int m2d_fields3d_download(m2d_fields3d *f, int which, double *host)
+{
+    M2D_CHECK(f && host, M2D_ERR_BAD_ARG, "m2d_fields3d_download: NULL argument");
+    double *d = field3_ptr(f, which);
+    M2D_CHECK(d, M2D_ERR_BAD_ARG, "m2d_fields3d_download: unknown field %d", which);
+    M2D_HIP(hipMemcpyAsync(host, d, 8 * f->n, hipMemcpyDeviceToHost, f->ctx->stream));
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+int m2d_fields3d_reset(m2d_fields3d *f)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields3d_reset: NULL argument");
+    return fill_bytes(f->ctx, f->rho, 0, 8 * f->n);
+}
+
+int m2d_fields3d_solve(m2d_fields3d *f, double tol, m2d_solve_info *info)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields3d_solve: NULL argument");
+    hipStream_t s = f->ctx->stream;
+    const size_t n = f->n;
+    const unsigned gb = blocks_for(n);
+    k3_rhs<<<gb, 256, 0, s>>>(f->mask, f->voltage, f->rho, f->u, n, f->p.rhs_scale);
+    M2D_HIP(hipGetLastError());
+    if (tol <= 0) tol = 1e-10;
+    int outer = 0, inner = 0;
+    double rel = 0;
+    const int max_outer = 12;
+    for (;;) {
+        k3_residual64<<<gb, 256, 0, s>>>(f->mask, f->rho, f->u, f->nx, f->ny, f->nz, f->r64, f->pr, f->partial);
+        k_sum_pairs<<<1, 1024, 0, s>>>(f->partial, (int)gb, f->dots + 8);
+        M2D_HIP(hipGetLastError());
+        M2D_TRY(read_dots3(f));
+        const double rn = sqrt(f->h_dots[8]), bn = sqrt(f->h_dots[9]);
+        rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
+        if (!std::isfinite(rn)) {
+            if (info) { info->outer_iters = outer; info->inner_iters = inner; info->rel_residual = rel; }
+            return set_error(M2D_ERR_NAN, "Fields3D solve: residual is not finite");
+        }
+        if (rel <= tol || outer >= max_outer) break;
+        int it = 0;
+        M2D_TRY(pcg3(f, 1e-5, 80, &it));
+        inner += it;
+        k3_add_correction<<<gb, 256, 0, s>>>(f->mask, f->u, f->pe, n);
+        M2D_HIP(hipGetLastError());
+        outer++;
+    }
+    if (info) { info->outer_iters = outer; info->inner_iters = inner; info->rel_residual = rel; }
+    M2D_TRY(build_tables3(f));
+    if (rel > tol)
+        return set_error(M2D_ERR_NOT_CONVERGED, "Fields3D solve: relative residual %.3e > %.3e after %d steps",
+                         rel, tol, outer);
+    return M2D_OK;
+}
+
+int m2d_accumulate_3d(m2d_ctx *ctx, const m2d_grid3d *g, const m2d_particles3d *p, double charge, double *rho)
+{
+    M2D_CHECK(ctx && rho, M2D_ERR_BAD_ARG, "m2d_accumulate_3d: NULL argument");
+    M2D_TRY(check_grid3(g));
+    M2D_TRY(check_parts3(p));
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    M2D_TRY(ensure_acc(ctx, n));
+    if (p->n > 0) {
+        k_accumulate3d<<<blocks_for((size_t)p->n), 256, 0, ctx->stream>>>(
+            p->x, p->y, p->z, p->n, grid3_of(*g), (unsigned long long *)ctx->acc.ptr, ctx->counters);
+        M2D_HIP(hipGetLastError());
+    }
+    k_flush<<<blocks_for(n), 256, 0, ctx->stream>>>((unsigned long long *)ctx->acc.ptr, rho, charge, n);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+int m2d_interpolate_3d(m2d_ctx *ctx, const m2d_grid3d *g, const double *data, const double *x, const double *y,
+                       const double *z, int64_t n, double *out)
+{
+    M2D_CHECK(ctx && data && (n == 0 || (x && y && z && out)), M2D_ERR_BAD_ARG, "m2d_interpolate_3d: NULL argument");
+    M2D_TRY(check_grid3(g));
+    if (n > 0) {
+        k_interp3d<<<blocks_for((size_t)n), 256, 0, ctx->stream>>>(data, grid3_of(*g), x, y, z, n, out);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_grad_3d(m2d_ctx *ctx, const m2d_grid3d *g, const double *data, const double *x, const double *y,
+                const double *z, int64_t n, double *gx, double *gy, double *gz)
+{
+    M2D_CHECK(ctx && data && (n == 0 || (x && y && z && gx && gy && gz)), M2D_ERR_BAD_ARG,
+              "m2d_grad_3d: NULL argument");
+    M2D_TRY(check_grid3(g));
+    if (n > 0) {
+        k_grad3d<<<blocks_for((size_t)n), 256, 0, ctx->stream>>>(data, grid3_of(*g), x, y, z, n, gx, gy, gz);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const m2d_particles3d *p,
+                   const m2d_mcc *mcc, double *rho, uint32_t flags, m2d_step_stats *stats)
+{
+    M2D_CHECK(ctx && f && s, M2D_ERR_BAD_ARG, "m2d_advance_3d: NULL argument");
+    M2D_CHECK(f->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_advance_3d: fields belong to another context");
+    M2D_TRY(check_parts3(p));
+    M2D_CHECK(s->mass > 0 && s->dt > 0, M2D_ERR_BAD_ARG, "m2d_advance_3d: mass and dt must be > 0");
+    M2D_CHECK((flags & ~(M2D_STEP_PUSH | M2D_STEP_BOUNDARY | M2D_STEP_DEPOSIT)) == 0, M2D_ERR_BAD_ARG,
+              "m2d_advance_3d: unknown flags 0x%x", flags);
+    const bool deposit = (flags & M2D_STEP_DEPOSIT) != 0;
+    M2D_CHECK(!deposit || rho, M2D_ERR_BAD_ARG, "m2d_advance_3d: DEPOSIT needs rho");
+    if (stats) M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
+    PushArgs3 a;
+    fill_args3(a, f, s, p, 1.0, 1.0);
+    a.push = (flags & M2D_STEP_PUSH) != 0;
+    a.boundary = (flags & M2D_STEP_BOUNDARY) != 0;
+    const bool use_mcc = a.push && mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
+    unsigned long long *next_len = nullptr;
+    if (use_mcc) {
+        a.trial_thr = trial_thr3(1.0 - exp(-s->dt / s->lifetime));  // species3d.cpp:8
+        a.mcc = mcc->d;
+        M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
+        a.cq = (uint32_t *)ctx->cq.ptr;
+        const int slot = (int)(ctx->launch_seq++ & 1);
+        a.cq_len = ctx->counters + CNT_QLEN + slot;
+        next_len = ctx->counters + CNT_QLEN + (slot ^ 1);
+    }
+    if (deposit) {
+        M2D_TRY(ensure_acc(ctx, f->n));
+        a.acc = (unsigned long long *)ctx->acc.ptr;
+    }
+    if (p->n > 0) {
+        hipEvent_t t0 = prof_begin(ctx);
+        const unsigned g = blocks_for((size_t)p->n);
+        if (deposit && use_mcc) k_advance3d<true, true><<<g, 256, 0, ctx->stream>>>(a);
+        else if (deposit) k_advance3d<true, false><<<g, 256, 0, ctx->stream>>>(a);
+        else if (use_mcc) k_advance3d<false, true><<<g, 256, 0, ctx->stream>>>(a);
+        else k_advance3d<false, false><<<g, 256, 0, ctx->stream>>>(a);
+        M2D_HIP(hipGetLastError());
+        prof_end(ctx, t0);
+        if (use_mcc) {
+            k_scatter3d<<<64, 256, 0, ctx->stream>>>(a, next_len);
+            M2D_HIP(hipGetLastError());
+        }
+    }
+    if (deposit) {
+        k_flush<<<blocks_for(f->n), 256, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n);
+        M2D_HIP(hipGetLastError());
+    }
+    if (stats) {
+        M2D_TRY(read_counters(ctx));
+        stats->removed = (int64_t)ctx->h_counters[CNT_REMOVED];
+        stats->collisions = (int64_t)ctx->h_counters[CNT_COLL];
+    }
+    return M2D_OK;
+}
+
+int m2d_advance_init_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const m2d_particles3d *p)
+{
+    M2D_CHECK(ctx && f && s, M2D_ERR_BAD_ARG, "m2d_advance_init_3d: NULL argument");
+    M2D_TRY(check_parts3(p));
+    M2D_CHECK(s->mass > 0 && s->dt > 0, M2D_ERR_BAD_ARG, "m2d_advance_init_3d: mass and dt must be > 0");
+    PushArgs3 a;
+    fill_args3(a, f, s, p, -0.5, -1.0);
+    if (p->n > 0) {
+        k_advance_init3d<<<blocks_for((size_t)p->n), 256, 0, ctx->stream>>>(a);
+        M2D_HIP(hipGetLastError());
+    }
+    return M2D_OK;
+}
+
+int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const m2d_particles3d *dst,
+                int64_t *n_live)
+{
+    M2D_CHECK(ctx && f && src && dst, M2D_ERR_BAD_ARG, "m2d_sort_3d: NULL argument");
+    M2D_TRY(check_parts3(src));
+    M2D_TRY(check_parts3(dst));
+    M2D_CHECK(dst->n == src->n, M2D_ERR_BAD_ARG, "m2d_sort_3d: dst holds %lld slots, src %lld",
+              (long long)dst->n, (long long)src->n);
+    M2D_CHECK(src->n == 0 || src->x != dst->x, M2D_ERR_BAD_ARG, "m2d_sort_3d: src and dst must differ");
+    const size_t n = (size_t)src->n;
+    if (n == 0) {
+        if (n_live) *n_live = 0;
+        return M2D_OK;
+    }
+    const Grid3 g = grid3_of(f->p.grid);
+    const uint64_t cells = (uint64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+    int bits = 0;
+    while ((1ull << bits) <= cells) bits++;
+    M2D_CHECK(bits <= 32, M2D_ERR_BAD_ARG, "m2d_sort_3d: grid too large for 32-bit keys");
+    const uint32_t dead_key = bits == 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
+    size_t temp = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, bits,
+                                             ctx->stream);
+    M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs size query: %s", hipGetErrorString(e));
+    const size_t arr = ((n * 4 + 255) / 256) * 256;
+    M2D_TRY(ensure(ctx, ctx->sortbuf, 4 * arr + temp + 256));
+    uint8_t *b = (uint8_t *)ctx->sortbuf.ptr;
+    uint32_t *k0 = (uint32_t *)b, *k1 = (uint32_t *)(b + arr), *v0 = (uint32_t *)(b + 2 * arr),
+             *v1 = (uint32_t *)(b + 3 * arr);
+    unsigned long long *live = ctx->counters + CNT_LIVE;
+    M2D_TRY(zero_words(ctx, live, 1));
+    k_sort_keys3d<<<blocks_for(n), 256, 0, ctx->stream>>>(src->x, src->y, src->z, src->n, g, dead_key, k0, v0);
+    M2D_HIP(hipGetLastError());
+    e = rocprim::radix_sort_pairs(b + 4 * arr, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
+    M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
+    k_sort_gather3d<<<blocks_for(n), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, dead_key, live);
+    M2D_HIP(hipGetLastError());
+    if (n_live) {
+        M2D_TRY(read_counters(ctx));
+        *n_live = (int64_t)ctx->h_counters[CNT_LIVE];
+    }
+    return M2D_OK;
+}
+
+int m2d_count_live_3d(m2d_ctx *ctx, const m2d_particles3d *p, int64_t *n_live)
+{
+    M2D_TRY(check_parts3(p));
+    const m2d_particles2d v{p->x, p->z, p->vx, p->vy, p->vz, p->n};
+    return m2d_count_live_2d(ctx, &v, n_live);
+}
+
+int m2d_mean_energy_3d(m2d_ctx *ctx, const m2d_particles3d *p, double mass, double *mean_eV, int64_t *n_live)
+{
+    M2D_TRY(check_parts3(p));
+    const m2d_particles2d v{p->x, p->z, p->vx, p->vy, p->vz, p->n};
+    return m2d_mean_energy_2d(ctx, &v, mass, mean_eV, n_live);
+}
+
+}  // extern "C"

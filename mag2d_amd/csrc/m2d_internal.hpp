@@ -116,6 +116,18 @@ void prof_end(m2d_ctx *ctx, hipEvent_t start);
 namespace m2d {
 
 int ensure(m2d_ctx *ctx, Scratch &s, size_t bytes);
+
+// vector kernels shared by the 2D and 3D solvers (m2d_fields2d.hip) and the
+// fixed-point flush (m2d_particles2d.hip)
+__global__ void __launch_bounds__(256) k_dot2(const float *a, const float *b, const float *c,
+                                              const float *d, size_t n, double *partial);
+__global__ void __launch_bounds__(1024) k_sum_pairs(const double *partial, int np, double *out);
+__global__ void k_cg_update(float *e, float *r, const float *p, const float *q, const double *sc, size_t n);
+__global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old, size_t n,
+                         int first);
+__global__ void k_copy_scalar(const double *src, double *dst);
+__global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n);
+int ensure_acc(m2d_ctx *ctx, size_t n);
 int fill_bytes(m2d_ctx *ctx, void *dst, int value, size_t bytes);
 int zero_words(m2d_ctx *ctx, unsigned long long *dst, size_t n);
 int read_counters(m2d_ctx *ctx);

@@ -1032,6 +1032,103 @@ int o_poisson3_apply(const o_grid3 *g, const uint8_t *mask, const double *x, dou
     return 0;
 }
 
+/* Solver::solve rhs (fields3d.cpp:84-93) */
+void o_poisson3_rhs(const o_grid3 *g, const uint8_t *mask, const double *voltage, double *rho,
+                    double scale)
+{
+    const long n = (long)g->nx * g->ny * g->nz;
+    for (long m = 0; m < n; m++)
+        rho[m] = (mask[m] == O_FIXED || mask[m] == O_FIXED_RF) ? voltage[m] : rho[m] * -scale;
+}
+
+/* Geometry::is_free (fields3d.hpp:48-55) */
+int o_is_free3(const o_grid3 *g, const uint8_t *mask, double x, double y, double z)
+{
+    if (!(x >= 0.0) || !(y >= 0.0) || !(z >= 0.0)) return 0;
+    int i = (int)(x * g->idx), j = (int)(y * g->idy), k = (int)(z * g->idz);
+    if (i > g->nx - 2 || j > g->ny - 2 || k > g->nz - 2) return 0;
+    for (int c = 0; c < 8; c++)
+        if (F3(mask, g, i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) == O_FREE) return 1;
+    return 0;
+}
+
+static void boris3(double qmdt2, double tx, double ty, double tz, double ex, double ey, double ez,
+                   o_particle *I)
+{
+    I->vx += ex * qmdt2;
+    I->vy += ey * qmdt2;
+    I->vz += ez * qmdt2;
+    /* species3d.cpp:30-50 */
+    double vpx = I->vx - I->vy * tz + I->vz * ty;
+    double vpy = I->vy - I->vz * tx + I->vx * tz;
+    double vpz = I->vz - I->vx * ty + I->vy * tx;
+    double tmp = 2.0 / (1 + SQR(tx) + SQR(ty) + SQR(tz));
+    double sx = tx * tmp, sy = ty * tmp, sz = tz * tmp;
+    I->vx = I->vx - vpy * sz + vpz * sy;
+    I->vy = I->vy - vpz * sx + vpx * sz;
+    I->vz = I->vz - vpx * sy + vpy * sx;
+}
+
+/* Species<CARTESIAN3D>::advance (species3d.cpp:3-94) */
+long o_advance3d(const o_fields3 *f, o_species *s, o_particle *P, long n, long *n_coll, long *n_bad)
+{
+    const double qmdt = s->charge / s->mass * s->dt;
+    const double prob = 1.0 - exp(-s->dt / s->lifetime);
+    const double tmp = s->charge * s->dt / (2.0 * s->mass);
+    long nc = 0, removed = 0, bad = 0;
+    for (long k = 0; k < n; k++) {
+        o_particle *I = &P[k];
+        if (I->empty) continue;
+        double ex, ey, ez;
+        o_f3_grad(&f->g, f->u, I->x, I->y, I->z, &ex, &ey, &ez);  /* ElMag3D::E */
+        ex = -ex; ey = -ey; ez = -ez;
+        /* half kick, rotation, half kick (species3d.cpp:24-56) */
+        boris3(qmdt / 2.0, f->Bx * tmp, f->By * tmp, f->Bz * tmp, ex, ey, ez, I);
+        I->vx += ex * qmdt / 2.0;
+        I->vy += ey * qmdt / 2.0;
+        I->vz += ez * qmdt / 2.0;
+        I->x += I->vx * s->dt;
+        I->y += I->vy * s->dt;
+        I->z += I->vz * s->dt;
+        if (s->mcc) {
+            o_rng local;
+            o_rng *r = species_rng(s, &local, k);
+            double u = s->rng_kind == O_RNG_SHR3 ? o_rng_uni(r)
+                                                 : o_trial_uniform(s->seed, s->stream,
+                                                                   (uint32_t)(k + s->slot0), s->niter);
+            if (u < prob) { o_scatter(s->mcc, I, r); nc++; }
+        }
+        if (I->x > f->x_max || I->x < 0 || I->y > f->y_max || I->y < 0 || I->z > f->z_max || I->z < 0) {
+            if (f->boundary == O_BC_FREE) { I->empty = 1; removed++; continue; }
+            I->x = o_mod(I->x, f->x_max);
+            I->y = o_mod(I->y, f->y_max);
+            I->z = o_mod(I->z, f->z_max);
+        }
+        if (!o_is_free3(&f->g, f->mask, I->x, I->y, I->z)) { I->empty = 1; removed++; continue; }
+        if (s->rho && o_f3_accumulate(&f->g, s->rho, s->charge, I->x, I->y, I->z)) bad++;
+    }
+    if (n_coll) *n_coll = nc;
+    if (n_bad) *n_bad = bad;
+    return removed;
+}
+
+void o_advance3d_init(const o_fields3 *f, o_species *s, o_particle *P, long n)
+{
+    const double qmdt = -s->charge / s->mass * s->dt;
+    const double tmp = -0.5 * s->charge * s->dt / (2.0 * s->mass);
+    for (long k = 0; k < n; k++) {
+        o_particle *I = &P[k];
+        if (I->empty) continue;
+        double ex, ey, ez;
+        o_f3_grad(&f->g, f->u, I->x, I->y, I->z, &ex, &ey, &ez);
+        ex = -ex; ey = -ey; ez = -ez;
+        boris3(0.0, f->Bx * tmp, f->By * tmp, f->Bz * tmp, 0, 0, 0, I);
+        I->vx += ex * qmdt / 2.0;
+        I->vy += ey * qmdt / 2.0;
+        I->vz += ez * qmdt / 2.0;
+    }
+}
+
 /* ======================================================================= */
 /* CPU baseline timing                                                     */
 /* ======================================================================= */

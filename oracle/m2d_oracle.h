@@ -232,6 +232,33 @@ int    o_f3_interpolate(const o_grid3 *g, const double *data, double x, double y
 void   o_f3_grad(const o_grid3 *g, const double *data, double x, double y, double z,
                  double *gx, double *gy, double *gz);
 int    o_poisson3_apply(const o_grid3 *g, const uint8_t *mask, const double *x, double *y);
+/* Solver::solve rhs (fields3d.cpp:84-93): FIXED rows take the voltage, free
+ * rows rho * -scale (the reference's scale is macroparticle_factor/eps_0) */
+void   o_poisson3_rhs(const o_grid3 *g, const uint8_t *mask, const double *voltage, double *rho,
+                      double scale);
+
+/* 3D fields of the 3D step (ElMag3D, fields3d.hpp:83-115): E = -grad u,
+ * B constant (the reference's species3d.cpp:17-18 sets it to zero; the
+ * magnetised 3D trap needs a uniform B, taken as a parameter here) */
+typedef struct {
+    o_grid3 g;                   /* origin at 0 */
+    double x_max, y_max, z_max;  /* box [0, max] (species3d.cpp:63-66) */
+    int boundary;                /* O_BC_FREE / O_BC_PERIODIC */
+    double Bx, By, Bz;
+    const double *u;
+    const uint8_t *mask;
+} o_fields3;
+/* Geometry::is_free (fields3d.hpp:48-55): any of the 8 corners FREE; cells
+ * outside [0, n-2] are not free (the reference reads past the mask there) */
+int  o_is_free3(const o_grid3 *g, const uint8_t *mask, double x, double y, double z);
+/* Species<CARTESIAN3D>::advance (species3d.cpp:3-94): E gather, Boris kick-
+ * rotate-kick in (x, y, z), drift, MCC trial + scatter, box (FREE removes,
+ * PERIODIC wraps with mod), is_free, deposit into rho (may be NULL). Returns
+ * removed particles. */
+long o_advance3d(const o_fields3 *f, o_species *s, o_particle *p, long n, long *n_coll, long *n_bad);
+/* leapfrog start of the 3D mover: rotation by -dt/2 and a backward half
+ * kick, the 2D advance_boris_init (particles.cpp:997-1050) on three axes */
+void o_advance3d_init(const o_fields3 *f, o_species *s, o_particle *p, long n);
 
 /* timing helper for the CPU baseline: n_steps of boris + boundary + deposit
  * with the SHR3 stream, returns wall seconds */

@@ -34,6 +34,25 @@ class Grid3(C.Structure):
                                   "xmax", "ymax", "zmax")]
 
 
+class Fields3(C.Structure):
+    _fields_ = [("g", Grid3), ("x_max", C.c_double), ("y_max", C.c_double), ("z_max", C.c_double),
+                ("boundary", C.c_int), ("Bx", C.c_double), ("By", C.c_double), ("Bz", C.c_double),
+                ("u", C.c_void_p), ("mask", C.c_void_p)]
+
+
+def grid3(nx, ny, nz, dx, dy, dz):
+    return Grid3(nx, ny, nz, dx, dy, dz, 1 / dx, 1 / dy, 1 / dz, 0, 0, 0, (nx - 1) * dx, (ny - 1) * dy,
+                 (nz - 1) * dz)
+
+
+def make_fields3(g, mask, u, *, boundary=0, B=(0.0, 0.0, 0.0)):
+    keep = [np.ascontiguousarray(mask, np.uint8), np.ascontiguousarray(u, np.float64)]
+    f = Fields3(g, g.xmax, g.ymax, g.zmax, boundary, B[0], B[1], B[2], keep[1].ctypes.data,
+                keep[0].ctypes.data)
+    f._keep = keep
+    return f
+
+
 class Particle(C.Structure):
     _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("vx", C.c_double),
                 ("vy", C.c_double), ("vz", C.c_double), ("time_to_death", C.c_double),
@@ -140,6 +159,11 @@ _SIGS = {
     "o_f3_interpolate": (C.c_int, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double, _PD]),
     "o_f3_grad": (None, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double, _PD, _PD, _PD]),
     "o_poisson3_apply": (C.c_int, [C.POINTER(Grid3), _P, _P, _P]),
+    "o_poisson3_rhs": (None, [C.POINTER(Grid3), _P, _P, _P, C.c_double]),
+    "o_is_free3": (C.c_int, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double]),
+    "o_advance3d": (C.c_long, [C.POINTER(Fields3), C.POINTER(Species), _P, C.c_long, C.POINTER(C.c_long),
+                               C.POINTER(C.c_long)]),
+    "o_advance3d_init": (None, [C.POINTER(Fields3), C.POINTER(Species), _P, C.c_long]),
     "o_bench_steps": (C.c_double, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long, C.c_int]),
     "o_bench_steps_omp": (C.c_double, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long, C.c_int,
                                        C.c_int]),
