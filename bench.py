@@ -206,13 +206,264 @@ def committed_traffic(particles, grid):
     return int(m.group(1))
 
 
+M_E = 9.11e-31
+M_40 = 40 * 1.66053906660e-27
+
+
+def _timed(ctx, one_step, steps, warmup):
+    """warmup untimed steps, then `steps` timed ones bracketed by device
+    synchronisation; returns (elapsed s, particle-pass ms, launches)."""
+    for k in range(warmup):
+        one_step(k)
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        one_step(k)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    kern_ms, launches = ctx.profile_read()
+    ctx.profile(False)
+    return elapsed, kern_ms, launches
+
+
+def run_penning2d(args):
+    """Self-consistent 2D Penning trap: 100 000 electrons, 128 x 128 grid over a
+    3 cm box, grounded frame with end caps at -10 V, Bz = 0.1 T, Maxwellian
+    1 eV in a centred 2 cm disk, dt = 1e-11 s; every step solves Poisson on
+    the deposited charge (Pic::advance, pic.cpp:330-358)."""
+    n, N, L = args.grid or 128, args.particles or 100_000, 3e-2
+    steps, warmup = args.steps or 1000, args.warmup if args.warmup is not None else 20
+    ctx = abi.Context(0, seed=1234)
+    dx = L / (n - 1)
+    g = abi.grid2d(n, n, dx, dx)
+    mask, volt = abi.grid_build(g, abi.GEO_EMPTY, L, L, extern_field=0.0)
+    h = 1.5 * dx
+    abi.square_electrode(g, mask, volt, 0.0, L, 0.0, h, -10.0)
+    abi.square_electrode(g, mask, volt, 0.0, L, L - h, L, -10.0)
+    density_total = 1e12
+    dV = (N / density_total) / ((n - 1) ** 2)  # Param::dV (param.cpp:132-134)
+    params = abi.FieldsParams(g, L, L, 1, 1, 0, abi.BC_FREE, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0.1, 0.0, dV)
+    fields = abi.Fields2D(ctx, params, mask, volt)
+    P = abi.ParticlesSoA(ctx, N)
+    T = 11604.5
+    vth = math.sqrt(abi.KB * T / M_E)
+    abi.add_particles_on_disk(ctx, fields, P, L / 2, L / 2, 1e-2, vth, stream=0)
+    sp = abi.Species2D(-abi.QE, M_E, 1e-11, float("inf"), 0, 0, 0)
+    rho = fields.ptr(abi.FIELD_RHO)
+    # Pic::advance_init (pic.cpp:359-384): deposit, solve, rebuild rho, leapfrog start
+    fields.reset()
+    abi.accumulate(ctx, g, P, -abi.QE, rho)
+    fields.solve(abi.SOLVE_STATIC, 1e-10)
+    fields.reset()
+    abi.accumulate(ctx, g, P, -abi.QE, rho)
+    abi.advance_init(ctx, fields, sp, P)
+    bufs = [P, abi.ParticlesSoA(ctx, N)]
+    cur = [0]
+    solve_info = []
+
+    def one_step(k):
+        if args.sort_every and k % args.sort_every == 0:
+            abi.sort(ctx, fields, bufs[cur[0]], bufs[1 - cur[0]], want_count=False)
+            cur[0] = 1 - cur[0]
+        solve_info.append(fields.solve(abi.SOLVE_STATIC, 1e-10))
+        fields.reset()
+        sp.niter = k
+        abi.advance(ctx, fields, sp, bufs[cur[0]], rho_ptr=rho,
+                    flags=abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT)
+
+    elapsed, kern_ms, launches = _timed(ctx, one_step, steps, warmup)
+    live = abi.count_live(ctx, bufs[cur[0]])
+    inner = [s.inner_iters for s in solve_info[warmup:]]
+    out = {
+        "metric": "particle-steps/s, self-consistent 2D Penning trap (solve every step)",
+        "value": live * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 push/deposit, f64 Poisson residual", "data": "synthetic",
+        "config": {"workload": "2D Penning trap, 100k electrons, 128x128, Bz 0.1 T, end caps -10 V, "
+                               "Poisson every step", "particles": N, "live_particles_end": live,
+                   "grid": [n, n], "dt": 1e-11, "sort_every": args.sort_every},
+        "particle_pass_us": kern_ms / max(launches, 1) * 1e3,
+        "poisson": {"tol": 1e-10, "mean_inner_pcg_iters": float(np.mean(inner)),
+                    "max_rel_residual": max(s.rel_residual for s in solve_info[warmup:])},
+        "device_flags": ctx.status(clear=True),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_penning(fields, mask, volt, n, L, dV, N, vth, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_penning(fields, mask, volt, n, L, dV, N, vth, budget_s):
+    """The oracle's Pic ordering on the same trap: FP64 direct Poisson solve
+    every step, Boris, boundary and deposit, serial."""
+    from oracle import oracle as O
+    import ctypes as C
+    dx = L / (n - 1)
+    og = O.grid2(n, n, dx, dx)
+    u = np.zeros(n * n)
+    of = O.make_fields(og, mask, u, x_max=L, z_max=L, geometry_empty=1, selfconsistent=1, Bz=0.1)
+    u = of._keep[1]
+    p = O.particles(N)
+    O.lib().o_add_particles_on_disk_philox(O.dptr(p), N, L / 2, L / 2, 1e-2, vth, C.byref(of), 1234, 0)
+    rho = np.zeros(n * n)
+    s = O.make_species(-abi.QE, M_E, 1e-11, rho=rho)
+    O.lib().o_accumulate(C.byref(of), C.byref(s), O.dptr(p), N)
+
+    def step():
+        b = rho.copy()
+        O.lib().o_poisson_rhs(C.byref(og), O.dptr(mask), O.dptr(volt), O.dptr(b), O.SOLVE_STATIC, dV)
+        out = np.zeros_like(b)
+        O.lib().o_poisson_solve(C.byref(og), O.dptr(mask), O.dptr(b), O.dptr(out))
+        u[:] = out
+        rho[:] = 0
+        O.lib().o_advance_boris(C.byref(of), C.byref(s), O.dptr(p), N, None)
+        O.lib().o_advance_boundary(C.byref(of), C.byref(s), O.dptr(p), N, None)
+        s.niter += 1
+
+    t0 = time.perf_counter()
+    step()
+    per = time.perf_counter() - t0
+    k = max(1, int(budget_s / per) - 1)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        step()
+    t = time.perf_counter() - t0
+    live = int((p["empty"] == 0).sum())
+    return {"value": live * k / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{N} electrons x {k} steps of the same trap, FP64 banded direct solve every step, "
+                      f"serial; {t:.1f} s"}
+
+
+def quad_trap_3d(n, L, V_rod=2.0, V_end=5.0, r0=0.3, a=0.08):
+    """Linear quadrupole in a box: four rods along z (+V, -V, +V, -V) at
+    radius r0 L, end caps (the z faces) at V_end, the other faces grounded."""
+    h = L / (n - 1)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    X, Y = i * h - L / 2, j * h - L / 2
+    mask = np.full((n, n, n), abi.FREE, np.uint8)
+    volt = np.zeros((n, n, n))
+    face = (i == 0) | (j == 0) | (k == 0) | (i == n - 1) | (j == n - 1) | (k == n - 1)
+    mask[face] = abi.FIXED
+    volt[(k == 0) | (k == n - 1)] = V_end
+    for q, (cx, cy) in enumerate([(r0, 0), (0, r0), (-r0, 0), (0, -r0)]):
+        rod = (X - cx * L) ** 2 + (Y - cy * L) ** 2 <= (a * L) ** 2
+        mask[rod & ~face] = abi.FIXED
+        volt[rod & ~face] = V_rod if q % 2 == 0 else -V_rod
+    return mask.ravel(), volt.ravel(), h
+
+
+def run_trap3d(args):
+    """3D trap: 128^3 grid over a 1 cm box, linear quadrupole rods with end
+    caps, Bz = 0.5 T, 16 777 216 ions of 40 u (Gaussian cloud, 0.05 eV),
+    space-charge Poisson (7-point multigrid PCG, FP64 refinement to 1e-10)
+    every step, fixed-point CIC, cell sort every 20 steps."""
+    n, N, L = args.grid or 128, args.particles or 16 * 1024 * 1024, 1e-2
+    steps, warmup = args.steps or 100, args.warmup if args.warmup is not None else 5
+    ctx = abi.Context(0, seed=1234)
+    mask, volt, h = quad_trap_3d(n, L)
+    g = abi.grid3d(n, n, n, h, h, h)
+    params = abi.Fields3DParams(g, L, L, L, abi.BC_FREE, 0.0, 0.0, 0.5, 1e13)
+    fields = abi.Fields3D(ctx, params, mask, volt)
+    rng = np.random.default_rng(1234)
+    pos = np.clip(L / 2 + rng.normal(size=(3, N)).astype(np.float32) * np.float32(0.1 * L),
+                  np.float32(0.2 * L), np.float32(0.8 * L)).astype(np.float32)
+    vth = math.sqrt(0.05 * abi.QE / M_40)
+    vel = (rng.normal(size=(3, N)).astype(np.float32) * np.float32(vth)).astype(np.float32)
+    P = abi.ParticlesSoA3D.from_host(ctx, *pos, *vel)
+    del pos, vel
+    sp = abi.Species2D(abi.QE, M_40, 1e-8, float("inf"), 0, 0, 0)
+    rho = fields.ptr(abi.FIELD_RHO)
+    fields.reset()
+    abi.accumulate_3d(ctx, g, P, abi.QE, rho)
+    t0 = time.perf_counter()
+    info0 = fields.solve(1e-10)
+    ctx.sync()
+    solve0_ms = (time.perf_counter() - t0) * 1e3
+    fields.reset()
+    abi.advance_init_3d(ctx, fields, sp, P)
+    bufs = [P, abi.ParticlesSoA3D(ctx, N)]
+    cur = [0]
+    infos = []
+
+    def one_step(k):
+        if args.sort_every and k % args.sort_every == 0:
+            abi.sort_3d(ctx, fields, bufs[cur[0]], bufs[1 - cur[0]])
+            cur[0] = 1 - cur[0]
+        if k > 0:
+            infos.append(fields.solve(1e-10))
+            fields.reset()
+        sp.niter = k
+        abi.advance_3d(ctx, fields, sp, bufs[cur[0]], rho_ptr=rho,
+                       flags=abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT)
+
+    elapsed, kern_ms, launches = _timed(ctx, one_step, steps, warmup)
+    live = abi.count_live_3d(ctx, bufs[cur[0]])
+    kern_us = kern_ms / max(launches, 1) * 1e3
+    bytes_launch = 48.0 * live + 12.0 * n ** 3  # 24 B state read+written, FP32 gradient tables once
+    out = {
+        "metric": "particle-steps/s, 3D trap with space-charge Poisson every step",
+        "value": live * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 push/deposit, f64 Poisson residual", "data": "synthetic",
+        "config": {"workload": "3D linear quadrupole + end caps, 128^3, Bz 0.5 T, 16M ions of 40 u, "
+                               "Poisson every step", "particles": N, "live_particles_end": live,
+                   "grid": [n, n, n], "dt": 1e-8, "sort_every": args.sort_every},
+        "roofline": {"kernel": "k_advance3d", "bound": "hbm", "achieved": bytes_launch / (kern_us * 1e-6) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_launch / (kern_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_avg_us": kern_us, "bytes_per_launch": bytes_launch},
+        "poisson": {"tol": 1e-10, "first_solve_ms": solve0_ms, "first_outer": info0.outer_iters,
+                    "first_inner": info0.inner_iters,
+                    "mean_inner_pcg_iters": float(np.mean([s.inner_iters for s in infos])) if infos else None,
+                    "max_rel_residual": max(s.rel_residual for s in infos) if infos else None},
+        "device_flags": ctx.status(clear=True),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_3d(fields, mask, n, L, h, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_3d(fields, mask, n, L, h, budget_s):
+    """The oracle's Species<CARTESIAN3D> step (push, boundary, deposit) on a
+    sample of the same trap; the 3D solve has no CPU restatement here."""
+    from oracle import oracle as O
+    import ctypes as C
+    u = fields.download(abi.FIELD_U)
+    og = O.grid3(n, n, n, h, h, h)
+    of = O.make_fields3(og, mask, u, B=(0.0, 0.0, 0.5))
+    rng = np.random.default_rng(1)
+    N = 200_000
+    p = O.particles(N)
+    for c in ("x", "y", "z"):
+        p[c] = np.clip(L / 2 + rng.normal(size=N) * 0.1 * L, 0.2 * L, 0.8 * L)
+    vth = math.sqrt(0.05 * abi.QE / M_40)
+    for c in ("vx", "vy", "vz"):
+        p[c] = rng.normal(size=N) * vth
+    rho = np.zeros(n ** 3)
+    s = O.make_species(abi.QE, M_40, 1e-8, rho=rho)
+    t0 = time.perf_counter()
+    O.lib().o_advance3d(C.byref(of), C.byref(s), O.dptr(p), N, None, None)
+    per = time.perf_counter() - t0
+    k = max(2, int(budget_s / per))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        O.lib().o_advance3d(C.byref(of), C.byref(s), O.dptr(p), N, None, None)
+    t = time.perf_counter() - t0
+    return {"value": N * k / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{N} ions x {k} steps of the same 3D trap (push, boundary, deposit; no solve), "
+                      f"serial FP64; {t:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--particles", type=int, default=4 * 1024 * 1024)
-    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--workload", choices=("rf22pt", "penning2d", "trap3d"), default="rf22pt",
+                    help="rf22pt: the headline RF trap (default); penning2d: self-consistent 2D Penning "
+                         "trap; trap3d: 3D quadrupole with space charge")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--particles", type=int, default=None)
+    ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--sort-every", type=int, default=20,
                     help="cell-sort the particle store every K steps (inside the timed region)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -237,6 +488,14 @@ def main():
               "particle pass would shard by particles with an int64 all-reduce of the deposit grid, "
               "which this build does not implement", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "penning2d":
+        return run_penning2d(args)
+    if args.workload == "trap3d":
+        return run_trap3d(args)
+    args.steps = args.steps if args.steps is not None else 200
+    args.warmup = args.warmup if args.warmup is not None else 20
+    args.particles = args.particles or 4 * 1024 * 1024
+    args.grid = args.grid or 512
 
     ctx = abi.Context(0, seed=1234)
     g, mask, volt, params, fields, mcc, poisson = build_scene(ctx, args.grid, args.particles)

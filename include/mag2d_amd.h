@@ -379,6 +379,8 @@ int m2d_advance_init_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s,
 /* stable re-order by cell into dst (same slot count), empty slots dropped to the end */
 int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const m2d_particles3d *dst,
                 int64_t *n_live);
+/* drop the tile schedule of a 3D store written by other means than m2d_sort_3d */
+int m2d_sort_forget_3d(m2d_ctx *ctx, const m2d_particles3d *p);
 /* n_particles() and energy_dist.mean_tot() of a 3D store */
 int m2d_count_live_3d(m2d_ctx *ctx, const m2d_particles3d *p, int64_t *n_live);
 int m2d_mean_energy_3d(m2d_ctx *ctx, const m2d_particles3d *p, double mass, double *mean_eV,

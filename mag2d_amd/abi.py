@@ -186,6 +186,7 @@ _SIGS = {
     "m2d_sort_3d": (C.c_int, [_P, _P, C.POINTER(Particles3D), C.POINTER(Particles3D),
                               C.POINTER(C.c_int64)]),
     "m2d_count_live_3d": (C.c_int, [_P, C.POINTER(Particles3D), C.POINTER(C.c_int64)]),
+    "m2d_sort_forget_3d": (C.c_int, [_P, C.POINTER(Particles3D)]),
     "m2d_mean_energy_3d": (C.c_int, [_P, C.POINTER(Particles3D), C.c_double, _PD,
                                      C.POINTER(C.c_int64)]),
 }
@@ -599,6 +600,8 @@ class ParticlesSoA3D:
             assert a.shape == (self.n,)
             if self.n:
                 self.arr[k].set(a)
+        s = self.struct()
+        check(lib().m2d_sort_forget_3d(self.ctx.h, C.byref(s)))
 
     def download(self):
         return {k: self.arr[k].get()[: self.n] for k in self.NAMES}

@@ -426,100 +426,216 @@ __device__ __forceinline__ double mod3(double x, double y)
     return x - y * (int)(x / y) + (x < 0 ? y : 0);
 }
 
+// 3D sort tiles: 4 x 4 x 4 cells, key = tile << 6 | cell in tile; the deposit
+// window of a tile spans its cells plus a halo of 2 cells: 9 nodes per axis
+constexpr int kTile3 = 4, kHalo3 = 2, kWin3 = kTile3 + 1 + 2 * kHalo3, kWinN3 = kWin3 * kWin3 * kWin3;
+
+struct Step3 {
+    bool removed = false, coll = false, nan_pos = false, bad = false;
+    int cell = -1;        // deposit cell (lower corner node), -1: none
+    int ci = 0, cj = 0, ck = 0;
+    uint32_t w8[8];
+};
+
+// Species<CARTESIAN3D>::advance for the particle in slot p (species3d.cpp:10-92):
+// gather, Boris, drift, MCC trial, box / mod, is_free, deposit weights
+template <bool DEPOSIT, bool MCC>
+__device__ __forceinline__ void step3(const PushArgs3 &a, int64_t p, Step3 &o)
+{
+    float x = a.x[p];
+    if (is_dead(x)) return;
+    float y = a.y[p], z = a.z[p];
+    float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+    if (a.push) {
+        // E = -grad u (ElMag3D::E, fields3d.hpp:91-97)
+        const double xd = x, yd = y, zd = z;
+        const float ex = -gather3_component(a.tx, a.g, xd, yd, zd, 1, 0, 0);
+        const float ey = -gather3_component(a.ty, a.g, xd, yd, zd, 0, 1, 0);
+        const float ez = -gather3_component(a.tz, a.g, xd, yd, zd, 0, 0, 1);
+        // species3d.cpp:24-61
+        vx = fmaf(ex, a.qmdt2, vx);
+        vy = fmaf(ey, a.qmdt2, vy);
+        vz = fmaf(ez, a.qmdt2, vz);
+        if (a.rotate) {
+            float vpx = vx - vy * a.tz_ + vz * a.ty_;
+            float vpy = vy - vz * a.tx_ + vx * a.tz_;
+            float vpz = vz - vx * a.ty_ + vy * a.tx_;
+            vx = vx - vpy * a.sz + vpz * a.sy;
+            vy = vy - vpz * a.sx + vpx * a.sz;
+            vz = vz - vpx * a.sy + vpy * a.sx;
+        }
+        vx = fmaf(ex, a.qmdt2, vx);
+        vy = fmaf(ey, a.qmdt2, vy);
+        vz = fmaf(ez, a.qmdt2, vz);
+        x = fmaf(vx, a.dt, x);
+        y = fmaf(vy, a.dt, y);
+        z = fmaf(vz, a.dt, z);
+        // MCC trial (species3d.cpp:64-67); the velocity change is applied after the pass
+        if (MCC) {
+            uint32_t c0 = (uint32_t)(p >> 2), c1 = (uint32_t)a.step, c2 = 0u;
+            uint32_t c3 = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
+            philox4x32_10(c0, c1, c2, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
+            const int q = (int)(p & 3);
+            const uint32_t tw = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : c3;
+            if ((uint64_t)tw < a.trial_thr) {
+                unsigned long long at = atomicAdd(a.cq_len, 1ull);
+                a.cq[at] = (uint32_t)p;
+                o.coll = true;
+            }
+        }
+    }
+    if (a.boundary) {
+        // species3d.cpp:69-85
+        if (x > a.bx1 || x < 0.f || y > a.by1 || y < 0.f || z > a.bz1 || z < 0.f || x != x || y != y || z != z) {
+            if (x != x || y != y || z != z) { o.removed = true; o.nan_pos = true; }
+            else if (!a.periodic) o.removed = true;
+            else {
+                x = (float)mod3(x, a.xper);
+                y = (float)mod3(y, a.yper);
+                z = (float)mod3(z, a.zper);
+            }
+        }
+        if (!o.removed) {
+            // Geometry::is_free (fields3d.hpp:48-55)
+            const double X = (double)x * a.g.idx, Y = (double)y * a.g.idy, Z = (double)z * a.g.idz;
+            bool fr = false;
+            if (X >= 0.0 && X < a.g.nx - 1.0 && Y >= 0.0 && Y < a.g.ny - 1.0 && Z >= 0.0 && Z < a.g.nz - 1.0)
+                fr = a.cellfree[((size_t)(int)X * a.g.ny + (int)Y) * a.g.nz + (int)Z] != 0;
+            o.removed = !fr;
+        }
+    }
+    if (DEPOSIT && !o.removed) {
+        const double X = (double)x * a.g.idx, Y = (double)y * a.g.idy, Z = (double)z * a.g.idz;
+        if (cic3(X, Y, Z, a.g, o.cell, o.w8)) {
+            o.ci = (int)X; o.cj = (int)Y; o.ck = (int)Z;
+        } else {
+            o.cell = -1;
+            o.bad = true;
+        }
+    }
+    if (o.removed) {
+        a.x[p] = dead_value();
+    } else {
+        a.x[p] = x; a.y[p] = y; a.z[p] = z;
+        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+    }
+}
+
+__device__ __forceinline__ void deposit8_global(unsigned long long *acc, const Grid3 &g, int cell,
+                                                const uint32_t w8[8])
+{
+    const size_t sx = (size_t)g.ny * g.nz;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const size_t m = (size_t)cell + ((q & 1) ? sx : 0) + (((q >> 1) & 1) ? (size_t)g.nz : 0) + (q >> 2);
+        if (w8[q]) atomicAdd(acc + m, (unsigned long long)w8[q]);
+    }
+}
+
+// untiled: one particle per thread, deposit summed across the wavefront
 template <bool DEPOSIT, bool MCC>
 __global__ void __launch_bounds__(256) k_advance3d(PushArgs3 a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = p < a.n;
-    float x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
-    if (live) {
-        x = a.x[p];
-        live = !is_dead(x);
-    }
-    bool removed = false, coll = false, nan_pos = false, bad = false;
-    int cell = -1;
-    uint32_t w8[8];
-    if (live) {
-        y = a.y[p]; z = a.z[p];
-        vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
-        if (a.push) {
-            // E = -grad u (ElMag3D::E, fields3d.hpp:91-97)
-            const double xd = x, yd = y, zd = z;
-            const float ex = -gather3_component(a.tx, a.g, xd, yd, zd, 1, 0, 0);
-            const float ey = -gather3_component(a.ty, a.g, xd, yd, zd, 0, 1, 0);
-            const float ez = -gather3_component(a.tz, a.g, xd, yd, zd, 0, 0, 1);
-            // species3d.cpp:24-61
-            vx = fmaf(ex, a.qmdt2, vx);
-            vy = fmaf(ey, a.qmdt2, vy);
-            vz = fmaf(ez, a.qmdt2, vz);
-            if (a.rotate) {
-                float vpx = vx - vy * a.tz_ + vz * a.ty_;
-                float vpy = vy - vz * a.tx_ + vx * a.tz_;
-                float vpz = vz - vx * a.ty_ + vy * a.tx_;
-                vx = vx - vpy * a.sz + vpz * a.sy;
-                vy = vy - vpz * a.sx + vpx * a.sz;
-                vz = vz - vpx * a.sy + vpy * a.sx;
-            }
-            vx = fmaf(ex, a.qmdt2, vx);
-            vy = fmaf(ey, a.qmdt2, vy);
-            vz = fmaf(ez, a.qmdt2, vz);
-            x = fmaf(vx, a.dt, x);
-            y = fmaf(vy, a.dt, y);
-            z = fmaf(vz, a.dt, z);
-            // MCC trial (species3d.cpp:64-67); the velocity change is applied after the pass
-            if (MCC) {
-                uint32_t c0 = (uint32_t)(p >> 2), c1 = (uint32_t)a.step, c2 = 0u;
-                uint32_t c3 = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
-                philox4x32_10(c0, c1, c2, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
-                const int q = (int)(p & 3);
-                const uint32_t tw = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : c3;
-                if ((uint64_t)tw < a.trial_thr) {
-                    unsigned long long at = atomicAdd(a.cq_len, 1ull);
-                    a.cq[at] = (uint32_t)p;
-                    coll = true;
+    Step3 o;
+    if (p < a.n) step3<DEPOSIT, MCC>(a, p, o);
+    if (DEPOSIT) wave_deposit3(o.cell, o.w8, a.g, a.acc);
+    wave_count(&a.counters[CNT_REMOVED], o.removed);
+    if (MCC) wave_count(&a.counters[CNT_COLL], o.coll);
+    if (__ballot(o.nan_pos) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+    if (__ballot(o.bad) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+}
+
+// tiled (store sorted by m2d_sort_3d): block b runs chunk sched[b] of one
+// sort tile and sums its deposits in an LDS window of the tile + halo,
+// written out with plain stores for k_reduce_tiles3; deposits that leave
+// the window go to the global grid
+template <bool MCC>
+__global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4 *sched,
+                                                         const unsigned int *n_chunks, int nty3, int ntz3,
+                                                         unsigned long long *partials)
+{
+    __shared__ unsigned long long win[kWinN3];
+    if (blockIdx.x >= *n_chunks) return;  // uniform, before any barrier
+    const int4 s = sched[blockIdx.x];
+    const int t = s.x;
+    const int wi0 = (t / (nty3 * ntz3)) * kTile3 - kHalo3, wj0 = ((t / ntz3) % nty3) * kTile3 - kHalo3,
+              wk0 = (t % ntz3) * kTile3 - kHalo3;
+    for (int k = threadIdx.x; k < kWinN3; k += 256) win[k] = 0ull;
+    __syncthreads();
+    bool removed_any = false, coll_any = false, nan_any = false, bad_any = false;
+    int n_removed = 0, n_coll = 0;
+    for (int64_t p = s.y + threadIdx.x; p < s.z; p += 256) {
+        Step3 o;
+        step3<true, MCC>(a, p, o);
+        n_removed += o.removed;
+        n_coll += o.coll;
+        nan_any |= o.nan_pos;
+        bad_any |= o.bad;
+        if (o.cell >= 0) {
+            const int da = o.ci - wi0, db = o.cj - wj0, dc = o.ck - wk0;
+            if ((unsigned)da < (unsigned)(kWin3 - 1) && (unsigned)db < (unsigned)(kWin3 - 1) &&
+                (unsigned)dc < (unsigned)(kWin3 - 1)) {
+                unsigned long long *w = win + (da * kWin3 + db) * kWin3 + dc;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int off = ((q & 1) ? kWin3 * kWin3 : 0) + (((q >> 1) & 1) ? kWin3 : 0) + (q >> 2);
+                    if (o.w8[q]) atomicAdd(w + off, (unsigned long long)o.w8[q]);
                 }
+            } else {
+                deposit8_global(a.acc, a.g, o.cell, o.w8);
             }
-        }
-        if (a.boundary) {
-            // species3d.cpp:69-85
-            if (x > a.bx1 || x < 0.f || y > a.by1 || y < 0.f || z > a.bz1 || z < 0.f || x != x || y != y ||
-                z != z) {
-                if (x != x || y != y || z != z) { removed = true; nan_pos = true; }
-                else if (!a.periodic) removed = true;
-                else {
-                    x = (float)mod3(x, a.xper);
-                    y = (float)mod3(y, a.yper);
-                    z = (float)mod3(z, a.zper);
-                }
-            }
-            if (!removed) {
-                // Geometry::is_free (fields3d.hpp:48-55)
-                const double X = (double)x * a.g.idx, Y = (double)y * a.g.idy, Z = (double)z * a.g.idz;
-                bool fr = false;
-                if (X >= 0.0 && X < a.g.nx - 1.0 && Y >= 0.0 && Y < a.g.ny - 1.0 && Z >= 0.0 &&
-                    Z < a.g.nz - 1.0)
-                    fr = a.cellfree[((size_t)(int)X * a.g.ny + (int)Y) * a.g.nz + (int)Z] != 0;
-                removed = !fr;
-            }
-        }
-        if (DEPOSIT && !removed) {
-            if (!cic3((double)x * a.g.idx, (double)y * a.g.idy, (double)z * a.g.idz, a.g, cell, w8)) {
-                cell = -1;
-                bad = true;
-            }
-        }
-        if (removed) {
-            a.x[p] = dead_value();
-        } else {
-            a.x[p] = x; a.y[p] = y; a.z[p] = z;
-            a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
         }
     }
-    if (DEPOSIT) wave_deposit3(cell, w8, a.g, a.acc);
-    wave_count(&a.counters[CNT_REMOVED], removed);
-    if (MCC) wave_count(&a.counters[CNT_COLL], coll);
-    if (__ballot(nan_pos) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
-    if (__ballot(bad) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+    (void)removed_any; (void)coll_any;
+    __syncthreads();
+    unsigned long long *out = partials + (size_t)blockIdx.x * kWinN3;
+    for (int k = threadIdx.x; k < kWinN3; k += 256) out[k] = win[k];
+    int sum = n_removed;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (__lane_id() == 0 && sum) atomicAdd(&a.counters[CNT_REMOVED], (unsigned long long)sum);
+    if (MCC) {
+        int c = n_coll;
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if (__lane_id() == 0 && c) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)c);
+    }
+    if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+    if (__ballot(bad_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+}
+
+// node sums of the tiled pass: every node of a 4 x 4 x 4 output tile adds the
+// windows of the chunks of the 27 neighbouring sort tiles that cover it and
+// the global fallback; rho += charge * sum * 2^-31; the global grid is cleared
+__global__ void __launch_bounds__(64) k_reduce_tiles3(const unsigned long long *partials, const int *chunk_first,
+                                                      const int *chunk_count, int ntx3, int nty3, int ntz3,
+                                                      unsigned long long *acc, double *rho, double charge, Grid3 g)
+{
+    const int TI = blockIdx.z, TJ = blockIdx.y, TK = blockIdx.x;
+    const int i = TI * kTile3 + threadIdx.x / 16, j = TJ * kTile3 + (threadIdx.x / 4) % 4,
+              k = TK * kTile3 + threadIdx.x % 4;
+    if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+    const size_t m = ((size_t)i * g.ny + j) * g.nz + k;
+    long long s = (long long)acc[m];
+    for (int ai = max(TI - 1, 0); ai <= min(TI + 1, ntx3 - 1); ai++) {
+        const int da = i - (ai * kTile3 - kHalo3);
+        if (da < 0 || da >= kWin3) continue;
+        for (int aj = max(TJ - 1, 0); aj <= min(TJ + 1, nty3 - 1); aj++) {
+            const int db = j - (aj * kTile3 - kHalo3);
+            if (db < 0 || db >= kWin3) continue;
+            for (int ak = max(TK - 1, 0); ak <= min(TK + 1, ntz3 - 1); ak++) {
+                const int dc = k - (ak * kTile3 - kHalo3);
+                if (dc < 0 || dc >= kWin3) continue;
+                const int t = (ai * nty3 + aj) * ntz3 + ak;
+                const int c0 = chunk_first[t], c1 = c0 + chunk_count[t];
+                const int off = (da * kWin3 + db) * kWin3 + dc;
+                for (int c = c0; c < c1; c++) s += (long long)partials[(size_t)c * kWinN3 + off];
+            }
+        }
+    }
+    if (s) {
+        rho[m] += charge * ((double)s * kInvFix);
+        if (acc[m]) acc[m] = 0ull;
+    }
 }
 
 // BaseSpecies::scatter for the queued slots (velocity only, after the pass)
@@ -588,8 +704,9 @@ __global__ void __launch_bounds__(256) k_accumulate3d(const float *xs, const flo
 }
 
 // ------------------------------------------------------------------ sort
-__global__ void k_sort_keys3d(const float *xs, const float *ys, const float *zs, int64_t n, Grid3 g,
-                              uint32_t dead_key, uint32_t *keys, uint32_t *vals)
+// tile-major cell key: (tile << 6) | cell in its 4 x 4 x 4 tile
+__global__ void k_sort_keys3d(const float *xs, const float *ys, const float *zs, int64_t n, Grid3 g, int nty3,
+                              int ntz3, uint32_t dead_key, uint32_t *keys, uint32_t *vals)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -599,7 +716,8 @@ __global__ void k_sort_keys3d(const float *xs, const float *ys, const float *zs,
         auto idx = [](double X, int nmax) { return X >= 0.0 ? (int)fmin(X, (double)nmax) : 0; };
         const int i = idx((double)x * g.idx, g.nx - 2), j = idx((double)ys[p] * g.idy, g.ny - 2),
                   k = idx((double)zs[p] * g.idz, g.nz - 2);
-        key = (uint32_t)(((size_t)i * (g.ny - 1) + j) * (g.nz - 1) + k);
+        const uint32_t tile = (uint32_t)(((i / kTile3) * nty3 + j / kTile3) * ntz3 + k / kTile3);
+        key = (tile << 6) | (uint32_t)((i % kTile3) * 16 + (j % kTile3) * 4 + k % kTile3);
     }
     keys[p] = key;
     vals[p] = (uint32_t)p;
@@ -1154,13 +1272,31 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
         M2D_TRY(ensure_acc(ctx, f->n));
         a.acc = (unsigned long long *)ctx->acc.ptr;
     }
+    // a store sorted by m2d_sort_3d deposits through per-chunk LDS windows
+    const int ntx3 = (f->nx - 2) / kTile3 + 1, nty3 = (f->ny - 2) / kTile3 + 1, ntz3 = (f->nz - 2) / kTile3 + 1;
+    const m2d_tiling *T = deposit ? find_tiling(ctx, p->x, p->n) : nullptr;
+    if (T && (T->ntx != ntx3 || T->nty != nty3 || T->ntz != ntz3)) T = nullptr;
+    unsigned long long *partials = nullptr;
+    if (T) {
+        M2D_TRY(ensure(ctx, ctx->partials, (size_t)T->max_chunks * kWinN3 * sizeof(unsigned long long)));
+        partials = (unsigned long long *)ctx->partials.ptr;
+    }
     if (p->n > 0) {
         hipEvent_t t0 = prof_begin(ctx);
-        const unsigned g = blocks_for((size_t)p->n);
-        if (deposit && use_mcc) k_advance3d<true, true><<<g, 256, 0, ctx->stream>>>(a);
-        else if (deposit) k_advance3d<true, false><<<g, 256, 0, ctx->stream>>>(a);
-        else if (use_mcc) k_advance3d<false, true><<<g, 256, 0, ctx->stream>>>(a);
-        else k_advance3d<false, false><<<g, 256, 0, ctx->stream>>>(a);
+        if (T) {
+            if (use_mcc)
+                k_advance3d_tiled<true><<<T->max_chunks, 256, 0, ctx->stream>>>(a, T->sched, T->n_chunks, nty3, ntz3,
+                                                                               partials);
+            else
+                k_advance3d_tiled<false><<<T->max_chunks, 256, 0, ctx->stream>>>(a, T->sched, T->n_chunks, nty3,
+                                                                                ntz3, partials);
+        } else {
+            const unsigned g = blocks_for((size_t)p->n);
+            if (deposit && use_mcc) k_advance3d<true, true><<<g, 256, 0, ctx->stream>>>(a);
+            else if (deposit) k_advance3d<true, false><<<g, 256, 0, ctx->stream>>>(a);
+            else if (use_mcc) k_advance3d<false, true><<<g, 256, 0, ctx->stream>>>(a);
+            else k_advance3d<false, false><<<g, 256, 0, ctx->stream>>>(a);
+        }
         M2D_HIP(hipGetLastError());
         prof_end(ctx, t0);
         if (use_mcc) {
@@ -1168,7 +1304,12 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
             M2D_HIP(hipGetLastError());
         }
     }
-    if (deposit) {
+    if (T) {
+        dim3 gr((f->nz + kTile3 - 1) / kTile3, (f->ny + kTile3 - 1) / kTile3, (f->nx + kTile3 - 1) / kTile3);
+        k_reduce_tiles3<<<gr, 64, 0, ctx->stream>>>(partials, T->chunk_first, T->chunk_count, ntx3, nty3, ntz3,
+                                                    a.acc, rho, s->charge, a.g);
+        M2D_HIP(hipGetLastError());
+    } else if (deposit) {
         k_flush<<<blocks_for(f->n), 256, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n);
         M2D_HIP(hipGetLastError());
     }
@@ -1204,14 +1345,17 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
               (long long)dst->n, (long long)src->n);
     M2D_CHECK(src->n == 0 || src->x != dst->x, M2D_ERR_BAD_ARG, "m2d_sort_3d: src and dst must differ");
     const size_t n = (size_t)src->n;
+    forget_tiling(ctx, src->x);
+    forget_tiling(ctx, dst->x);
     if (n == 0) {
         if (n_live) *n_live = 0;
         return M2D_OK;
     }
     const Grid3 g = grid3_of(f->p.grid);
-    const uint64_t cells = (uint64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+    const int ntx3 = (g.nx - 2) / kTile3 + 1, nty3 = (g.ny - 2) / kTile3 + 1, ntz3 = (g.nz - 2) / kTile3 + 1;
+    const uint64_t keyspace = ((uint64_t)ntx3 * nty3 * ntz3) << 6;
     int bits = 0;
-    while ((1ull << bits) <= cells) bits++;
+    while ((1ull << bits) <= keyspace) bits++;
     M2D_CHECK(bits <= 32, M2D_ERR_BAD_ARG, "m2d_sort_3d: grid too large for 32-bit keys");
     const uint32_t dead_key = bits == 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
     size_t temp = 0;
@@ -1226,16 +1370,25 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
              *v1 = (uint32_t *)(b + 3 * arr);
     unsigned long long *live = ctx->counters + CNT_LIVE;
     M2D_TRY(zero_words(ctx, live, 1));
-    k_sort_keys3d<<<blocks_for(n), 256, 0, ctx->stream>>>(src->x, src->y, src->z, src->n, g, dead_key, k0, v0);
+    k_sort_keys3d<<<blocks_for(n), 256, 0, ctx->stream>>>(src->x, src->y, src->z, src->n, g, nty3, ntz3, dead_key,
+                                                          k0, v0);
     M2D_HIP(hipGetLastError());
     e = rocprim::radix_sort_pairs(b + 4 * arr, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather3d<<<blocks_for(n), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, dead_key, live);
     M2D_HIP(hipGetLastError());
+    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, dead_key, ntx3, nty3, ntz3));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];
     }
+    return M2D_OK;
+}
+
+int m2d_sort_forget_3d(m2d_ctx *ctx, const m2d_particles3d *p)
+{
+    M2D_CHECK(ctx && p, M2D_ERR_BAD_ARG, "m2d_sort_forget_3d: NULL argument");
+    forget_tiling(ctx, p->x);
     return M2D_OK;
 }
 

@@ -67,7 +67,7 @@ struct Scratch {
 struct m2d_tiling {
     const float *x = nullptr;
     int64_t n = 0;
-    int ntx = 0, ntz = 0;      // sort tiles along x and z
+    int ntx = 0, nty = 1, ntz = 0;  // sort tiles along x, (y,) z
     int max_chunks = 0;        // launch bound
     m2d::Scratch buf;          // device: tile_start, tile_end, chunk_first, chunk_count, sched, n_chunks
     int *tile_start = nullptr, *tile_end = nullptr, *chunk_first = nullptr, *chunk_count = nullptr;
@@ -103,6 +103,8 @@ struct m2d_ctx {
 namespace m2d {
 // tile schedules of sorted stores (m2d_sort.hip)
 m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n);
+int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
+                 int nty, int ntz);
 void forget_tiling(m2d_ctx *ctx, const float *x);
 void free_tilings(m2d_ctx *ctx);
 // objects that hold a context keep it alive until they are destroyed

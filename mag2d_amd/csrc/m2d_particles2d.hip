@@ -915,7 +915,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     // a store sorted by m2d_sort_2d against these fields deposits through
     // per-chunk LDS windows reduced by k_reduce_tiles
     const m2d_tiling *T = deposit ? find_tiling(ctx, p->x, p->n) : nullptr;
-    if (T && (T->ntx != (f->nx - 2) / kTile + 1 || T->ntz != (f->nz - 2) / kTile + 1)) T = nullptr;
+    if (T && (T->nty != 1 || T->ntx != (f->nx - 2) / kTile + 1 || T->ntz != (f->nz - 2) / kTile + 1)) T = nullptr;
     if (deposit) {
         M2D_TRY(ensure_acc(ctx, f->n));
         a.acc = (unsigned long long *)ctx->acc.ptr;

@@ -132,11 +132,12 @@ void free_tilings(m2d_ctx *ctx)
     ctx->tilings.clear();
 }
 
-// build the schedule of the store `dst` from the sorted keys
-static int build_tiling(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *dst, const uint32_t *keys,
-                        uint32_t dead_key, int ntx, int ntz)
+// build the schedule of the store at x (n slots) from its sorted keys; tiles
+// number ntx * nty * ntz (nty = 1 in 2D), 64 cells each
+int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
+                 int nty, int ntz)
 {
-    forget_tiling(ctx, dst->x);
+    forget_tiling(ctx, x);
     m2d_tiling *T = nullptr;
     for (m2d_tiling *t : ctx->tilings)
         if (t->x == nullptr) { T = t; break; }
@@ -151,8 +152,8 @@ static int build_tiling(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *ds
         ctx->tilings.push_back(T);
     }
     T->x = nullptr;
-    const int ntiles = ntx * ntz;
-    const int max_chunks = ntiles + (int)(dst->n / kChunkCap) + 1;
+    const int ntiles = ntx * nty * ntz;
+    const int max_chunks = ntiles + (int)(n / kChunkCap) + 1;
     const size_t words = 4 * (size_t)ntiles + 4 * (size_t)max_chunks + 4;
     M2D_TRY(ensure(ctx, T->buf, words * sizeof(int)));
     int *b = (int *)T->buf.ptr;
@@ -163,7 +164,6 @@ static int build_tiling(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *ds
     T->sched = (int4 *)(b + 4 * ntiles);
     T->n_chunks = (unsigned int *)(b + 4 * ntiles + 4 * max_chunks);
     M2D_TRY(fill_bytes(ctx, T->tile_start, 0, sizeof(int) * 2 * (size_t)ntiles));
-    const int64_t n = dst->n;
     k_tile_bounds<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(keys, n, dead_key, T->tile_start,
                                                                         T->tile_end);
     k_tile_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, ctx->stream>>>(T->tile_start, T->tile_end,
@@ -172,11 +172,11 @@ static int build_tiling(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *ds
                                                  T->chunk_first, T->sched, T->n_chunks);
     M2D_HIP(hipGetLastError());
     T->ntx = ntx;
+    T->nty = nty;
     T->ntz = ntz;
     T->max_chunks = max_chunks;
     T->n = n;
-    T->x = dst->x;
-    (void)f;
+    T->x = x;
     return M2D_OK;
 }
 
@@ -235,7 +235,7 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key, live);
     M2D_HIP(hipGetLastError());
-    M2D_TRY(build_tiling(ctx, f, dst, k1, g.dead_key, ntx, g.ntz));
+    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, g.dead_key, ntx, 1, g.ntz));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];
