@@ -334,6 +334,68 @@ def cpu_baseline_penning(fields, mask, volt, n, L, dV, N, vth, budget_s):
                       f"serial; {t:.1f} s"}
 
 
+def run_adjoint2d(args):
+    """Adjoint sensitivity: the mean ion kinetic energy after 2000 self-consistent
+    steps of a 2D trap (256 x 256 over 2 cm, Bz = 0.1 T, 500 000 ions of 3 u at
+    0.1 eV on a 5 mm disk, dt = 1e-9 s), differentiated with respect to 8
+    segmented wall electrodes (voltages U[-5, 5] V, seed 1234) with
+    checkpoints every 100 steps.  Reports the gradient's wall time over one
+    plain forward run of the same steps."""
+    n, N, L = args.grid or 256, args.particles or 500_000, 2e-2
+    steps = args.steps or 2000
+    K = 100
+    ctx = abi.Context(0, seed=1234)
+    dx = L / (n - 1)
+    g = abi.grid2d(n, n, dx, dx)
+    mask, volt = abi.grid_build(g, abi.GEO_EMPTY, L, L, extern_field=0.0)
+    group = np.full(n * n, -1, np.int32)
+    G = group.reshape(n, n)
+    h = n // 2
+    G[0, :h], G[0, h:], G[-1, :h], G[-1, h:] = 0, 1, 2, 3
+    G[1:-1, 0][: h - 1], G[1:-1, 0][h - 1:] = 4, 5
+    G[1:-1, -1][: h - 1], G[1:-1, -1][h - 1:] = 6, 7
+    V = np.random.default_rng(1234).uniform(-5, 5, 8)
+    volt = volt + np.where(group >= 0, V[np.maximum(group, 0)], 0.0)
+    dV = (N / 1e12) / ((n - 1) ** 2)
+    params = abi.FieldsParams(g, L, L, 1, 1, 0, abi.BC_FREE, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0.1, 0.0, dV)
+    fields = abi.Fields2D(ctx, params, mask, volt)
+    vth = math.sqrt(0.1 * abi.QE / M_ION)
+    sp = abi.Species2D(abi.QE, M_ION, 1e-9, float("inf"), 0, 0, 0)
+
+    def fresh():
+        # the disk sample, cell-sorted as a production run keeps its store
+        Q = abi.ParticlesSoA(ctx, N)
+        abi.add_particles_on_disk(ctx, fields, Q, L / 2, L / 2, 5e-3, vth, stream=0)
+        P = abi.ParticlesSoA(ctx, N)
+        abi.sort(ctx, fields, Q, P, want_count=False)
+        fields.upload(abi.FIELD_U, np.zeros(n * n))
+        ctx.sync()
+        return P
+
+    P = fresh()
+    t0 = time.perf_counter()
+    J0, _, info0 = abi.adjoint_energy(ctx, fields, sp, P, steps)
+    t_fwd = time.perf_counter() - t0
+    P = fresh()
+    t0 = time.perf_counter()
+    J1, grad, info1 = abi.adjoint_energy(ctx, fields, sp, P, steps, group=group, n_groups=8, checkpoint_every=K)
+    t_grad = time.perf_counter() - t0
+    out = {
+        "metric": "adjoint/forward time ratio (gradient of mean ion energy w.r.t. 8 electrode voltages)",
+        "value": t_grad / t_fwd, "unit": "x forward", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": t_grad / steps * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 push/deposit, f64 adjoint and Poisson residual", "data": "synthetic",
+        "config": {"workload": "2D trap, 500k ions of 3 u, 256x256, 8 segmented electrodes, Poisson every "
+                               "step, checkpoint every 100", "particles": N, "live_particles_end": info1.n_live,
+                   "grid": [n, n], "dt": 1e-9, "checkpoint_every": K},
+        "forward_s": t_fwd, "gradient_s": t_grad,
+        "gradient_split_ms": {"forward": info1.forward_ms, "replay_and_reverse": info1.backward_ms},
+        "J_eV": J1, "J_forward_only_eV": J0, "grad_eV_per_V": list(grad),
+        "device_flags": ctx.status(clear=True),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def quad_trap_3d(n, L, V_rod=2.0, V_end=5.0, r0=0.3, a=0.08):
     """Linear quadrupole in a box: four rods along z (+V, -V, +V, -V) at
     radius r0 L, end caps (the z faces) at V_end, the other faces grounded."""
@@ -457,7 +519,7 @@ def cpu_baseline_3d(fields, mask, n, L, h, budget_s):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("rf22pt", "penning2d", "trap3d"), default="rf22pt",
+    ap.add_argument("--workload", choices=("rf22pt", "penning2d", "trap3d", "adjoint2d"), default="rf22pt",
                     help="rf22pt: the headline RF trap (default); penning2d: self-consistent 2D Penning "
                          "trap; trap3d: 3D quadrupole with space charge")
     ap.add_argument("--steps", type=int, default=None)
@@ -492,6 +554,8 @@ def main():
         return run_penning2d(args)
     if args.workload == "trap3d":
         return run_trap3d(args)
+    if args.workload == "adjoint2d":
+        return run_adjoint2d(args)
     args.steps = args.steps if args.steps is not None else 200
     args.warmup = args.warmup if args.warmup is not None else 20
     args.particles = args.particles or 4 * 1024 * 1024

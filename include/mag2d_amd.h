@@ -317,6 +317,35 @@ int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_partic
                                  double cx, double cz, double radius, double vth,
                                  uint32_t stream);
 
+/* ======================= adjoint ========================================= */
+
+typedef struct {
+    int32_t steps;               /* self-consistent steps of the run */
+    int32_t checkpoint_every;    /* particle checkpoints every K steps; the reverse sweep replays
+                                    the pushes of one K-step segment from its checkpoint */
+    int32_t n_groups;            /* electrode voltages differentiated, <= 64 */
+    const int32_t *group;        /* host, nx*nz: group of each FIXED node, -1 for none */
+} m2d_adjoint_cfg;
+
+typedef struct {
+    int64_t n_live;              /* live particles at the end (the mean's denominator) */
+    double forward_ms;           /* the forward run alone */
+    double backward_ms;          /* replay + reverse sweep */
+} m2d_adjoint_info;
+
+/* J = energy_dist.mean_tot() (histogram.cpp:68-71) after cfg->steps iterations
+ * of Pic::advance with a solve every step (pic.cpp:330-358; one species, no
+ * collisions, no RF), starting from the store p already started by
+ * advance_boris_init, whose charge is deposited first (pic.cpp:362-375).
+ * grad (may be NULL: forward only) receives dJ/dV_g for each group of FIXED
+ * nodes whose voltages move together, by reverse-mode differentiation of
+ * the push, the deposit and the Poisson solve (self-adjoint on the free
+ * nodes); particles removed by the boundary are frozen out.  The store,
+ * Fields::u and rho end as after a plain forward run.  No counterpart in the
+ * reference, which has no derivatives. */
+int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const m2d_particles2d *p,
+                          const m2d_adjoint_cfg *cfg, double *J, double *grad, m2d_adjoint_info *info);
+
 /* ======================= 3D (Field3D, Species<CARTESIAN3D>) ============== */
 
 typedef struct m2d_fields3d m2d_fields3d;

@@ -96,6 +96,15 @@ class StepStats(C.Structure):
     _fields_ = [("removed", C.c_int64), ("collisions", C.c_int64)]
 
 
+class AdjointCfg(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("checkpoint_every", C.c_int32), ("n_groups", C.c_int32),
+                ("group", C.POINTER(C.c_int32))]
+
+
+class AdjointInfo(C.Structure):
+    _fields_ = [("n_live", C.c_int64), ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
+
+
 class Grid3D(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("dx", C.c_double),
                 ("dy", C.c_double), ("dz", C.c_double)]
@@ -170,6 +179,8 @@ _SIGS = {
     "m2d_sort_forget_2d": (C.c_int, [_P, C.POINTER(Particles2D)]),
     "m2d_add_particles_on_disk_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.c_uint32]),
+    "m2d_adjoint_energy_2d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D),
+                                        C.POINTER(AdjointCfg), _PD, _PD, C.POINTER(AdjointInfo)]),
     "m2d_fields3d_create": (C.c_int, [_P, C.POINTER(Fields3DParams), _PU8, _PD, C.POINTER(_P)]),
     "m2d_fields3d_destroy": (C.c_int, [_P]),
     "m2d_fields3d_data": (_P, [_P, C.c_int]),
@@ -719,3 +730,21 @@ def grad_3d(ctx, g, data_ptr, x, y, z):
     check(lib().m2d_grad_3d(ctx.h, C.byref(g), C.c_void_p(data_ptr), *(C.c_void_p(a.ptr) for a in d), n,
                             *(C.c_void_p(o.ptr) for o in out)))
     return tuple(o.get() for o in out)
+
+
+# --------------------------------------------------------------------- adjoint
+
+def adjoint_energy(ctx, fields: Fields2D, species: Species2D, parts: ParticlesSoA, steps, group=None,
+                   n_groups=0, checkpoint_every=100):
+    """J (mean kinetic energy in eV after `steps` self-consistent steps) and,
+    when `group` (nx*nz int array, -1 for none) is given, dJ/dV per group."""
+    g = None if group is None else np.ascontiguousarray(group, np.int32)
+    cfg = AdjointCfg(steps, checkpoint_every, n_groups,
+                     g.ctypes.data_as(C.POINTER(C.c_int32)) if g is not None else None)
+    J = C.c_double()
+    grad = np.zeros(max(n_groups, 1))
+    info = AdjointInfo()
+    s = parts.struct()
+    check(lib().m2d_adjoint_energy_2d(ctx.h, fields.h, C.byref(species), C.byref(s), C.byref(cfg), C.byref(J),
+                                      _ptr(grad, np.float64) if g is not None else None, C.byref(info)))
+    return J.value, (grad[:n_groups] if g is not None else None), info

@@ -1129,6 +1129,88 @@ void o_advance3d_init(const o_fields3 *f, o_species *s, o_particle *P, long n)
     }
 }
 
+/* Pic<CARTESIAN>::advance_init / advance with a solve every step
+ * (pic.cpp:330-384) for one species and no collisions, from a store already
+ * started by advance_boris_init: rho^0 = accumulate(p); then per step
+ * u = boundary_solve(rho), reset, advance_boris, advance_boundary (deposit).
+ * The band Cholesky factor of the Dirichlet-eliminated matrix is built once.
+ * Returns the mean kinetic energy (eV) of the live particles at the end, or
+ * NaN when the matrix cannot be factored. */
+double o_selfconsistent_run(o_fields *f, o_species *s, o_particle *P, long np, const double *voltage,
+                            double dV, int steps)
+{
+    const o_grid2 *g = &f->g;
+    const int M = g->nx, N = g->nz;
+    const long n = (long)M * N;
+    const int bw = N;
+    const uint8_t *mask = f->mask;
+    double *L = (double *)calloc((size_t)n * (bw + 1), sizeof(double));
+    double *c = (double *)malloc(sizeof(double) * n);
+    double *rho = (double *)calloc((size_t)n, sizeof(double));
+    double *u = (double *)f->u;
+    if (!L || !c || !rho) { free(L); free(c); free(rho); return NAN; }
+#define LB(k, d) L[(size_t)(k) * (bw + 1) + (d)]
+#define DIR(k) (mask[k] == O_FIXED || mask[k] == O_FIXED_RF)
+    for (long k = 0; k < n; k++) {
+        if (DIR(k)) { LB(k, 0) = 1.0; continue; }
+        LB(k, 0) = 4.0;
+        if (!DIR(k - N)) LB(k, N) = -1.0;
+        if (!DIR(k - 1)) LB(k, 1) = -1.0;
+    }
+    for (long k = 0; k < n; k++) {
+        long dmax = k < bw ? k : bw;
+        for (long d = dmax; d >= 1; d--) {
+            long j = k - d;
+            double sm = LB(k, d);
+            long m0 = k - bw > j - bw ? k - bw : j - bw;
+            if (m0 < 0) m0 = 0;
+            for (long m = m0; m < j; m++) sm -= LB(k, k - m) * LB(j, j - m);
+            LB(k, d) = sm / LB(j, 0);
+        }
+        double sm = LB(k, 0);
+        for (long m = (k - bw > 0 ? k - bw : 0); m < k; m++) sm -= SQR(LB(k, k - m));
+        if (!(sm > 0.0)) { free(L); free(c); free(rho); return NAN; }
+        LB(k, 0) = sqrt(sm);
+    }
+    double *sr = s->rho;
+    s->rho = rho;
+    o_accumulate(f, s, P, np);
+    const double scale = -(g->dx * g->dx) / O_EPS0 / dV;
+    for (int st = 0; st < steps; st++) {
+        /* boundary_solve (fields.cpp:278-312) on the factor */
+        for (long k = 0; k < n; k++) {
+            /* STATIC rows (fields.cpp:297-310): FIXED -> voltage, FIXED_RF -> 0 */
+            if (DIR(k)) { c[k] = mask[k] == O_FIXED ? voltage[k] : 0.0; continue; }
+            double rhs = rho[k] * scale;
+            const long nb[4] = {k - N, k - 1, k + 1, k + N};
+            for (int q = 0; q < 4; q++)
+                if (DIR(nb[q])) rhs -= mask[nb[q]] == O_FIXED ? voltage[nb[q]] : 0.0;
+            c[k] = -rhs;
+        }
+        for (long k = 0; k < n; k++) {
+            double sm = c[k];
+            for (long m = (k - bw > 0 ? k - bw : 0); m < k; m++) sm -= LB(k, k - m) * c[m];
+            c[k] = sm / LB(k, 0);
+        }
+        for (long k = n - 1; k >= 0; k--) {
+            double sm = c[k];
+            long mmax = k + bw < n - 1 ? k + bw : n - 1;
+            for (long m = k + 1; m <= mmax; m++) sm -= LB(m, m - k) * c[m];
+            c[k] = sm / LB(k, 0);
+        }
+        memcpy(u, c, sizeof(double) * n);
+        memset(rho, 0, sizeof(double) * n);
+        o_advance_boris(f, s, P, np, NULL);
+        o_advance_boundary(f, s, P, np, NULL);
+        s->niter++;
+    }
+#undef LB
+#undef DIR
+    s->rho = sr;
+    free(L); free(c); free(rho);
+    return o_mean_energy_eV(P, np, s->mass);
+}
+
 /* ======================================================================= */
 /* CPU baseline timing                                                     */
 /* ======================================================================= */

@@ -220,6 +220,12 @@ long o_add_particles_on_disk(o_particle *p, long cap, long n_req, double cx, dou
 void o_add_particles_on_disk_philox(o_particle *p, long n, double cx, double cz, double radius,
                                     double vth, const o_fields *f, uint64_t seed, uint32_t stream);
 
+/* Pic<CARTESIAN>::advance_init / advance with a solve every step (pic.cpp:330-384),
+ * one species, no collisions, from a store already started by advance_boris_init;
+ * f->u is the potential array it writes.  Returns the final mean kinetic energy (eV). */
+double o_selfconsistent_run(o_fields *f, o_species *s, o_particle *p, long n, const double *voltage,
+                            double dV, int steps);
+
 /* ---------------- Field3D (Field3D.hpp:40-163) --------------------------- */
 typedef struct {
     int nx, ny, nz;

@@ -160,6 +160,8 @@ _SIGS = {
     "o_f3_grad": (None, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double, _PD, _PD, _PD]),
     "o_poisson3_apply": (C.c_int, [C.POINTER(Grid3), _P, _P, _P]),
     "o_poisson3_rhs": (None, [C.POINTER(Grid3), _P, _P, _P, C.c_double]),
+    "o_selfconsistent_run": (C.c_double, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long, _P, C.c_double,
+                                          C.c_int]),
     "o_is_free3": (C.c_int, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double]),
     "o_advance3d": (C.c_long, [C.POINTER(Fields3), C.POINTER(Species), _P, C.c_long, C.POINTER(C.c_long),
                                C.POINTER(C.c_long)]),
