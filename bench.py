@@ -396,6 +396,90 @@ def run_adjoint2d(args):
     print(json.dumps(out), flush=True)
 
 
+def run_batch2d(args):
+    """Batched ensemble: 256 independent RF 22-pole runs on a 128 x 128 grid
+    (2 cm), ions of 3 u at 0.1 eV on a 5 mm disk, dt = 1e-9 s; per run
+    (seed 1234) N ~ lognormal clipped to [1e3, 1e5], He pressure log-uniform
+    in [1e-4, 1e-1] Pa at 300 K (constant k = 1e-15 m^3/s elastic) and V_rf
+    uniform in [10, 50] V; 5000 steps in one hipGraph of persistent launches
+    with the tuned shape.  Compared with the same runs stepped one after
+    another with the plain pass (a sample of runs and steps, as a rate)."""
+    n, L = args.grid or 128, 2e-2
+    steps = args.steps or 5000
+    R = 256
+    ctx = abi.Context(0, seed=1234)
+    dx = L / (n - 1)
+    g = abi.grid2d(n, n, dx, dx)
+    mask, volt = abi.grid_build(g, abi.GEO_RF_22PT, L, L)
+    omega = 2 * math.pi * 17e6
+
+    def fields_for(amp):
+        prm = abi.FieldsParams(g, L, L, 0, 0, 0, abi.BC_FREE, 0.0, 1, amp, omega, 0.0, 0.0, 0.0, 0.0, 1e-12)
+        return abi.Fields2D(ctx, prm, mask, volt)
+
+    fields = fields_for(30.0)
+    fields.solve(abi.SOLVE_RF, 1e-10)
+    urf = fields.download(abi.FIELD_URF)
+    rng = np.random.default_rng(1234)
+    counts = np.clip(np.exp(rng.normal(math.log(1e4), 1.0, R)), 1e3, 1e5).astype(np.int64)
+    pressure = 10 ** rng.uniform(-4, -1, R)
+    amps = rng.uniform(10, 50, R)
+    dens = pressure / (abi.KB * 300.0)
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    N = int(offsets[-1])
+    P = abi.ParticlesSoA(ctx, N)
+    vth = math.sqrt(0.1 * abi.QE / M_ION)
+    abi.add_particles_on_disk(ctx, fields, P, L / 2, L / 2, 5e-3, vth, stream=0)
+    mccs = [abi.Mcc(ctx, M_ION, [abi.Target(M_HE, d, 300.0, 0.0, abi.Particles2D())],
+                    [abi.Interaction(abi.ELASTIC, 0, 0.0, 1e-15, 0.0, 0, 0)], 1.0) for d in dens]
+    sp = abi.Species2D(abi.QE, M_ION, 1e-9, float("inf"), 0, 0, 0)
+    batch = abi.Batch(ctx, fields, sp, P, offsets, amps, mccs)
+    t0 = time.perf_counter()
+    shape = batch.tune(0, 50, 1000)
+    t_tune = time.perf_counter() - t0
+    ctx.sync()
+    t0 = time.perf_counter()
+    batch.advance(0, steps)
+    ctx.sync()
+    t_batch = time.perf_counter() - t0
+    mean, live = batch.energy()
+    rate_batch = N * steps / t_batch
+    # sequential plain runs: every 32nd run alone, its own Fields (amplitude) and collisions
+    seq_steps = min(200, steps)
+    sample = list(range(0, R, 32))
+    t_seq, ps_seq = 0.0, 0
+    for r in sample:
+        fr = fields_for(float(amps[r]))
+        fr.upload(abi.FIELD_URF, urf)
+        Q = abi.ParticlesSoA(ctx, int(counts[r]))
+        abi.add_particles_on_disk(ctx, fr, Q, L / 2, L / 2, 5e-3, vth, stream=r)
+        s1 = abi.Species2D(abi.QE, M_ION, 1e-9, mccs[r].lifetime, 0, r, 0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(seq_steps):
+            s1.niter = k
+            abi.advance(ctx, fr, s1, Q, mcc=mccs[r], flags=abi.STEP_PUSH | abi.STEP_BOUNDARY)
+        ctx.sync()
+        t_seq += time.perf_counter() - t0
+        ps_seq += int(counts[r]) * seq_steps
+    rate_seq = ps_seq / t_seq
+    out = {
+        "metric": "batched ensemble speed-up over sequential plain runs (particle-steps/s ratio)",
+        "value": rate_batch / rate_seq, "unit": "x sequential", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": t_batch / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "256 RF 22-pole runs, 128x128, ragged N lognormal [1e3,1e5], pressure log-uniform "
+                               "[1e-4,1e-1] Pa, V_rf U[10,50] V", "runs": R, "particles": N,
+                   "grid": [n, n], "dt": 1e-9, "shape": {"block": shape[0], "ppt": shape[1], "seg_steps": shape[2]}},
+        "batched_particle_steps_per_s": rate_batch, "batched_s": t_batch, "tune_s": t_tune,
+        "sequential_particle_steps_per_s": rate_seq,
+        "sequential_sample": f"{len(sample)} runs x {seq_steps} steps stepped alone with the plain pass",
+        "live_particles_end": int(live.sum()), "mean_energy_eV_range": [float(mean.min()), float(mean.max())],
+        "device_flags": ctx.status(clear=True),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def quad_trap_3d(n, L, V_rod=2.0, V_end=5.0, r0=0.3, a=0.08):
     """Linear quadrupole in a box: four rods along z (+V, -V, +V, -V) at
     radius r0 L, end caps (the z faces) at V_end, the other faces grounded."""
@@ -519,7 +603,8 @@ def cpu_baseline_3d(fields, mask, n, L, h, budget_s):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("rf22pt", "penning2d", "trap3d", "adjoint2d"), default="rf22pt",
+    ap.add_argument("--workload", choices=("rf22pt", "penning2d", "trap3d", "adjoint2d", "batch2d"),
+                    default="rf22pt",
                     help="rf22pt: the headline RF trap (default); penning2d: self-consistent 2D Penning "
                          "trap; trap3d: 3D quadrupole with space charge")
     ap.add_argument("--steps", type=int, default=None)
@@ -556,6 +641,8 @@ def main():
         return run_trap3d(args)
     if args.workload == "adjoint2d":
         return run_adjoint2d(args)
+    if args.workload == "batch2d":
+        return run_batch2d(args)
     args.steps = args.steps if args.steps is not None else 200
     args.warmup = args.warmup if args.warmup is not None else 20
     args.particles = args.particles or 4 * 1024 * 1024

@@ -346,6 +346,47 @@ typedef struct {
 int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const m2d_particles2d *p,
                           const m2d_adjoint_cfg *cfg, double *J, double *grad, m2d_adjoint_info *info);
 
+/* ======================= batched ensembles =============================== */
+
+typedef struct m2d_batch m2d_batch;
+
+/* Independent runs sharing a field geometry and species constants (the
+ * one-process-per-run sweeps of drive_22PT_pressure.py:145-160 and
+ * drive_8PT_pressure.py:151-166): run r owns slots [offset[r], offset[r+1])
+ * of one store, with its own RF amplitude, collision tables and random
+ * stream. */
+typedef struct {
+    int32_t n_runs;
+    const int64_t *offset;        /* host, n_runs + 1, offset[0] = 0, offset[n_runs] = store slots */
+    const double *rf_amplitude;   /* host, per run V_rf (ignored when the fields have rf = 0) */
+    const m2d_mcc *const *mcc;    /* host, per run collision tables; NULL or NULL entries: none */
+    const uint32_t *stream;       /* host, per run RNG stream; NULL: the run index */
+} m2d_batch_desc;
+
+/* launch shape of the persistent pass */
+typedef struct {
+    int32_t block;                /* threads per block: 64, 128 or 256 */
+    int32_t ppt;                  /* particles per thread: 1, 2 or 4 */
+    int32_t seg_steps;            /* steps per launch */
+} m2d_batch_shape;
+
+/* The batch over fields f (no space charge) and species constants s
+ * (charge, mass, dt); the store p is advanced in place by m2d_batch_advance. */
+int m2d_batch_create(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const m2d_particles2d *p,
+                     const m2d_batch_desc *d, m2d_batch **out);
+int m2d_batch_destroy(m2d_batch *b);
+/* time every shape on a copy of the store for probe_steps steps from niter0
+ * and keep the fastest (seg_steps as given); the batch's state is unchanged */
+int m2d_batch_tune(m2d_batch *b, uint64_t niter0, int32_t probe_steps, int32_t seg_steps, m2d_batch_shape *best);
+int m2d_batch_set_shape(m2d_batch *b, const m2d_batch_shape *sh);
+/* Species<CARTESIAN>::advance of every run for steps niter0 .. niter0+steps-1
+ * (particles.hpp:342-349, no deposit): one persistent launch per seg_steps
+ * steps, the call's launches captured and replayed as one hipGraph.  Bit for
+ * bit the states m2d_advance_2d gives each run step by step. */
+int m2d_batch_advance(m2d_batch *b, uint64_t niter0, int32_t steps);
+/* per run: energy_dist.mean_tot() (eV) and live count */
+int m2d_batch_energy(m2d_batch *b, double *mean_eV, int64_t *n_live);
+
 /* ======================= 3D (Field3D, Species<CARTESIAN3D>) ============== */
 
 typedef struct m2d_fields3d m2d_fields3d;

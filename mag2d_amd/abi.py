@@ -105,6 +105,15 @@ class AdjointInfo(C.Structure):
     _fields_ = [("n_live", C.c_int64), ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
 
 
+class BatchDesc(C.Structure):
+    _fields_ = [("n_runs", C.c_int32), ("offset", C.POINTER(C.c_int64)), ("rf_amplitude", C.POINTER(C.c_double)),
+                ("mcc", C.POINTER(C.c_void_p)), ("stream", C.POINTER(C.c_uint32))]
+
+
+class BatchShape(C.Structure):
+    _fields_ = [("block", C.c_int32), ("ppt", C.c_int32), ("seg_steps", C.c_int32)]
+
+
 class Grid3D(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("dx", C.c_double),
                 ("dy", C.c_double), ("dz", C.c_double)]
@@ -181,6 +190,13 @@ _SIGS = {
                                                C.c_double, C.c_double, C.c_double, C.c_uint32]),
     "m2d_adjoint_energy_2d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D),
                                         C.POINTER(AdjointCfg), _PD, _PD, C.POINTER(AdjointInfo)]),
+    "m2d_batch_create": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D), C.POINTER(BatchDesc),
+                                   C.POINTER(_P)]),
+    "m2d_batch_destroy": (C.c_int, [_P]),
+    "m2d_batch_tune": (C.c_int, [_P, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(BatchShape)]),
+    "m2d_batch_set_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
+    "m2d_batch_advance": (C.c_int, [_P, C.c_uint64, C.c_int32]),
+    "m2d_batch_energy": (C.c_int, [_P, _PD, C.POINTER(C.c_int64)]),
     "m2d_fields3d_create": (C.c_int, [_P, C.POINTER(Fields3DParams), _PU8, _PD, C.POINTER(_P)]),
     "m2d_fields3d_destroy": (C.c_int, [_P]),
     "m2d_fields3d_data": (_P, [_P, C.c_int]),
@@ -748,3 +764,57 @@ def adjoint_energy(ctx, fields: Fields2D, species: Species2D, parts: ParticlesSo
     check(lib().m2d_adjoint_energy_2d(ctx.h, fields.h, C.byref(species), C.byref(s), C.byref(cfg), C.byref(J),
                                       _ptr(grad, np.float64) if g is not None else None, C.byref(info)))
     return J.value, (grad[:n_groups] if g is not None else None), info
+
+
+# --------------------------------------------------------------------- batches
+
+class Batch:
+    """m2d_batch: independent runs in one store, advanced by the persistent pass."""
+
+    def __init__(self, ctx, fields, species: Species2D, parts: ParticlesSoA, offsets, rf_amplitude, mccs=None,
+                 streams=None):
+        self.ctx, self.fields, self.parts = ctx, fields, parts
+        self.offsets = np.ascontiguousarray(offsets, np.int64)
+        self.amp = np.ascontiguousarray(rf_amplitude, np.float64)
+        R = len(self.offsets) - 1
+        self.mcc_ptrs = (C.c_void_p * R)(*[(m.h.value if m is not None else None) for m in mccs]) if mccs else None
+        self.streams = np.ascontiguousarray(streams, np.uint32) if streams is not None else None
+        self._mccs = mccs
+        d = BatchDesc(R, self.offsets.ctypes.data_as(C.POINTER(C.c_int64)),
+                      self.amp.ctypes.data_as(C.POINTER(C.c_double)),
+                      C.cast(self.mcc_ptrs, C.POINTER(C.c_void_p)) if self.mcc_ptrs is not None else None,
+                      self.streams.ctypes.data_as(C.POINTER(C.c_uint32)) if self.streams is not None else None)
+        self._s = parts.struct()
+        h = C.c_void_p()
+        check(lib().m2d_batch_create(ctx.h, fields.h, C.byref(species), C.byref(self._s), C.byref(d), C.byref(h)))
+        self.h = h
+        self.n_runs = R
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().m2d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, niter0, probe_steps, seg_steps):
+        sh = BatchShape()
+        check(lib().m2d_batch_tune(self.h, niter0, probe_steps, seg_steps, C.byref(sh)))
+        return sh.block, sh.ppt, sh.seg_steps
+
+    def set_shape(self, block, ppt, seg_steps):
+        sh = BatchShape(block, ppt, seg_steps)
+        check(lib().m2d_batch_set_shape(self.h, C.byref(sh)))
+
+    def advance(self, niter0, steps):
+        check(lib().m2d_batch_advance(self.h, niter0, steps))
+
+    def energy(self):
+        e = np.zeros(self.n_runs)
+        n = np.zeros(self.n_runs, np.int64)
+        check(lib().m2d_batch_energy(self.h, _ptr(e, np.float64), n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return e, n

@@ -1,0 +1,60 @@
+// m2d_gather2d.hpp -- Fields::E at a particle (fields.hpp:124-150) through
+// the padded FP32 edge-gradient tables, shared by the single-run and the
+// batched 2D passes so both compute the same bits.
+#pragma once
+#include "m2d_internal.hpp"
+
+namespace m2d {
+
+// ------------------------------------------------------------------ gather
+// Fields::E through the padded edge-gradient tables.  Index arithmetic is
+// FP64 on the FP32 position (exactly the reference's x*idx), weights FP32.
+// A is any argument block with the table pointers gx, gz, the grid nx, nz,
+// idx, idz, xmin, zmin, the constant-field switch econst/ecx/ecz and the
+// status counters.
+template <class A>
+__device__ __forceinline__ void gather_E(const A &a, double xd, double zd, float ph, float &ex, float &ez)
+{
+    if (a.econst) { ex = a.ecx; ez = a.ecz; return; }
+    const int nx = a.nx, nz = a.nz;
+    double X = (xd - a.xmin) * a.idx;
+    double Z = (zd - a.zmin) * a.idz;
+    // x component: i = (int)(x idx + .5), j = min((int)(z idz), lmax-2)
+    int ih = (int)(X + 0.5);
+    int jx = min((int)Z, nz - 2);
+    // z component: i = min((int)(x idx), jmax-2), j = (int)(z idz + .5)
+    int i0 = min((int)X, nx - 2);
+    int jh = (int)(Z + 0.5);
+    bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
+    if (bad) {
+        raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+        X = fmin(fmax(X, 0.0), (double)(nx - 1));
+        Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
+        ih = (int)(X + 0.5); jx = min((int)Z, nz - 2);
+        i0 = min((int)X, nx - 2); jh = (int)(Z + 0.5);
+    }
+    ih = max(0, min(ih, nx - 1));
+    jx = max(jx, 0);
+    i0 = max(i0, 0);
+    jh = max(0, min(jh, nz - 1));
+    const float fx = (float)(X - ih + 0.5), fyx = (float)(Z - jx);
+    const float fxz = (float)(X - i0), fz = (float)(Z - jh + 0.5);
+
+    const float2 *r0 = a.gx + (ih * nz + jx);
+    const float2 *r1 = r0 + nz;
+    float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
+    const float2 *q0 = a.gz + (i0 * (nz + 1) + jh);
+    const float2 *q1 = q0 + (nz + 1);
+    float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
+
+    float g00 = fmaf(ph, p00.y, p00.x), g01 = fmaf(ph, p01.y, p01.x);
+    float g10 = fmaf(ph, p10.y, p10.x), g11 = fmaf(ph, p11.y, p11.x);
+    float gxv = (1.f - fx) * ((1.f - fyx) * g00 + fyx * g01) + fx * ((1.f - fyx) * g10 + fyx * g11);
+    float h00 = fmaf(ph, s00.y, s00.x), h01 = fmaf(ph, s01.y, s01.x);
+    float h10 = fmaf(ph, s10.y, s10.x), h11 = fmaf(ph, s11.y, s11.x);
+    float gzv = (1.f - fxz) * ((1.f - fz) * h00 + fz * h01) + fxz * ((1.f - fz) * h10 + fz * h11);
+    ex = -gxv;
+    ez = -gzv;
+}
+
+}  // namespace m2d

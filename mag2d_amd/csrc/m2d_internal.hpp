@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cmath>
 #include <cstdio>
 #include <cstdarg>
 #include <string>
@@ -208,6 +209,17 @@ __device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred)
 __device__ __forceinline__ void raise_flag(unsigned long long *counters, unsigned long long f)
 {
     atomicOr(&counters[CNT_FLAGS], f);
+}
+
+// cos of the RF phase at step niter: t = niter dt, cos(mod(omega t, 1e7 pi))
+// (Fields::E, fields.hpp:140-144; mod, mymath.cpp:62-67), on the host
+inline double rf_cos(uint64_t niter, double dt, double omega)
+{
+    const double t = (double)niter * dt;
+    double ph = t * omega;
+    const double y = 10000000 * M_PI;
+    if (!(ph >= 0.0 && ph <= y)) ph = ph - y * (int)(ph / y) + (ph < 0 ? y : 0);
+    return cos(ph);
 }
 
 inline int grid_for(int64_t n, int per_thread, int block = kBlock)

@@ -8,6 +8,7 @@
 // One read and one write of the 20-byte SoA state per live particle.
 #include "m2d_objects.hpp"
 #include "m2d_scatter.hpp"
+#include "m2d_gather2d.hpp"
 #include <cmath>
 
 namespace m2d {
@@ -49,55 +50,6 @@ struct PushArgs {
     int ntz;                       // sort tiles along z
     unsigned long long *partials;  // chunk -> kWin x kWin window
 };
-
-// ------------------------------------------------------------------ gather
-// Fields::E through the padded edge-gradient tables.  Index arithmetic is
-// FP64 on the FP32 position (exactly the reference's x*idx), weights FP32.
-__device__ __forceinline__ void gather_E(const PushArgs &a, double xd, double zd, float &ex,
-                                         float &ez)
-{
-    if (a.econst) { ex = a.ecx; ez = a.ecz; return; }
-    const int nx = a.nx, nz = a.nz;
-    double X = (xd - a.xmin) * a.idx;
-    double Z = (zd - a.zmin) * a.idz;
-    // x component: i = (int)(x idx + .5), j = min((int)(z idz), lmax-2)
-    int ih = (int)(X + 0.5);
-    int jx = min((int)Z, nz - 2);
-    // z component: i = min((int)(x idx), jmax-2), j = (int)(z idz + .5)
-    int i0 = min((int)X, nx - 2);
-    int jh = (int)(Z + 0.5);
-    bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
-    if (bad) {
-        raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
-        X = fmin(fmax(X, 0.0), (double)(nx - 1));
-        Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
-        ih = (int)(X + 0.5); jx = min((int)Z, nz - 2);
-        i0 = min((int)X, nx - 2); jh = (int)(Z + 0.5);
-    }
-    ih = max(0, min(ih, nx - 1));
-    jx = max(jx, 0);
-    i0 = max(i0, 0);
-    jh = max(0, min(jh, nz - 1));
-    const float fx = (float)(X - ih + 0.5), fyx = (float)(Z - jx);
-    const float fxz = (float)(X - i0), fz = (float)(Z - jh + 0.5);
-    const float ph = a.phase;
-
-    const float2 *r0 = a.gx + (ih * nz + jx);
-    const float2 *r1 = r0 + nz;
-    float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
-    const float2 *q0 = a.gz + (i0 * (nz + 1) + jh);
-    const float2 *q1 = q0 + (nz + 1);
-    float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
-
-    float g00 = fmaf(ph, p00.y, p00.x), g01 = fmaf(ph, p01.y, p01.x);
-    float g10 = fmaf(ph, p10.y, p10.x), g11 = fmaf(ph, p11.y, p11.x);
-    float gxv = (1.f - fx) * ((1.f - fyx) * g00 + fyx * g01) + fx * ((1.f - fyx) * g10 + fyx * g11);
-    float h00 = fmaf(ph, s00.y, s00.x), h01 = fmaf(ph, s01.y, s01.x);
-    float h10 = fmaf(ph, s10.y, s10.x), h11 = fmaf(ph, s11.y, s11.x);
-    float gzv = (1.f - fxz) * ((1.f - fz) * h00 + fz * h01) + fxz * ((1.f - fz) * h10 + fz * h11);
-    ex = -gxv;
-    ez = -gzv;
-}
 
 // ------------------------------------------------------------------ deposit
 // CIC weights (Field2D.hpp:45-62) as 2^-31 fixed point.  The cell offsets
@@ -278,7 +230,7 @@ k_advance2d(PushArgs a)
             if (a.push) {
                 // ---- advance_boris (particles.cpp:945-988)
                 float ex, ez;
-                gather_E(a, (double)x, (double)z, ex, ez);
+                gather_E(a, (double)x, (double)z, a.phase, ex, ez);
                 vx = fmaf(ex, a.qmdt2, vx);
                 vz = fmaf(ez, a.qmdt2, vz);
                 if (a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
@@ -478,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_advance_init2d(PushArgs a)
     float z = a.z[p];
     float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
     float ex, ez;
-    gather_E(a, (double)x, (double)z, ex, ez);
+    gather_E(a, (double)x, (double)z, a.phase, ex, ez);
     float vpr = vx - vy * a.tz + vz * a.ty;
     float vpz = vz - vx * a.ty + vy * a.tx;
     float vpt = vy - vz * a.tx + vx * a.tz;
@@ -497,7 +449,7 @@ __global__ void k_fields_E(PushArgs a, const double *xs, const double *zs, int64
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     float fx, fz;
-    gather_E(a, xs[k], zs[k], fx, fz);
+    gather_E(a, xs[k], zs[k], a.phase, fx, fz);
     ex[k] = fx;
     ez[k] = fz;
 }
@@ -852,13 +804,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.ecx = 0.f;
     a.ecz = (float)P.extern_field;
     double phase = 0.0;
-    if (P.rf) {
-        double t = (double)s->niter * s->dt;
-        double ph = t * P.rf_omega;
-        double y = 10000000 * M_PI;
-        if (!(ph >= 0.0 && ph <= y)) ph = ph - y * (int)(ph / y) + (ph < 0 ? y : 0);  // mymath.cpp:62-67
-        phase = P.rf_amplitude * cos(ph) + P.rf_U0;
-    }
+    if (P.rf) phase = P.rf_amplitude * rf_cos(s->niter, s->dt, P.rf_omega) + P.rf_U0;
     a.phase = (float)phase;
     a.boundary = 0; a.periodic = P.boundary == M2D_BC_PERIODIC;
     a.checkfree = !P.electric_field_from_file;
