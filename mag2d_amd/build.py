@@ -33,6 +33,11 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", f"-I{ROOT / 'include'}",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a * b + c is fused where the source writes it in one expression, never
+    # across statements: the fusing then does not depend on the code around an
+    # inlined function, so kernels that share a device function (the single-run
+    # and batched passes, forward and adjoint replay) compute the same bits
+    "-ffp-contract=on",
 ]
 
 

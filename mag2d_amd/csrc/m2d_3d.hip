@@ -1257,7 +1257,8 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
     fill_args3(a, f, s, p, 1.0, 1.0);
     a.push = (flags & M2D_STEP_PUSH) != 0;
     a.boundary = (flags & M2D_STEP_BOUNDARY) != 0;
-    const bool use_mcc = a.push && mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
+    // an empty store queues nothing (and leaves the alternating counters)
+    const bool use_mcc = p->n > 0 && a.push && mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
     unsigned long long *next_len = nullptr;
     if (use_mcc) {
         a.trial_thr = trial_thr3(1.0 - exp(-s->dt / s->lifetime));  // species3d.cpp:8
@@ -1377,7 +1378,7 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather3d<<<blocks_for(n), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, dead_key, live);
     M2D_HIP(hipGetLastError());
-    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, dead_key, ntx3, nty3, ntz3));
+    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, dead_key, ntx3, nty3, ntz3, kChunkCap3));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];

@@ -10,29 +10,32 @@ namespace m2d {
 // Fields::E through the padded edge-gradient tables.  Index arithmetic is
 // FP64 on the FP32 position (exactly the reference's x*idx), weights FP32.
 // A is any argument block with the table pointers gx, gz, the grid nx, nz,
-// idx, idz, xmin, zmin, the constant-field switch econst/ecx/ecz and the
-// status counters.
-template <class A>
-__device__ __forceinline__ void gather_E(const A &a, double xd, double zd, float ph, float &ex, float &ez)
+// idx, idz, xmin, zmin and the constant-field switch econst/ecx/ecz.
+// ORIGIN0: the caller guarantees xmin = zmin = 0 and no constant field
+// (x - 0 is x exactly, so the result is the same).  Returns true when the
+// point lies outside the grid (a NaN included): it is then clamped onto the
+// grid, and the caller raises M2D_FLAG_OUT_OF_RANGE.  Every table access
+// stays inside the tables for any input, so the gather may run ahead of the
+// test that discards its result.
+template <bool ORIGIN0 = false, class A>
+__device__ __forceinline__ bool gather_E_clamped(const A &a, double xd, double zd, float ph, float &ex,
+                                                 float &ez)
 {
-    if (a.econst) { ex = a.ecx; ez = a.ecz; return; }
+    if (!ORIGIN0 && a.econst) { ex = a.ecx; ez = a.ecz; return false; }
     const int nx = a.nx, nz = a.nz;
-    double X = (xd - a.xmin) * a.idx;
-    double Z = (zd - a.zmin) * a.idz;
+    double X = ORIGIN0 ? xd * a.idx : (xd - a.xmin) * a.idx;
+    double Z = ORIGIN0 ? zd * a.idz : (zd - a.zmin) * a.idz;
+    const bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
+    if (bad) {  // fmax of a NaN and 0 is 0
+        X = fmin(fmax(X, 0.0), (double)(nx - 1));
+        Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
+    }
     // x component: i = (int)(x idx + .5), j = min((int)(z idz), lmax-2)
     int ih = (int)(X + 0.5);
     int jx = min((int)Z, nz - 2);
     // z component: i = min((int)(x idx), jmax-2), j = (int)(z idz + .5)
     int i0 = min((int)X, nx - 2);
     int jh = (int)(Z + 0.5);
-    bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
-    if (bad) {
-        raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
-        X = fmin(fmax(X, 0.0), (double)(nx - 1));
-        Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
-        ih = (int)(X + 0.5); jx = min((int)Z, nz - 2);
-        i0 = min((int)X, nx - 2); jh = (int)(Z + 0.5);
-    }
     ih = max(0, min(ih, nx - 1));
     jx = max(jx, 0);
     i0 = max(i0, 0);
@@ -55,6 +58,14 @@ __device__ __forceinline__ void gather_E(const A &a, double xd, double zd, float
     float gzv = (1.f - fxz) * ((1.f - fz) * h00 + fz * h01) + fxz * ((1.f - fz) * h10 + fz * h11);
     ex = -gxv;
     ez = -gzv;
+    return bad;
+}
+
+// the same, raising M2D_FLAG_OUT_OF_RANGE in a.counters itself
+template <bool ORIGIN0 = false, class A>
+__device__ __forceinline__ void gather_E(const A &a, double xd, double zd, float ph, float &ex, float &ez)
+{
+    if (gather_E_clamped<ORIGIN0>(a, xd, zd, ph, ex, ez)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
 }
 
 }  // namespace m2d

@@ -29,8 +29,9 @@ constexpr int kTile = 8;               // cells per side of the sort tile
 constexpr int kTileShift = 6;          // key = tile << 6 | cell in tile
 constexpr int kHalo = 4;               // deposit window halo (cells) around a tile
 constexpr int kWin = kTile + 1 + 2 * kHalo;  // window side in nodes
-constexpr int kChunkCap = 1984;        // particles per block of the tiled pass: with its start
-                                       // rounded down to 64 slots, two staged rounds of 1024
+constexpr int kChunkCap = 960;         // particles per block of the tiled 2D pass: with its
+                                       // start rounded down to 64 slots, two staged rounds of 512
+constexpr int kChunkCap3 = 1984;       // particles per block of the tiled 3D pass
 
 // ---------------------------------------------------------------- errors
 int set_error(int code, const char *fmt, ...);
@@ -63,7 +64,7 @@ struct Scratch {
 // Tile schedule of a cell-sorted store (built by m2d_sort_2d): the live
 // particles of sort tile t are slots [tile_start[t], tile_end[t]); chunk c
 // of the tiled pass covers sched[c] = (tile, start, end), at most kChunkCap
-// particles of one tile.  Valid for the store at `x` with `n` slots until
+// (2D) or kChunkCap3 (3D) particles of one tile.  Valid for the store at `x` with `n` slots until
 // the store is written by anything but the particle pass.
 struct m2d_tiling {
     const float *x = nullptr;
@@ -92,8 +93,8 @@ struct m2d_ctx {
     m2d::Scratch tmp;       // general temporaries (compaction offsets)
     m2d::Scratch sortbuf;   // keys, slots and radix-sort temporaries
     m2d::Scratch cq;        // MCC collision queue (slot indices)
+    m2d::Scratch partials;  // per-block deposit windows of the tiled 3D pass
     uint64_t launch_seq = 0;  // selects the collision-queue counter of a launch
-    m2d::Scratch partials;  // per-block deposit windows of the tiled pass
     std::vector<struct m2d_tiling *> tilings;  // schedules of recently sorted stores
     // kernel timing: event pairs around particle-pass launches
     bool profile = false;
@@ -105,7 +106,7 @@ namespace m2d {
 // tile schedules of sorted stores (m2d_sort.hip)
 m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n);
 int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
-                 int nty, int ntz);
+                 int nty, int ntz, int cap);
 void forget_tiling(m2d_ctx *ctx, const float *x);
 void free_tilings(m2d_ctx *ctx);
 // objects that hold a context keep it alive until they are destroyed
@@ -143,13 +144,15 @@ __device__ __forceinline__ float dead_value() { return __uint_as_float(M2D_DEAD_
 __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2,
                                               uint32_t &c3, uint32_t k0, uint32_t k1)
 {
+    // each 32 x 32 -> 64 product is one v_mad_u64_u32 (the separate low and
+    // high multiplies would cost two)
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0;
-        uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
 }

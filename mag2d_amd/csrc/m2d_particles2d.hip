@@ -48,7 +48,6 @@ struct PushArgs {
     const int4 *sched;             // chunk -> (tile, first slot, end slot)
     const unsigned int *n_chunks;
     int ntz;                       // sort tiles along z
-    unsigned long long *partials;  // chunk -> kWin x kWin window
 };
 
 // ------------------------------------------------------------------ deposit
@@ -93,24 +92,30 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 }
 
 // ------------------------------------------------------------------ the pass
-// A block owns kRounds x 1024 consecutive slots.  Each round is staged
-// through LDS: global loads and stores are fully coalesced (thread t moves
-// slots t, t+256, ...), while the compute phase gives each thread 4
-// consecutive slots, so consecutive particles of a sorted store share a cell.
+// A block of kPass threads owns a chunk of consecutive slots, taken in rounds
+// of 4 kPass.  Thread t holds slots 4t..4t+3 of a round in registers: each
+// component moves as one aligned float4 per thread (a wavefront moves 1 KB
+// contiguous per component), and consecutive particles of a sorted store,
+// which mostly share a cell, meet in one thread.
 // Deposits are summed twice before they reach memory: in registers over a
 // thread's run of equal cells, then across the wavefront by a segmented scan
 // over the lanes' last runs (lanes hold consecutive slots, so a wavefront of
 // a sorted store spans a few cells); only the last lane of each segment
 // issues the four 64-bit atomics.
-constexpr int kRound = kBlock * 4;
-constexpr int kRounds = 2;
-constexpr int kChunk = kRound * kRounds;
-constexpr int kPPT = kChunk / kBlock;       // slots per thread (grid sizing)
+// Two-wavefront blocks with 2.3 KB of LDS (the deposit window): a launch of
+// a few thousand chunks is mostly resident at once.
+constexpr int kPass = 128;
+constexpr int kRound = kPass * 4;
+constexpr int kChunk = 2048;                // slots per block of the untiled pass
+constexpr int kPPT = kChunk / kBlock;       // slots per thread of the helper kernels
 
-// LDS stage index of slot s of a round: one pad word per 32 keeps both the
-// coalesced (stride 1) and the per-thread (stride 4) accesses free of bank
-// conflicts
-__device__ __forceinline__ int lds_at(int s) { return s + (s >> 5); }
+// component c of the state (x, z, vx, vy, vz); c is a constant after
+// unrolling, so the pointer stays in scalar registers (an array of the five
+// pointers would live in scratch memory)
+__device__ __forceinline__ float *comp(const PushArgs &a, int c)
+{
+    return c == 0 ? a.x : c == 1 ? a.z : c == 2 ? a.vx : c == 3 ? a.vy : a.vz;
+}
 
 // running sum of one thread's deposits while consecutive particles share a cell
 struct DepRun {
@@ -157,18 +162,21 @@ __device__ __forceinline__ void wave_deposit(DepRun &d, unsigned long long *acc,
     d.ci = -1;
 }
 
-template <bool DEPOSIT, bool MCC, bool TILED>
-// six wavefronts per SIMD: the pass is bound by memory latency, and the LDS
-// stage allows six blocks per CU
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+// FAST: the common configuration fixed at compile time (B = 0, no periodic
+// wrap, no constant field, boundary and free-cell tests on, origin at 0),
+// which frees the scalar registers and branches the generic flags cost
+template <bool DEPOSIT, bool MCC, bool TILED, bool FAST>
+// four wavefronts per SIMD (128 VGPRs): a thread holds its four particles and
+// two gathers in registers (with fewer VGPRs the scalar and vector spills cost
+// more than the occupancy gains)
+__global__ void __launch_bounds__(kPass) __attribute__((amdgpu_waves_per_eu(4)))
 k_advance2d(PushArgs a)
 {
-    __shared__ float st[5][kRound + kRound / 32];
     __shared__ unsigned long long win[TILED ? kWin * kWin : 1];
     const int tid = threadIdx.x;
-    // slots [lo, hi) of this block; rounds start at lo rounded down to 64,
-    // so every wavefront moves whole aligned 256-byte segments and each
-    // thread's 4 slots share one trial Philox block
+    // slots [lo, hi) of this block; rounds start at lo rounded down to 64, so
+    // a thread's 4 slots are one aligned float4 of each component and share
+    // one trial Philox block
     int64_t lo, hi, co0;
     int wi0 = 0, wj0 = 0;
     bool wfree = false;
@@ -181,156 +189,196 @@ k_advance2d(PushArgs a)
         wi0 = (s.x / a.ntz) * kTile - kHalo;
         wj0 = (s.x % a.ntz) * kTile - kHalo;
         wfree = a.winfree[s.x] != 0;
-        for (int k = tid; k < kWin * kWin; k += kBlock) win[k] = 0ull;
+        for (int k = tid; k < kWin * kWin; k += kPass) win[k] = 0ull;
+        __syncthreads();
     } else {
         lo = (int64_t)blockIdx.x * kChunk;
         hi = min(lo + (int64_t)kChunk, a.n);
         co0 = lo;
     }
-    float *const arr[5] = {a.x, a.z, a.vx, a.vy, a.vz};
     int n_removed = 0, n_coll = 0;
-    bool nan_any = false;
+    bool nan_any = false, gbad = false;
     DepRun run;
     run.ci = -1;
-    for (int r = 0; r < kRounds; r++) {
-        const int64_t co = co0 + (int64_t)r * kRound;
-        if (co >= hi) break;  // uniform over the block
-        // ---- coalesced load into LDS
-        float v[5][4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int64_t idx = co + k * kBlock + tid;
-            const bool in = idx >= lo && idx < hi;
-#pragma unroll
-            for (int c = 0; c < 5; c++) v[c][k] = in ? arr[c][idx] : (c == 0 ? dead_value() : 0.f);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-#pragma unroll
-            for (int c = 0; c < 5; c++) st[c][lds_at(k * kBlock + tid)] = v[c][k];
-        __syncthreads();
-        // ---- compute on 4 consecutive slots, one particle live at a time
+#pragma unroll 1
+    for (int64_t co = co0; co < hi; co += kRound) {  // uniform over the block
+        // ---- this thread's slots pb..pb+3; slots outside [lo, hi) belong to
+        // other blocks: they are neither computed nor written
         const int64_t pb = co + 4 * tid;
-        uint32_t tw0 = 0u, tw1 = 0u, tw2 = 0u, tw3 = 0u;
+        const bool full = pb >= lo && pb + 4 <= hi;
+        float v[5][4];
+        if (full) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                const float4 t = *reinterpret_cast<const float4 *>(comp(a, c) + pb);
+                v[c][0] = t.x; v[c][1] = t.y; v[c][2] = t.z; v[c][3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (pb + k >= lo && pb + k < hi) {
+#pragma unroll
+                    for (int c = 0; c < 5; c++) v[c][k] = comp(a, c)[pb + k];
+                } else {
+                    v[0][k] = dead_value();
+#pragma unroll
+                    for (int c = 1; c < 5; c++) v[c][k] = 0.f;
+                }
+            }
+        }
+        uint32_t tw[4] = {0u, 0u, 0u, 0u};
         if (MCC) {
             // trial uniforms of slots pb..pb+3: one Philox block
-            tw0 = (uint32_t)(pb >> 2); tw1 = (uint32_t)a.step; tw2 = 0u;
-            tw3 = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
-            philox4x32_10(tw0, tw1, tw2, tw3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
+            tw[0] = (uint32_t)(pb >> 2); tw[1] = (uint32_t)a.step; tw[2] = 0u;
+            tw[3] = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
+            philox4x32_10(tw[0], tw[1], tw[2], tw[3], (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
         }
-#pragma unroll 1
-        for (int q = 0; q < 4; q++) {
-            const int li = lds_at(4 * tid + q);
-            float x = st[0][li];
-            if (is_dead(x)) continue;
-            float z = st[1][li], vx = st[2][li], vy = st[3][li], vz = st[4][li];
-            const uint32_t twq = q == 0 ? tw0 : q == 1 ? tw1 : q == 2 ? tw2 : tw3;
-
-            if (a.push) {
-                // ---- advance_boris (particles.cpp:945-988)
-                float ex, ez;
-                gather_E(a, (double)x, (double)z, a.phase, ex, ez);
-                vx = fmaf(ex, a.qmdt2, vx);
-                vz = fmaf(ez, a.qmdt2, vz);
-                if (a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
-                    float vpr = vx - vy * a.tz + vz * a.ty;
-                    float vpz = vz - vx * a.ty + vy * a.tx;
-                    float vpt = vy - vz * a.tx + vx * a.tz;
-                    vx = vx - vpt * a.sz + vpz * a.sy;
-                    vy = vy - vpz * a.sx + vpr * a.sz;
-                    vz = vz - vpr * a.sy + vpt * a.sx;
+        // ---- particles in pairs: both gathers are issued before either is
+        // used, so two table fetches are in flight per thread.  The body
+        // works on entries 0 and 1 and the arrays rotate by two after each
+        // pair (register moves once the loop is unrolled).
+#pragma unroll
+        for (int q0 = 0; q0 < 4; q0 += 2) {
+            float gex[2] = {0.f, 0.f}, gez[2] = {0.f, 0.f};
+            if (FAST || a.push) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    // a dead slot gathers at a clamped point and is discarded
+                    const bool b = gather_E_clamped<FAST>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
+                                                          gez[h]);
+                    gbad |= b && !is_dead(v[0][h]);
                 }
-                vx = fmaf(ex, a.qmdt2, vx);
-                vz = fmaf(ez, a.qmdt2, vz);
-                x = fmaf(vx, a.dt, x);
-                z = fmaf(vz, a.dt, z);
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int h = q;
+                float x = v[0][q];
+                if (is_dead(x)) continue;
+                float z = v[1][q], vx = v[2][q], vy = v[3][q], vz = v[4][q];
 
-                // ---- MCC trial (particles.cpp:990-993).  A hit only changes the
-                // velocity after the drift, so it cannot change the boundary test
-                // or the deposit below: the slot is queued and k_scatter2d applies
-                // BaseSpecies::scatter afterwards.
-                if (MCC) {
-                    if ((uint64_t)twq < a.trial_thr) {
-                        unsigned long long at = atomicAdd(a.cq_len, 1ull);
-                        a.cq[at] = (uint32_t)(pb + q);
-                        n_coll++;
+                if (FAST || a.push) {
+                    // ---- advance_boris (particles.cpp:945-988)
+                    const float ex = gex[h], ez = gez[h];
+                    vx = fmaf(ex, a.qmdt2, vx);
+                    vz = fmaf(ez, a.qmdt2, vz);
+                    if (!FAST && a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
+                        float vpr = vx - vy * a.tz + vz * a.ty;
+                        float vpz = vz - vx * a.ty + vy * a.tx;
+                        float vpt = vy - vz * a.tx + vx * a.tz;
+                        vx = vx - vpt * a.sz + vpz * a.sy;
+                        vy = vy - vpz * a.sx + vpr * a.sz;
+                        vz = vz - vpr * a.sy + vpt * a.sx;
+                    }
+                    vx = fmaf(ex, a.qmdt2, vx);
+                    vz = fmaf(ez, a.qmdt2, vz);
+                    x = fmaf(vx, a.dt, x);
+                    z = fmaf(vz, a.dt, z);
+
+                    // ---- MCC trial (particles.cpp:990-993).  A hit only changes the
+                    // velocity after the drift, so it cannot change the boundary test
+                    // or the deposit below: the slot is queued and k_flush_scatter
+                    // applies BaseSpecies::scatter afterwards.
+                    if (MCC) {
+                        if ((uint64_t)tw[q] < a.trial_thr) {
+                            unsigned long long at = atomicAdd(a.cq_len, 1ull);
+                            a.cq[at] = (uint32_t)(pb + q0 + q);
+                            n_coll++;
+                        }
                     }
                 }
-            }
 
-            // ---- advance_boundary (particles.hpp:376-406)
-            bool dead = false;
-            if (a.boundary) {
-                bool inside = x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1;
-                if (!inside) {
-                    if (x != x || z != z) { dead = true; nan_any = true; }
-                    else if (!a.periodic) dead = true;
-                    else {
-                        double xm = fmod((double)x, a.xper), zm = fmod((double)z, a.zper);
-                        if (xm < 0) xm += a.xper;
-                        if (zm < 0) zm += a.zper;
-                        x = (float)xm; z = (float)zm;
+                // ---- advance_boundary (particles.hpp:376-406)
+                bool dead = false;
+                if (FAST) {
+                    // a NaN fails every comparison, so it lands outside
+                    dead = !(x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1);
+                    nan_any |= dead && (x != x || z != z);
+                } else if (a.boundary) {
+                    bool inside = x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1;
+                    if (!inside) {
+                        if (x != x || z != z) { dead = true; nan_any = true; }
+                        else if (!a.periodic) dead = true;
+                        else {
+                            double xm = fmod((double)x, a.xper), zm = fmod((double)z, a.zper);
+                            if (xm < 0) xm += a.xper;
+                            if (zm < 0) zm += a.zper;
+                            x = (float)xm; z = (float)zm;
+                        }
                     }
                 }
-            }
-            const double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
-            if (!dead && a.boundary && a.checkfree) {
-                bool free_ = false;
-                if (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz) {
-                    int ci = (int)X, cj = (int)Z;
-                    // a cell of an all-free window needs no table lookup
-                    if (wfree && (unsigned)(ci - wi0) < (unsigned)(kWin - 1) &&
-                        (unsigned)(cj - wj0) < (unsigned)(kWin - 1))
-                        free_ = true;
-                    else if (ci <= a.nx - 1 && cj <= a.nz - 1)
-                        free_ = a.cellfree[ci * a.nz + cj] != 0;
+                const double X = FAST ? (double)x * a.idx : ((double)x - a.xmin) * a.idx;
+                const double Z = FAST ? (double)z * a.idz : ((double)z - a.zmin) * a.idz;
+                if (!dead && (FAST || (a.boundary && a.checkfree))) {
+                    bool free_ = false;
+                    if (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz) {
+                        int ci = (int)X, cj = (int)Z;
+                        // a cell of an all-free window needs no table lookup
+                        if (wfree && (unsigned)(ci - wi0) < (unsigned)(kWin - 1) &&
+                            (unsigned)(cj - wj0) < (unsigned)(kWin - 1))
+                            free_ = true;
+                        else if (ci <= a.nx - 1 && cj <= a.nz - 1)
+                            free_ = a.cellfree[ci * a.nz + cj] != 0;
+                    }
+                    dead = !free_;
                 }
-                dead = !free_;
-            }
-            if (dead) {
-                st[0][li] = dead_value();
-                n_removed++;
-                continue;
-            }
-            // ---- deposit (particles.hpp:408-409), summed per run of equal cells
-            if (DEPOSIT) {
-                int ci, cj;
-                uint32_t w[4];
-                if (cic_weights(X, Z, a.nx, a.nz, ci, cj, w)) {
-                    if (ci == run.ci && cj == run.cj) {
-                        run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
+                if (dead) {
+                    v[0][q] = dead_value();
+                    n_removed++;
+                    continue;
+                }
+                // ---- deposit (particles.hpp:408-409), summed per run of equal cells
+                if (DEPOSIT) {
+                    int ci, cj;
+                    uint32_t w[4];
+                    if (cic_weights(X, Z, a.nx, a.nz, ci, cj, w)) {
+                        if (ci == run.ci && cj == run.cj) {
+                            run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
+                        } else {
+                            if constexpr (TILED) run_to_window(run, win, wi0, wj0, a.acc, a.nz);
+                            else if (run.ci >= 0) deposit_global(a.acc, a.nz, run.ci, run.cj, run.w);
+                            run.ci = ci; run.cj = cj;
+                            run.w[0] = w[0]; run.w[1] = w[1]; run.w[2] = w[2]; run.w[3] = w[3];
+                        }
                     } else {
-                        if constexpr (TILED) run_to_window(run, win, wi0, wj0, a.acc, a.nz);
-                        else if (run.ci >= 0) deposit_global(a.acc, a.nz, run.ci, run.cj, run.w);
-                        run.ci = ci; run.cj = cj;
-                        run.w[0] = w[0]; run.w[1] = w[1]; run.w[2] = w[2]; run.w[3] = w[3];
+                        raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
                     }
-                } else {
-                    raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
                 }
+                v[0][q] = x; v[1][q] = z; v[2][q] = vx; v[3][q] = vy; v[4][q] = vz;
             }
-            st[0][li] = x; st[1][li] = z; st[2][li] = vx; st[3][li] = vy; st[4][li] = vz;
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                float t0 = v[c][0], t1 = v[c][1];
+                v[c][0] = v[c][2]; v[c][1] = v[c][3]; v[c][2] = t0; v[c][3] = t1;
+            }
+            uint32_t u0 = tw[0], u1 = tw[1];
+            tw[0] = tw[2]; tw[1] = tw[3]; tw[2] = u0; tw[3] = u1;
         }
         if constexpr (DEPOSIT && !TILED) wave_deposit(run, a.acc, a.nz);
-        __syncthreads();
-        // ---- coalesced store
+        // ---- store
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int64_t idx = co + k * kBlock + tid;
-            if (idx >= lo && idx < hi) {
+            for (int c = 0; c < 5; c++)
+                *reinterpret_cast<float4 *>(comp(a, c) + pb) = make_float4(v[c][0], v[c][1], v[c][2], v[c][3]);
+        } else {
 #pragma unroll
-                for (int c = 0; c < 5; c++) arr[c][idx] = st[c][lds_at(k * kBlock + tid)];
+            for (int k = 0; k < 4; k++) {
+                if (pb + k >= lo && pb + k < hi) {
+#pragma unroll
+                    for (int c = 0; c < 5; c++) comp(a, c)[pb + k] = v[c][k];
+                }
             }
         }
     }
     if constexpr (TILED) {
-        // the window goes out with plain stores; k_reduce_tiles adds the
-        // overlapping windows of neighbouring chunks
+        // the window's nonzero nodes go to the global grid as 64-bit atomics
+        // (integer sums: the result does not depend on their order).  A
+        // nonzero node belongs to a deposited cell, so it lies in the grid.
         run_to_window(run, win, wi0, wj0, a.acc, a.nz);
         __syncthreads();
-        unsigned long long *out = a.partials + (size_t)blockIdx.x * (kWin * kWin);
-        for (int k = tid; k < kWin * kWin; k += kBlock) out[k] = win[k];
+        for (int k = tid; k < kWin * kWin; k += kPass) {
+            const unsigned long long v = win[k];
+            if (v) atomicAdd(a.acc + (size_t)(wi0 + k / kWin) * a.nz + (wj0 + k % kWin), v);
+        }
     }
     // ---- counters: one atomic per wavefront
     int sum = n_removed;
@@ -342,6 +390,7 @@ k_advance2d(PushArgs a)
         if (__lane_id() == 0 && c) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)c);
     }
     if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
+    if (__ballot(gbad) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
 }
 
 // BaseSpecies::scatter for the slots queued by k_advance2d, threads
@@ -364,60 +413,26 @@ __device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long l
     }
 }
 
-struct ReduceArgs {
-    const unsigned long long *partials;
-    const int *chunk_first, *chunk_count;
-    int ntx, ntz;
-    unsigned long long *acc;
-    double *rho;
-    double charge;
-    int nx, nz;
-};
-
-// Node sums of the tiled pass: every node of an 8 x 8 output tile adds the
-// windows of the chunks of the 3 x 3 neighbouring sort tiles that cover it,
-// plus the fallback atomics in acc; then rho += charge * sum * 2^-31 and acc
-// is cleared (the flush of the untiled pass).  With MCC, one extra row of
-// blocks applies the collision queue meanwhile.
+// After the pass: rho += charge * acc * 2^-31 and acc = 0 over the grid
+// (blocks [0, nflush)); with MCC, the blocks after them apply the collision
+// queue meanwhile.
 template <bool MCC>
-__global__ void __launch_bounds__(64) k_reduce_tiles(ReduceArgs r, PushArgs a, unsigned long long *next_len)
+__global__ void __launch_bounds__(kBlock) k_flush_scatter(unsigned long long *acc, double *rho, double charge,
+                                                         size_t n, unsigned nflush, PushArgs a,
+                                                         unsigned long long *next_len)
 {
-    if (MCC && blockIdx.y == gridDim.y - 1) {
-        scatter_queue(a, (unsigned long long)blockIdx.x * 64 + threadIdx.x,
-                      (unsigned long long)gridDim.x * 64, next_len);
+    if (MCC && blockIdx.x >= nflush) {
+        scatter_queue(a, (unsigned long long)(blockIdx.x - nflush) * kBlock + threadIdx.x,
+                      (unsigned long long)(gridDim.x - nflush) * kBlock, next_len);
         return;
     }
-    const unsigned long long *partials = r.partials;
-    const int *chunk_first = r.chunk_first, *chunk_count = r.chunk_count;
-    const int ntx = r.ntx, ntz = r.ntz, nx = r.nx, nz = r.nz;
-    unsigned long long *acc = r.acc;
-    const int TI = blockIdx.y, TJ = blockIdx.x;
-    const int i = TI * kTile + threadIdx.x / kTile, j = TJ * kTile + threadIdx.x % kTile;
-    if (i >= nx || j >= nz) return;
-    const size_t k = (size_t)i * nz + j;
-    long long s = (long long)acc[k];
-    for (int ai = TI - 1; ai <= TI + 1; ai++) {
-        if (ai < 0 || ai >= ntx) continue;
-        for (int aj = TJ - 1; aj <= TJ + 1; aj++) {
-            if (aj < 0 || aj >= ntz) continue;
-            const int t = ai * ntz + aj;
-            const int a = i - (ai * kTile - kHalo), b = j - (aj * kTile - kHalo);
-            if (a < 0 || a >= kWin || b < 0 || b >= kWin) continue;
-            const int c0 = chunk_first[t], c1 = c0 + chunk_count[t];
-            for (int c = c0; c < c1; c++) s += (long long)partials[(size_t)c * (kWin * kWin) + a * kWin + b];
-        }
+    const size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const long long v = (long long)acc[k];
+    if (v) {
+        rho[k] += charge * ((double)v * kInvFix);
+        acc[k] = 0ull;
     }
-    if (s) {
-        r.rho[k] += r.charge * ((double)s * kInvFix);
-        if (acc[k]) acc[k] = 0ull;
-    }
-}
-
-// the collision queue of an untiled launch
-__global__ void __launch_bounds__(kBlock) k_scatter2d(PushArgs a, unsigned long long *next_len)
-{
-    scatter_queue(a, (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x,
-                  (unsigned long long)gridDim.x * blockDim.x, next_len);
 }
 
 // Species<CARTESIAN>::advance_boris_init (particles.cpp:997-1050)
@@ -815,7 +830,6 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.sched = nullptr;
     a.n_chunks = nullptr;
     a.ntz = 0;
-    a.partials = nullptr;
     a.acc = nullptr;
     a.counters = f->ctx->counters;
 }
@@ -844,8 +858,9 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     fill_push_args(a, f, s, p, 1.0, 1.0);
     a.boundary = (flags & M2D_STEP_BOUNDARY) != 0;
     a.push = (flags & M2D_STEP_PUSH) != 0;
-    // the MCC trial belongs to advance_boris
-    bool use_mcc = a.push && mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
+    // the MCC trial belongs to advance_boris; an empty store queues nothing
+    // (and leaves the alternating queue counters as they are)
+    bool use_mcc = p->n > 0 && a.push && mcc && std::isfinite(s->lifetime) && s->lifetime > 0;
     unsigned long long *next_len = nullptr;
     if (use_mcc) {
         a.trial_thr = trial_threshold(1.0 - exp(-s->dt / s->lifetime));  // particles.cpp:937
@@ -859,7 +874,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         next_len = ctx->counters + CNT_QLEN + (slot ^ 1);
     }
     // a store sorted by m2d_sort_2d against these fields deposits through
-    // per-chunk LDS windows reduced by k_reduce_tiles
+    // per-chunk LDS windows
     const m2d_tiling *T = deposit ? find_tiling(ctx, p->x, p->n) : nullptr;
     if (T && (T->nty != 1 || T->ntx != (f->nx - 2) / kTile + 1 || T->ntz != (f->nz - 2) / kTile + 1)) T = nullptr;
     if (deposit) {
@@ -867,43 +882,51 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         a.acc = (unsigned long long *)ctx->acc.ptr;
     }
     if (T) {
-        M2D_TRY(ensure(ctx, ctx->partials, (size_t)T->max_chunks * kWin * kWin * sizeof(unsigned long long)));
         a.sched = T->sched;
         a.n_chunks = T->n_chunks;
         a.ntz = T->ntz;
-        a.partials = (unsigned long long *)ctx->partials.ptr;
     }
     if (p->n > 0) {
         hipEvent_t t0 = prof_begin(ctx);
+        // the common configuration runs the compile-time specialised pass
+        const bool fast = a.push && a.boundary && a.checkfree && !a.rotate && !a.periodic && !a.econst &&
+                          a.xmin == 0.0 && a.zmin == 0.0;
+        hipStream_t st = ctx->stream;
         if (T) {
             const int g = T->max_chunks;
-            if (use_mcc) k_advance2d<true, true, true><<<g, kBlock, 0, ctx->stream>>>(a);
-            else k_advance2d<true, false, true><<<g, kBlock, 0, ctx->stream>>>(a);
+            if (fast) {
+                if (use_mcc) k_advance2d<true, true, true, true><<<g, kPass, 0, st>>>(a);
+                else k_advance2d<true, false, true, true><<<g, kPass, 0, st>>>(a);
+            } else {
+                if (use_mcc) k_advance2d<true, true, true, false><<<g, kPass, 0, st>>>(a);
+                else k_advance2d<true, false, true, false><<<g, kPass, 0, st>>>(a);
+            }
         } else {
-            const int g = grid_for(p->n, kPPT);
-            if (deposit && use_mcc) k_advance2d<true, true, false><<<g, kBlock, 0, ctx->stream>>>(a);
-            else if (deposit) k_advance2d<true, false, false><<<g, kBlock, 0, ctx->stream>>>(a);
-            else if (use_mcc) k_advance2d<false, true, false><<<g, kBlock, 0, ctx->stream>>>(a);
-            else k_advance2d<false, false, false><<<g, kBlock, 0, ctx->stream>>>(a);
+            const int g = (int)((p->n + kChunk - 1) / kChunk);
+            if (fast) {
+                if (deposit && use_mcc) k_advance2d<true, true, false, true><<<g, kPass, 0, st>>>(a);
+                else if (deposit) k_advance2d<true, false, false, true><<<g, kPass, 0, st>>>(a);
+                else if (use_mcc) k_advance2d<false, true, false, true><<<g, kPass, 0, st>>>(a);
+                else k_advance2d<false, false, false, true><<<g, kPass, 0, st>>>(a);
+            } else {
+                if (deposit && use_mcc) k_advance2d<true, true, false, false><<<g, kPass, 0, st>>>(a);
+                else if (deposit) k_advance2d<true, false, false, false><<<g, kPass, 0, st>>>(a);
+                else if (use_mcc) k_advance2d<false, true, false, false><<<g, kPass, 0, st>>>(a);
+                else k_advance2d<false, false, false, false><<<g, kPass, 0, st>>>(a);
+            }
         }
         M2D_HIP(hipGetLastError());
         prof_end(ctx, t0);
-        if (use_mcc && !T) {
-            k_scatter2d<<<64, kBlock, 0, ctx->stream>>>(a, next_len);
-            M2D_HIP(hipGetLastError());
-        }
     }
-    if (T) {
-        // the tile reduction; with MCC its last row of blocks runs the scatter
-        const bool sc = use_mcc && p->n > 0;
-        dim3 g((f->nz + kTile - 1) / kTile, (f->nx + kTile - 1) / kTile + (sc ? 1 : 0));
-        ReduceArgs ra{a.partials, T->chunk_first, T->chunk_count, T->ntx, T->ntz, a.acc, rho, s->charge,
-                      f->nx, f->nz};
-        if (sc) k_reduce_tiles<true><<<g, kTile * kTile, 0, ctx->stream>>>(ra, a, next_len);
-        else k_reduce_tiles<false><<<g, kTile * kTile, 0, ctx->stream>>>(ra, a, next_len);
+    if (deposit || use_mcc) {
+        // the grid flush and the collision queue in one launch
+        const unsigned nfl = deposit ? (unsigned)((f->n + kBlock - 1) / kBlock) : 0u;
+        const unsigned g = nfl + (use_mcc ? 64u : 0u);
+        if (use_mcc)
+            k_flush_scatter<true><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n, nfl, a, next_len);
+        else
+            k_flush_scatter<false><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n, nfl, a, next_len);
         M2D_HIP(hipGetLastError());
-    } else if (deposit) {
-        M2D_TRY(flush(ctx, f->n, rho, s->charge));
     }
     if (stats) {
         M2D_TRY(read_counters(ctx));

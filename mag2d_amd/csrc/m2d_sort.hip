@@ -72,17 +72,17 @@ __global__ void k_tile_bounds(const uint32_t *keys, int64_t n, uint32_t dead_key
     if (i == n - 1 || keys[i + 1] == dead_key || (keys[i + 1] >> kTileShift) != t) tend[t] = (int)(i + 1);
 }
 
-__global__ void k_tile_chunks(const int *tstart, const int *tend, int ntiles, int *count)
+__global__ void k_tile_chunks(const int *tstart, const int *tend, int ntiles, int cap, int *count)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     int c = tend[t] - tstart[t];
-    count[t] = c > 0 ? (c + kChunkCap - 1) / kChunkCap : 0;
+    count[t] = c > 0 ? (c + cap - 1) / cap : 0;
 }
 
 // exclusive scan of the per-tile chunk counts (one block) and the schedule
 __global__ void __launch_bounds__(1024) k_tile_schedule(const int *tstart, const int *tend,
-                                                        const int *count, int ntiles, int *first,
+                                                        const int *count, int ntiles, int cap, int *first,
                                                         int4 *sched, unsigned int *n_chunks)
 {
     __shared__ int s[1024];
@@ -102,8 +102,8 @@ __global__ void __launch_bounds__(1024) k_tile_schedule(const int *tstart, const
     for (int t = lo; t < hi; t++) {
         first[t] = run;
         for (int c = 0; c < count[t]; c++) {
-            int a = tstart[t] + c * kChunkCap;
-            sched[run + c] = make_int4(t, a, min(tend[t], a + kChunkCap), 0);
+            int a = tstart[t] + c * cap;
+            sched[run + c] = make_int4(t, a, min(tend[t], a + cap), 0);
         }
         run += count[t];
     }
@@ -133,9 +133,10 @@ void free_tilings(m2d_ctx *ctx)
 }
 
 // build the schedule of the store at x (n slots) from its sorted keys; tiles
-// number ntx * nty * ntz (nty = 1 in 2D), 64 cells each
+// number ntx * nty * ntz (nty = 1 in 2D), 64 cells each; a tile's slots are
+// cut into chunks of at most cap
 int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
-                 int nty, int ntz)
+                 int nty, int ntz, int cap)
 {
     forget_tiling(ctx, x);
     m2d_tiling *T = nullptr;
@@ -153,7 +154,7 @@ int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, 
     }
     T->x = nullptr;
     const int ntiles = ntx * nty * ntz;
-    const int max_chunks = ntiles + (int)(n / kChunkCap) + 1;
+    const int max_chunks = ntiles + (int)(n / cap) + 1;
     const size_t words = 4 * (size_t)ntiles + 4 * (size_t)max_chunks + 4;
     M2D_TRY(ensure(ctx, T->buf, words * sizeof(int)));
     int *b = (int *)T->buf.ptr;
@@ -167,8 +168,8 @@ int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, 
     k_tile_bounds<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(keys, n, dead_key, T->tile_start,
                                                                         T->tile_end);
     k_tile_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, ctx->stream>>>(T->tile_start, T->tile_end,
-                                                                            ntiles, T->chunk_count);
-    k_tile_schedule<<<1, 1024, 0, ctx->stream>>>(T->tile_start, T->tile_end, T->chunk_count, ntiles,
+                                                                            ntiles, cap, T->chunk_count);
+    k_tile_schedule<<<1, 1024, 0, ctx->stream>>>(T->tile_start, T->tile_end, T->chunk_count, ntiles, cap,
                                                  T->chunk_first, T->sched, T->n_chunks);
     M2D_HIP(hipGetLastError());
     T->ntx = ntx;
@@ -235,7 +236,7 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key, live);
     M2D_HIP(hipGetLastError());
-    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, g.dead_key, ntx, 1, g.ntz));
+    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, g.dead_key, ntx, 1, g.ntz, kChunkCap));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];
