@@ -62,12 +62,12 @@ int fields_build_tables(m2d_fields2d *f)
 // ------------------------------------------------------------------ FP64 kernels
 __device__ __forceinline__ bool is_dir(uint8_t m) { return m == M2D_FIXED || m == M2D_FIXED_RF; }
 
-// rhs in place (fields.cpp:297-310, 333-346), Dirichlet values into u
-__global__ void k_rhs(const uint8_t *mask, const double *voltage, double *rho, double *u, size_t n,
-                      int mode, double scale)
+// rhs in place (fields.cpp:297-310, 333-346), Dirichlet values into u.
+// The node bodies below are shared by the grid kernels and the
+// single-workgroup solve, so both paths compute each node the same way.
+__device__ __forceinline__ void node_rhs(const uint8_t *mask, const double *voltage, double *rho, double *u,
+                                         size_t k, int mode, double scale)
 {
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
     uint8_t m = mask[k];
     double b;
     if (mode == M2D_SOLVE_RAW) b = rho[k];
@@ -76,6 +76,36 @@ __global__ void k_rhs(const uint8_t *mask, const double *voltage, double *rho, d
     else b = rho[k] * scale;
     rho[k] = b;
     if (is_dir(m)) u[k] = b;
+}
+
+__global__ void k_rhs(const uint8_t *mask, const double *voltage, double *rho, double *u, size_t n,
+                      int mode, double scale)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) node_rhs(mask, voltage, rho, u, k, mode, scale);
+}
+
+// r = b - A u at node k (0 on Dirichlet nodes), d = -r in FP32; rr = r^2 and
+// bb = b_eff^2 (b with the Dirichlet neighbours moved to the right-hand side)
+__device__ __forceinline__ void node_residual64(const uint8_t *mask, const double *b, const double *u, int nz,
+                                                double *r, float *d, size_t k, double &rr, double &bb)
+{
+    double rv = 0;
+    bb = 0;
+    if (!is_dir(mask[k])) {
+        // free rows never sit on the frame (checked at creation)
+        double au = u[k - nz] + u[k - 1] - 4.0 * u[k] + u[k + 1] + u[k + nz];
+        rv = b[k] - au;
+        double be = b[k];
+        if (is_dir(mask[k - nz])) be -= u[k - nz];
+        if (is_dir(mask[k - 1])) be -= u[k - 1];
+        if (is_dir(mask[k + 1])) be -= u[k + 1];
+        if (is_dir(mask[k + nz])) be -= u[k + nz];
+        bb = be * be;
+    }
+    r[k] = rv;
+    d[k] = (float)(-rv);
+    rr = rv * rv;
 }
 
 // r = b - A u on free nodes (rows of fields.cpp:229-237), 0 on Dirichlet nodes;
@@ -88,25 +118,7 @@ __global__ void __launch_bounds__(256) k_residual64(const uint8_t *mask, const d
     size_t n = (size_t)nx * nz;
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0, bb = 0;
-    if (k < n) {
-        double rv = 0;
-        if (!is_dir(mask[k])) {
-            int i = (int)(k / nz), j = (int)(k % nz);
-            // free rows never sit on the frame (checked at creation)
-            double au = u[k - nz] + u[k - 1] - 4.0 * u[k] + u[k + 1] + u[k + nz];
-            rv = b[k] - au;
-            double be = b[k];
-            if (is_dir(mask[k - nz])) be -= u[k - nz];
-            if (is_dir(mask[k - 1])) be -= u[k - 1];
-            if (is_dir(mask[k + 1])) be -= u[k + 1];
-            if (is_dir(mask[k + nz])) be -= u[k + nz];
-            bb = be * be;
-            (void)i; (void)j;
-        }
-        r[k] = rv;
-        d[k] = (float)(-rv);
-        rr = rv * rv;
-    }
+    if (k < n) node_residual64(mask, b, u, nz, r, d, k, rr, bb);
     s1[threadIdx.x] = rr;
     s2[threadIdx.x] = bb;
     __syncthreads();
@@ -140,6 +152,7 @@ __global__ void __launch_bounds__(1024) k_sum_pairs(const double *partial, int n
 }
 
 // ------------------------------------------------------------------ FP32 MG kernels
+
 // 9-point operator y = A x (Dirichlet rows give 0)
 __device__ __forceinline__ float apply9(const float *st, const float *x, int i, int j, int nx, int nz)
 {
@@ -172,13 +185,10 @@ __device__ __forceinline__ float apply5(const uint8_t *dir, const float *x, size
     return s;
 }
 
-// red-black Gauss-Seidel half sweep on the finest level (colour = (i+j)&1)
-__global__ void k_rbgs5(const uint8_t *dir, float *x, const float *b, int nx, int nz, int colour)
+// red-black Gauss-Seidel update of node (i, j) of colour (i+j)&1 on the
+// finest level
+__device__ __forceinline__ void node_rbgs5(const uint8_t *dir, float *x, const float *b, int nz, size_t k)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
-    if (j >= nz || ((i + j) & 1) != colour) return;
-    size_t k = (size_t)i * nz + j;
     if (dir[k]) return;
     float s = b[k];
     if (!dir[k - nz]) s += x[k - nz];
@@ -188,13 +198,27 @@ __global__ void k_rbgs5(const uint8_t *dir, float *x, const float *b, int nx, in
     x[k] = 0.25f * s;
 }
 
+// half sweep (colour = (i+j)&1)
+__global__ void k_rbgs5(const uint8_t *dir, float *x, const float *b, int nx, int nz, int colour)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nz || ((i + j) & 1) != colour) return;
+    node_rbgs5(dir, x, b, nz, (size_t)i * nz + j);
+}
+
+__device__ __forceinline__ float node_resid5(const uint8_t *dir, const float *x, const float *b, int nz, size_t k)
+{
+    return dir[k] ? 0.f : b[k] - apply5(dir, x, k, nz);
+}
+
 __global__ void k_resid5(const uint8_t *dir, const float *x, const float *b, float *r, int nx, int nz)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     if (j >= nz) return;
     size_t k = (size_t)i * nz + j;
-    r[k] = dir[k] ? 0.f : b[k] - apply5(dir, x, k, nz);
+    r[k] = node_resid5(dir, x, b, nz, k);
 }
 
 __global__ void k_apply5(const uint8_t *dir, const float *x, float *y, int nx, int nz)
@@ -207,15 +231,28 @@ __global__ void k_apply5(const uint8_t *dir, const float *x, float *y, int nx, i
 }
 
 // weighted Jacobi on a 9-point level: xo = xi + w D^-1 (b - A xi)
+__device__ __forceinline__ float node_jacobi9(const float *st, const float *dinv, const float *xi, const float *b,
+                                              int nx, int nz, float w, int i, int j)
+{
+    size_t k = (size_t)i * nz + j;
+    float dv = dinv[k];
+    return dv == 0.f ? 0.f : xi[k] + w * dv * (b[k] - apply9(st, xi, i, j, nx, nz));
+}
+
 __global__ void k_jacobi9(const float *st, const float *dinv, const float *xi, float *xo,
                           const float *b, int nx, int nz, float w)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     if (j >= nz) return;
+    xo[(size_t)i * nz + j] = node_jacobi9(st, dinv, xi, b, nx, nz, w, i, j);
+}
+
+__device__ __forceinline__ float node_resid9(const float *st, const float *dinv, const float *x, const float *b,
+                                             int nx, int nz, int i, int j)
+{
     size_t k = (size_t)i * nz + j;
-    float dv = dinv[k];
-    xo[k] = dv == 0.f ? 0.f : xi[k] + w * dv * (b[k] - apply9(st, xi, i, j, nx, nz));
+    return dinv[k] == 0.f ? 0.f : b[k] - apply9(st, x, i, j, nx, nz);
 }
 
 __global__ void k_resid9(const float *st, const float *dinv, const float *x, const float *b, float *r,
@@ -224,19 +261,15 @@ __global__ void k_resid9(const float *st, const float *dinv, const float *x, con
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     if (j >= nz) return;
-    size_t k = (size_t)i * nz + j;
-    r[k] = dinv[k] == 0.f ? 0.f : b[k] - apply9(st, x, i, j, nx, nz);
+    r[(size_t)i * nz + j] = node_resid9(st, dinv, x, b, nx, nz, i, j);
 }
 
 // b_c = P^T r_f with P the bilinear interpolation masked on fine Dirichlet nodes
-__global__ void k_restrict(const float *rf, int nxf, int nzf, float *bc, const float *dinvc, int nxc,
-                           int nzc)
+__device__ __forceinline__ float node_restrict(const float *rf, int nxf, int nzf, const float *dinvc, int nzc,
+                                               int I, int J)
 {
-    int J = blockIdx.x * blockDim.x + threadIdx.x;
-    int I = blockIdx.y;
-    if (J >= nzc) return;
     size_t K = (size_t)I * nzc + J;
-    if (dinvc[K] == 0.f) { bc[K] = 0.f; return; }
+    if (dinvc[K] == 0.f) return 0.f;
     float s = 0.f;
     for (int di = -1; di <= 1; di++) {
         int i = 2 * I + di;
@@ -249,16 +282,22 @@ __global__ void k_restrict(const float *rf, int nxf, int nzf, float *bc, const f
             s = fmaf(wi * wj, rf[(size_t)i * nzf + j], s);  // r is 0 on Dirichlet nodes
         }
     }
-    bc[K] = s;
+    return s;
+}
+
+__global__ void k_restrict(const float *rf, int nxf, int nzf, float *bc, const float *dinvc, int nxc,
+                           int nzc)
+{
+    int J = blockIdx.x * blockDim.x + threadIdx.x;
+    int I = blockIdx.y;
+    if (J >= nzc) return;
+    bc[(size_t)I * nzc + J] = node_restrict(rf, nxf, nzf, dinvc, nzc, I, J);
 }
 
 // x_f += P x_c on free fine nodes
-__global__ void k_prolong(float *xf, const uint8_t *dirf, int nxf, int nzf, const float *xc, int nxc,
-                          int nzc)
+__device__ __forceinline__ void node_prolong(float *xf, const uint8_t *dirf, int nzf, const float *xc, int nxc,
+                                             int nzc, int i, int j)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
-    if (j >= nzf) return;
     size_t k = (size_t)i * nzf + j;
     if (dirf[k]) return;
     int I0 = i >> 1, J0 = j >> 1;
@@ -285,6 +324,15 @@ __global__ void k_prolong(float *xf, const uint8_t *dirf, int nxf, int nzf, cons
         s = 0.5f * (a + b);
     }
     xf[k] += s;
+}
+
+__global__ void k_prolong(float *xf, const uint8_t *dirf, int nxf, int nzf, const float *xc, int nxc,
+                          int nzc)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nzf) return;
+    node_prolong(xf, dirf, nzf, xc, nxc, nzc, i, j);
 }
 
 // dense coarsest solve x = A^-1 b, one block
@@ -562,6 +610,8 @@ static void mg_free(Mg &mg)
 }
 
 // ------------------------------------------------------------------ host: V-cycle
+// smoothing sweeps before and after the coarse correction, and the damping
+// of the Jacobi sweeps on the Galerkin levels
 constexpr int kPre = 2, kPost = 2;
 constexpr float kOmega = 0.8f;
 
@@ -693,9 +743,9 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
     const unsigned gb = (unsigned)((n + 255) / 256);
     double *u = mode == M2D_SOLVE_RF ? f->urf : f->u;
     double scale = -(f->p.grid.dx * f->p.grid.dx) / kEps0 / f->p.dV;  // fields.cpp:307
+    if (tol <= 0) tol = 1e-10;
     k_rhs<<<gb, 256, 0, s>>>(f->mask, f->voltage, f->rho, u, n, mode, scale);
     M2D_HIP(hipGetLastError());
-    if (tol <= 0) tol = 1e-10;
     int outer = 0, inner_total = 0;
     double rel = 0;
     const int max_outer = 12;
@@ -740,6 +790,9 @@ int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_
     const m2d_grid2d &g = p->grid;
     M2D_CHECK(g.nx >= 3 && g.nz >= 3 && g.dx > 0 && g.dz > 0, M2D_ERR_BAD_ARG,
               "m2d_fields2d_create: grid %dx%d spacing %g,%g", g.nx, g.nz, g.dx, g.dz);
+    // node indices are 32-bit and the gathers multiply in 24 bits
+    M2D_CHECK(g.nx < (1 << 24) && g.nz < (1 << 24) && (size_t)(g.nx + 1) * (g.nz + 1) < (1ull << 31),
+              M2D_ERR_BAD_ARG, "m2d_fields2d_create: grid %dx%d too large", g.nx, g.nz);
     M2D_CHECK(p->dV > 0, M2D_ERR_BAD_ARG, "m2d_fields2d_create: dV must be > 0");
     const size_t n = (size_t)g.nx * g.nz;
     // a free 5-point row on the frame would reach outside the grid (fields.cpp:229-237)

@@ -43,10 +43,12 @@ __device__ __forceinline__ bool gather_E_clamped(const A &a, double xd, double z
     const float fx = (float)(X - ih + 0.5), fyx = (float)(Z - jx);
     const float fxz = (float)(X - i0), fz = (float)(Z - jh + 0.5);
 
-    const float2 *r0 = a.gx + (ih * nz + jx);
+    // indices are clamped non-negative and the tables hold fewer than 2^24
+    // rows: 24-bit products are exact (full-rate multiplies)
+    const float2 *r0 = a.gx + (__umul24(ih, nz) + jx);
     const float2 *r1 = r0 + nz;
     float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
-    const float2 *q0 = a.gz + (i0 * (nz + 1) + jh);
+    const float2 *q0 = a.gz + (__umul24(i0, nz + 1) + jh);
     const float2 *q1 = q0 + (nz + 1);
     float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
 

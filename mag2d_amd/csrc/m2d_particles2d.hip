@@ -106,6 +106,7 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 // a few thousand chunks is mostly resident at once.
 constexpr int kPass = 128;
 constexpr int kRound = kPass * 4;
+constexpr int kGather = 2;                  // particles whose gathers are in flight together
 constexpr int kChunk = 2048;                // slots per block of the untiled pass
 constexpr int kPPT = kChunk / kBlock;       // slots per thread of the helper kernels
 
@@ -233,16 +234,19 @@ k_advance2d(PushArgs a)
             tw[3] = (kPurposeTrial << 24) | ((uint32_t)(a.step >> 32) & 0xFFFFFFu);
             philox4x32_10(tw[0], tw[1], tw[2], tw[3], (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ a.stream);
         }
-        // ---- particles in pairs: both gathers are issued before either is
-        // used, so two table fetches are in flight per thread.  The body
-        // works on entries 0 and 1 and the arrays rotate by two after each
-        // pair (register moves once the loop is unrolled).
+        // ---- particles in groups of kGather: the group's gathers are issued
+        // before any is used, so kGather table fetches are in flight per
+        // thread.  The body works on entries 0..kGather-1 and the arrays
+        // rotate by kGather after each group (register renaming once the
+        // loop is unrolled).
 #pragma unroll
-        for (int q0 = 0; q0 < 4; q0 += 2) {
-            float gex[2] = {0.f, 0.f}, gez[2] = {0.f, 0.f};
+        for (int q0 = 0; q0 < 4; q0 += kGather) {
+            float gex[kGather], gez[kGather];
+#pragma unroll
+            for (int h = 0; h < kGather; h++) gex[h] = gez[h] = 0.f;
             if (FAST || a.push) {
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
+                for (int h = 0; h < kGather; h++) {
                     // a dead slot gathers at a clamped point and is discarded
                     const bool b = gather_E_clamped<FAST>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
                                                           gez[h]);
@@ -250,7 +254,7 @@ k_advance2d(PushArgs a)
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
+            for (int q = 0; q < kGather; q++) {
                 const int h = q;
                 float x = v[0][q];
                 if (is_dead(x)) continue;
@@ -347,11 +351,17 @@ k_advance2d(PushArgs a)
             }
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                float t0 = v[c][0], t1 = v[c][1];
-                v[c][0] = v[c][2]; v[c][1] = v[c][3]; v[c][2] = t0; v[c][3] = t1;
+                float t[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) t[q] = v[c][(q + kGather) & 3];
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[c][q] = t[q];
             }
-            uint32_t u0 = tw[0], u1 = tw[1];
-            tw[0] = tw[2]; tw[1] = tw[3]; tw[2] = u0; tw[3] = u1;
+            uint32_t u[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) u[q] = tw[(q + kGather) & 3];
+#pragma unroll
+            for (int q = 0; q < 4; q++) tw[q] = u[q];
         }
         if constexpr (DEPOSIT && !TILED) wave_deposit(run, a.acc, a.nz);
         // ---- store
