@@ -108,27 +108,6 @@ __device__ __forceinline__ void node_residual64(const uint8_t *mask, const doubl
     rr = rv * rv;
 }
 
-// r = b - A u on free nodes (rows of fields.cpp:229-237), 0 on Dirichlet nodes;
-// also d = -r in FP32 and per-block partial sums of r^2 and b_eff^2
-__global__ void __launch_bounds__(256) k_residual64(const uint8_t *mask, const double *b,
-                                                    const double *u, int nx, int nz, double *r,
-                                                    float *d, double *partial)
-{
-    __shared__ double s1[256], s2[256];
-    size_t n = (size_t)nx * nz;
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double rr = 0, bb = 0;
-    if (k < n) node_residual64(mask, b, u, nz, r, d, k, rr, bb);
-    s1[threadIdx.x] = rr;
-    s2[threadIdx.x] = bb;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
-}
-
 // u += e on free nodes
 __global__ void k_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
 {
@@ -152,6 +131,11 @@ __global__ void __launch_bounds__(1024) k_sum_pairs(const double *partial, int n
 }
 
 // ------------------------------------------------------------------ FP32 MG kernels
+// smoothing sweeps before and after the coarse correction, and the damping
+// of the Jacobi sweeps on the Galerkin levels
+constexpr int kPre = 2, kPost = 2;
+constexpr float kOmega = 0.8f;
+
 
 // 9-point operator y = A x (Dirichlet rows give 0)
 __device__ __forceinline__ float apply9(const float *st, const float *x, int i, int j, int nx, int nz)
@@ -210,24 +194,6 @@ __global__ void k_rbgs5(const uint8_t *dir, float *x, const float *b, int nx, in
 __device__ __forceinline__ float node_resid5(const uint8_t *dir, const float *x, const float *b, int nz, size_t k)
 {
     return dir[k] ? 0.f : b[k] - apply5(dir, x, k, nz);
-}
-
-__global__ void k_resid5(const uint8_t *dir, const float *x, const float *b, float *r, int nx, int nz)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
-    if (j >= nz) return;
-    size_t k = (size_t)i * nz + j;
-    r[k] = node_resid5(dir, x, b, nz, k);
-}
-
-__global__ void k_apply5(const uint8_t *dir, const float *x, float *y, int nx, int nz)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
-    if (j >= nz) return;
-    size_t k = (size_t)i * nz + j;
-    y[k] = dir[k] ? 0.f : apply5(dir, x, k, nz);
 }
 
 // weighted Jacobi on a 9-point level: xo = xi + w D^-1 (b - A xi)
@@ -394,6 +360,535 @@ __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const d
 }
 
 __global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
+
+// ------------------------------------------------------------------ fused solver kernels
+// The solve is launch-bound on small grids, so its kernels carry their own
+// reductions and fuse neighbouring sweeps.  finish_sums: each block sums its
+// values in a fixed order and writes a partial; the block that takes the last
+// ticket sums the partials in block order, writes out[] and resets the
+// ticket, so a reduction costs no extra launch and gives the same bits on
+// every run.
+constexpr int kRedBlock = 256;
+
+template <int NV>
+__device__ __forceinline__ void finish_sums(double (&v)[NV], double *partial, unsigned int *ticket, double *out)
+{
+    __shared__ double sh[NV][kRedBlock / 64];
+    __shared__ bool last;
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o);
+        if ((threadIdx.x & 63) == 0) sh[q][w] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double t = 0;
+            for (int k = 0; k < kRedBlock / 64; k++) t += sh[q][k];
+            partial[NV * blockIdx.x + q] = t;
+        }
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;  // uniform over the block
+    __threadfence();
+    double t[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) t[q] = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kRedBlock)
+#pragma unroll
+        for (int q = 0; q < NV; q++) t[q] += partial[NV * b + q];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) t[q] += __shfl_down(t[q], o);
+        if ((threadIdx.x & 63) == 0) sh[q][w] = t[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double s = 0;
+            for (int k = 0; k < kRedBlock / 64; k++) s += sh[q][k];
+            out[q] = s;
+        }
+        *ticket = 0u;
+    }
+}
+
+// out = a.b (FP64 accumulation)
+__global__ void __launch_bounds__(kRedBlock) k_dot_fin(const float *a, const float *b, size_t n, double *partial,
+                                                      unsigned int *ticket, double *out)
+{
+    double v[1] = {0};
+    for (size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (size_t)gridDim.x * kRedBlock)
+        v[0] += (double)a[k] * b[k];
+    finish_sums<1>(v, partial, ticket, out);
+}
+
+// q = A' p on the finest level, out = p.q
+__global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, const float *p, float *q, size_t n,
+                                                         int nz, double *partial, unsigned int *ticket, double *out)
+{
+    double v[1] = {0};
+    const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    if (k < n) {
+        const float y = dir[k] ? 0.f : apply5(dir, p, k, nz);
+        q[k] = y;
+        v[0] = (double)p[k] * y;
+    }
+    finish_sums<1>(v, partial, ticket, out);
+}
+
+// e += alpha p, r -= alpha q with alpha = rz / pq; out = r.r
+// (first: e starts at 0, so e = alpha p without reading e)
+__global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r, const float *p, const float *q,
+                                                            const double *rz, const double *pq, size_t n, int first,
+                                                            double *partial, unsigned int *ticket, double *out)
+{
+    double v[1] = {0};
+    const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    if (k < n) {
+        const double den = *pq;
+        const float alpha = den != 0.0 ? (float)(*rz / den) : 0.f;
+        e[k] = fmaf(alpha, p[k], first ? 0.f : e[k]);
+        const float rk = fmaf(-alpha, q[k], r[k]);
+        r[k] = rk;
+        v[0] = (double)rk * rk;
+    }
+    finish_sums<1>(v, partial, ticket, out);
+}
+
+// the FP64 residual with its norms finished in the same launch: out[0] =
+// |r|^2, out[1] = |b_eff|^2, out[2] = |d|^2 of the FP32 copy d = -r
+__global__ void __launch_bounds__(kRedBlock) k_residual64_fin(const uint8_t *mask, const double *b, const double *u,
+                                                             size_t n, int nz, double *r, float *d, double *partial,
+                                                             unsigned int *ticket, double *out)
+{
+    double v[3] = {0, 0, 0};
+    const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    if (k < n) {
+        node_residual64(mask, b, u, nz, r, d, k, v[0], v[1]);
+        v[2] = (double)d[k] * d[k];
+    }
+    finish_sums<3>(v, partial, ticket, out);
+}
+
+// prolongation value at fine node (i, j) (the increment of node_prolong)
+__device__ __forceinline__ float prolong_at(const float *xc, int nxc, int nzc, int i, int j)
+{
+    int I0 = i >> 1, J0 = j >> 1;
+    float s = 0.f;
+    if ((i & 1) == 0) {
+        if ((j & 1) == 0) s = xc[(size_t)I0 * nzc + J0];
+        else {
+            s = 0.5f * xc[(size_t)I0 * nzc + J0];
+            if (J0 + 1 < nzc) s += 0.5f * xc[(size_t)I0 * nzc + J0 + 1];
+        }
+    } else {
+        float a = 0.f, c = 0.f;
+        if ((j & 1) == 0) {
+            a = xc[(size_t)I0 * nzc + J0];
+            if (I0 + 1 < nxc) c = xc[(size_t)(I0 + 1) * nzc + J0];
+        } else {
+            a = 0.5f * xc[(size_t)I0 * nzc + J0];
+            if (J0 + 1 < nzc) a += 0.5f * xc[(size_t)I0 * nzc + J0 + 1];
+            if (I0 + 1 < nxc) {
+                c = 0.5f * xc[(size_t)(I0 + 1) * nzc + J0];
+                if (J0 + 1 < nzc) c += 0.5f * xc[(size_t)(I0 + 1) * nzc + J0 + 1];
+            }
+        }
+        s = 0.5f * (a + c);
+    }
+    return s;
+}
+
+// ---- the finest level in LDS tiles.  A block holds a tile of kT0 x kT0
+// nodes (32, or 16 on grids of at most 256^2 nodes, for more blocks) plus a halo and runs several red-black half sweeps on it with
+// workgroup barriers in between.  A node at distance d from the halo's edge
+// keeps the exact global value for d half sweeps after the first, so the
+// halo is as wide as the sweeps (and the residual stencil) need.  Updates
+// use the node_rbgs5 order of additions, so the result is the same as the
+// half sweeps run one launch each.
+constexpr int kH0d = 2 * kPre + 1;   // descent: 2 kPre - 1 sweeps, residual, restriction
+constexpr int kH0u = 2 * kPost;      // ascent: 2 kPost sweeps
+constexpr int kT0Threads = 256;
+
+// one red-black half sweep over the held region but its outermost ring
+template <int S>
+__device__ __forceinline__ void tile_rbgs(float *sx, const float *sb, const uint8_t *sd, int i0, int j0, int colour)
+{
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int li = t / S, lj = t % S;
+        if (li == 0 || li == S - 1 || lj == 0 || lj == S - 1) continue;
+        if (((i0 + li + j0 + lj) & 1) != colour || sd[t]) continue;
+        float s = sb[t];
+        if (!sd[t - S]) s += sx[t - S];
+        if (!sd[t - 1]) s += sx[t - 1];
+        if (!sd[t + 1]) s += sx[t + 1];
+        if (!sd[t + S]) s += sx[t + S];
+        sx[t] = 0.25f * s;
+    }
+}
+
+// descent on the finest level: pre-smoothing from x = 0, the residual and its
+// restriction to the coarse nodes whose fine node (2I, 2J) lies in the tile;
+// writes the smoothed x of the tile and those coarse right-hand sides
+template <int kT0>
+__global__ void __launch_bounds__(kT0Threads) k_l0_down(const uint8_t *dir, const float *b, float *x, int nx, int nz,
+                                                      float *bc, const float *dinvc, int nxc, int nzc)
+{
+    constexpr int S = kT0 + 2 * kH0d;
+    __shared__ float sx[S * S], sb[S * S], sr[S * S];
+    __shared__ uint8_t sd[S * S];
+    const int gi0 = blockIdx.y * kT0, gj0 = blockIdx.x * kT0;
+    const int i0 = gi0 - kH0d, j0 = gj0 - kH0d;
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int i = i0 + t / S, j = j0 + t % S;
+        const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
+        const size_t k = (size_t)i * nz + j;
+        const uint8_t d = in ? dir[k] : (uint8_t)1;
+        const float bv = in ? b[k] : 0.f;
+        sd[t] = d;
+        sb[t] = bv;
+        // the first red half sweep from zero reads no neighbour
+        sx[t] = (((i + j) & 1) == 0 && !d) ? 0.25f * (bv + 0.f) : 0.f;
+    }
+    __syncthreads();
+    for (int h = 1; h < 2 * kPre; h++) {
+        tile_rbgs<S>(sx, sb, sd, i0, j0, h & 1);
+        __syncthreads();
+    }
+    // residual (node_resid5) where it reads only exact x: the 2 kPre - 1
+    // sweeps after the first left the outer 2 kPre - 1 rings inexact
+    constexpr int R = 2 * kPre;
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int li = t / S, lj = t % S;
+        if (li < R || li >= S - R || lj < R || lj >= S - R) continue;
+        float r = 0.f;
+        if (!sd[t]) {
+            float s = 4.f * sx[t];
+            if (!sd[t - S]) s -= sx[t - S];
+            if (!sd[t - 1]) s -= sx[t - 1];
+            if (!sd[t + 1]) s -= sx[t + 1];
+            if (!sd[t + S]) s -= sx[t + S];
+            r = sb[t] - s;
+        }
+        sr[t] = r;
+    }
+    __syncthreads();
+    // restriction (node_restrict) for the coarse nodes of this tile
+    for (int t = threadIdx.x; t < (kT0 / 2) * (kT0 / 2); t += kT0Threads) {
+        const int I = gi0 / 2 + t / (kT0 / 2), J = gj0 / 2 + t % (kT0 / 2);
+        if (I >= nxc || J >= nzc) continue;
+        const size_t K = (size_t)I * nzc + J;
+        if (dinvc[K] == 0.f) { bc[K] = 0.f; continue; }
+        float s = 0.f;
+        for (int di = -1; di <= 1; di++) {
+            const int i = 2 * I + di;
+            if (i < 0 || i >= nx) continue;
+            const float wi = di == 0 ? 1.f : 0.5f;
+            for (int dj = -1; dj <= 1; dj++) {
+                const int j = 2 * J + dj;
+                if (j < 0 || j >= nz) continue;
+                const float wj = dj == 0 ? 1.f : 0.5f;
+                s = fmaf(wi * wj, sr[(i - i0) * S + (j - j0)], s);
+            }
+        }
+        bc[K] = s;
+    }
+    for (int t = threadIdx.x; t < kT0 * kT0; t += kT0Threads) {
+        const int i = gi0 + t / kT0, j = gj0 + t % kT0;
+        if (i < nx && j < nz) x[(size_t)i * nz + j] = sx[(i - i0) * S + (j - j0)];
+    }
+}
+
+// ascent on the finest level: x += P x_c, then the post-smoothing sweeps
+template <int kT0>
+__global__ void __launch_bounds__(kT0Threads) k_l0_up(const uint8_t *dir, const float *b, float *x, int nx, int nz,
+                                                    const float *xc, int nxc, int nzc)
+{
+    constexpr int S = kT0 + 2 * kH0u;
+    __shared__ float sx[S * S], sb[S * S];
+    __shared__ uint8_t sd[S * S];
+    const int gi0 = blockIdx.y * kT0, gj0 = blockIdx.x * kT0;
+    const int i0 = gi0 - kH0u, j0 = gj0 - kH0u;
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int i = i0 + t / S, j = j0 + t % S;
+        const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
+        const size_t k = (size_t)i * nz + j;
+        const uint8_t d = in ? dir[k] : (uint8_t)1;
+        float xv = in ? x[k] : 0.f;
+        if (!d) xv += prolong_at(xc, nxc, nzc, i, j);  // node_prolong
+        sd[t] = d;
+        sb[t] = in ? b[k] : 0.f;
+        sx[t] = xv;
+    }
+    __syncthreads();
+    for (int h = 0; h < 2 * kPost; h++) {
+        tile_rbgs<S>(sx, sb, sd, i0, j0, (h & 1) ^ 1);
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < kT0 * kT0; t += kT0Threads) {
+        const int i = gi0 + t / kT0, j = gj0 + t % kT0;
+        if (i < nx && j < nz) x[(size_t)i * nz + j] = sx[(i - i0) * S + (j - j0)];
+    }
+}
+
+// ---- the Galerkin levels in LDS tiles (damped Jacobi, 9-point operator).
+// The first sweep starts from x = 0 and reads no neighbour; each later sweep,
+// the residual and the restriction widen the inexact border by one ring.
+constexpr int kTm = 16;
+constexpr int kHmd = kPre + 1;  // descent: kPre - 1 sweeps, residual, restriction
+constexpr int kHmu = kPost;     // ascent: kPost sweeps
+
+// apply9 on tile-local arrays (same loop order and skips as apply9)
+template <int S>
+__device__ __forceinline__ float tile_apply9(const float *sst, const float *sx, int t, int i, int j, int nx, int nz)
+{
+    const float *c = sst + 9 * t;
+    float s = 0.f;
+#pragma unroll
+    for (int di = -1; di <= 1; di++) {
+        const int ii = i + di;
+        if (ii < 0 || ii >= nx) continue;
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++) {
+            const int jj = j + dj;
+            if (jj < 0 || jj >= nz) continue;
+            const float a = c[(di + 1) * 3 + (dj + 1)];
+            if (a != 0.f) s = fmaf(a, sx[t + di * S + dj], s);
+        }
+    }
+    return s;
+}
+
+// one Jacobi sweep (node_jacobi9) over the held region but its outer ring
+template <int S>
+__device__ __forceinline__ void tile_jacobi(const float *sst, const float *sdv, const float *sb, const float *xi,
+                                            float *xo, int i0, int j0, int nx, int nz, float w)
+{
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int li = t / S, lj = t % S;
+        if (li == 0 || li == S - 1 || lj == 0 || lj == S - 1) continue;
+        const float dv = sdv[t];
+        xo[t] = dv == 0.f ? 0.f : xi[t] + w * dv * (sb[t] - tile_apply9<S>(sst, xi, t, i0 + li, j0 + lj, nx, nz));
+    }
+}
+
+// load the coefficients, inverse diagonal and right-hand side of a tile
+// (zero outside the level: such nodes act as pinned)
+template <int S>
+__device__ __forceinline__ void tile_load9(const float *st, const float *dinv, const float *b, int i0, int j0, int nx,
+                                           int nz, float *sst, float *sdv, float *sb)
+{
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int i = i0 + t / S, j = j0 + t % S;
+        const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
+        const size_t k = (size_t)i * nz + j;
+        sdv[t] = in ? dinv[k] : 0.f;
+        sb[t] = in ? b[k] : 0.f;
+    }
+    for (int t = threadIdx.x; t < 9 * S * S; t += kT0Threads) {
+        const int node = t / 9, q = t % 9;
+        const int i = i0 + node / S, j = j0 + node % S;
+        const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
+        sst[t] = in ? st[9 * ((size_t)i * nz + j) + q] : 0.f;
+    }
+}
+
+// descent on a Galerkin level: smoothing from x = 0, residual, restriction
+// to the next level's right-hand side; writes the smoothed x of the tile
+__global__ void __launch_bounds__(kT0Threads) k_lm_down(const float *st, const float *dinv, const float *b, float *x,
+                                                      int nx, int nz, float *bc, const float *dinvc, int nxc, int nzc,
+                                                      float w)
+{
+    constexpr int S = kTm + 2 * kHmd;
+    __shared__ float sst[9 * S * S], sdv[S * S], sb[S * S], sx[S * S], sx2[S * S];
+    const int gi0 = blockIdx.y * kTm, gj0 = blockIdx.x * kTm;
+    const int i0 = gi0 - kHmd, j0 = gj0 - kHmd;
+    tile_load9<S>(st, dinv, b, i0, j0, nx, nz, sst, sdv, sb);
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) sx[t] = 0.f;
+    __syncthreads();
+    float *xa = sx, *xb = sx2;
+    for (int it = 0; it < kPre; it++) {
+        // the first sweep starts from x = +0, where the operator gives +0
+        // (every term of apply9 is then a signed zero added to +0): it reads
+        // no neighbour and is exact on the outer ring too
+        if (it == 0) {
+            for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+                const float dv = sdv[t];
+                xb[t] = dv == 0.f ? 0.f : xa[t] + w * dv * (sb[t] - 0.f);
+            }
+        } else {
+            tile_jacobi<S>(sst, sdv, sb, xa, xb, i0, j0, nx, nz, w);
+        }
+        __syncthreads();
+        float *tmp = xa; xa = xb; xb = tmp;
+    }
+    // residual (node_resid9) into xb where xa is exact
+    constexpr int R = kPre;
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int li = t / S, lj = t % S;
+        if (li < R || li >= S - R || lj < R || lj >= S - R) continue;
+        xb[t] = sdv[t] == 0.f ? 0.f : sb[t] - tile_apply9<S>(sst, xa, t, i0 + li, j0 + lj, nx, nz);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < (kTm / 2) * (kTm / 2); t += kT0Threads) {
+        const int I = gi0 / 2 + t / (kTm / 2), J = gj0 / 2 + t % (kTm / 2);
+        if (I >= nxc || J >= nzc) continue;
+        const size_t K = (size_t)I * nzc + J;
+        if (dinvc[K] == 0.f) { bc[K] = 0.f; continue; }
+        float s = 0.f;
+        for (int di = -1; di <= 1; di++) {
+            const int i = 2 * I + di;
+            if (i < 0 || i >= nx) continue;
+            const float wi = di == 0 ? 1.f : 0.5f;
+            for (int dj = -1; dj <= 1; dj++) {
+                const int j = 2 * J + dj;
+                if (j < 0 || j >= nz) continue;
+                const float wj = dj == 0 ? 1.f : 0.5f;
+                s = fmaf(wi * wj, xb[(i - i0) * S + (j - j0)], s);
+            }
+        }
+        bc[K] = s;
+    }
+    for (int t = threadIdx.x; t < kTm * kTm; t += kT0Threads) {
+        const int i = gi0 + t / kTm, j = gj0 + t % kTm;
+        if (i < nx && j < nz) x[(size_t)i * nz + j] = xa[(i - i0) * S + (j - j0)];
+    }
+}
+
+// ascent on a Galerkin level: x += P x_c on free nodes, then the sweeps
+__global__ void __launch_bounds__(kT0Threads) k_lm_up(const float *st, const float *dinv, const uint8_t *dir,
+                                                    const float *b, float *x, int nx, int nz, const float *xc,
+                                                    int nxc, int nzc, float w)
+{
+    constexpr int S = kTm + 2 * kHmu;
+    __shared__ float sst[9 * S * S], sdv[S * S], sb[S * S], sx[S * S], sx2[S * S];
+    const int gi0 = blockIdx.y * kTm, gj0 = blockIdx.x * kTm;
+    const int i0 = gi0 - kHmu, j0 = gj0 - kHmu;
+    tile_load9<S>(st, dinv, b, i0, j0, nx, nz, sst, sdv, sb);
+    for (int t = threadIdx.x; t < S * S; t += kT0Threads) {
+        const int i = i0 + t / S, j = j0 + t % S;
+        const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
+        const size_t k = (size_t)i * nz + j;
+        float xv = in ? x[k] : 0.f;
+        if (in && !dir[k]) xv += prolong_at(xc, nxc, nzc, i, j);  // node_prolong
+        sx[t] = xv;
+    }
+    __syncthreads();
+    float *xa = sx, *xb = sx2;
+    for (int it = 0; it < kPost; it++) {
+        tile_jacobi<S>(sst, sdv, sb, xa, xb, i0, j0, nx, nz, w);
+        __syncthreads();
+        float *tmp = xa; xa = xb; xb = tmp;
+    }
+    for (int t = threadIdx.x; t < kTm * kTm; t += kT0Threads) {
+        const int i = gi0 + t / kTm, j = gj0 + t % kTm;
+        if (i < nx && j < nz) x[(size_t)i * nz + j] = xa[(i - i0) * S + (j - j0)];
+    }
+}
+
+// the V-cycle below level lc in one workgroup: descent from b on level lc,
+// dense solve, ascent to x on level lc
+constexpr int kCoarseThreads = 1024;
+constexpr int kDenseParts = 4;  // threads per row of the dense coarsest solve
+constexpr int kMaxLevels = 16;
+constexpr size_t kBlockLevelNodes = 1024;  // levels this small run inside the workgroup
+
+struct BLevel {
+    int nx, nz, n;
+    const float *st, *dinv;
+    const uint8_t *dir;
+    float *x, *x2, *r, *b;
+};
+
+struct CoarseArgs {
+    int lc, L;
+    BLevel lv[kMaxLevels];
+
+    const float *invT;  // the dense inverse, transposed (row r of A^-1 is column r)
+    int coarse_n;
+    float omega;
+    int pre, post;
+};
+
+__device__ void c_jacobi(const BLevel &A, float *x, int sweeps, float w)
+{
+    float *xa = x, *xb = A.x2;
+    for (int it = 0; it < sweeps; it++) {
+        for (int k = threadIdx.x; k < A.n; k += kCoarseThreads)
+            xb[k] = node_jacobi9(A.st, A.dinv, xa, A.b, A.nx, A.nz, w, k / A.nz, k % A.nz);
+        __syncthreads();
+        float *t = xa; xa = xb; xb = t;
+    }
+    if (xa != x) {
+        for (int k = threadIdx.x; k < A.n; k += kCoarseThreads) x[k] = xa[k];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse(CoarseArgs C)
+{
+    extern __shared__ float sb[];  // coarse_n right-hand side words, then kDenseParts partial rows
+    for (int l = C.lc; l < C.L - 1; l++) {
+        const BLevel &A = C.lv[l];
+        for (int k = threadIdx.x; k < A.n; k += kCoarseThreads) A.x[k] = 0.f;
+        __syncthreads();
+        c_jacobi(A, A.x, C.pre, C.omega);
+        for (int k = threadIdx.x; k < A.n; k += kCoarseThreads)
+            A.r[k] = node_resid9(A.st, A.dinv, A.x, A.b, A.nx, A.nz, k / A.nz, k % A.nz);
+        __syncthreads();
+        const BLevel &Cn = C.lv[l + 1];
+        for (int k = threadIdx.x; k < Cn.n; k += kCoarseThreads)
+            Cn.b[k] = node_restrict(A.r, A.nx, A.nz, Cn.dinv, Cn.nz, k / Cn.nz, k % Cn.nz);
+        __syncthreads();
+    }
+    {
+        // dense solve x = A^-1 b: kDenseParts threads per row, each summing a
+        // contiguous quarter of the columns (8 loads in flight), the quarters
+        // added in order
+        const BLevel &Z = C.lv[C.L - 1];
+        const int n = C.coarse_n;
+        float *part = sb + n;
+        for (int k = threadIdx.x; k < n; k += kCoarseThreads) sb[k] = Z.b[k];
+        __syncthreads();
+        const int len = (n + kDenseParts - 1) / kDenseParts;
+        for (int t = threadIdx.x; t < n * kDenseParts; t += kCoarseThreads) {
+            const int row = t % n, p = t / n;
+            const int c0 = p * len, c1 = min(n, c0 + len);
+            float s = 0.f;
+            int c = c0;
+            for (; c + 8 <= c1; c += 8) {
+                float a[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = C.invT[(size_t)(c + q) * n + row];
+#pragma unroll
+                for (int q = 0; q < 8; q++) s = fmaf(a[q], sb[c + q], s);
+            }
+            for (; c < c1; c++) s = fmaf(C.invT[(size_t)c * n + row], sb[c], s);
+            part[p * n + row] = s;
+        }
+        __syncthreads();
+        for (int row = threadIdx.x; row < n; row += kCoarseThreads) {
+            float s = part[row];
+            for (int p = 1; p < kDenseParts; p++) s += part[p * n + row];
+            Z.x[row] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = C.L - 2; l >= C.lc; l--) {
+        const BLevel &A = C.lv[l];
+        const BLevel &Cn = C.lv[l + 1];
+        for (int k = threadIdx.x; k < A.n; k += kCoarseThreads)
+            node_prolong(A.x, A.dir, A.nz, Cn.x, Cn.nx, Cn.nz, k / A.nz, k % A.nz);
+        __syncthreads();
+        c_jacobi(A, A.x, C.post, C.omega);
+    }
+}
 
 // ------------------------------------------------------------------ host: hierarchy
 static void p1d(int i, int nc, int I[2], float w[2], int &cnt)
@@ -581,6 +1076,14 @@ static int mg_setup(m2d_fields2d *f)
     mg.coarse_n = (int)C.n;
     M2D_TRY(dmalloc((void **)&mg.coarse_inv, sizeof(float) * inv.size()));
     M2D_HIP(hipMemcpy(mg.coarse_inv, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice));
+    {
+        const size_t cn = C.n;
+        std::vector<float> invT(cn * cn);
+        for (size_t r = 0; r < cn; r++)
+            for (size_t c = 0; c < cn; c++) invT[c * cn + r] = inv[r * cn + c];
+        M2D_TRY(dmalloc((void **)&mg.coarse_invT, sizeof(float) * invT.size()));
+        M2D_HIP(hipMemcpy(mg.coarse_invT, invT.data(), sizeof(float) * invT.size(), hipMemcpyHostToDevice));
+    }
     const size_t n = f->n;
     M2D_TRY(dmalloc((void **)&mg.pr, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.pz, sizeof(float) * n));
@@ -589,9 +1092,19 @@ static int mg_setup(m2d_fields2d *f)
     M2D_TRY(dmalloc((void **)&mg.pe, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.r64, sizeof(double) * n));
     M2D_TRY(dmalloc((void **)&mg.dots, sizeof(double) * 16));
+    M2D_TRY(dmalloc((void **)&mg.ticket, sizeof(unsigned int)));
+    M2D_HIP(hipMemset(mg.ticket, 0, sizeof(unsigned int)));
+    // the coarse part of the V-cycle runs in one workgroup from the first
+    // level of at most kBlockLevelNodes nodes (the dense level at the latest)
+    mg.lc = (int)mg.lv.size() - 1;
+    for (int l = 1; l < (int)mg.lv.size(); l++)
+        if (mg.lv[l].n <= kBlockLevelNodes) { mg.lc = l; break; }
+    M2D_CHECK(mg.lv.size() <= (size_t)kMaxLevels && mg.coarse_n <= 3200, M2D_ERR_BAD_ARG,
+              "Fields: multigrid hierarchy of %d levels, coarsest %d nodes", (int)mg.lv.size(), mg.coarse_n);
     M2D_HIP(hipHostMalloc((void **)&mg.h_dots, sizeof(double) * 16));
     mg.npartial = (int)std::min<size_t>((n + 255) / 256, 65536);
-    M2D_TRY(dmalloc((void **)&mg.partial, sizeof(double) * 2 * std::max<size_t>(mg.npartial, (n + 255) / 256)));
+    // up to three sums per block of the fused reductions
+    M2D_TRY(dmalloc((void **)&mg.partial, sizeof(double) * 3 * std::max<size_t>(mg.npartial, (n + 255) / 256)));
     return M2D_OK;
 }
 
@@ -603,91 +1116,79 @@ static void mg_free(Mg &mg)
             if (p) (void)hipFree(p);
     mg.lv.clear();
     for (void *p : {(void *)mg.coarse_inv, (void *)mg.pr, (void *)mg.pz, (void *)mg.pp, (void *)mg.pq,
-                    (void *)mg.pe, (void *)mg.r64, (void *)mg.dots, (void *)mg.partial})
+                    (void *)mg.pe, (void *)mg.r64, (void *)mg.dots, (void *)mg.partial, (void *)mg.ticket, (void *)mg.coarse_invT})
         if (p) (void)hipFree(p);
     if (mg.h_dots) (void)hipHostFree(mg.h_dots);
     mg = Mg();
 }
 
 // ------------------------------------------------------------------ host: V-cycle
-// smoothing sweeps before and after the coarse correction, and the damping
-// of the Jacobi sweeps on the Galerkin levels
-constexpr int kPre = 2, kPost = 2;
-constexpr float kOmega = 0.8f;
 
+// one V-cycle: the finest level in grid kernels (first sweep from zero,
+// fused residual + restriction, fused prolongation + black sweep), middle
+// levels in grid kernels, and every level from mg.lc down in one workgroup
 static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
 {
     Mg &mg = f->mg;
     hipStream_t s = f->ctx->stream;
     const int L = (int)mg.lv.size();
-    // level 0 works on (out, rhs); coarse levels on (x, b)
-    for (int l = 0; l < L - 1; l++) {
-        MgLevel &A = mg.lv[l];
-        float *x = l == 0 ? out : A.x;
-        const float *b = l == 0 ? rhs : A.b;
-        dim3 g((A.nz + 255) / 256, A.nx);
-        M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * A.n));
-        if (l == 0) {
-            for (int it = 0; it < kPre; it++) {
-                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 0);
-                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 1);
-            }
-            k_resid5<<<g, 256, 0, s>>>(A.dir, x, b, A.r, A.nx, A.nz);
-        } else {
-            float *xa = x, *xb = A.x2;
-            for (int it = 0; it < kPre; it++) {
-                k_jacobi9<<<g, 256, 0, s>>>(A.st, A.dinv, xa, xb, b, A.nx, A.nz, kOmega);
-                std::swap(xa, xb);
-            }
-            if (xa != x) M2D_HIP(hipMemcpyAsync(x, xa, sizeof(float) * A.n, hipMemcpyDeviceToDevice, s));
-            k_resid9<<<g, 256, 0, s>>>(A.st, A.dinv, x, b, A.r, A.nx, A.nz);
-        }
-        MgLevel &C = mg.lv[l + 1];
-        k_restrict<<<dim3((C.nz + 255) / 256, C.nx), 256, 0, s>>>(A.r, A.nx, A.nz, C.b, C.dinv, C.nx, C.nz);
+    MgLevel &A0 = mg.lv[0];
+    if (L == 1) {
+        k_coarse_solve<<<1, 256, sizeof(float) * mg.coarse_n, s>>>(mg.coarse_inv, rhs, out, mg.coarse_n);
+        M2D_HIP(hipGetLastError());
+        return M2D_OK;
     }
-    MgLevel &C = mg.lv[L - 1];
+    // ---- level 0 descent (LDS tiles: smoothing, residual, restriction)
+    const bool small = A0.n <= 65536;
+    const int T0 = small ? 16 : 32;
+    const dim3 gt((A0.nz + T0 - 1) / T0, (A0.nx + T0 - 1) / T0);
     {
-        float *x = L == 1 ? out : C.x;
-        const float *b = L == 1 ? rhs : C.b;
-        k_coarse_solve<<<1, 256, sizeof(float) * mg.coarse_n, s>>>(mg.coarse_inv, b, x, mg.coarse_n);
+        MgLevel &C = mg.lv[1];
+        if (small) k_l0_down<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz);
+        else k_l0_down<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz);
     }
-    for (int l = L - 2; l >= 0; l--) {
+    // ---- middle levels [1, lc) in LDS tiles
+    for (int l = 1; l < mg.lc; l++) {
+        MgLevel &A = mg.lv[l];
+        MgLevel &C = mg.lv[l + 1];
+        const dim3 g((A.nz + kTm - 1) / kTm, (A.nx + kTm - 1) / kTm);
+        k_lm_down<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.b, A.x, A.nx, A.nz, C.b, C.dinv, C.nx, C.nz, kOmega);
+    }
+    // ---- levels [lc, L) in one workgroup
+    {
+        CoarseArgs C;
+        C.lc = mg.lc;
+        C.L = L;
+        for (int l = 0; l < L; l++) {
+            const MgLevel &A = mg.lv[l];
+            C.lv[l] = BLevel{A.nx, A.nz, (int)A.n, A.st, A.dinv, A.dir, A.x, A.x2, A.r, A.b};
+        }
+        C.invT = mg.coarse_invT;
+        C.coarse_n = mg.coarse_n;
+        C.omega = kOmega;
+        C.pre = kPre;
+        C.post = kPost;
+        k_vcycle_coarse<<<1, kCoarseThreads, sizeof(float) * mg.coarse_n * (1 + kDenseParts), s>>>(C);
+    }
+    // ---- middle levels ascent
+    for (int l = mg.lc - 1; l >= 1; l--) {
         MgLevel &A = mg.lv[l];
         MgLevel &Cc = mg.lv[l + 1];
-        float *x = l == 0 ? out : A.x;
-        const float *b = l == 0 ? rhs : A.b;
-        dim3 g((A.nz + 255) / 256, A.nx);
-        k_prolong<<<g, 256, 0, s>>>(x, A.dir, A.nx, A.nz, Cc.x, Cc.nx, Cc.nz);
-        if (l == 0) {
-            for (int it = 0; it < kPost; it++) {
-                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 1);
-                k_rbgs5<<<g, 256, 0, s>>>(A.dir, x, b, A.nx, A.nz, 0);
-            }
-        } else {
-            float *xa = x, *xb = A.x2;
-            for (int it = 0; it < kPost; it++) {
-                k_jacobi9<<<g, 256, 0, s>>>(A.st, A.dinv, xa, xb, b, A.nx, A.nz, kOmega);
-                std::swap(xa, xb);
-            }
-            if (xa != x) M2D_HIP(hipMemcpyAsync(x, xa, sizeof(float) * A.n, hipMemcpyDeviceToDevice, s));
-        }
+        const dim3 g((A.nz + kTm - 1) / kTm, (A.nx + kTm - 1) / kTm);
+        k_lm_up<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.dir, A.b, A.x, A.nx, A.nz, Cc.x, Cc.nx, Cc.nz, kOmega);
+    }
+    // ---- level 0 ascent (LDS tiles: prolongation, smoothing)
+    {
+        MgLevel &C = mg.lv[1];
+        if (small) k_l0_up<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz);
+        else k_l0_up<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz);
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
 }
 
-// dots[slot] (and dots[slot+1]) <- a.b (and c.d)
-static int dot2(m2d_fields2d *f, const float *a, const float *b, const float *c, const float *d,
-                int slot)
-{
-    Mg &mg = f->mg;
-    hipStream_t s = f->ctx->stream;
-    int nb = std::min(mg.npartial, 1024);
-    k_dot2<<<nb, 256, 0, s>>>(a, b, c, d, f->n, mg.partial);
-    k_sum_pairs<<<1, 1024, 0, s>>>(mg.partial, nb, mg.dots + slot);
-    M2D_HIP(hipGetLastError());
-    return M2D_OK;
-}
+// fused-reduction launch shape over the finest level
+static unsigned red_blocks(size_t n) { return (unsigned)((n + kRedBlock - 1) / kRedBlock); }
 
 static int read_dots(m2d_fields2d *f)
 {
@@ -697,39 +1198,35 @@ static int read_dots(m2d_fields2d *f)
     return M2D_OK;
 }
 
-// FP32 PCG on A' e = d (d in mg.pr on entry), e in mg.pe; stops at
-// ||r|| <= rtol ||d|| or max_it.  Returns iterations.
-static int pcg(m2d_fields2d *f, double rtol, int max_it, int *iters)
+// FP32 PCG on A' e = d (d in mg.pr on entry, |d| = r0), e in mg.pe; stops
+// at ||r|| <= rtol r0 or max_it.  One host read per iteration (the
+// convergence test); the dot products are finished on the device.
+// dots: [0], [4] r.z of alternate iterations, [1] p.q, [2] r.r
+static int pcg(m2d_fields2d *f, double rtol, int max_it, double r0, int *iters)
 {
     Mg &mg = f->mg;
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
-    const unsigned gb = (unsigned)((n + 255) / 256);
+    const unsigned gb = (unsigned)((n + 255) / 256), rb = red_blocks(n);
     const MgLevel &L0 = mg.lv[0];
-    dim3 g2((L0.nz + 255) / 256, L0.nx);
-    M2D_TRY(fill_bytes(f->ctx, mg.pe, 0, sizeof(float) * n));
-    // dots layout: [0] rz, [1] pq, [2] rr, [3] unused, [4] rz_old
-    M2D_TRY(dot2(f, mg.pr, mg.pr, nullptr, nullptr, 2));
-    M2D_TRY(read_dots(f));
-    double r0 = sqrt(mg.h_dots[2]);
     *iters = 0;
-    if (r0 == 0.0) return M2D_OK;
+    if (r0 == 0.0) return fill_bytes(f->ctx, mg.pe, 0, sizeof(float) * n);  // e = 0
     M2D_TRY(vcycle(f, mg.pr, mg.pz));
-    M2D_TRY(dot2(f, mg.pr, mg.pz, nullptr, nullptr, 0));
+    int cur = 0;
+    k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots);
     k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1);
     for (int it = 1; it <= max_it; it++) {
-        k_apply5<<<g2, 256, 0, s>>>(L0.dir, mg.pp, mg.pq, L0.nx, L0.nz);
-        // dots[0] = rz (kept), dots[1] = p.q
-        M2D_TRY(dot2(f, mg.pp, mg.pq, nullptr, nullptr, 1));
-        k_cg_update<<<gb, 256, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots, n);
-        M2D_TRY(dot2(f, mg.pr, mg.pr, nullptr, nullptr, 2));
+        k_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pp, mg.pq, n, L0.nz, mg.partial, mg.ticket, mg.dots + 1);
+        k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
+                                                 it == 1, mg.partial, mg.ticket, mg.dots + 2);
+        M2D_HIP(hipGetLastError());
         *iters = it;
         M2D_TRY(read_dots(f));
         if (sqrt(mg.h_dots[2]) <= rtol * r0) break;
         M2D_TRY(vcycle(f, mg.pr, mg.pz));
-        k_copy_scalar<<<1, 1, 0, s>>>(mg.dots, mg.dots + 4);
-        M2D_TRY(dot2(f, mg.pr, mg.pz, nullptr, nullptr, 0));
-        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 0);
+        k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1));
+        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0);
+        cur ^= 1;
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -750,8 +1247,8 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
     double rel = 0;
     const int max_outer = 12;
     for (;;) {
-        k_residual64<<<gb, 256, 0, s>>>(f->mask, f->rho, u, f->nx, f->nz, mg.r64, mg.pr, mg.partial);
-        k_sum_pairs<<<1, 1024, 0, s>>>(mg.partial, (int)gb, mg.dots + 8);
+        k_residual64_fin<<<red_blocks(n), kRedBlock, 0, s>>>(f->mask, f->rho, u, n, f->nz, mg.r64, mg.pr, mg.partial,
+                                                             mg.ticket, mg.dots + 8);
         M2D_HIP(hipGetLastError());
         M2D_TRY(read_dots(f));
         double rn = sqrt(mg.h_dots[8]), bn = sqrt(mg.h_dots[9]);
@@ -762,7 +1259,7 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
         }
         if (rel <= tol || outer >= max_outer) break;
         int it = 0;
-        M2D_TRY(pcg(f, 1e-5, 60, &it));
+        M2D_TRY(pcg(f, 1e-5, 60, sqrt(mg.h_dots[10]), &it));
         inner_total += it;
         k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n);
         M2D_HIP(hipGetLastError());

@@ -54,6 +54,7 @@ struct MgLevel {
 struct Mg {
     std::vector<MgLevel> lv;
     float *coarse_inv = nullptr;  // dense inverse of the coarsest operator (n x n)
+    float *coarse_invT = nullptr; // the same, transposed
     int coarse_n = 0;
     // FP32 PCG vectors on the finest level
     float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
@@ -63,6 +64,8 @@ struct Mg {
     double *h_dots = nullptr;     // pinned mirror
     double *partial = nullptr;    // reduction partials
     int npartial = 0;
+    unsigned int *ticket = nullptr;  // block counter of the fused reductions (0 between launches)
+    int lc = 0;                   // first level of the one-workgroup coarse V-cycle
 };
 
 }  // namespace m2d
