@@ -890,6 +890,79 @@ __global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse(CoarseArgs C)
     }
 }
 
+// The common case of k_vcycle_coarse -- one Galerkin level of at most 1024
+// nodes above a dense level of at most 256 -- with both levels held in LDS
+// for the whole V-cycle (the same node bodies on LDS arrays, so the same
+// bits).  Writes x of the Galerkin level.
+constexpr int kLdsLevelNodes = 1024, kLdsDenseNodes = 256;
+
+__global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse_lds(CoarseArgs C)
+{
+    __shared__ float s_st[9 * kLdsLevelNodes], s_dv[kLdsLevelNodes], s_b[kLdsLevelNodes];
+    __shared__ float s_x[kLdsLevelNodes], s_x2[kLdsLevelNodes], s_r[kLdsLevelNodes];
+    __shared__ uint8_t s_dir[kLdsLevelNodes];
+    __shared__ float s_zb[kLdsDenseNodes], s_zx[kLdsDenseNodes], s_part[kDenseParts * kLdsDenseNodes];
+    const BLevel &A = C.lv[C.lc];
+    const BLevel &Z = C.lv[C.lc + 1];
+    const int n = A.n, nx = A.nx, nz = A.nz, cn = C.coarse_n;
+    for (int k = threadIdx.x; k < 9 * n; k += kCoarseThreads) s_st[k] = A.st[k];
+    for (int k = threadIdx.x; k < n; k += kCoarseThreads) {
+        s_dv[k] = A.dinv[k];
+        s_b[k] = A.b[k];
+        s_dir[k] = A.dir[k];
+        s_x[k] = 0.f;
+    }
+    __syncthreads();
+    float *xa = s_x, *xb = s_x2;
+    for (int it = 0; it < C.pre; it++) {
+        for (int k = threadIdx.x; k < n; k += kCoarseThreads)
+            xb[k] = node_jacobi9(s_st, s_dv, xa, s_b, nx, nz, C.omega, k / nz, k % nz);
+        __syncthreads();
+        float *t = xa; xa = xb; xb = t;
+    }
+    for (int k = threadIdx.x; k < n; k += kCoarseThreads)
+        s_r[k] = node_resid9(s_st, s_dv, xa, s_b, nx, nz, k / nz, k % nz);
+    __syncthreads();
+    for (int k = threadIdx.x; k < cn; k += kCoarseThreads)
+        s_zb[k] = node_restrict(s_r, nx, nz, Z.dinv, Z.nz, k / Z.nz, k % Z.nz);
+    __syncthreads();
+    {
+        const int len = (cn + kDenseParts - 1) / kDenseParts;
+        for (int t = threadIdx.x; t < cn * kDenseParts; t += kCoarseThreads) {
+            const int row = t % cn, p = t / cn;
+            const int c0 = p * len, c1 = min(cn, c0 + len);
+            float s = 0.f;
+            int c = c0;
+            for (; c + 8 <= c1; c += 8) {
+                float a[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = C.invT[(size_t)(c + q) * cn + row];
+#pragma unroll
+                for (int q = 0; q < 8; q++) s = fmaf(a[q], s_zb[c + q], s);
+            }
+            for (; c < c1; c++) s = fmaf(C.invT[(size_t)c * cn + row], s_zb[c], s);
+            s_part[p * cn + row] = s;
+        }
+        __syncthreads();
+        for (int row = threadIdx.x; row < cn; row += kCoarseThreads) {
+            float s = s_part[row];
+            for (int p = 1; p < kDenseParts; p++) s += s_part[p * cn + row];
+            s_zx[row] = s;
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < n; k += kCoarseThreads)
+        node_prolong(xa, s_dir, nz, s_zx, Z.nx, Z.nz, k / nz, k % nz);
+    __syncthreads();
+    for (int it = 0; it < C.post; it++) {
+        for (int k = threadIdx.x; k < n; k += kCoarseThreads)
+            xb[k] = node_jacobi9(s_st, s_dv, xa, s_b, nx, nz, C.omega, k / nz, k % nz);
+        __syncthreads();
+        float *t = xa; xa = xb; xb = t;
+    }
+    for (int k = threadIdx.x; k < n; k += kCoarseThreads) A.x[k] = xa[k];
+}
+
 // ------------------------------------------------------------------ host: hierarchy
 static void p1d(int i, int nc, int I[2], float w[2], int &cnt)
 {
@@ -1168,7 +1241,10 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
         C.omega = kOmega;
         C.pre = kPre;
         C.post = kPost;
-        k_vcycle_coarse<<<1, kCoarseThreads, sizeof(float) * mg.coarse_n * (1 + kDenseParts), s>>>(C);
+        if (L - 1 - mg.lc == 1 && mg.lv[mg.lc].n <= (size_t)kLdsLevelNodes && mg.coarse_n <= kLdsDenseNodes)
+            k_vcycle_coarse_lds<<<1, kCoarseThreads, 0, s>>>(C);
+        else
+            k_vcycle_coarse<<<1, kCoarseThreads, sizeof(float) * mg.coarse_n * (1 + kDenseParts), s>>>(C);
     }
     // ---- middle levels ascent
     for (int l = mg.lc - 1; l >= 1; l--) {
