@@ -39,6 +39,8 @@ struct m2d_fields3d {
     float *tx = nullptr, *ty = nullptr, *tz = nullptr;  // (u_m - u_{m-e}) * 1/h per axis, FP32
     std::vector<m2d::Mg3Level> lv;
     float *coarse_inv = nullptr;
+    float *coarse_invT = nullptr;  // the same, transposed (read by k_dense_solve)
+    unsigned int *ticket = nullptr;  // block counter of the fused reductions
     int coarse_n = 0;
     bool coarse_dense = true;
     float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
@@ -86,15 +88,22 @@ __global__ void k3_rhs(const uint8_t *mask, const double *voltage, double *rho, 
 }
 
 // r = b - A u on free rows (matrix_init, fields3d.cpp:35-76); d = -r in FP32;
-// per-block partial sums of r^2 and of the rhs with Dirichlet neighbours moved over
-__global__ void __launch_bounds__(256) k3_residual64(const uint8_t *mask, const double *b, const double *u,
-                                                     int nx, int ny, int nz, double *r, float *d,
-                                                     double *partial)
+__global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
 {
-    __shared__ double s1[256], s2[256];
-    const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
     size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double rr = 0, bb = 0;
+    if (m < n && !dir3(mask[m])) u[m] += (double)e[m];
+}
+
+// r = b - A u on free nodes (0 on Dirichlet nodes) with its norms finished in
+// the same launch: out[0] = |r|^2, out[1] = |b_eff|^2, out[2] = |d|^2, d = -r in FP32
+__global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *mask, const double *b, const double *u,
+                                                              int nx, int ny, int nz, double *r, float *d,
+                                                              double *partial, unsigned int *ticket, double *out,
+                                                              int fin)
+{
+    const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
+    const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    double v[3] = {0, 0, 0};
     if (m < n) {
         double rv = 0;
         if (!dir3(mask[m])) {
@@ -107,26 +116,15 @@ __global__ void __launch_bounds__(256) k3_residual64(const uint8_t *mask, const 
                 if (dir3(mask[nb[q]])) be -= u[nb[q]];
             }
             rv = b[m] - au;
-            bb = be * be;
+            v[1] = be * be;
         }
         r[m] = rv;
-        d[m] = (float)(-rv);
-        rr = rv * rv;
+        const float dm = (float)(-rv);
+        d[m] = dm;
+        v[0] = rv * rv;
+        v[2] = (double)dm * dm;
     }
-    s1[threadIdx.x] = rr;
-    s2[threadIdx.x] = bb;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
-}
-
-__global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
-{
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < n && !dir3(mask[m])) u[m] += (double)e[m];
+    finish_sums<3>(v, partial, ticket, out, fin);
 }
 
 // A' x = 6 x - sum over free neighbours (Dirichlet rows: 0)
@@ -142,15 +140,35 @@ __device__ __forceinline__ float apply7(const uint8_t *dir, const float *x, size
     return s;
 }
 
-// red-black Gauss-Seidel half sweep, colour = (i+j+k) & 1
-__global__ void k3_rbgs(const uint8_t *dir, float *x, const float *b, int nx, int ny, int nz, int colour)
+// y = A' x and out = x.y in one launch
+__global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *dir, const float *x, float *y, int nx, int ny,
+                                                         int nz, double *partial, unsigned int *ticket, double *out,
+                                                         int fin)
 {
-    const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n || dir[m]) return;
-    int i, j, k;
-    node3(m, ny, nz, i, j, k);
-    if (((i + j + k) & 1) != colour) return;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    double v[1] = {0};
+    if (m < n) {
+        const float ym = dir[m] ? 0.f : apply7(dir, x, m, (size_t)ny * nz, nz);
+        y[m] = ym;
+        v[0] = (double)x[m] * ym;
+    }
+    finish_sums<1>(v, partial, ticket, out, fin);
+}
+
+// red-black Gauss-Seidel half sweep, colour = (i+j+k) & 1.  Threads cover
+// only the nodes of the colour: thread (kk, j, i) updates k = 2 kk + parity,
+// blocks of 64 x 4 (kk, j) and a grid of (nz/2 / 64, ny / 4, nx).
+constexpr int kRbgsX = 64, kRbgsY = 4;
+__global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *dir, float *x, const float *b, int nx,
+                                                           int ny, int nz, int colour)
+{
+    const size_t sx = (size_t)ny * nz;
+    const int i = blockIdx.z, j = blockIdx.y * kRbgsY + threadIdx.y;
+    const int k = 2 * (blockIdx.x * kRbgsX + threadIdx.x) + ((i + j + colour) & 1);
+    if (j >= ny || k >= nz) return;
+    const size_t m = (size_t)i * sx + (size_t)j * nz + k;
+    if (dir[m]) return;
     float s = b[m];
     if (!dir[m - sx]) s += x[m - sx];
     if (!dir[m - nz]) s += x[m - nz];
@@ -169,13 +187,6 @@ __global__ void k3_resid(const uint8_t *dir, const float *x, const float *b, flo
     r[m] = dir[m] ? 0.f : b[m] - apply7(dir, x, m, (size_t)ny * nz, nz);
 }
 
-__global__ void k3_apply(const uint8_t *dir, const float *x, float *y, int nx, int ny, int nz)
-{
-    const size_t n = (size_t)nx * ny * nz;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    y[m] = dir[m] ? 0.f : apply7(dir, x, m, (size_t)ny * nz, nz);
-}
 
 // 1D weights of the trilinear prolongation: fine i from coarse I
 __device__ __forceinline__ float w1(int i, int I) { int d = i - 2 * I; return d == 0 ? 1.f : (d == 1 || d == -1 ? 0.5f : 0.f); }
@@ -234,19 +245,6 @@ __global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int f
             for (int c = 0; c < nk; c++)
                 s = fmaf(wa[a] * wb[b] * wc[c], xc[((size_t)Ia[a] * cy + Ja[b]) * cz + Ka[c]], s);
     xf[m] += s;
-}
-
-__global__ void __launch_bounds__(256) k3_coarse_solve(const float *inv, const float *b, float *x, int n)
-{
-    extern __shared__ float sb[];
-    for (int k = threadIdx.x; k < n; k += blockDim.x) sb[k] = b[k];
-    __syncthreads();
-    for (int row = threadIdx.x; row < n; row += blockDim.x) {
-        const float *a = inv + (size_t)row * n;
-        float s = 0.f;
-        for (int c = 0; c < n; c++) s = fmaf(a[c], sb[c], s);
-        x[row] = s;
-    }
 }
 
 // ------------------------------------------------------------------ point kernels (FP64)
@@ -874,6 +872,12 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
         f->coarse_n = (int)C.n;
         M2D_TRY(dmalloc3((void **)&f->coarse_inv, sizeof(float) * inv.size()));
         M2D_HIP(hipMemcpy(f->coarse_inv, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice));
+        const size_t cn = (size_t)f->coarse_n;
+        std::vector<float> invT(cn * cn);
+        for (size_t r = 0; r < cn; r++)
+            for (size_t c = 0; c < cn; c++) invT[c * cn + r] = inv[r * cn + c];
+        M2D_TRY(dmalloc3((void **)&f->coarse_invT, sizeof(float) * invT.size()));
+        M2D_HIP(hipMemcpy(f->coarse_invT, invT.data(), sizeof(float) * invT.size(), hipMemcpyHostToDevice));
     }
     const size_t n = f->n;
     for (float **q : {&f->pr, &f->pz, &f->pp, &f->pq, &f->pe}) M2D_TRY(dmalloc3((void **)q, sizeof(float) * n));
@@ -881,7 +885,10 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
     M2D_TRY(dmalloc3((void **)&f->dots, sizeof(double) * 16));
     M2D_HIP(hipHostMalloc((void **)&f->h_dots, sizeof(double) * 16));
     f->npartial = (int)std::min<size_t>((n + 255) / 256, 65536);
-    M2D_TRY(dmalloc3((void **)&f->partial, sizeof(double) * 2 * std::max<size_t>(f->npartial, (n + 255) / 256)));
+    // up to three sums per block of the fused reductions
+    M2D_TRY(dmalloc3((void **)&f->partial, sizeof(double) * 3 * std::max<size_t>(f->npartial, (n + 255) / 256)));
+    M2D_TRY(dmalloc3((void **)&f->ticket, sizeof(unsigned int)));
+    M2D_HIP(hipMemset(f->ticket, 0, sizeof(unsigned int)));
     return M2D_OK;
 }
 
@@ -894,10 +901,12 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
     hipStream_t s = f->ctx->stream;
     const int L = (int)f->lv.size();
     auto smooth = [&](Mg3Level &V, float *x, const float *b, int sweeps, bool forward) {
-        const unsigned g = blocks_for(V.n);
+        const dim3 g((unsigned)(((V.nz + 1) / 2 + kRbgsX - 1) / kRbgsX), (unsigned)((V.ny + kRbgsY - 1) / kRbgsY),
+                     (unsigned)V.nx);
+        const dim3 blk(kRbgsX, kRbgsY);
         for (int it = 0; it < sweeps; it++) {
-            k3_rbgs<<<g, 256, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1);
-            k3_rbgs<<<g, 256, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
+            k3_rbgs<<<g, blk, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1);
+            k3_rbgs<<<g, blk, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
         }
     };
     for (int l = 0; l < L - 1; l++) {
@@ -915,7 +924,8 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         float *x = L == 1 ? out : C.x;
         const float *b = L == 1 ? rhs : C.b;
         if (f->coarse_dense) {
-            k3_coarse_solve<<<1, 256, sizeof(float) * f->coarse_n, s>>>(f->coarse_inv, b, x, f->coarse_n);
+            k_dense_solve<<<1, kCoarseThreads, sizeof(float) * f->coarse_n * (1 + kDenseParts), s>>>(
+                f->coarse_invT, b, x, f->coarse_n);
         } else {
             M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * C.n));
             smooth(C, x, b, kCoarseSweeps / 2, true);
@@ -934,16 +944,6 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
     return M2D_OK;
 }
 
-static int dot3(m2d_fields3d *f, const float *a, const float *b, int slot)
-{
-    hipStream_t s = f->ctx->stream;
-    const int nb = std::min(f->npartial, 1024);
-    k_dot2<<<nb, 256, 0, s>>>(a, b, nullptr, nullptr, f->n, f->partial);
-    k_sum_pairs<<<1, 1024, 0, s>>>(f->partial, nb, f->dots + slot);
-    M2D_HIP(hipGetLastError());
-    return M2D_OK;
-}
-
 static int read_dots3(m2d_fields3d *f)
 {
     M2D_HIP(hipMemcpyAsync(f->h_dots, f->dots, sizeof(double) * 16, hipMemcpyDeviceToHost, f->ctx->stream));
@@ -951,34 +951,42 @@ static int read_dots3(m2d_fields3d *f)
     return M2D_OK;
 }
 
-// FP32 PCG on A' e = d (d in pr), e in pe
-static int pcg3(m2d_fields3d *f, double rtol, int max_it, int *iters)
+// FP32 PCG on A' e = d (d in pr, |d| = r0), e in pe; the dot products finish
+// on the device, one host read per iteration (the convergence test).
+// dots: [0], [4] r.z of alternate iterations, [1] p.q, [2] r.r
+static int pcg3(m2d_fields3d *f, double rtol, int max_it, double r0, int *iters)
 {
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
-    const unsigned gb = blocks_for(n);
+    const unsigned gb = blocks_for(n), rb = (unsigned)((n + kRedBlock - 1) / kRedBlock);
     const Mg3Level &V = f->lv[0];
-    M2D_TRY(fill_bytes(f->ctx, f->pe, 0, sizeof(float) * n));
-    M2D_TRY(dot3(f, f->pr, f->pr, 2));
-    M2D_TRY(read_dots3(f));
-    const double r0 = sqrt(f->h_dots[2]);
     *iters = 0;
-    if (r0 == 0.0) return M2D_OK;
+    if (r0 == 0.0) return fill_bytes(f->ctx, f->pe, 0, sizeof(float) * n);  // e = 0
     M2D_TRY(vcycle3(f, f->pr, f->pz));
-    M2D_TRY(dot3(f, f->pr, f->pz, 0));
+    int cur = 0;
+    reduce_launch<1>(rb, f->partial, f->dots, s, [&](int fin) {
+        k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin);
+    });
     k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1);
     for (int it = 1; it <= max_it; it++) {
-        k3_apply<<<gb, 256, 0, s>>>(V.dir, f->pp, f->pq, V.nx, V.ny, V.nz);
-        M2D_TRY(dot3(f, f->pp, f->pq, 1));
-        k_cg_update<<<gb, 256, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots, n);
-        M2D_TRY(dot3(f, f->pr, f->pr, 2));
+        reduce_launch<1>(rb, f->partial, f->dots + 1, s, [&](int fin) {
+            k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.dir, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
+                                                  f->dots + 1, fin);
+        });
+        reduce_launch<1>(rb, f->partial, f->dots + 2, s, [&](int fin) {
+            k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots + 4 * cur, f->dots + 1, n,
+                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin);
+        });
+        M2D_HIP(hipGetLastError());
         *iters = it;
         M2D_TRY(read_dots3(f));
         if (sqrt(f->h_dots[2]) <= rtol * r0) break;
         M2D_TRY(vcycle3(f, f->pr, f->pz));
-        k_copy_scalar<<<1, 1, 0, s>>>(f->dots, f->dots + 4);
-        M2D_TRY(dot3(f, f->pr, f->pz, 0));
-        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 0);
+        reduce_launch<1>(rb, f->partial, f->dots + 4 * (cur ^ 1), s, [&](int fin) {
+            k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1), fin);
+        });
+        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0);
+        cur ^= 1;
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -1050,7 +1058,7 @@ static void free3(m2d_fields3d *f)
     for (void *q : {(void *)f->mask, (void *)f->cellfree, (void *)f->voltage, (void *)f->u, (void *)f->rho,
                     (void *)f->tx, (void *)f->ty, (void *)f->tz, (void *)f->coarse_inv, (void *)f->pr,
                     (void *)f->pz, (void *)f->pp, (void *)f->pq, (void *)f->pe, (void *)f->r64,
-                    (void *)f->dots, (void *)f->partial})
+                    (void *)f->dots, (void *)f->partial, (void *)f->coarse_invT, (void *)f->ticket})
         if (q) (void)hipFree(q);
     for (Mg3Level &V : f->lv)
         for (void *q : {(void *)V.dir, (void *)V.x, (void *)V.b, (void *)V.r})
@@ -1173,8 +1181,11 @@ int m2d_fields3d_solve(m2d_fields3d *f, double tol, m2d_solve_info *info)
     double rel = 0;
     const int max_outer = 12;
     for (;;) {
-        k3_residual64<<<gb, 256, 0, s>>>(f->mask, f->rho, f->u, f->nx, f->ny, f->nz, f->r64, f->pr, f->partial);
-        k_sum_pairs<<<1, 1024, 0, s>>>(f->partial, (int)gb, f->dots + 8);
+        const unsigned rb = (unsigned)((n + kRedBlock - 1) / kRedBlock);
+        reduce_launch<3>(rb, f->partial, f->dots + 8, s, [&](int fin) {
+            k3_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, f->u, f->nx, f->ny, f->nz, f->r64, f->pr,
+                                                       f->partial, f->ticket, f->dots + 8, fin);
+        });
         M2D_HIP(hipGetLastError());
         M2D_TRY(read_dots3(f));
         const double rn = sqrt(f->h_dots[8]), bn = sqrt(f->h_dots[9]);
@@ -1185,7 +1196,8 @@ int m2d_fields3d_solve(m2d_fields3d *f, double tol, m2d_solve_info *info)
         }
         if (rel <= tol || outer >= max_outer) break;
         int it = 0;
-        M2D_TRY(pcg3(f, 1e-5, 80, &it));
+        // solved to 1e-5, or only as far as the FP64 target needs
+        M2D_TRY(pcg3(f, std::max(1e-5, std::min(0.1, 0.5 * tol / rel)), 80, sqrt(f->h_dots[10]), &it));
         inner += it;
         k3_add_correction<<<gb, 256, 0, s>>>(f->mask, f->u, f->pe, n);
         M2D_HIP(hipGetLastError());

@@ -115,21 +115,6 @@ __global__ void k_add_correction(const uint8_t *mask, double *u, const float *e,
     if (k < n && !is_dir(mask[k])) u[k] += (double)e[k];
 }
 
-// fixed-order sum of `np` pairs of partials into out[0..1]
-__global__ void __launch_bounds__(1024) k_sum_pairs(const double *partial, int np, double *out)
-{
-    __shared__ double s1[1024], s2[1024];
-    double a = 0, b = 0;
-    for (int k = threadIdx.x; k < np; k += 1024) { a += partial[2 * k]; b += partial[2 * k + 1]; }
-    s1[threadIdx.x] = a; s2[threadIdx.x] = b;
-    __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { out[0] = s1[0]; out[1] = s2[0]; }
-}
-
 // ------------------------------------------------------------------ FP32 MG kernels
 // smoothing sweeps before and after the coarse correction, and the damping
 // of the Jacobi sweeps on the Galerkin levels
@@ -315,38 +300,6 @@ __global__ void __launch_bounds__(256) k_coarse_solve(const float *inv, const fl
     }
 }
 
-// ------------------------------------------------------------------ PCG helpers
-// per-block partial dot products (FP64 accumulation) of up to two pairs
-__global__ void __launch_bounds__(256) k_dot2(const float *a, const float *b, const float *c,
-                                              const float *d, size_t n, double *partial)
-{
-    __shared__ double s1[256], s2[256];
-    double x = 0, y = 0;
-    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
-        x += (double)a[k] * b[k];
-        if (c) y += (double)c[k] * d[k];
-    }
-    s1[threadIdx.x] = x; s2[threadIdx.x] = y;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
-}
-
-// e += alpha p ; r -= alpha q with alpha = dots[0] / dots[1] read on the device
-__global__ void k_cg_update(float *e, float *r, const float *p, const float *q, const double *sc,
-                            size_t n)
-{
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    double den = sc[1];
-    float alpha = den != 0.0 ? (float)(sc[0] / den) : 0.f;
-    e[k] = fmaf(alpha, p[k], e[k]);
-    r[k] = fmaf(-alpha, q[k], r[k]);
-}
-
 // p = z + beta p, beta = rz_new / rz_old
 __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old,
                          size_t n, int first)
@@ -359,8 +312,6 @@ __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const d
     p[k] = fmaf(beta, p[k], z[k]);
 }
 
-__global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
-
 // ------------------------------------------------------------------ fused solver kernels
 // The solve is launch-bound on small grids, so its kernels carry their own
 // reductions and fuse neighbouring sweeps.  finish_sums: each block sums its
@@ -368,69 +319,20 @@ __global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
 // ticket sums the partials in block order, writes out[] and resets the
 // ticket, so a reduction costs no extra launch and gives the same bits on
 // every run.
-constexpr int kRedBlock = 256;
-
-template <int NV>
-__device__ __forceinline__ void finish_sums(double (&v)[NV], double *partial, unsigned int *ticket, double *out)
-{
-    __shared__ double sh[NV][kRedBlock / 64];
-    __shared__ bool last;
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o);
-        if ((threadIdx.x & 63) == 0) sh[q][w] = v[q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double t = 0;
-            for (int k = 0; k < kRedBlock / 64; k++) t += sh[q][k];
-            partial[NV * blockIdx.x + q] = t;
-        }
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;  // uniform over the block
-    __threadfence();
-    double t[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) t[q] = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kRedBlock)
-#pragma unroll
-        for (int q = 0; q < NV; q++) t[q] += partial[NV * b + q];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        for (int o = 32; o > 0; o >>= 1) t[q] += __shfl_down(t[q], o);
-        if ((threadIdx.x & 63) == 0) sh[q][w] = t[q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double s = 0;
-            for (int k = 0; k < kRedBlock / 64; k++) s += sh[q][k];
-            out[q] = s;
-        }
-        *ticket = 0u;
-    }
-}
-
 // out = a.b (FP64 accumulation)
 __global__ void __launch_bounds__(kRedBlock) k_dot_fin(const float *a, const float *b, size_t n, double *partial,
-                                                      unsigned int *ticket, double *out)
+                                                      unsigned int *ticket, double *out, int fin)
 {
     double v[1] = {0};
     for (size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (size_t)gridDim.x * kRedBlock)
         v[0] += (double)a[k] * b[k];
-    finish_sums<1>(v, partial, ticket, out);
+    finish_sums<1>(v, partial, ticket, out, fin);
 }
 
 // q = A' p on the finest level, out = p.q
 __global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, const float *p, float *q, size_t n,
-                                                         int nz, double *partial, unsigned int *ticket, double *out)
+                                                         int nz, double *partial, unsigned int *ticket, double *out,
+                                                         int fin)
 {
     double v[1] = {0};
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
@@ -439,14 +341,15 @@ __global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, co
         q[k] = y;
         v[0] = (double)p[k] * y;
     }
-    finish_sums<1>(v, partial, ticket, out);
+    finish_sums<1>(v, partial, ticket, out, fin);
 }
 
 // e += alpha p, r -= alpha q with alpha = rz / pq; out = r.r
 // (first: e starts at 0, so e = alpha p without reading e)
 __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r, const float *p, const float *q,
                                                             const double *rz, const double *pq, size_t n, int first,
-                                                            double *partial, unsigned int *ticket, double *out)
+                                                            double *partial, unsigned int *ticket, double *out,
+                                                            int fin)
 {
     double v[1] = {0};
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
@@ -458,14 +361,14 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r,
         r[k] = rk;
         v[0] = (double)rk * rk;
     }
-    finish_sums<1>(v, partial, ticket, out);
+    finish_sums<1>(v, partial, ticket, out, fin);
 }
 
 // the FP64 residual with its norms finished in the same launch: out[0] =
 // |r|^2, out[1] = |b_eff|^2, out[2] = |d|^2 of the FP32 copy d = -r
 __global__ void __launch_bounds__(kRedBlock) k_residual64_fin(const uint8_t *mask, const double *b, const double *u,
                                                              size_t n, int nz, double *r, float *d, double *partial,
-                                                             unsigned int *ticket, double *out)
+                                                             unsigned int *ticket, double *out, int fin)
 {
     double v[3] = {0, 0, 0};
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
@@ -473,7 +376,7 @@ __global__ void __launch_bounds__(kRedBlock) k_residual64_fin(const uint8_t *mas
         node_residual64(mask, b, u, nz, r, d, k, v[0], v[1]);
         v[2] = (double)d[k] * d[k];
     }
-    finish_sums<3>(v, partial, ticket, out);
+    finish_sums<3>(v, partial, ticket, out, fin);
 }
 
 // prolongation value at fine node (i, j) (the increment of node_prolong)
@@ -794,8 +697,6 @@ __global__ void __launch_bounds__(kT0Threads) k_lm_up(const float *st, const flo
 
 // the V-cycle below level lc in one workgroup: descent from b on level lc,
 // dense solve, ascent to x on level lc
-constexpr int kCoarseThreads = 1024;
-constexpr int kDenseParts = 4;  // threads per row of the dense coarsest solve
 constexpr int kMaxLevels = 16;
 constexpr size_t kBlockLevelNodes = 1024;  // levels this small run inside the workgroup
 
@@ -961,6 +862,39 @@ __global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse_lds(CoarseArgs
         float *t = xa; xa = xb; xb = t;
     }
     for (int k = threadIdx.x; k < n; k += kCoarseThreads) A.x[k] = xa[k];
+}
+
+// dense solve x = A^-1 b with the transposed inverse, one workgroup of
+// kCoarseThreads: kDenseParts threads per row (the same sums as the dense
+// step of k_vcycle_coarse); dynamic LDS of (1 + kDenseParts) n words
+__global__ void __launch_bounds__(kCoarseThreads) k_dense_solve(const float *invT, const float *b, float *x, int n)
+{
+    extern __shared__ float sb[];
+    float *part = sb + n;
+    for (int k = threadIdx.x; k < n; k += kCoarseThreads) sb[k] = b[k];
+    __syncthreads();
+    const int len = (n + kDenseParts - 1) / kDenseParts;
+    for (int t = threadIdx.x; t < n * kDenseParts; t += kCoarseThreads) {
+        const int row = t % n, p = t / n;
+        const int c0 = p * len, c1 = min(n, c0 + len);
+        float s = 0.f;
+        int c = c0;
+        for (; c + 8 <= c1; c += 8) {
+            float a[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) a[q] = invT[(size_t)(c + q) * n + row];
+#pragma unroll
+            for (int q = 0; q < 8; q++) s = fmaf(a[q], sb[c + q], s);
+        }
+        for (; c < c1; c++) s = fmaf(invT[(size_t)c * n + row], sb[c], s);
+        part[p * n + row] = s;
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += kCoarseThreads) {
+        float s = part[row];
+        for (int p = 1; p < kDenseParts; p++) s += part[p * n + row];
+        x[row] = s;
+    }
 }
 
 // ------------------------------------------------------------------ host: hierarchy
@@ -1289,18 +1223,27 @@ static int pcg(m2d_fields2d *f, double rtol, int max_it, double r0, int *iters)
     if (r0 == 0.0) return fill_bytes(f->ctx, mg.pe, 0, sizeof(float) * n);  // e = 0
     M2D_TRY(vcycle(f, mg.pr, mg.pz));
     int cur = 0;
-    k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots);
+    reduce_launch<1>(rb, mg.partial, mg.dots, s, [&](int fin) {
+        k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots, fin);
+    });
     k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1);
     for (int it = 1; it <= max_it; it++) {
-        k_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pp, mg.pq, n, L0.nz, mg.partial, mg.ticket, mg.dots + 1);
-        k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
-                                                 it == 1, mg.partial, mg.ticket, mg.dots + 2);
+        reduce_launch<1>(rb, mg.partial, mg.dots + 1, s, [&](int fin) {
+            k_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pp, mg.pq, n, L0.nz, mg.partial, mg.ticket, mg.dots + 1,
+                                                  fin);
+        });
+        reduce_launch<1>(rb, mg.partial, mg.dots + 2, s, [&](int fin) {
+            k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
+                                                     it == 1, mg.partial, mg.ticket, mg.dots + 2, fin);
+        });
         M2D_HIP(hipGetLastError());
         *iters = it;
         M2D_TRY(read_dots(f));
         if (sqrt(mg.h_dots[2]) <= rtol * r0) break;
         M2D_TRY(vcycle(f, mg.pr, mg.pz));
-        k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1));
+        reduce_launch<1>(rb, mg.partial, mg.dots + 4 * (cur ^ 1), s, [&](int fin) {
+            k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1), fin);
+        });
         k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0);
         cur ^= 1;
     }
@@ -1310,6 +1253,10 @@ static int pcg(m2d_fields2d *f, double rtol, int max_it, double r0, int *iters)
 
 static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
 {
+    // the FP32 correction is solved to 1e-5 of its right-hand side, or only
+    // as far as the FP64 target needs (half of tol / rel, at most 0.1): the
+    // last refinement pass then takes one or two CG iterations, not five
+    auto inner_rtol = [](double rel, double tol) { return std::max(1e-5, std::min(0.1, 0.5 * tol / rel)); };
     hipStream_t s = f->ctx->stream;
     Mg &mg = f->mg;
     const size_t n = f->n;
@@ -1323,8 +1270,11 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
     double rel = 0;
     const int max_outer = 12;
     for (;;) {
-        k_residual64_fin<<<red_blocks(n), kRedBlock, 0, s>>>(f->mask, f->rho, u, n, f->nz, mg.r64, mg.pr, mg.partial,
-                                                             mg.ticket, mg.dots + 8);
+        const unsigned rb = red_blocks(n);
+        reduce_launch<3>(rb, mg.partial, mg.dots + 8, s, [&](int fin) {
+            k_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, u, n, f->nz, mg.r64, mg.pr, mg.partial, mg.ticket,
+                                                      mg.dots + 8, fin);
+        });
         M2D_HIP(hipGetLastError());
         M2D_TRY(read_dots(f));
         double rn = sqrt(mg.h_dots[8]), bn = sqrt(mg.h_dots[9]);
@@ -1335,7 +1285,7 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
         }
         if (rel <= tol || outer >= max_outer) break;
         int it = 0;
-        M2D_TRY(pcg(f, 1e-5, 60, sqrt(mg.h_dots[10]), &it));
+        M2D_TRY(pcg(f, inner_rtol(rel, tol), 60, sqrt(mg.h_dots[10]), &it));
         inner_total += it;
         k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n);
         M2D_HIP(hipGetLastError());

@@ -123,14 +123,17 @@ int ensure(m2d_ctx *ctx, Scratch &s, size_t bytes);
 
 // vector kernels shared by the 2D and 3D solvers (m2d_fields2d.hip) and the
 // fixed-point flush (m2d_particles2d.hip)
-__global__ void __launch_bounds__(256) k_dot2(const float *a, const float *b, const float *c,
-                                              const float *d, size_t n, double *partial);
-__global__ void __launch_bounds__(1024) k_sum_pairs(const double *partial, int np, double *out);
-__global__ void k_cg_update(float *e, float *r, const float *p, const float *q, const double *sc, size_t n);
 __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old, size_t n,
                          int first);
-__global__ void k_copy_scalar(const double *src, double *dst);
 __global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n);
+__global__ void k_dot_fin(const float *a, const float *b, size_t n, double *partial, unsigned int *ticket,
+                          double *out, int fin);
+__global__ void k_cg_update_dot(float *e, float *r, const float *p, const float *q, const double *rz,
+                                const double *pq, size_t n, int first, double *partial, unsigned int *ticket,
+                                double *out, int fin);
+constexpr int kCoarseThreads = 1024;
+constexpr int kDenseParts = 4;  // threads per row of the dense coarsest solves
+__global__ void k_dense_solve(const float *invT, const float *b, float *x, int n);
 int ensure_acc(m2d_ctx *ctx, size_t n);
 int fill_bytes(m2d_ctx *ctx, void *dst, int value, size_t bytes);
 int zero_words(m2d_ctx *ctx, unsigned long long *dst, size_t n);
@@ -229,6 +232,95 @@ inline int grid_for(int64_t n, int per_thread, int block = kBlock)
 {
     int64_t g = (n + (int64_t)block * per_thread - 1) / ((int64_t)block * per_thread);
     return (int)(g < 1 ? 1 : g);
+}
+
+
+// Reductions of the solver kernels (m2d_fields2d.hip, m2d_3d.hip).  Each
+// block sums its values in a fixed order and writes a partial.  The partials
+// are then summed in block order, by sum_partials, either in a kernel of its
+// own (k_sum_partials) or, with fin set, by the block that takes the last
+// ticket, which writes out[] and resets the ticket: no extra launch, which
+// pays on small grids.  The last-ticket path costs each block a device-scope
+// fence, so it is for launches of at most kFinBlocks blocks.  Both paths add
+// in the same order and give the same bits on every run.  Blocks of
+// kRedBlock threads; every thread of every block must call finish_sums.
+constexpr int kRedBlock = 256;
+constexpr unsigned kFinBlocks = 256;
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *partial, unsigned nb, double *out)
+{
+    __shared__ double sh[NV][kRedBlock / 64];
+    const int w = threadIdx.x >> 6;
+    double t[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) t[q] = 0;
+    for (unsigned b = threadIdx.x; b < nb; b += kRedBlock)
+#pragma unroll
+        for (int q = 0; q < NV; q++) t[q] += partial[NV * b + q];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) t[q] += __shfl_down(t[q], o);
+        if ((threadIdx.x & 63) == 0) sh[q][w] = t[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double s = 0;
+            for (int k = 0; k < kRedBlock / 64; k++) s += sh[q][k];
+            out[q] = s;
+        }
+    }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(kRedBlock) k_sum_partials(const double *partial, unsigned nb, double *out)
+{
+    sum_partials<NV>(partial, nb, out);
+}
+
+template <int NV>
+__device__ __forceinline__ void finish_sums(double (&v)[NV], double *partial, unsigned int *ticket, double *out,
+                                            int fin)
+{
+    __shared__ double sh[NV][kRedBlock / 64];
+    __shared__ bool last;
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o);
+        if ((threadIdx.x & 63) == 0) sh[q][w] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double t = 0;
+            for (int k = 0; k < kRedBlock / 64; k++) t += sh[q][k];
+            partial[NV * blockIdx.x + q] = t;
+        }
+        if (fin) {
+            __threadfence();
+            last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        }
+    }
+    if (!fin) return;  // uniform: a kernel argument
+    __syncthreads();
+    if (!last) return;  // uniform over the block
+    __threadfence();
+    sum_partials<NV>(partial, gridDim.x, out);
+    if (threadIdx.x == 0) *ticket = 0u;
+}
+
+// launch a reduction kernel over nb blocks and finish its sums: in the
+// kernel (fin) when nb <= kFinBlocks, else with k_sum_partials
+template <int NV, class Launch>
+inline void reduce_launch(unsigned nb, double *partial, double *out, hipStream_t s, Launch launch)
+{
+    const int fin = nb <= kFinBlocks;
+    launch(fin);
+    if (!fin) k_sum_partials<NV><<<1, kRedBlock, 0, s>>>(partial, nb, out);
 }
 
 }  // namespace m2d
