@@ -22,6 +22,7 @@ struct Mg3Level {
     int nx = 0, ny = 0, nz = 0;
     size_t n = 0;
     uint8_t *dir = nullptr;   // 1 on Dirichlet nodes (and the coarse faces)
+    uint8_t *nbm = nullptr;   // per node: bits 0-5 set for free neighbours -x,-y,-z,+z,+y,+x; 0x80 if pinned
     float *x = nullptr, *b = nullptr, *r = nullptr;
 };
 
@@ -127,21 +128,26 @@ __global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *ma
     finish_sums<3>(v, partial, ticket, out, fin);
 }
 
+// Neighbour masks of the 7-point levels: one byte per node replaces the
+// seven Dirichlet flags a stencil update reads.  Bits 0-5: neighbours
+// -x, -y, -z, +z, +y, +x free; kNbSelf: the node itself is pinned.
+constexpr uint8_t kNbSelf = 0x80;
+
 // A' x = 6 x - sum over free neighbours (Dirichlet rows: 0)
-__device__ __forceinline__ float apply7(const uint8_t *dir, const float *x, size_t m, size_t sx, int nz)
+__device__ __forceinline__ float apply7(uint8_t nb, const float *x, size_t m, size_t sx, int nz)
 {
     float s = 6.f * x[m];
-    if (!dir[m - sx]) s -= x[m - sx];
-    if (!dir[m - nz]) s -= x[m - nz];
-    if (!dir[m - 1]) s -= x[m - 1];
-    if (!dir[m + 1]) s -= x[m + 1];
-    if (!dir[m + nz]) s -= x[m + nz];
-    if (!dir[m + sx]) s -= x[m + sx];
+    if (nb & 1) s -= x[m - sx];
+    if (nb & 2) s -= x[m - nz];
+    if (nb & 4) s -= x[m - 1];
+    if (nb & 8) s -= x[m + 1];
+    if (nb & 16) s -= x[m + nz];
+    if (nb & 32) s -= x[m + sx];
     return s;
 }
 
 // y = A' x and out = x.y in one launch
-__global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *dir, const float *x, float *y, int nx, int ny,
+__global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *nbm, const float *x, float *y, int nx, int ny,
                                                          int nz, double *partial, unsigned int *ticket, double *out,
                                                          int fin)
 {
@@ -149,7 +155,8 @@ __global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *dir, co
     const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     double v[1] = {0};
     if (m < n) {
-        const float ym = dir[m] ? 0.f : apply7(dir, x, m, (size_t)ny * nz, nz);
+        const uint8_t nb = nbm[m];
+        const float ym = (nb & kNbSelf) ? 0.f : apply7(nb, x, m, (size_t)ny * nz, nz);
         y[m] = ym;
         v[0] = (double)x[m] * ym;
     }
@@ -160,7 +167,7 @@ __global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *dir, co
 // only the nodes of the colour: thread (kk, j, i) updates k = 2 kk + parity,
 // blocks of 64 x 4 (kk, j) and a grid of (nz/2 / 64, ny / 4, nx).
 constexpr int kRbgsX = 64, kRbgsY = 4;
-__global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *dir, float *x, const float *b, int nx,
+__global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, float *x, const float *b, int nx,
                                                            int ny, int nz, int colour)
 {
     const size_t sx = (size_t)ny * nz;
@@ -168,23 +175,25 @@ __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *dir, f
     const int k = 2 * (blockIdx.x * kRbgsX + threadIdx.x) + ((i + j + colour) & 1);
     if (j >= ny || k >= nz) return;
     const size_t m = (size_t)i * sx + (size_t)j * nz + k;
-    if (dir[m]) return;
+    const uint8_t nb = nbm[m];
+    if (nb & kNbSelf) return;
     float s = b[m];
-    if (!dir[m - sx]) s += x[m - sx];
-    if (!dir[m - nz]) s += x[m - nz];
-    if (!dir[m - 1]) s += x[m - 1];
-    if (!dir[m + 1]) s += x[m + 1];
-    if (!dir[m + nz]) s += x[m + nz];
-    if (!dir[m + sx]) s += x[m + sx];
+    if (nb & 1) s += x[m - sx];
+    if (nb & 2) s += x[m - nz];
+    if (nb & 4) s += x[m - 1];
+    if (nb & 8) s += x[m + 1];
+    if (nb & 16) s += x[m + nz];
+    if (nb & 32) s += x[m + sx];
     x[m] = s * (1.f / 6.f);
 }
 
-__global__ void k3_resid(const uint8_t *dir, const float *x, const float *b, float *r, int nx, int ny, int nz)
+__global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx, int ny, int nz)
 {
     const size_t n = (size_t)nx * ny * nz;
     size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
-    r[m] = dir[m] ? 0.f : b[m] - apply7(dir, x, m, (size_t)ny * nz, nz);
+    const uint8_t nb = nbm[m];
+    r[m] = (nb & kNbSelf) ? 0.f : b[m] - apply7(nb, x, m, (size_t)ny * nz, nz);
 }
 
 
@@ -856,10 +865,29 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
         V.nx = dims[3 * l]; V.ny = dims[3 * l + 1]; V.nz = dims[3 * l + 2];
         V.n = (size_t)V.nx * V.ny * V.nz;
         M2D_TRY(dmalloc3((void **)&V.dir, V.n));
+        M2D_TRY(dmalloc3((void **)&V.nbm, V.n));
         M2D_TRY(dmalloc3((void **)&V.x, sizeof(float) * V.n));
         M2D_TRY(dmalloc3((void **)&V.b, sizeof(float) * V.n));
         M2D_TRY(dmalloc3((void **)&V.r, sizeof(float) * V.n));
         M2D_HIP(hipMemcpy(V.dir, dirs[l].data(), V.n, hipMemcpyHostToDevice));
+        // neighbour masks (a free node never sits on a face)
+        const std::vector<uint8_t> &d = dirs[l];
+        std::vector<uint8_t> nbm(V.n);
+        const size_t sx = (size_t)V.ny * V.nz;
+        for (int i = 0; i < V.nx; i++)
+            for (int j = 0; j < V.ny; j++)
+                for (int k = 0; k < V.nz; k++) {
+                    const size_t m = (size_t)i * sx + (size_t)j * V.nz + k;
+                    if (d[m]) { nbm[m] = kNbSelf; continue; }
+                    const bool inside = i > 0 && i < V.nx - 1 && j > 0 && j < V.ny - 1 && k > 0 && k < V.nz - 1;
+                    M2D_CHECK(inside, M2D_ERR_BAD_ARG, "Fields3D: free node on a face of level %d", (int)l);
+                    const size_t nb[6] = {m - sx, m - (size_t)V.nz, m - 1, m + 1, m + (size_t)V.nz, m + sx};
+                    uint8_t bits = 0;
+                    for (int q = 0; q < 6; q++)
+                        if (!d[nb[q]]) bits |= (uint8_t)(1u << q);
+                    nbm[m] = bits;
+                }
+        M2D_HIP(hipMemcpy(V.nbm, nbm.data(), V.n, hipMemcpyHostToDevice));
     }
     const Mg3Level &C = f->lv.back();
     size_t nfree = 0;
@@ -905,8 +933,8 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
                      (unsigned)V.nx);
         const dim3 blk(kRbgsX, kRbgsY);
         for (int it = 0; it < sweeps; it++) {
-            k3_rbgs<<<g, blk, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1);
-            k3_rbgs<<<g, blk, 0, s>>>(V.dir, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
+            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1);
+            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
         }
     };
     for (int l = 0; l < L - 1; l++) {
@@ -915,7 +943,7 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         const float *b = l == 0 ? rhs : V.b;
         M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * V.n));
         smooth(V, x, b, kPre3, true);
-        k3_resid<<<blocks_for(V.n), 256, 0, s>>>(V.dir, x, b, V.r, V.nx, V.ny, V.nz);
+        k3_resid<<<blocks_for(V.n), 256, 0, s>>>(V.nbm, x, b, V.r, V.nx, V.ny, V.nz);
         Mg3Level &C = f->lv[l + 1];
         k3_restrict<<<blocks_for(C.n), 256, 0, s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz);
     }
@@ -970,7 +998,7 @@ static int pcg3(m2d_fields3d *f, double rtol, int max_it, double r0, int *iters)
     k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1);
     for (int it = 1; it <= max_it; it++) {
         reduce_launch<1>(rb, f->partial, f->dots + 1, s, [&](int fin) {
-            k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.dir, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
+            k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.nbm, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
                                                   f->dots + 1, fin);
         });
         reduce_launch<1>(rb, f->partial, f->dots + 2, s, [&](int fin) {
@@ -1061,7 +1089,7 @@ static void free3(m2d_fields3d *f)
                     (void *)f->dots, (void *)f->partial, (void *)f->coarse_invT, (void *)f->ticket})
         if (q) (void)hipFree(q);
     for (Mg3Level &V : f->lv)
-        for (void *q : {(void *)V.dir, (void *)V.x, (void *)V.b, (void *)V.r})
+        for (void *q : {(void *)V.dir, (void *)V.nbm, (void *)V.x, (void *)V.b, (void *)V.r})
             if (q) (void)hipFree(q);
     if (f->h_dots) (void)hipHostFree(f->h_dots);
 }

@@ -255,6 +255,8 @@ __device__ __forceinline__ void sum_partials(const double *partial, unsigned nb,
     double t[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) t[q] = 0;
+    // unrolled so that several loads are in flight; the adds keep block order
+#pragma unroll 8
     for (unsigned b = threadIdx.x; b < nb; b += kRedBlock)
 #pragma unroll
         for (int q = 0; q < NV; q++) t[q] += partial[NV * b + q];
