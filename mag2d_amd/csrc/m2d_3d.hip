@@ -44,6 +44,7 @@ struct m2d_fields3d {
     unsigned int *ticket = nullptr;  // block counter of the fused reductions
     int coarse_n = 0;
     bool coarse_dense = true;
+    int lc3 = 1;                   // first level of the one-workgroup coarse V-cycle
     float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
     double *r64 = nullptr, *dots = nullptr, *h_dots = nullptr, *partial = nullptr;
     int npartial = 0;
@@ -163,6 +164,22 @@ __global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *nbm, co
     finish_sums<1>(v, partial, ticket, out, fin);
 }
 
+// Gauss-Seidel update of node m (the node bodies here are shared by the grid
+// kernels and the one-workgroup coarse V-cycle)
+__device__ __forceinline__ void node3_rbgs(const uint8_t *nbm, float *x, const float *b, size_t m, size_t sx, int nz)
+{
+    const uint8_t nb = nbm[m];
+    if (nb & kNbSelf) return;
+    float s = b[m];
+    if (nb & 1) s += x[m - sx];
+    if (nb & 2) s += x[m - nz];
+    if (nb & 4) s += x[m - 1];
+    if (nb & 8) s += x[m + 1];
+    if (nb & 16) s += x[m + nz];
+    if (nb & 32) s += x[m + sx];
+    x[m] = s * (1.f / 6.f);
+}
+
 // red-black Gauss-Seidel half sweep, colour = (i+j+k) & 1.  Threads cover
 // only the nodes of the colour: thread (kk, j, i) updates k = 2 kk + parity,
 // blocks of 64 x 4 (kk, j) and a grid of (nz/2 / 64, ny / 4, nx).
@@ -174,17 +191,7 @@ __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, f
     const int i = blockIdx.z, j = blockIdx.y * kRbgsY + threadIdx.y;
     const int k = 2 * (blockIdx.x * kRbgsX + threadIdx.x) + ((i + j + colour) & 1);
     if (j >= ny || k >= nz) return;
-    const size_t m = (size_t)i * sx + (size_t)j * nz + k;
-    const uint8_t nb = nbm[m];
-    if (nb & kNbSelf) return;
-    float s = b[m];
-    if (nb & 1) s += x[m - sx];
-    if (nb & 2) s += x[m - nz];
-    if (nb & 4) s += x[m - 1];
-    if (nb & 8) s += x[m + 1];
-    if (nb & 16) s += x[m + nz];
-    if (nb & 32) s += x[m + sx];
-    x[m] = s * (1.f / 6.f);
+    node3_rbgs(nbm, x, b, (size_t)i * sx + (size_t)j * nz + k, sx, nz);
 }
 
 __global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx, int ny, int nz)
@@ -200,14 +207,11 @@ __global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, flo
 // 1D weights of the trilinear prolongation: fine i from coarse I
 __device__ __forceinline__ float w1(int i, int I) { int d = i - 2 * I; return d == 0 ? 1.f : (d == 1 || d == -1 ? 0.5f : 0.f); }
 
-// b_c = P^T r_f / 2 on free coarse nodes
-__global__ void k3_restrict(const float *rf, int fx, int fy, int fz, float *bc, const uint8_t *dirc, int cx,
-                            int cy, int cz)
+// b_c = P^T r_f / 2 at free coarse node M
+__device__ __forceinline__ float node3_restrict(const float *rf, int fx, int fy, int fz, const uint8_t *dirc, int cy,
+                                                int cz, size_t M)
 {
-    const size_t N = (size_t)cx * cy * cz;
-    size_t M = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (M >= N) return;
-    if (dirc[M]) { bc[M] = 0.f; return; }
+    if (dirc[M]) return 0.f;
     int I, J, K;
     node3(M, cy, cz, I, J, K);
     float s = 0.f;
@@ -222,16 +226,22 @@ __global__ void k3_restrict(const float *rf, int fx, int fy, int fz, float *bc, 
             }
         }
     }
-    bc[M] = 0.5f * s;
+    return 0.5f * s;
 }
 
-// x_f += P x_c on free fine nodes
-__global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int fz, const float *xc, int cx,
-                           int cy, int cz)
+__global__ void k3_restrict(const float *rf, int fx, int fy, int fz, float *bc, const uint8_t *dirc, int cx,
+                            int cy, int cz)
 {
-    const size_t n = (size_t)fx * fy * fz;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n || dirf[m]) return;
+    const size_t N = (size_t)cx * cy * cz;
+    size_t M = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (M >= N) return;
+    bc[M] = node3_restrict(rf, fx, fy, fz, dirc, cy, cz, M);
+}
+
+// x_f += P x_c at fine node m (a free node)
+__device__ __forceinline__ void node3_prolong_free(float *xf, int fy, int fz, const float *xc, int cx, int cy, int cz,
+                                                   size_t m)
+{
     int i, j, k;
     node3(m, fy, fz, i, j, k);
     int Ia[2], Ja[2], Ka[2];
@@ -254,6 +264,215 @@ __global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int f
             for (int c = 0; c < nk; c++)
                 s = fmaf(wa[a] * wb[b] * wc[c], xc[((size_t)Ia[a] * cy + Ja[b]) * cz + Ka[c]], s);
     xf[m] += s;
+}
+
+// x_f += P x_c at fine node m when it is free
+__device__ __forceinline__ void node3_prolong(float *xf, const uint8_t *dirf, int fy, int fz, const float *xc, int cx,
+                                              int cy, int cz, size_t m)
+{
+    if (!dirf[m]) node3_prolong_free(xf, fy, fz, xc, cx, cy, cz, m);
+}
+
+// x_f += P x_c on free fine nodes
+__global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int fz, const float *xc, int cx,
+                           int cy, int cz)
+{
+    const size_t n = (size_t)fx * fy * fz;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    node3_prolong(xf, dirf, fy, fz, xc, cx, cy, cz, m);
+}
+
+// ---- the coarse levels [lc, L) of the V-cycle in one workgroup: the same
+// node bodies and sweep order as the grid kernels, a workgroup barrier
+// between sweeps instead of a launch
+constexpr int kMaxLevels3 = 16;
+constexpr size_t kBlockLevelNodes3 = 4096;  // levels this small run inside the workgroup
+
+struct BLevel3 {
+    int nx, ny, nz, n;
+    const uint8_t *dir, *nbm;
+    float *x, *b, *r;
+};
+
+struct Coarse3Args {
+    int lc, L;
+    BLevel3 lv[kMaxLevels3];
+    int dense;          // dense coarsest solve (else sweeps symmetric RB-GS sweeps)
+    const float *invT;  // the dense inverse, transposed
+    int coarse_n, sweeps, pre, post;
+};
+
+__device__ void c3_smooth(const BLevel3 &V, float *x, const float *b, int sweeps, bool forward)
+{
+    const size_t sx = (size_t)V.ny * V.nz;
+    for (int it = 0; it < sweeps; it++) {
+        for (int h = 0; h < 2; h++) {
+            const int colour = forward ? h : 1 - h;
+            for (int m = threadIdx.x; m < V.n; m += kCoarseThreads) {
+                int i, j, k;
+                node3((size_t)m, V.ny, V.nz, i, j, k);
+                if (((i + j + k) & 1) == colour) node3_rbgs(V.nbm, x, b, (size_t)m, sx, V.nz);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kCoarseThreads) k3_vcycle_coarse(Coarse3Args C)
+{
+    extern __shared__ float sb[];  // dense solve: coarse_n words, then kDenseParts partial rows
+    for (int l = C.lc; l < C.L - 1; l++) {
+        const BLevel3 &V = C.lv[l];
+        for (int m = threadIdx.x; m < V.n; m += kCoarseThreads) V.x[m] = 0.f;
+        __syncthreads();
+        c3_smooth(V, V.x, V.b, C.pre, true);
+        const size_t sx = (size_t)V.ny * V.nz;
+        for (int m = threadIdx.x; m < V.n; m += kCoarseThreads) {
+            const uint8_t nb = V.nbm[m];
+            V.r[m] = (nb & kNbSelf) ? 0.f : V.b[m] - apply7(nb, V.x, (size_t)m, sx, V.nz);
+        }
+        __syncthreads();
+        const BLevel3 &W = C.lv[l + 1];
+        for (int M = threadIdx.x; M < W.n; M += kCoarseThreads)
+            W.b[M] = node3_restrict(V.r, V.nx, V.ny, V.nz, W.dir, W.ny, W.nz, (size_t)M);
+        __syncthreads();
+    }
+    {
+        const BLevel3 &Z = C.lv[C.L - 1];
+        if (C.dense) {
+            const int n = C.coarse_n;
+            float *part = sb + n;
+            for (int k = threadIdx.x; k < n; k += kCoarseThreads) sb[k] = Z.b[k];
+            __syncthreads();
+            const int len = (n + kDenseParts - 1) / kDenseParts;
+            for (int t = threadIdx.x; t < n * kDenseParts; t += kCoarseThreads) {
+                const int row = t % n, p = t / n;
+                const int c0 = p * len, c1 = min(n, c0 + len);
+                float s = 0.f;
+                int c = c0;
+                for (; c + 8 <= c1; c += 8) {
+                    float a[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) a[q] = C.invT[(size_t)(c + q) * n + row];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) s = fmaf(a[q], sb[c + q], s);
+                }
+                for (; c < c1; c++) s = fmaf(C.invT[(size_t)c * n + row], sb[c], s);
+                part[p * n + row] = s;
+            }
+            __syncthreads();
+            for (int row = threadIdx.x; row < n; row += kCoarseThreads) {
+                float s = part[row];
+                for (int p = 1; p < kDenseParts; p++) s += part[p * n + row];
+                Z.x[row] = s;
+            }
+            __syncthreads();
+        } else {
+            for (int m = threadIdx.x; m < Z.n; m += kCoarseThreads) Z.x[m] = 0.f;
+            __syncthreads();
+            c3_smooth(Z, Z.x, Z.b, C.sweeps / 2, true);
+            c3_smooth(Z, Z.x, Z.b, C.sweeps / 2, false);
+        }
+    }
+    for (int l = C.L - 2; l >= C.lc; l--) {
+        const BLevel3 &V = C.lv[l];
+        const BLevel3 &W = C.lv[l + 1];
+        for (int m = threadIdx.x; m < V.n; m += kCoarseThreads)
+            node3_prolong(V.x, V.dir, V.ny, V.nz, W.x, W.nx, W.ny, W.nz, (size_t)m);
+        __syncthreads();
+        c3_smooth(V, V.x, V.b, C.post, false);
+    }
+}
+
+// The common case -- one red-black level of at most 4096 nodes above a dense
+// level of at most 512 -- with both levels in LDS and each thread's nodes and
+// their colours computed once.  Same node updates and order, same bits.
+constexpr int kLds3Nodes = 4096, kLds3Dense = 512, kLds3PerThread = kLds3Nodes / kCoarseThreads;
+
+__global__ void __launch_bounds__(kCoarseThreads) k3_vcycle_coarse_lds(Coarse3Args C)
+{
+    __shared__ float s_x[kLds3Nodes], s_b[kLds3Nodes], s_r[kLds3Nodes];
+    __shared__ uint8_t s_nb[kLds3Nodes];
+    __shared__ float s_zb[kLds3Dense], s_zx[kLds3Dense];
+    const BLevel3 &V = C.lv[C.lc];
+    const BLevel3 &Z = C.lv[C.lc + 1];
+    const int n = V.n, ny = V.ny, nz = V.nz, cn = C.coarse_n;
+    const size_t sx = (size_t)ny * nz;
+    int par[kLds3PerThread];
+#pragma unroll
+    for (int q = 0; q < kLds3PerThread; q++) {
+        const int m = threadIdx.x + q * kCoarseThreads;
+        par[q] = 2;
+        if (m < n) {
+            int i, j, k;
+            node3((size_t)m, ny, nz, i, j, k);
+            par[q] = (i + j + k) & 1;
+            s_nb[m] = V.nbm[m];
+            s_b[m] = V.b[m];
+            s_x[m] = 0.f;
+        }
+    }
+    __syncthreads();
+    auto sweep = [&](int colour) {
+#pragma unroll
+        for (int q = 0; q < kLds3PerThread; q++)
+            if (par[q] == colour) node3_rbgs(s_nb, s_x, s_b, (size_t)(threadIdx.x + q * kCoarseThreads), sx, nz);
+        __syncthreads();
+    };
+    for (int it = 0; it < C.pre; it++) { sweep(0); sweep(1); }
+#pragma unroll
+    for (int q = 0; q < kLds3PerThread; q++) {
+        const int m = threadIdx.x + q * kCoarseThreads;
+        if (m < n) {
+            const uint8_t nb = s_nb[m];
+            s_r[m] = (nb & kNbSelf) ? 0.f : s_b[m] - apply7(nb, s_x, (size_t)m, sx, nz);
+        }
+    }
+    __syncthreads();
+    for (int M = threadIdx.x; M < cn; M += kCoarseThreads)
+        s_zb[M] = node3_restrict(s_r, V.nx, ny, nz, Z.dir, Z.ny, Z.nz, (size_t)M);
+    __syncthreads();
+    {
+        // the dense solve of k3_vcycle_coarse; its partial rows reuse s_r
+        float *part = s_r;
+        const int len = (cn + kDenseParts - 1) / kDenseParts;
+        for (int t = threadIdx.x; t < cn * kDenseParts; t += kCoarseThreads) {
+            const int row = t % cn, p = t / cn;
+            const int c0 = p * len, c1 = min(cn, c0 + len);
+            float s = 0.f;
+            int c = c0;
+            for (; c + 8 <= c1; c += 8) {
+                float a[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = C.invT[(size_t)(c + q) * cn + row];
+#pragma unroll
+                for (int q = 0; q < 8; q++) s = fmaf(a[q], s_zb[c + q], s);
+            }
+            for (; c < c1; c++) s = fmaf(C.invT[(size_t)c * cn + row], s_zb[c], s);
+            part[p * cn + row] = s;
+        }
+        __syncthreads();
+        for (int row = threadIdx.x; row < cn; row += kCoarseThreads) {
+            float s = part[row];
+            for (int p = 1; p < kDenseParts; p++) s += part[p * cn + row];
+            s_zx[row] = s;
+        }
+        __syncthreads();
+    }
+    // node3_prolong with the pinned flag from the neighbour mask
+#pragma unroll
+    for (int q = 0; q < kLds3PerThread; q++) {
+        const int m = threadIdx.x + q * kCoarseThreads;
+        if (m < n && !(s_nb[m] & kNbSelf)) node3_prolong_free(s_x, ny, nz, s_zx, Z.nx, Z.ny, Z.nz, (size_t)m);
+    }
+    __syncthreads();
+    for (int it = 0; it < C.post; it++) { sweep(1); sweep(0); }
+#pragma unroll
+    for (int q = 0; q < kLds3PerThread; q++) {
+        const int m = threadIdx.x + q * kCoarseThreads;
+        if (m < n) V.x[m] = s_x[m];
+    }
 }
 
 // ------------------------------------------------------------------ point kernels (FP64)
@@ -889,6 +1108,10 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
                 }
         M2D_HIP(hipMemcpy(V.nbm, nbm.data(), V.n, hipMemcpyHostToDevice));
     }
+    M2D_CHECK(L <= (size_t)kMaxLevels3, M2D_ERR_BAD_ARG, "Fields3D: %d multigrid levels", (int)L);
+    f->lc3 = (int)L - 1;
+    for (size_t l = 1; l < L; l++)
+        if (f->lv[l].n <= kBlockLevelNodes3) { f->lc3 = (int)l; break; }
     const Mg3Level &C = f->lv.back();
     size_t nfree = 0;
     for (uint8_t d : dirs.back()) nfree += !d;
@@ -937,7 +1160,9 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
             k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
         }
     };
-    for (int l = 0; l < L - 1; l++) {
+    // levels [lc, L) run in one workgroup (k3_vcycle_coarse) when lc > 0
+    const int lc = L > 1 ? f->lc3 : L;
+    for (int l = 0; l < std::min(lc, L - 1); l++) {
         Mg3Level &V = f->lv[l];
         float *x = l == 0 ? out : V.x;
         const float *b = l == 0 ? rhs : V.b;
@@ -947,7 +1172,27 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         Mg3Level &C = f->lv[l + 1];
         k3_restrict<<<blocks_for(C.n), 256, 0, s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz);
     }
-    {
+    if (lc < L) {
+        Coarse3Args C;
+        C.lc = lc;
+        C.L = L;
+        for (int l = 0; l < L; l++) {
+            const Mg3Level &V = f->lv[l];
+            C.lv[l] = BLevel3{V.nx, V.ny, V.nz, (int)V.n, V.dir, V.nbm, V.x, V.b, V.r};
+        }
+        C.dense = f->coarse_dense;
+        C.invT = f->coarse_invT;
+        C.coarse_n = f->coarse_n;
+        C.sweeps = kCoarseSweeps;
+        C.pre = kPre3;
+        C.post = kPost3;
+        if (L - 1 - lc == 1 && f->lv[lc].n <= (size_t)kLds3Nodes && C.dense && f->coarse_n <= kLds3Dense) {
+            k3_vcycle_coarse_lds<<<1, kCoarseThreads, 0, s>>>(C);
+        } else {
+            const size_t shm = C.dense ? sizeof(float) * f->coarse_n * (1 + kDenseParts) : 0;
+            k3_vcycle_coarse<<<1, kCoarseThreads, shm, s>>>(C);
+        }
+    } else {
         Mg3Level &C = f->lv[L - 1];
         float *x = L == 1 ? out : C.x;
         const float *b = L == 1 ? rhs : C.b;
@@ -960,7 +1205,7 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
             smooth(C, x, b, kCoarseSweeps / 2, false);
         }
     }
-    for (int l = L - 2; l >= 0; l--) {
+    for (int l = std::min(lc, L - 1) - 1; l >= 0; l--) {
         Mg3Level &V = f->lv[l];
         Mg3Level &C = f->lv[l + 1];
         float *x = l == 0 ? out : V.x;
