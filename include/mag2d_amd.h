@@ -325,6 +325,8 @@ typedef struct {
                                     the pushes of one K-step segment from its checkpoint */
     int32_t n_groups;            /* electrode voltages differentiated, <= 64 */
     const int32_t *group;        /* host, nx*nz: group of each FIXED node, -1 for none */
+    double adjoint_tol;          /* FP64 relative residual of the adjoint solves; 0: 1e-10, the
+                                    forward solves' (the forward run always solves to 1e-10) */
 } m2d_adjoint_cfg;
 
 typedef struct {

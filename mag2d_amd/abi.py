@@ -98,7 +98,7 @@ class StepStats(C.Structure):
 
 class AdjointCfg(C.Structure):
     _fields_ = [("steps", C.c_int32), ("checkpoint_every", C.c_int32), ("n_groups", C.c_int32),
-                ("group", C.POINTER(C.c_int32))]
+                ("group", C.POINTER(C.c_int32)), ("adjoint_tol", C.c_double)]
 
 
 class AdjointInfo(C.Structure):
@@ -751,12 +751,13 @@ def grad_3d(ctx, g, data_ptr, x, y, z):
 # --------------------------------------------------------------------- adjoint
 
 def adjoint_energy(ctx, fields: Fields2D, species: Species2D, parts: ParticlesSoA, steps, group=None,
-                   n_groups=0, checkpoint_every=100):
+                   n_groups=0, checkpoint_every=100, adjoint_tol=0.0):
     """J (mean kinetic energy in eV after `steps` self-consistent steps) and,
-    when `group` (nx*nz int array, -1 for none) is given, dJ/dV per group."""
+    when `group` (nx*nz int array, -1 for none) is given, dJ/dV per group;
+    adjoint_tol: relative residual of the adjoint solves (0: 1e-10)."""
     g = None if group is None else np.ascontiguousarray(group, np.int32)
     cfg = AdjointCfg(steps, checkpoint_every, n_groups,
-                     g.ctypes.data_as(C.POINTER(C.c_int32)) if g is not None else None)
+                     g.ctypes.data_as(C.POINTER(C.c_int32)) if g is not None else None, adjoint_tol)
     J = C.c_double()
     grad = np.zeros(max(n_groups, 1))
     info = AdjointInfo()

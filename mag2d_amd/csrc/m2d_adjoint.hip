@@ -293,6 +293,9 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
               "m2d_adjoint_energy_2d: 1..%d electrode groups with a node map", kMaxGroups);
     M2D_CHECK(p->n > 0 && p->x && p->z && p->vx && p->vy && p->vz, M2D_ERR_BAD_ARG,
               "m2d_adjoint_energy_2d: empty particle store");
+    M2D_CHECK(cfg->adjoint_tol >= 0 && cfg->adjoint_tol < 1, M2D_ERR_BAD_ARG,
+              "m2d_adjoint_energy_2d: adjoint_tol %g out of [0, 1)", cfg->adjoint_tol);
+    const double adj_tol = cfg->adjoint_tol > 0 ? cfg->adjoint_tol : 1e-10;
     const int N = cfg->steps, K = cfg->checkpoint_every;
     const size_t nc = f->n, np = (size_t)p->n;
     hipStream_t st = ctx->stream;
@@ -432,7 +435,7 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
             // A_FF y = lu_F
             k_adj_rhs<<<gb, 256, 0, st>>>(f->mask, lu.as<double>(), af->rho, nc);
             M2D_HIP(hipGetLastError());
-            M2D_TRY(m2d_fields2d_solve(af, M2D_SOLVE_RAW, 1e-10, &si));
+            M2D_TRY(m2d_fields2d_solve(af, M2D_SOLVE_RAW, adj_tol, &si));
             k_adj_groups<<<gb, 256, 0, st>>>(f->mask, dgroup.as<int>(), lu.as<double>(), af->u, f->nx, f->nz,
                                              cfg->n_groups, gpart.as<double>());
             k_adj_groups_sum<<<1, 64, 0, st>>>(gpart.as<double>(), (int)gb, cfg->n_groups, dgrad.as<double>());

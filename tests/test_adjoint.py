@@ -96,7 +96,9 @@ def test_adjoint_gradient_matches_fp64_finite_differences(ctx, K):
     sp = abi.Species2D(-QE, M_E, dt, float("inf"), 0, 0, 0)
     zero = np.zeros(n * n)
     fields.upload(abi.FIELD_U, zero)  # the solves warm-start from u: start both runs alike
-    J, grad, info = abi.adjoint_energy(ctx, fields, sp, P, steps, group=group, n_groups=8, checkpoint_every=K)
+    # the adjoint solves at 1e-8, as the benchmark runs them (the forward run solves at 1e-10)
+    J, grad, info = abi.adjoint_energy(ctx, fields, sp, P, steps, group=group, n_groups=8, checkpoint_every=K,
+                                       adjoint_tol=1e-8)
     assert ctx.status(clear=True) == 0
     # the store ends as a plain forward run leaves it
     P2 = abi.ParticlesSoA.from_host(ctx, x, z, v[0], v[1], v[2])
