@@ -280,6 +280,7 @@ def run_penning2d(args):
     elapsed, kern_ms, launches = _timed(ctx, one_step, steps, warmup)
     live = abi.count_live(ctx, bufs[cur[0]])
     inner = [s.inner_iters for s in solve_info[warmup:]]
+    pass_us = kern_ms / max(launches, 1) * 1e3
     out = {
         "metric": "particle-steps/s, self-consistent 2D Penning trap (solve every step)",
         "value": live * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
@@ -288,7 +289,7 @@ def run_penning2d(args):
         "config": {"workload": "2D Penning trap, 100k electrons, 128x128, Bz 0.1 T, end caps -10 V, "
                                "Poisson every step", "particles": N, "live_particles_end": live,
                    "grid": [n, n], "dt": 1e-11, "sort_every": args.sort_every},
-        "particle_pass_us": kern_ms / max(launches, 1) * 1e3,
+        "particle_pass_us": pass_us,
         "poisson": {"tol": 1e-10, "mean_inner_pcg_iters": float(np.mean(inner)),
                     "max_rel_residual": max(s.rel_residual for s in solve_info[warmup:])},
         "device_flags": ctx.status(clear=True),

@@ -1568,23 +1568,15 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
         partials = (unsigned long long *)ctx->partials.ptr;
     }
     if (p->n > 0) {
-        hipEvent_t t0 = prof_begin(ctx);
         if (T) {
-            if (use_mcc)
-                k_advance3d_tiled<true><<<T->max_chunks, 256, 0, ctx->stream>>>(a, T->sched, T->n_chunks, nty3, ntz3,
-                                                                               partials);
-            else
-                k_advance3d_tiled<false><<<T->max_chunks, 256, 0, ctx->stream>>>(a, T->sched, T->n_chunks, nty3,
-                                                                                ntz3, partials);
+            launch_timed(ctx, use_mcc ? k_advance3d_tiled<true> : k_advance3d_tiled<false>, dim3(T->max_chunks),
+                         dim3(256), a, T->sched, T->n_chunks, nty3, ntz3, partials);
         } else {
-            const unsigned g = blocks_for((size_t)p->n);
-            if (deposit && use_mcc) k_advance3d<true, true><<<g, 256, 0, ctx->stream>>>(a);
-            else if (deposit) k_advance3d<true, false><<<g, 256, 0, ctx->stream>>>(a);
-            else if (use_mcc) k_advance3d<false, true><<<g, 256, 0, ctx->stream>>>(a);
-            else k_advance3d<false, false><<<g, 256, 0, ctx->stream>>>(a);
+            void (*kern)(PushArgs3) = deposit ? (use_mcc ? k_advance3d<true, true> : k_advance3d<true, false>)
+                                              : (use_mcc ? k_advance3d<false, true> : k_advance3d<false, false>);
+            launch_timed(ctx, kern, dim3(blocks_for((size_t)p->n)), dim3(256), a);
         }
         M2D_HIP(hipGetLastError());
-        prof_end(ctx, t0);
         if (use_mcc) {
             k_scatter3d<<<64, 256, 0, ctx->stream>>>(a, next_len);
             M2D_HIP(hipGetLastError());

@@ -100,21 +100,18 @@ static hipEvent_t pool_get(m2d_ctx *ctx)
     return e;
 }
 
-hipEvent_t prof_begin(m2d_ctx *ctx)
+bool prof_pair(m2d_ctx *ctx, hipEvent_t &start, hipEvent_t &stop)
 {
-    if (!ctx->profile) return nullptr;
-    hipEvent_t e = pool_get(ctx);
-    if (e) (void)hipEventRecord(e, ctx->stream);
-    return e;
-}
-
-void prof_end(m2d_ctx *ctx, hipEvent_t start)
-{
-    if (!start) return;
-    hipEvent_t e = pool_get(ctx);
-    if (!e) return;
-    (void)hipEventRecord(e, ctx->stream);
-    ctx->ev_pending.push_back({start, e});
+    if (!ctx->profile) return false;
+    start = pool_get(ctx);
+    stop = pool_get(ctx);
+    if (!start || !stop) {
+        if (start) ctx->ev_pool.push_back(start);
+        if (stop) ctx->ev_pool.push_back(stop);
+        return false;
+    }
+    ctx->ev_pending.push_back({start, stop});
+    return true;
 }
 
 }  // namespace m2d

@@ -1,6 +1,7 @@
 // m2d_internal.hpp -- shared internals of libmag2d_amd (HIP, gfx950).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <cmath>
 #include <cstdio>
@@ -113,8 +114,20 @@ void free_tilings(m2d_ctx *ctx);
 void ctx_retain(m2d_ctx *ctx);
 void ctx_release(m2d_ctx *ctx);
 // bracket a launch with events when profiling is on
-hipEvent_t prof_begin(m2d_ctx *ctx);
-void prof_end(m2d_ctx *ctx, hipEvent_t start);
+// Kernel timing (m2d_ctx_profile): launch_timed launches through
+// hipExtLaunchKernel with start and stop events that the runtime stamps at the
+// kernel's own dispatch and completion, so the pair holds the kernel alone
+// (an event pair recorded around a launch can also hold the wait of an idle
+// queue for the launch).
+bool prof_pair(m2d_ctx *ctx, hipEvent_t &start, hipEvent_t &stop);
+
+template <typename... Args, typename F = void (*)(Args...)>
+inline void launch_timed(m2d_ctx *ctx, F kernel, dim3 grid, dim3 block, Args... args)
+{
+    hipEvent_t e0, e1;
+    if (prof_pair(ctx, e0, e1)) hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+}
 }  // namespace m2d
 
 namespace m2d {

@@ -897,36 +897,31 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         a.ntz = T->ntz;
     }
     if (p->n > 0) {
-        hipEvent_t t0 = prof_begin(ctx);
         // the common configuration runs the compile-time specialised pass
         const bool fast = a.push && a.boundary && a.checkfree && !a.rotate && !a.periodic && !a.econst &&
                           a.xmin == 0.0 && a.zmin == 0.0;
-        hipStream_t st = ctx->stream;
+        void (*kern)(PushArgs) = nullptr;
+        int g = 0;
         if (T) {
-            const int g = T->max_chunks;
-            if (fast) {
-                if (use_mcc) k_advance2d<true, true, true, true><<<g, kPass, 0, st>>>(a);
-                else k_advance2d<true, false, true, true><<<g, kPass, 0, st>>>(a);
-            } else {
-                if (use_mcc) k_advance2d<true, true, true, false><<<g, kPass, 0, st>>>(a);
-                else k_advance2d<true, false, true, false><<<g, kPass, 0, st>>>(a);
-            }
+            g = T->max_chunks;
+            if (fast) kern = use_mcc ? k_advance2d<true, true, true, true> : k_advance2d<true, false, true, true>;
+            else kern = use_mcc ? k_advance2d<true, true, true, false> : k_advance2d<true, false, true, false>;
         } else {
-            const int g = (int)((p->n + kChunk - 1) / kChunk);
+            g = (int)((p->n + kChunk - 1) / kChunk);
             if (fast) {
-                if (deposit && use_mcc) k_advance2d<true, true, false, true><<<g, kPass, 0, st>>>(a);
-                else if (deposit) k_advance2d<true, false, false, true><<<g, kPass, 0, st>>>(a);
-                else if (use_mcc) k_advance2d<false, true, false, true><<<g, kPass, 0, st>>>(a);
-                else k_advance2d<false, false, false, true><<<g, kPass, 0, st>>>(a);
+                if (deposit && use_mcc) kern = k_advance2d<true, true, false, true>;
+                else if (deposit) kern = k_advance2d<true, false, false, true>;
+                else if (use_mcc) kern = k_advance2d<false, true, false, true>;
+                else kern = k_advance2d<false, false, false, true>;
             } else {
-                if (deposit && use_mcc) k_advance2d<true, true, false, false><<<g, kPass, 0, st>>>(a);
-                else if (deposit) k_advance2d<true, false, false, false><<<g, kPass, 0, st>>>(a);
-                else if (use_mcc) k_advance2d<false, true, false, false><<<g, kPass, 0, st>>>(a);
-                else k_advance2d<false, false, false, false><<<g, kPass, 0, st>>>(a);
+                if (deposit && use_mcc) kern = k_advance2d<true, true, false, false>;
+                else if (deposit) kern = k_advance2d<true, false, false, false>;
+                else if (use_mcc) kern = k_advance2d<false, true, false, false>;
+                else kern = k_advance2d<false, false, false, false>;
             }
         }
+        launch_timed(ctx, kern, dim3(g), dim3(kPass), a);
         M2D_HIP(hipGetLastError());
-        prof_end(ctx, t0);
     }
     if (deposit || use_mcc) {
         // the grid flush and the collision queue in one launch
