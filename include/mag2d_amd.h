@@ -46,6 +46,7 @@ extern "C" {
 #define M2D_FLAG_OUT_OF_RANGE  1u  /* a gather/deposit index left the grid (value clamped / skipped) */
 #define M2D_FLAG_NAN           2u  /* a live particle had a NaN position and was removed */
 #define M2D_FLAG_PARTNER       4u  /* MCC found no live partner particle in the retry budget */
+#define M2D_FLAG_EVENTS        8u  /* the multicoll mover hit its per-particle event budget in a step */
 
 /* mask values (fields.hpp:20) */
 #define M2D_FIXED    0
@@ -284,6 +285,27 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
 /* Species<CARTESIAN>::advance_boris_init (particles.cpp:997-1050) */
 int m2d_advance_init_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
                         const m2d_particles2d *p);
+/* Species<CARTESIAN>::advance_multicoll (particles.cpp:813-861), the mover
+ * of Param::ADVANCE_MULTICOLL: every live particle flies in the constant
+ * field of Fields::E(0, 0) (fields.hpp:126-131) from one collision event to
+ * the next inside the step; each event calls BaseSpecies::scatter (`mcc`
+ * NULL: a null collision) and draws time_to_death = lifetime * Exp(1).
+ * time_to_death[p->n] (FP64, device) is t_particle::time_to_death
+ * (particles.hpp:31), per slot.  Like the reference it requires B = 0, the
+ * EMPTY geometry and selfconsistent = 0 (M2D_ERR_UNSUPPORTED otherwise), and
+ * reproduces its position update x += (v + a t/2) t with v already kicked.
+ * dt / lifetime must be at most 256; a particle with more than 4096 events
+ * in one step raises M2D_FLAG_EVENTS and flies the rest of the step freely.
+ * advance_boundary is a separate call (m2d_advance_2d with BOUNDARY only).
+ * stats->collisions counts events.  Does not increment niter. */
+int m2d_advance_multicoll_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
+                             const m2d_particles2d *p, double *time_to_death, const m2d_mcc *mcc,
+                             m2d_step_stats *stats);
+/* The time_to_death draw of BaseSpecies::lifetime_init (particles.cpp:163-169):
+ * time_to_death = s->lifetime * Exp(1) for the live slots of p (empty slots
+ * keep theirs).  Draws are keyed by (slot, s->niter, s->stream). */
+int m2d_time_to_death_init_2d(m2d_ctx *ctx, const m2d_species2d *s, const m2d_particles2d *p,
+                              double *time_to_death);
 /* energy_dist.mean_tot() after energy_dist_compute (particles.cpp:408-414,
  * histogram.cpp:68-71): mean kinetic energy in eV over live particles */
 int m2d_mean_energy_2d(m2d_ctx *ctx, const m2d_particles2d *p, double mass, double *mean_eV,

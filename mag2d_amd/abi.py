@@ -18,7 +18,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "libmag2d_amd.so"
 M2D_OK = 0
 STATUS = {-1: "BAD_ARG", -2: "OUT_OF_RANGE", -3: "NAN", -4: "HIP_ERROR", -5: "NOT_CONVERGED",
           -6: "UNSUPPORTED", -7: "NO_MEMORY"}
-FLAG_OUT_OF_RANGE, FLAG_NAN, FLAG_PARTNER = 1, 2, 4
+FLAG_OUT_OF_RANGE, FLAG_NAN, FLAG_PARTNER, FLAG_EVENTS = 1, 2, 4, 8
 FIXED, FIXED_RF, FREE, BOUNDARY = 0, 1, 2, 3
 DEAD_BITS = 0xFFC0DEAD
 (GEO_EMPTY, GEO_PROBE, GEO_RF_22PT, GEO_RF_8PT, GEO_RF_HAITRAP, GEO_RF_QUAD, GEO_MAC,
@@ -178,6 +178,9 @@ _SIGS = {
     "m2d_advance_2d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D), _P, _P,
                                  C.c_uint32, C.POINTER(StepStats)]),
     "m2d_advance_init_2d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D)]),
+    "m2d_advance_multicoll_2d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D), _P, _P,
+                                           C.POINTER(StepStats)]),
+    "m2d_time_to_death_init_2d": (C.c_int, [_P, C.POINTER(Species2D), C.POINTER(Particles2D), _P]),
     "m2d_mean_energy_2d": (C.c_int, [_P, C.POINTER(Particles2D), C.c_double, _PD,
                                      C.POINTER(C.c_int64)]),
     "m2d_count_live_2d": (C.c_int, [_P, C.POINTER(Particles2D), C.POINTER(C.c_int64)]),
@@ -515,6 +518,22 @@ def advance(ctx, fields, species: Species2D, parts: ParticlesSoA, mcc=None, rho_
 def advance_init(ctx, fields, species, parts):
     s = parts.struct()
     check(lib().m2d_advance_init_2d(ctx.h, fields.h, C.byref(species), C.byref(s)))
+
+
+def advance_multicoll(ctx, fields, species: Species2D, parts: ParticlesSoA, ttd_ptr: int, mcc=None,
+                      want_stats=False):
+    """Species<CARTESIAN>::advance_multicoll; ttd_ptr: device FP64 time_to_death per slot."""
+    st = StepStats()
+    s = parts.struct()
+    check(lib().m2d_advance_multicoll_2d(ctx.h, fields.h, C.byref(species), C.byref(s), C.c_void_p(ttd_ptr),
+                                         mcc.h if mcc is not None else None,
+                                         C.byref(st) if want_stats else None))
+    return st
+
+
+def time_to_death_init(ctx, species: Species2D, parts: ParticlesSoA, ttd_ptr: int):
+    s = parts.struct()
+    check(lib().m2d_time_to_death_init_2d(ctx.h, C.byref(species), C.byref(s), C.c_void_p(ttd_ptr)))
 
 
 def accumulate(ctx, g: Grid2D, parts: ParticlesSoA, charge, rho_ptr):

@@ -25,6 +25,9 @@ constexpr int kBlock = 256;                  // threads per block of the particl
 constexpr uint32_t kPurposeMcc = 0;    // scatter draws of one slot: counter (slot, step, block)
 constexpr uint32_t kPurposeInit = 1;   // initial sampling of one slot
 constexpr uint32_t kPurposeTrial = 2;  // MCC trial: counter (slot/4, step, 0), word slot%4
+constexpr uint32_t kPurposeMulticoll = 3;  // multicoll events of one slot in one step: the
+                                           // time_to_death draws and scatter, in event order
+constexpr uint32_t kPurposeTtd = 4;    // the first time_to_death of one slot
 
 constexpr int kTile = 8;               // cells per side of the sort tile
 constexpr int kTileShift = 6;          // key = tile << 6 | cell in tile
@@ -151,6 +154,8 @@ int ensure_acc(m2d_ctx *ctx, size_t n);
 int fill_bytes(m2d_ctx *ctx, void *dst, int value, size_t bytes);
 int zero_words(m2d_ctx *ctx, unsigned long long *dst, size_t n);
 int read_counters(m2d_ctx *ctx);
+// null, count, non-null and 16-byte aligned arrays of a 2D store
+int check_parts(const m2d_particles2d *p);
 
 // ---------------------------------------------------------------- device helpers
 __device__ __forceinline__ bool is_dead(float x) { return __float_as_uint(x) == M2D_DEAD_BITS; }

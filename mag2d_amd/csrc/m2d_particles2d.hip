@@ -750,7 +750,7 @@ __global__ void __launch_bounds__(kBlock) k_disk2d(m2d_particles2d P, double cx,
 }
 
 // ------------------------------------------------------------------ host helpers
-static int check_parts(const m2d_particles2d *p)
+int check_parts(const m2d_particles2d *p)
 {
     M2D_CHECK(p, M2D_ERR_BAD_ARG, "particles is NULL");
     M2D_CHECK(p->n >= 0 && p->n < (1ll << 32), M2D_ERR_BAD_ARG, "particle count %lld out of range",

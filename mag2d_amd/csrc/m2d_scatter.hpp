@@ -76,11 +76,11 @@ __device__ inline void deflect(Rng &r, double angle, double &x, double &y, doubl
 }
 
 // BaseSpecies::scatter (particles.cpp:208-365), FP64, same draw order as
-// the reference: species pick, partner, interaction pick, angles.
-__device__ inline void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, float &vzf,
-                        unsigned long long *counters)
+// the reference: species pick, partner, interaction pick, angles.  The
+// velocity changes only when an interaction fires.
+__device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy, double &vz,
+                                 unsigned long long *counters)
 {
-    double vx = vxf, vy = vyf, vz = vzf;
     double gamma = r.uni() / m.lifetime;
     double tmp = 0.0;
     int sid;
@@ -163,6 +163,14 @@ __device__ inline void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, f
     default:
         return;
     }
+}
+
+// the same on an FP32 particle state
+__device__ inline void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, float &vzf,
+                               unsigned long long *counters)
+{
+    double vx = vxf, vy = vyf, vz = vzf;
+    scatter_d(m, r, vx, vy, vz, counters);
     vxf = (float)vx; vyf = (float)vy; vzf = (float)vz;
 }
 

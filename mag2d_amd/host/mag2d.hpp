@@ -185,6 +185,10 @@ enum coord_type { CARTESIAN = 0 };
 // (param.cpp:14-142).  Call derive() after changing sizes or counts.
 struct Param {
     enum boundary_type { FREE = M2D_BC_FREE, PERIODIC = M2D_BC_PERIODIC };
+    // Param::Mover (param.hpp:21); the config accepts ADVANCE_BORIS, the
+    // default, and ADVANCE_MULTICOLL (param.cpp:97-105)
+    enum Mover { ADVANCE_BORIS, ADVANCE_MULTICOLL, ADVANCE_LEAPFROG };
+    Mover mover = ADVANCE_BORIS;
     int x_sampl = 101, z_sampl = 101;
     double x_max = 1e-2, z_max = 1e-2, x_min = 0.0, z_min = 0.0;
     geometry_type geometry = EMPTY;
@@ -294,7 +298,7 @@ public:
 
     // the operator API (particles.hpp:281-291)
     void advance();           // advance_position + advance_boundary, niter++, t += dt
-    void advance_init();      // advance_boris_init
+    void advance_init();      // advance_position_init: advance_boris_init, nothing for multicoll
     void accumulate();        // rho.accumulate over live particles
     void advance_boundary();  // box, is_free, deposit when selfconsistent
     // particles.cpp:728-749; velocities Maxwellian at the species temperature
@@ -303,7 +307,8 @@ public:
     void lifetime_init(const std::vector<Species *> &speclist);
     // energy_dist.mean_tot() (histogram.cpp:68-71): mean kinetic energy in eV
     double mean_energy();
-    // re-order the device store by cell (drops holes; host mirror follows)
+    // re-order the device store by cell (drops holes; host mirror follows).
+    // Boris only: the multicoll mover keeps time_to_death per slot.
     void sort();
     // the container calls of BaseSpecies on the host mirror
     int insert();
@@ -324,6 +329,7 @@ private:
     SpeciesParams params_;
     uint32_t disk_calls_ = 0;
     DeviceBuffer soa_[5], tmp_[5];
+    DeviceBuffer ttd_;                  // time_to_death per slot (FP64), multicoll mover only
     int64_t cap_ = 0;
     bool host_newer_ = false, dev_newer_ = false;
     m2d_mcc *mcc_ = nullptr;
@@ -334,6 +340,10 @@ private:
     void reserve_device(int64_t n);
     m2d_species2d desc() const;
     void run_pass(uint32_t flags);
+    bool multicoll() const { return p_->mover == Param::ADVANCE_MULTICOLL; }
+    void ensure_ttd();
+    void bind_partners();
+    void build_mcc(const std::vector<Species *> &speclist);
 };
 
 // ------------------------------------------------------------------ Pic

@@ -554,6 +554,12 @@ void Species::upload()
     }
     for (int c = 0; c < 5; c++)
         if (n) check(m2d_memcpy_h2d(dev_->ctx(), soa_[c].get(), h[c].data(), (size_t)n * sizeof(float)));
+    if (multicoll()) {
+        ensure_ttd();
+        std::vector<double> t((size_t)n);
+        for (int64_t i = 0; i < n; i++) t[(size_t)i] = particles[(size_t)i].time_to_death;
+        if (n) check(m2d_memcpy_h2d(dev_->ctx(), ttd_.get(), t.data(), (size_t)n * sizeof(double)));
+    }
     m2d_particles2d s = store();
     check(m2d_sort_forget_2d(dev_->ctx(), &s));
     host_newer_ = dev_newer_ = false;
@@ -568,7 +574,13 @@ void Species::download()
         h[c].resize((size_t)n);
         if (n) check(m2d_memcpy_d2h(dev_->ctx(), h[c].data(), soa_[c].get(), (size_t)n * sizeof(float)));
     }
-    // y and time_to_death are not on the device; slots keep them where they persist
+    // y is not on the device, nor is time_to_death outside the multicoll
+    // mover: slots keep them where they persist
+    std::vector<double> t;
+    if (multicoll() && ttd_.bytes() == (size_t)std::max<int64_t>(n, 1) * sizeof(double)) {
+        t.resize((size_t)n);
+        if (n) check(m2d_memcpy_d2h(dev_->ctx(), t.data(), ttd_.get(), (size_t)n * sizeof(double)));
+    }
     particles.resize((size_t)n, t_particle{0, 0, 0, 0, 0, 0, 0, true});
     for (int64_t i = 0; i < n; i++) {
         t_particle &p = particles[(size_t)i];
@@ -579,6 +591,7 @@ void Species::download()
         p.vx = h[2][i];
         p.vy = h[3][i];
         p.vz = h[4][i];
+        if (!t.empty()) p.time_to_death = t[(size_t)i];
     }
     rebuild_free_list();
     dev_newer_ = false;
@@ -633,16 +646,33 @@ m2d_species2d Species::desc() const
     return d;
 }
 
-void Species::run_pass(uint32_t flags)
+// the multicoll mover's time_to_death array, one FP64 per slot; slots that
+// had none start at 0 (a value-initialised t_particle), so their first event
+// comes at once
+void Species::ensure_ttd()
 {
-    ensure_device();
-    if (cap_ == 0) return;
+    const size_t bytes = (size_t)std::max<int64_t>(cap_, 1) * sizeof(double);
+    if (ttd_.bytes() == bytes) return;
+    ttd_ = DeviceBuffer(dev_, bytes);
+    check(m2d_memset(dev_->ctx(), ttd_.get(), 0, bytes));
+}
+
+// point the MCC targets that are live species at their current stores
+void Species::bind_partners()
+{
     for (size_t k = 0; k < partners_.size(); k++)
         if (partners_[k]) {
             partners_[k]->ensure_device();
             m2d_particles2d ps = partners_[k]->store();
             check(m2d_mcc_set_partner(mcc_, (int32_t)k, &ps));
         }
+}
+
+void Species::run_pass(uint32_t flags)
+{
+    ensure_device();
+    if (cap_ == 0) return;
+    bind_partners();
     m2d_species2d d = desc();
     m2d_particles2d s = store();
     const bool dep = (flags & M2D_STEP_DEPOSIT) != 0;
@@ -654,8 +684,29 @@ void Species::run_pass(uint32_t flags)
 
 void Species::advance()
 {
-    // advance_position + advance_boundary (particles.hpp:342-349) in one pass
-    run_pass(M2D_STEP_PUSH | M2D_STEP_BOUNDARY | (p_->selfconsistent ? M2D_STEP_DEPOSIT : 0u));
+    // advance_position + advance_boundary (particles.hpp:342-349); the mover
+    // of Param::mover (particles.cpp:775-792)
+    if (p_->mover == Param::ADVANCE_BORIS) {
+        // Boris push, MCC trial, boundary and deposit in one pass
+        run_pass(M2D_STEP_PUSH | M2D_STEP_BOUNDARY | (p_->selfconsistent ? M2D_STEP_DEPOSIT : 0u));
+    } else if (p_->mover == Param::ADVANCE_MULTICOLL) {
+        ensure_device();
+        if (cap_ > 0) {
+            ensure_ttd();
+            bind_partners();
+            m2d_species2d d = desc();
+            m2d_particles2d s = store();
+            check(m2d_advance_multicoll_2d(dev_->ctx(), field->handle(), &d, &s, (double *)ttd_.get(), mcc_,
+                                           nullptr));
+            dev_newer_ = true;
+        }
+        advance_boundary();
+        // advance_multicoll counts the step itself as well (particles.cpp:858-859)
+        niter++;
+        t += dt;
+    } else {
+        throw std::runtime_error("Species<CARTESIAN>: advance method not implemented");
+    }
     niter++;
     t += dt;
 }
@@ -667,6 +718,10 @@ void Species::advance_boundary()
 
 void Species::advance_init()
 {
+    // advance_position_init (particles.cpp:795-811): nothing for multicoll
+    if (p_->mover == Param::ADVANCE_MULTICOLL) return;
+    if (p_->mover != Param::ADVANCE_BORIS)
+        throw std::runtime_error("Species<CARTESIAN>: advance method not implemented");
     ensure_device();
     if (cap_ == 0) return;
     m2d_species2d d = desc();
@@ -704,6 +759,14 @@ void Species::add_particles_on_disk(int nparticles, double centerx, double cente
         check(m2d_memcpy_d2d(dev_->ctx(), (float *)grown[c].get() + old, tmp[c].get(),
                              (size_t)nparticles * sizeof(float)));
     }
+    if (multicoll()) {
+        // the new slots' time_to_death is 0, as in a freshly inserted t_particle
+        DeviceBuffer t(dev_, (size_t)n * sizeof(double));
+        check(m2d_memset(dev_->ctx(), t.get(), 0, (size_t)n * sizeof(double)));
+        if (old && ttd_.bytes() == (size_t)old * sizeof(double))
+            check(m2d_memcpy_d2d(dev_->ctx(), t.get(), ttd_.get(), (size_t)old * sizeof(double)));
+        ttd_ = std::move(t);
+    }
     dev_->synchronize();
     for (int c = 0; c < 5; c++) soa_[c] = std::move(grown[c]);
     cap_ = n;
@@ -718,7 +781,23 @@ void Species::lifetime_init(const std::vector<Species *> &speclist)
     }
     partners_.clear();
     lifetime = INFINITY;
-    if (params_.interactions.empty()) return;
+    if (!params_.interactions.empty()) build_mcc(speclist);
+    // time_to_death of the live particles (particles.cpp:163-169); only the
+    // multicoll mover reads it
+    if (multicoll()) {
+        ensure_device();
+        if (cap_ > 0) {
+            ensure_ttd();
+            m2d_species2d d = desc();
+            m2d_particles2d s = store();
+            check(m2d_time_to_death_init_2d(dev_->ctx(), &d, &s, (double *)ttd_.get()));
+            dev_newer_ = true;
+        }
+    }
+}
+
+void Species::build_mcc(const std::vector<Species *> &speclist)
+{
     std::vector<m2d_target> targets;
     std::vector<m2d_interaction> inter;
     std::vector<double> tabE, tabS;
@@ -785,6 +864,8 @@ double Species::mean_energy()
 
 void Species::sort()
 {
+    if (multicoll())
+        throw std::runtime_error("Species::sort: the multicoll mover keeps time_to_death per slot");
     ensure_device();
     if (cap_ == 0) return;
     for (int c = 0; c < 5; c++)
@@ -1023,6 +1104,82 @@ int m2dh_trap_run(const m2dh_trap_cfg *c, float *out_x, float *out_z, float *out
             if (out_vx) out_vx[k] = (float)q.vx;
             if (out_vy) out_vy[k] = (float)q.vy;
             if (out_vz) out_vz[k] = (float)q.vz;
+            k++;
+        }
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+typedef struct {
+    int32_t grid, n_particles, steps, mover;  // mover: Param::Mover
+    double length, dt, extern_field, temperature_K, mass, charge, radius;
+    double gas_mass, gas_temperature_K, gas_density, rate;  // one ELASTIC interaction with a background gas
+    uint64_t seed;
+} m2dh_mcc_cfg;
+
+// A run in the style of config_MCC.txt: EMPTY geometry, constant
+// extern_field, not self-consistent, one particle species that collides
+// elastically with a background gas (a species that is not a particle
+// ensemble), advanced by Pic::advance with the mover of cfg->mover.  Writes
+// the live particles (slot order) and their time_to_death, and metrics =
+// {mean energy eV, live particles, niter, lifetime}.
+int m2dh_mcc_run(const m2dh_mcc_cfg *c, float *out_x, float *out_z, float *out_vx, float *out_vy, float *out_vz,
+                 double *out_ttd, int64_t n_out, double *metrics, char *msg, int len)
+{
+    try {
+        Param p;
+        p.x_sampl = p.z_sampl = c->grid;
+        p.x_max = p.z_max = c->length;
+        p.geometry = EMPTY;
+        p.selfconsistent = 0;
+        p.extern_field = c->extern_field;
+        p.mover = (Param::Mover)c->mover;
+        p.n_particles_total = c->n_particles;
+        p.seed = c->seed;
+        p.derive();
+        SpeciesParams ion;
+        ion.name = "ION";
+        ion.mass = c->mass;
+        ion.charge = c->charge;
+        ion.temperature = c->temperature_K;
+        ion.dt = c->dt;
+        InteractionParams ip;
+        ip.type = M2D_ELASTIC;
+        ip.target = "GAS";
+        ip.rate = c->rate;
+        ion.interactions.push_back(ip);
+        SpeciesParams gas;
+        gas.name = "GAS";
+        gas.mass = c->gas_mass;
+        gas.temperature = c->gas_temperature_K;
+        gas.density = c->gas_density;
+        gas.dt = c->dt;
+        gas.particle = false;
+        Pic run(p, {ion, gas});
+        Species &s = *run.speclist[0];
+        s.add_particles_on_disk(c->n_particles, 0.5 * p.x_max, 0.5 * p.z_max, c->radius);
+        run.advance_init();
+        for (int k = 0; k < c->steps; k++) run.advance();
+        s.download();
+        if (metrics) {
+            metrics[0] = s.mean_energy();
+            metrics[1] = s.n_particles();
+            metrics[2] = (double)s.niter;
+            metrics[3] = s.lifetime;
+        }
+        int64_t k = 0;
+        for (const t_particle &q : s.particles) {
+            if (q.empty || k >= n_out) continue;
+            if (out_x) out_x[k] = (float)q.x;
+            if (out_z) out_z[k] = (float)q.z;
+            if (out_vx) out_vx[k] = (float)q.vx;
+            if (out_vy) out_vy[k] = (float)q.vy;
+            if (out_vz) out_vz[k] = (float)q.vz;
+            if (out_ttd) out_ttd[k] = q.time_to_death;
             k++;
         }
     } catch (const std::exception &e) {

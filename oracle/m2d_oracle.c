@@ -837,6 +837,78 @@ void o_advance_boris_init(const o_fields *f, o_species *s, o_particle *P, long n
     }
 }
 
+/* Exp(1) variate.  The reference draws it by the ziggurat (t_random::rexp,
+ * random.cpp:54); only the distribution matters (statistical parity).  Both
+ * sources here invert a uniform, as the device library does. */
+double o_rng_rexp(o_rng *r)
+{
+    double u = o_rng_uni(r);
+    if (u <= 0.0) u = 1e-300;
+    return -log(u);
+}
+
+/* BaseSpecies::lifetime_init (particles.cpp:163-169): time_to_death of the
+ * live particles.  Philox: counter (slot, niter, block, 4 << 24). */
+void o_ttd_init(const o_species *s, o_particle *P, long n)
+{
+    for (long k = 0; k < n; k++) {
+        o_particle *I = &P[k];
+        if (I->empty) continue;
+        o_rng local, *r = s->shr3;
+        if (s->rng_kind != O_RNG_SHR3) {
+            o_rng_philox_begin(&local, s->seed, s->stream, (uint32_t)(k + s->slot0), s->niter, 4);
+            r = &local;
+        }
+        I->time_to_death = o_rng_rexp(r) * s->lifetime;
+    }
+}
+
+/* Species<CARTESIAN>::advance_multicoll (particles.cpp:813-861).  Returns 0,
+ * -1 for B != 0, -2 for a selfconsistent or non-EMPTY run (the reference
+ * throws there).  Philox: one stream per slot and step, counter (slot,
+ * niter, block, 3 << 24), drawn in event order (scatter, then the next
+ * time_to_death).  s->mcc NULL: every event is a null collision.  niter is
+ * left to the caller. */
+int o_advance_multicoll(const o_fields *f, o_species *s, o_particle *P, long n, long *n_events)
+{
+    if (f->Br != 0. || f->Bz != 0. || f->Bt != 0.) return -1;
+    if (f->selfconsistent || !f->geometry_empty) return -2;
+    double fx, fz;
+    o_fields_E(f, 0., 0., 0., &fx, &fz);
+    const double qm = s->charge / s->mass;
+    const double dt = s->dt;
+    long ne = 0;
+    for (long k = 0; k < n; k++) {
+        o_particle *I = &P[k];
+        if (I->empty) continue;
+        o_rng local, *r = s->shr3;
+        if (s->rng_kind != O_RNG_SHR3) {
+            o_rng_philox_begin(&local, s->seed, s->stream, (uint32_t)(k + s->slot0), s->niter, 3);
+            r = &local;
+        }
+        double local_time = 0., ttd;
+        while (local_time + I->time_to_death < dt) {
+            ttd = I->time_to_death;
+            I->vx += fx * qm * ttd;
+            I->vz += fz * qm * ttd;
+            I->x += (I->vx + 0.5 * fx * qm * ttd) * ttd;
+            I->z += (I->vz + 0.5 * fz * qm * ttd) * ttd;
+            local_time += I->time_to_death;
+            if (s->mcc) o_scatter(s->mcc, I, r);
+            I->time_to_death = s->lifetime * o_rng_rexp(r);
+            ne++;
+        }
+        ttd = dt - local_time;
+        I->vx += fx * qm * ttd;
+        I->vz += fz * qm * ttd;
+        I->x += (I->vx + 0.5 * fx * qm * ttd) * ttd;
+        I->z += (I->vz + 0.5 * fz * qm * ttd) * ttd;
+        I->time_to_death -= ttd;
+    }
+    if (n_events) *n_events = ne;
+    return 0;
+}
+
 /* Species<D>::advance_boundary (particles.hpp:370-411) */
 long o_advance_boundary(const o_fields *f, o_species *s, o_particle *P, long n, long *n_bad)
 {

@@ -208,6 +208,12 @@ typedef struct {
 void o_fields_E(const o_fields *f, double x, double z, double t, double *ex, double *ez); /* fields.hpp:124-150 */
 void o_advance_boris(const o_fields *f, o_species *s, o_particle *p, long n, long *n_coll);
 void o_advance_boris_init(const o_fields *f, o_species *s, o_particle *p, long n);
+double o_rng_rexp(o_rng *r);   /* Exp(1) by inversion (the reference's ziggurat: statistical parity) */
+/* BaseSpecies::lifetime_init's time_to_death draw (particles.cpp:163-169) */
+void o_ttd_init(const o_species *s, o_particle *p, long n);
+/* Species<CARTESIAN>::advance_multicoll (particles.cpp:813-861); 0, or -1 (B != 0) / -2 (not EMPTY
+ * or selfconsistent) where the reference throws */
+int  o_advance_multicoll(const o_fields *f, o_species *s, o_particle *p, long n, long *n_events);
 /* advance_boundary incl. deposit; returns particles removed; *n_bad counts
  * deposits outside the grid (Field2D::accumulate throws there). */
 long o_advance_boundary(const o_fields *f, o_species *s, o_particle *p, long n, long *n_bad);

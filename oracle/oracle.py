@@ -146,6 +146,10 @@ _SIGS = {
     "o_advance_boris": (None, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long,
                                C.POINTER(C.c_long)]),
     "o_advance_boris_init": (None, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long]),
+    "o_rng_rexp": (C.c_double, [C.POINTER(Rng)]),
+    "o_ttd_init": (None, [C.POINTER(Species), _P, C.c_long]),
+    "o_advance_multicoll": (C.c_int, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long,
+                                      C.POINTER(C.c_long)]),
     "o_advance_boundary": (C.c_long, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long,
                                       C.POINTER(C.c_long)]),
     "o_accumulate": (C.c_long, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long]),

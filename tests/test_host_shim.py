@@ -28,6 +28,17 @@ class TrapCfg(C.Structure):
                 ("density_total", C.c_double), ("seed", C.c_uint64)]
 
 
+class MccCfg(C.Structure):
+    _fields_ = [("grid", C.c_int32), ("n_particles", C.c_int32), ("steps", C.c_int32), ("mover", C.c_int32),
+                ("length", C.c_double), ("dt", C.c_double), ("extern_field", C.c_double),
+                ("temperature_K", C.c_double), ("mass", C.c_double), ("charge", C.c_double),
+                ("radius", C.c_double), ("gas_mass", C.c_double), ("gas_temperature_K", C.c_double),
+                ("gas_density", C.c_double), ("rate", C.c_double), ("seed", C.c_uint64)]
+
+
+ADVANCE_BORIS, ADVANCE_MULTICOLL = 0, 1
+
+
 def host():
     abi.lib()  # libmag2d_amd first: the host layer links against it
     L = C.CDLL(str(HOST_LIB))
@@ -37,12 +48,15 @@ def host():
     L.m2dh_trap_run.restype = C.c_int
     L.m2dh_trap_run.argtypes = [C.POINTER(TrapCfg), _f, _f, _f, _f, _f, C.c_int64, C.POINTER(C.c_double),
                                 C.c_char_p, C.c_int]
+    L.m2dh_mcc_run.restype = C.c_int
+    L.m2dh_mcc_run.argtypes = [C.POINTER(MccCfg), _f, _f, _f, _f, _f, C.POINTER(C.c_double), C.c_int64,
+                               C.POINTER(C.c_double), C.c_char_p, C.c_int]
     return L
 
 
 def test_host_library_loads_and_exports():
     L = host()
-    for name in ("m2dh_list_selftest", "m2dh_trap_run"):
+    for name in ("m2dh_list_selftest", "m2dh_trap_run", "m2dh_mcc_run"):
         assert hasattr(L, name)
 
 
@@ -143,3 +157,70 @@ def test_pic_selfconsistent_trap_matches_oracle():
     assert abs(m[3] - u[c]) <= 1e-6 * max(1.0, abs(u).max())
     e_or = O.lib().o_mean_energy_eV(O.dptr(p), n, cfg.mass)
     assert math.isclose(m[0], e_or, rel_tol=1e-4)
+
+
+def oracle_mcc(cfg):
+    """The same run in the oracle: samples of the Philox disk sampler, every
+    time_to_death 0 (the particles come after lifetime_init, as the
+    initscript's do), then advance_multicoll + advance_boundary per step with
+    niter advancing by two (particles.cpp:858-859 and particles.hpp:346-347)."""
+    n, L = cfg.grid, cfg.length
+    dx = L / (n - 1)
+    og = O.grid2(n, n, dx, dx)
+    mask = np.zeros(n * n, np.uint8)
+    volt = np.zeros(n * n)
+    O.lib().o_grid_build(C.byref(og), C.byref(O.Geom(abi.GEO_EMPTY, L, L, 1e-3, 0.0, cfg.extern_field)),
+                         O.dptr(mask), O.dptr(volt))
+    of = O.make_fields(og, mask, np.zeros(n * n), x_max=L, z_max=L, geometry_empty=1,
+                       extern_field=cfg.extern_field)
+    vth = math.sqrt(abi.KB * cfg.temperature_K / cfg.mass)
+    p = O.particles(cfg.n_particles)
+    O.lib().o_add_particles_on_disk_philox(O.dptr(p), cfg.n_particles, 0.5 * L, 0.5 * L, cfg.radius, vth,
+                                           C.byref(of), cfg.seed, 0)
+    E_max = cfg.temperature_K * abi.KB / abi.QE * 10.0
+    mcc = O.make_mcc(cfg.mass, E_max, [dict(mass=cfg.gas_mass, density=cfg.gas_density,
+                                            temperature=cfg.gas_temperature_K,
+                                            inters=[O.inter(O.ELASTIC, cfg.mass, cfg.gas_mass, rate=cfg.rate)])])
+    s = O.make_species(cfg.charge, cfg.mass, cfg.dt, lifetime=mcc.lifetime, mcc=mcc, seed=cfg.seed, stream=0)
+    N = cfg.n_particles
+    events = 0
+    for k in range(cfg.steps):
+        s.niter = 2 * k
+        ne = C.c_long(0)
+        assert O.lib().o_advance_multicoll(C.byref(of), C.byref(s), O.dptr(p), N, C.byref(ne)) == 0
+        O.lib().o_advance_boundary(C.byref(of), C.byref(s), O.dptr(p), N, None)
+        events += ne.value
+    return p, mcc.lifetime, events
+
+
+@pytest.mark.gpu
+def test_pic_multicoll_run_matches_oracle():
+    """Param::mover = ADVANCE_MULTICOLL through Pic::advance: the same events,
+    velocities and positions as the oracle's restatement."""
+    cfg = MccCfg(grid=64, n_particles=4000, steps=12, mover=ADVANCE_MULTICOLL, length=2e-2, dt=1e-9,
+                 extern_field=1000.0, temperature_K=1160.45, mass=3 * 1.66053906660e-27, charge=abi.QE,
+                 radius=5e-3, gas_mass=6.69e-27, gas_temperature_K=300.0, gas_density=1e24, rate=1e-15,
+                 seed=1234)
+    n = cfg.n_particles
+    out = {k: np.zeros(n, np.float32) for k in ("x", "z", "vx", "vy", "vz")}
+    ttd = np.zeros(n)
+    m = (C.c_double * 4)()
+    msg = C.create_string_buffer(512)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    rc = host().m2dh_mcc_run(C.byref(cfg), fp(out["x"]), fp(out["z"]), fp(out["vx"]), fp(out["vy"]),
+                             fp(out["vz"]), ttd.ctypes.data_as(C.POINTER(C.c_double)), n, m, msg, 512)
+    assert rc == 0, msg.value.decode()
+    p, lifetime, events = oracle_mcc(cfg)
+    assert m[3] == pytest.approx(lifetime, rel=1e-14)
+    assert int(m[2]) == 2 * cfg.steps
+    # every particle also has an event at t = 0 of its first step (time_to_death 0)
+    assert events == pytest.approx(n + cfg.steps * n * cfg.dt / lifetime, rel=0.05)
+    alive = p["empty"] == 0
+    k = int(alive.sum())
+    assert int(m[1]) == k and k > 0.9 * n
+    np.testing.assert_allclose(ttd[:k], p["time_to_death"][alive], rtol=1e-12, atol=1e-24)
+    vscale = np.abs(p["vx"][alive]).max()
+    for c in ("vx", "vy", "vz"):
+        assert np.abs(out[c][:k] - p[c][alive]).max() <= 1e-4 * vscale, c
+    for c in ("x", "z"):
+        assert np.abs(out[c][:k] - p[c][alive]).max() <= 4 * cfg.steps * np.spacing(np.float32(cfg.length)), c
