@@ -338,6 +338,19 @@ int m2d_sort_forget_2d(m2d_ctx *ctx, const m2d_particles2d *p);
 int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p,
                                  double cx, double cz, double radius, double vth,
                                  uint32_t stream);
+/* Species<CARTESIAN>::add_particles_bessel (particles.cpp:701-726) into the
+ * n slots of p: density J0(2.405 r / radius) on the disk, by rejection from
+ * the square with acceptance uni() <= J0; velocities as rndv.  Samples
+ * outside the box become empty slots, as the reference skips them. */
+int m2d_add_particles_bessel_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p,
+                                double cx, double cz, double radius, double vth, uint32_t stream);
+/* Species<CARTESIAN>::add_particles_everywhere (particles.cpp:686-699) into the
+ * n slots of p: uniform in the box [xmin, x_max] x [zmin, z_max], Maxwellian
+ * velocities of thermal speed vth, and (time_to_death not NULL, FP64 device
+ * array of n) time_to_death = lifetime * Exp(1). */
+int m2d_add_particles_everywhere_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p,
+                                    double vth, double *time_to_death, double lifetime,
+                                    uint32_t stream);
 
 /* ======================= adjoint ========================================= */
 

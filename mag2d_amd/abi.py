@@ -191,6 +191,10 @@ _SIGS = {
     "m2d_sort_forget_2d": (C.c_int, [_P, C.POINTER(Particles2D)]),
     "m2d_add_particles_on_disk_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.c_uint32]),
+    "m2d_add_particles_bessel_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_uint32]),
+    "m2d_add_particles_everywhere_2d": (C.c_int, [_P, _P, C.POINTER(Particles2D), C.c_double, _P, C.c_double,
+                                                  C.c_uint32]),
     "m2d_adjoint_energy_2d": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D),
                                         C.POINTER(AdjointCfg), _PD, _PD, C.POINTER(AdjointInfo)]),
     "m2d_batch_create": (C.c_int, [_P, _P, C.POINTER(Species2D), C.POINTER(Particles2D), C.POINTER(BatchDesc),
@@ -588,6 +592,17 @@ def sort_forget(ctx, parts):
 def add_particles_on_disk(ctx, fields, parts, cx, cz, radius, vth, stream=0):
     s = parts.struct()
     check(lib().m2d_add_particles_on_disk_2d(ctx.h, fields.h, C.byref(s), cx, cz, radius, vth, stream))
+
+
+def add_particles_bessel(ctx, fields, parts, cx, cz, radius, vth, stream=0):
+    s = parts.struct()
+    check(lib().m2d_add_particles_bessel_2d(ctx.h, fields.h, C.byref(s), cx, cz, radius, vth, stream))
+
+
+def add_particles_everywhere(ctx, fields, parts, vth, ttd_ptr=None, lifetime=float("inf"), stream=0):
+    s = parts.struct()
+    check(lib().m2d_add_particles_everywhere_2d(ctx.h, fields.h, C.byref(s), vth,
+                                                C.c_void_p(ttd_ptr) if ttd_ptr else None, lifetime, stream))
 
 
 def interpolate(ctx, g, data_ptr, x, z):

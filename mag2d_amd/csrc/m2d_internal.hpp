@@ -217,6 +217,9 @@ struct Rng {
         double a = uni(), b = uni();
         return sqrt(-2.0 * log(a)) * cos(2.0 * M_PI * b);
     }
+    // Exp(1) by inversion; t_random::rexp (random.cpp:54) draws the same
+    // distribution by the ziggurat
+    __device__ __forceinline__ double exponential() { return -log(uni()); }
 };
 
 // wave-aggregated counter increment (one atomic per wavefront)

@@ -29,10 +29,6 @@ struct MulticollArgs {
     unsigned long long *counters;
 };
 
-// t_random::rexp (random.cpp:54) draws Exp(1) by the ziggurat; inversion of
-// a uniform in (0, 1) draws the same distribution
-__device__ __forceinline__ double exp_draw(Rng &r) { return -log(r.uni()); }
-
 // flight of duration t in the constant field, in the reference's order:
 // kick, then x += (v + a t / 2) t with the kicked v (particles.cpp:841-844)
 __device__ __forceinline__ void fly(const MulticollArgs &a, double t, double &x, double &z, double &vx,
@@ -64,7 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_multicoll2d(MulticollArgs a)
             fly(a, ttd, x, z, vx, vz);
             local += ttd;
             if (a.mcc) scatter_d(*a.mcc, rg, vx, vy, vz, a.counters);
-            ttd = a.lifetime * exp_draw(rg);
+            ttd = a.lifetime * rg.exponential();
             events++;
         }
         // particles.cpp:850-855
@@ -91,7 +87,7 @@ __global__ void __launch_bounds__(kBlock) k_ttd_init2d(const float *x, double *t
     if (p >= n || is_dead(x[p])) return;
     Rng rg;
     rg.begin(seed, stream, (uint32_t)p, step, kPurposeTtd);
-    ttd[p] = lifetime * exp_draw(rg);
+    ttd[p] = lifetime * rg.exponential();
 }
 
 }  // namespace m2d

@@ -749,6 +749,71 @@ __global__ void __launch_bounds__(kBlock) k_disk2d(m2d_particles2d P, double cx,
     P.vx[p] = (float)a; P.vz[p] = (float)b; P.vy[p] = (float)c;
 }
 
+// J0 by its power series sum_k (-x^2/4)^k / (k!)^2; for 0 <= x <= 2.5 the
+// terms fall below 1e-20 within 20 terms (the sampler below needs x <= 2.41)
+__device__ __forceinline__ double bessel_j0_series(double x)
+{
+    const double q = -0.25 * x * x;
+    double t = 1.0, s = 1.0;
+#pragma unroll
+    for (int k = 1; k <= 20; k++) {
+        t *= q / (double)(k * k);
+        s += t;
+    }
+    return s;
+}
+
+// add_particles_bessel (particles.cpp:701-726): density J0(2.405 r / R) on the
+// disk by rejection from the square, one Philox stream per slot
+__global__ void __launch_bounds__(kBlock) k_bessel2d(m2d_particles2d P, double cx, double cz, double R,
+                                                     double vth, double bx0, double bx1, double bz0, double bz1,
+                                                     uint64_t seed, uint32_t stream)
+{
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    Rng r;
+    r.begin(seed, stream, (uint32_t)p, 0, kPurposeInit);
+    const double root = 2.404825557695773;  // first zero of J0 (particles.cpp:708)
+    double x = 0, y = 0;
+    // acceptance is about 1/3 per trial: 256 trials fail with probability 1e-46
+    for (int t = 0; t < 256; t++) {
+        x = r.uni() * 2 - 1.0;
+        y = r.uni() * 2 - 1.0;
+        const double rr = sqrt(x * x + y * y);
+        if (rr > 1.0) continue;
+        if (!(r.uni() > bessel_j0_series(rr * root))) break;
+    }
+    x = x * R + cx;
+    y = y * R + cz;
+    if (x < bx0 || x > bx1 || y < bz0 || y > bz1) {
+        P.x[p] = dead_value();
+        return;
+    }
+    // rndv(vx, vz, vy) (particles.hpp:188-193)
+    double a = r.normal() * vth, b = r.normal() * vth, c = r.normal() * vth;
+    P.x[p] = (float)x; P.z[p] = (float)y;
+    P.vx[p] = (float)a; P.vz[p] = (float)b; P.vy[p] = (float)c;
+}
+
+// add_particles_everywhere (particles.cpp:686-699): uniform in the box, one
+// Philox stream per slot; time_to_death (may be null) = lifetime Exp(1)
+__global__ void __launch_bounds__(kBlock) k_everywhere2d(m2d_particles2d P, double x0, double x1, double z0,
+                                                         double z1, double vth, double *ttd, double lifetime,
+                                                         uint64_t seed, uint32_t stream)
+{
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    Rng r;
+    r.begin(seed, stream, (uint32_t)p, 0, kPurposeInit);
+    const double x = (x1 - x0) * r.uni() + x0;
+    (void)r.uni();  // y: the reference draws it; the Cartesian store has no y
+    const double z = (z1 - z0) * r.uni() + z0;
+    const double a = r.normal() * vth, b = r.normal() * vth, c = r.normal() * vth;
+    P.x[p] = (float)x; P.z[p] = (float)z;
+    P.vx[p] = (float)a; P.vy[p] = (float)b; P.vz[p] = (float)c;
+    if (ttd) ttd[p] = lifetime * r.exponential();
+}
+
 // ------------------------------------------------------------------ host helpers
 int check_parts(const m2d_particles2d *p)
 {
@@ -1107,6 +1172,38 @@ int m2d_add_particles_on_disk_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_partic
     k_disk2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(*p, cx, cz, radius, vth, f->p.grid.xmin,
                                                              f->p.x_max, f->p.grid.zmin, f->p.z_max,
                                                              ctx->seed, stream);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+int m2d_add_particles_bessel_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p, double cx, double cz,
+                                double radius, double vth, uint32_t stream)
+{
+    M2D_CHECK(ctx && f, M2D_ERR_BAD_ARG, "m2d_add_particles_bessel_2d: NULL argument");
+    M2D_TRY(check_parts(p));
+    M2D_CHECK(radius > 0 && vth >= 0, M2D_ERR_BAD_ARG, "m2d_add_particles_bessel_2d: radius/vth");
+    if (p->n == 0) return M2D_OK;
+    forget_tiling(ctx, p->x);
+    k_bessel2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(*p, cx, cz, radius, vth, f->p.grid.xmin,
+                                                               f->p.x_max, f->p.grid.zmin, f->p.z_max,
+                                                               ctx->seed, stream);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
+}
+
+int m2d_add_particles_everywhere_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *p, double vth,
+                                    double *time_to_death, double lifetime, uint32_t stream)
+{
+    M2D_CHECK(ctx && f, M2D_ERR_BAD_ARG, "m2d_add_particles_everywhere_2d: NULL argument");
+    M2D_TRY(check_parts(p));
+    M2D_CHECK(vth >= 0, M2D_ERR_BAD_ARG, "m2d_add_particles_everywhere_2d: vth must be >= 0");
+    M2D_CHECK(!time_to_death || lifetime > 0, M2D_ERR_BAD_ARG,
+              "m2d_add_particles_everywhere_2d: lifetime must be > 0");
+    if (p->n == 0) return M2D_OK;
+    forget_tiling(ctx, p->x);
+    k_everywhere2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(*p, f->p.grid.xmin, f->p.x_max,
+                                                                   f->p.grid.zmin, f->p.z_max, vth,
+                                                                   time_to_death, lifetime, ctx->seed, stream);
     M2D_HIP(hipGetLastError());
     return M2D_OK;
 }

@@ -19,6 +19,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <iosfwd>
 #include <memory>
 #include <stdexcept>
@@ -303,6 +304,10 @@ public:
     void advance_boundary();  // box, is_free, deposit when selfconsistent
     // particles.cpp:728-749; velocities Maxwellian at the species temperature
     void add_particles_on_disk(int nparticles, double centerx, double centery, double radius);
+    // particles.cpp:701-726: density J0(2.405 r / radius) on the disk
+    void add_particles_bessel(int nparticles, double centerx, double centery, double radius);
+    // particles.cpp:686-699: uniform in the box, time_to_death = rexp() lifetime
+    void add_particles_everywhere(int nparticles);
     // BaseSpecies::lifetime_init (particles.cpp:142-170) over the species list
     void lifetime_init(const std::vector<Species *> &speclist);
     // energy_dist.mean_tot() (histogram.cpp:68-71): mean kinetic energy in eV
@@ -344,6 +349,8 @@ private:
     void ensure_ttd();
     void bind_partners();
     void build_mcc(const std::vector<Species *> &speclist);
+    void append_samples(int nparticles, const std::function<void(const m2d_particles2d &, double *)> &fill);
+    uint32_t next_sample_stream();
 };
 
 // ------------------------------------------------------------------ Pic

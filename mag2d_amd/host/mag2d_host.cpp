@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <sstream>
 
@@ -738,20 +739,27 @@ void Species::accumulate()
     rho.mark_device_written();
 }
 
-void Species::add_particles_on_disk(int nparticles, double centerx, double centery, double radius)
+// Sample n particles into a fresh store with `fill`, then append them to
+// this one.  `fill` gets the new store and, for the multicoll mover, the new
+// slots' time_to_death (set to 0 before, as in a freshly inserted
+// t_particle), else null.
+void Species::append_samples(int nparticles, const std::function<void(const m2d_particles2d &, double *)> &fill)
 {
     if (nparticles <= 0) return;
     ensure_device();
-    // sample into a fresh store, then append it to this one; every call draws
-    // from its own stream
     DeviceBuffer tmp[5];
     for (auto &b : tmp) b = DeviceBuffer(dev_, (size_t)nparticles * sizeof(float));
     m2d_particles2d t{(float *)tmp[0].get(), (float *)tmp[1].get(), (float *)tmp[2].get(),
                       (float *)tmp[3].get(), (float *)tmp[4].get(), nparticles};
-    const double vth = v_max * M_SQRT1_2;  // rndv: rnor() v_max / sqrt(2)
-    const uint32_t stream = stream_ ^ (disk_calls_++ << 16);
-    check(m2d_add_particles_on_disk_2d(dev_->ctx(), field->handle(), &t, centerx, centery, radius, vth, stream));
     const int64_t old = cap_, n = cap_ + nparticles;
+    DeviceBuffer ttd;
+    if (multicoll()) {
+        ttd = DeviceBuffer(dev_, (size_t)n * sizeof(double));
+        check(m2d_memset(dev_->ctx(), ttd.get(), 0, (size_t)n * sizeof(double)));
+        if (old && ttd_.bytes() == (size_t)old * sizeof(double))
+            check(m2d_memcpy_d2d(dev_->ctx(), ttd.get(), ttd_.get(), (size_t)old * sizeof(double)));
+    }
+    fill(t, multicoll() ? (double *)ttd.get() + old : nullptr);
     DeviceBuffer grown[5];
     for (int c = 0; c < 5; c++) {
         grown[c] = DeviceBuffer(dev_, (size_t)n * sizeof(float));
@@ -759,18 +767,41 @@ void Species::add_particles_on_disk(int nparticles, double centerx, double cente
         check(m2d_memcpy_d2d(dev_->ctx(), (float *)grown[c].get() + old, tmp[c].get(),
                              (size_t)nparticles * sizeof(float)));
     }
-    if (multicoll()) {
-        // the new slots' time_to_death is 0, as in a freshly inserted t_particle
-        DeviceBuffer t(dev_, (size_t)n * sizeof(double));
-        check(m2d_memset(dev_->ctx(), t.get(), 0, (size_t)n * sizeof(double)));
-        if (old && ttd_.bytes() == (size_t)old * sizeof(double))
-            check(m2d_memcpy_d2d(dev_->ctx(), t.get(), ttd_.get(), (size_t)old * sizeof(double)));
-        ttd_ = std::move(t);
-    }
     dev_->synchronize();
     for (int c = 0; c < 5; c++) soa_[c] = std::move(grown[c]);
+    if (multicoll()) ttd_ = std::move(ttd);
     cap_ = n;
     dev_newer_ = true;
+}
+
+// each sampler call draws from its own stream
+uint32_t Species::next_sample_stream() { return stream_ ^ (disk_calls_++ << 16); }
+
+void Species::add_particles_on_disk(int nparticles, double centerx, double centery, double radius)
+{
+    const double vth = v_max * M_SQRT1_2;  // rndv: rnor() v_max / sqrt(2)
+    append_samples(nparticles, [&](const m2d_particles2d &t, double *) {
+        check(m2d_add_particles_on_disk_2d(dev_->ctx(), field->handle(), &t, centerx, centery, radius, vth,
+                                           next_sample_stream()));
+    });
+}
+
+void Species::add_particles_bessel(int nparticles, double centerx, double centery, double radius)
+{
+    const double vth = v_max * M_SQRT1_2;
+    append_samples(nparticles, [&](const m2d_particles2d &t, double *) {
+        check(m2d_add_particles_bessel_2d(dev_->ctx(), field->handle(), &t, centerx, centery, radius, vth,
+                                          next_sample_stream()));
+    });
+}
+
+void Species::add_particles_everywhere(int nparticles)
+{
+    const double vth = v_max * M_SQRT1_2;  // rnor() v_max / M_SQRT2 (particles.cpp:694-696)
+    append_samples(nparticles, [&](const m2d_particles2d &t, double *ttd) {
+        check(m2d_add_particles_everywhere_2d(dev_->ctx(), field->handle(), &t, vth, ttd, lifetime,
+                                              next_sample_stream()));
+    });
 }
 
 void Species::lifetime_init(const std::vector<Species *> &speclist)
@@ -933,8 +964,13 @@ void Pic::run_initscript(const std::string &filename)
             p.vx = std::stod(toks[4]);
             p.vy = std::stod(toks[5]);
             p.vz = std::stod(toks[6]);
-        } else if (toks[0] == "add_particles_everywhere" || toks[0] == "add_particles_bessel") {
-            throw std::runtime_error("Pic::run_initscript: " + toks[0] + " is not provided by this build");
+        } else if (toks[0] == "add_particles_bessel") {
+            Species *sp = species(6);
+            sp->add_particles_bessel(std::stoi(toks[2]), std::stod(toks[3]), std::stod(toks[4]),
+                                     std::stod(toks[5]));
+        } else if (toks[0] == "add_particles_everywhere") {
+            Species *sp = species(3);
+            sp->add_particles_everywhere(std::stoi(toks[2]));
         }
     }
 }
@@ -1106,6 +1142,51 @@ int m2dh_trap_run(const m2dh_trap_cfg *c, float *out_x, float *out_z, float *out
             if (out_vz) out_vz[k] = (float)q.vz;
             k++;
         }
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+// Pic::run_initscript (pic.cpp:241-326) on a box of grid^2 nodes and side
+// `length` (EMPTY geometry) with one species "ION" of the given mass and
+// temperature.  Writes the live particles in slot order (up to n_out) and
+// returns their count in *n_live.
+int m2dh_initscript_run(const char *script, int32_t grid, double length, double mass, double temperature_K,
+                        uint64_t seed, float *out_x, float *out_z, float *out_vx, float *out_vy, float *out_vz,
+                        int64_t n_out, int64_t *n_live, char *msg, int len)
+{
+    try {
+        Param p;
+        p.x_sampl = p.z_sampl = grid;
+        p.x_max = p.z_max = length;
+        p.geometry = EMPTY;
+        p.seed = seed;
+        p.derive();
+        SpeciesParams ion;
+        ion.name = "ION";
+        ion.mass = mass;
+        ion.charge = physconst::q_e;
+        ion.temperature = temperature_K;
+        Pic run(p, {ion});
+        run.run_initscript(script);
+        Species &s = *run.speclist[0];
+        s.download();
+        int64_t k = 0;
+        for (const t_particle &q : s.particles) {
+            if (q.empty) continue;
+            if (k < n_out) {
+                out_x[k] = (float)q.x;
+                out_z[k] = (float)q.z;
+                out_vx[k] = (float)q.vx;
+                out_vy[k] = (float)q.vy;
+                out_vz[k] = (float)q.vz;
+            }
+            k++;
+        }
+        *n_live = k;
     } catch (const std::exception &e) {
         std::snprintf(msg, (size_t)len, "%s", e.what());
         return -1;

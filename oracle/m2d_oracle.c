@@ -999,6 +999,57 @@ void o_add_particles_on_disk_philox(o_particle *P, long n, double cx, double cz,
     }
 }
 
+/* add_particles_bessel (particles.cpp:701-726) on the Philox streams of the
+ * device sampler (slot k: counter (k, 0, block, 1 << 24)); J0 is the C
+ * library's j0, where the reference calls std::tr1::cyl_bessel_j(0, .) */
+void o_add_particles_bessel_philox(o_particle *P, long n, double cx, double cz, double radius,
+                                   double vth, const o_fields *f, uint64_t seed, uint32_t stream)
+{
+    const double bessel_root = 2.404825557695773;
+    for (long k = 0; k < n; k++) {
+        o_rng r;
+        o_rng_philox_begin(&r, seed, stream, (uint32_t)k, 0, 1);
+        double x = 0, y = 0;
+        for (int t = 0; t < 256; t++) {
+            x = o_rng_uni(&r) * 2 - 1.0;
+            y = o_rng_uni(&r) * 2 - 1.0;
+            double rr = sqrt(SQR(x) + SQR(y));
+            if (rr > 1.0) continue;
+            if (!(o_rng_uni(&r) > j0(rr * bessel_root))) break;
+        }
+        x = x * radius + cx;
+        y = y * radius + cz;
+        o_particle *p = &P[k];
+        memset(p, 0, sizeof(*p));
+        if (x < f->x_min || x > f->x_max || y < f->z_min || y > f->z_max) { p->empty = 1; continue; }
+        p->x = x; p->z = y;
+        p->vx = o_rng_normal(&r) * vth;
+        p->vz = o_rng_normal(&r) * vth;
+        p->vy = o_rng_normal(&r) * vth;
+    }
+}
+
+/* add_particles_everywhere (particles.cpp:686-699) on the same streams: x, y,
+ * z uniform in the box (y is drawn and dropped: the Cartesian store has no
+ * y), rnor velocities vx, vy, vz, then time_to_death = rexp() * lifetime */
+void o_add_particles_everywhere_philox(o_particle *P, long n, double vth, double lifetime,
+                                       const o_fields *f, uint64_t seed, uint32_t stream)
+{
+    for (long k = 0; k < n; k++) {
+        o_rng r;
+        o_rng_philox_begin(&r, seed, stream, (uint32_t)k, 0, 1);
+        o_particle *p = &P[k];
+        memset(p, 0, sizeof(*p));
+        p->x = (f->x_max - f->x_min) * o_rng_uni(&r) + f->x_min;
+        (void)o_rng_uni(&r);
+        p->z = (f->z_max - f->z_min) * o_rng_uni(&r) + f->z_min;
+        p->vx = o_rng_normal(&r) * vth;
+        p->vy = o_rng_normal(&r) * vth;
+        p->vz = o_rng_normal(&r) * vth;
+        p->time_to_death = o_rng_rexp(&r) * lifetime;
+    }
+}
+
 /* ======================================================================= */
 /* Field3D                                                                 */
 /* ======================================================================= */

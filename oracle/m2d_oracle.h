@@ -225,6 +225,12 @@ long o_add_particles_on_disk(o_particle *p, long cap, long n_req, double cx, dou
                              double radius, double vth, const o_fields *f, o_rng *r);
 void o_add_particles_on_disk_philox(o_particle *p, long n, double cx, double cz, double radius,
                                     double vth, const o_fields *f, uint64_t seed, uint32_t stream);
+/* add_particles_bessel (particles.cpp:701-726), add_particles_everywhere (particles.cpp:686-699),
+ * on the device sampler's Philox streams */
+void o_add_particles_bessel_philox(o_particle *p, long n, double cx, double cz, double radius,
+                                   double vth, const o_fields *f, uint64_t seed, uint32_t stream);
+void o_add_particles_everywhere_philox(o_particle *p, long n, double vth, double lifetime,
+                                       const o_fields *f, uint64_t seed, uint32_t stream);
 
 /* Pic<CARTESIAN>::advance_init / advance with a solve every step (pic.cpp:330-384),
  * one species, no collisions, from a store already started by advance_boris_init;

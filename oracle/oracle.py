@@ -158,6 +158,10 @@ _SIGS = {
                                            C.c_double, C.c_double, C.POINTER(Fields), C.POINTER(Rng)]),
     "o_add_particles_on_disk_philox": (None, [_P, C.c_long, C.c_double, C.c_double, C.c_double,
                                               C.c_double, C.POINTER(Fields), C.c_uint64, C.c_uint32]),
+    "o_add_particles_bessel_philox": (None, [_P, C.c_long, C.c_double, C.c_double, C.c_double,
+                                             C.c_double, C.POINTER(Fields), C.c_uint64, C.c_uint32]),
+    "o_add_particles_everywhere_philox": (None, [_P, C.c_long, C.c_double, C.c_double, C.POINTER(Fields),
+                                                 C.c_uint64, C.c_uint32]),
     "o_f3_accumulate": (C.c_int, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double,
                                   C.c_double]),
     "o_f3_interpolate": (C.c_int, [C.POINTER(Grid3), _P, C.c_double, C.c_double, C.c_double, _PD]),

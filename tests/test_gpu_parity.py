@@ -56,7 +56,7 @@ def test_accumulate_matches_oracle_and_is_bit_reproducible(ctx):
         O.lib().o_f2_accumulate(C.byref(og), O.dptr(ref), q, a, b)
     nz = np.abs(ref) > 0
     assert np.max(np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])) < 1e-6  # north-star bound; ~1e-9 seen
-    # every particle deposits exactly 2^32 fixed-point units: total charge is N q to FP64 rounding
+    # every particle deposits exactly 2^31 fixed-point units: total charge is N q to FP64 rounding
     assert got.sum() == pytest.approx(N * q, rel=1e-12)
     rho2 = ctx.to_device(np.zeros(n * n))
     abi.accumulate(ctx, g, P, q, rho2.ptr)
@@ -438,6 +438,46 @@ def test_disk_sampler_matches_oracle_philox(ctx):
     assert r.max() <= 0.005 * (1 + 1e-6)
     # uniform on the disk: r^2 / R^2 ~ U(0,1)
     assert stats.kstest((r / 0.005) ** 2, "uniform").pvalue > 0.01
+
+
+def test_bessel_sampler_matches_oracle_philox(ctx):
+    """add_particles_bessel (particles.cpp:701-726) on the same Philox streams:
+    the same accepted samples (the disk crosses the box edge, so some slots stay
+    empty), positions and velocities to FP32 rounding."""
+    sc = Scene(ctx, n=64, geometry=abi.GEO_EMPTY, rf=False, solve_oracle=False)
+    n = 20000
+    P = abi.ParticlesSoA(ctx, n)
+    vth = math.sqrt(0.1 * QE / M_ION)
+    abi.add_particles_bessel(ctx, sc.fields, P, 0.004, 0.01, 0.005, vth, stream=5)
+    d = P.download()
+    op = O.particles(n)
+    O.lib().o_add_particles_bessel_philox(O.dptr(op), n, 0.004, 0.01, 0.005, vth, C.byref(sc.of), 1234, 5)
+    alive = P.alive()
+    assert np.array_equal(alive, op["empty"] == 0)
+    assert 0 < (~alive).sum() < 0.5 * n
+    for k in ("x", "z"):
+        np.testing.assert_allclose(d[k][alive], op[k][alive], rtol=1e-7)
+    for k in ("vx", "vy", "vz"):
+        np.testing.assert_allclose(d[k][alive], op[k][alive], rtol=1e-6, atol=1e-6 * vth)
+
+
+def test_everywhere_sampler_matches_oracle_philox(ctx):
+    """add_particles_everywhere (particles.cpp:686-699), with time_to_death."""
+    sc = Scene(ctx, n=64, geometry=abi.GEO_EMPTY, rf=False, solve_oracle=False)
+    n = 20000
+    P = abi.ParticlesSoA(ctx, n)
+    vth = math.sqrt(0.1 * QE / M_ION)
+    ttd = ctx.to_device(np.zeros(n))
+    abi.add_particles_everywhere(ctx, sc.fields, P, vth, ttd.ptr, 2e-9, stream=6)
+    d = P.download()
+    op = O.particles(n)
+    O.lib().o_add_particles_everywhere_philox(O.dptr(op), n, vth, 2e-9, C.byref(sc.of), 1234, 6)
+    assert P.alive().all()
+    for k in ("x", "z"):
+        np.testing.assert_allclose(d[k], op[k], rtol=1e-7)
+    for k in ("vx", "vy", "vz"):
+        np.testing.assert_allclose(d[k], op[k], rtol=1e-6, atol=1e-6 * vth)
+    np.testing.assert_allclose(ttd.get(), op["time_to_death"], rtol=1e-13)
 
 
 def test_out_of_range_gather_is_flagged_not_faulting(ctx):

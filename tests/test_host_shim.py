@@ -48,6 +48,9 @@ def host():
     L.m2dh_trap_run.restype = C.c_int
     L.m2dh_trap_run.argtypes = [C.POINTER(TrapCfg), _f, _f, _f, _f, _f, C.c_int64, C.POINTER(C.c_double),
                                 C.c_char_p, C.c_int]
+    L.m2dh_initscript_run.restype = C.c_int
+    L.m2dh_initscript_run.argtypes = [C.c_char_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                      _f, _f, _f, _f, _f, C.c_int64, C.POINTER(C.c_int64), C.c_char_p, C.c_int]
     L.m2dh_mcc_run.restype = C.c_int
     L.m2dh_mcc_run.argtypes = [C.POINTER(MccCfg), _f, _f, _f, _f, _f, C.POINTER(C.c_double), C.c_int64,
                                C.POINTER(C.c_double), C.c_char_p, C.c_int]
@@ -56,7 +59,7 @@ def host():
 
 def test_host_library_loads_and_exports():
     L = host()
-    for name in ("m2dh_list_selftest", "m2dh_trap_run", "m2dh_mcc_run"):
+    for name in ("m2dh_list_selftest", "m2dh_trap_run", "m2dh_mcc_run", "m2dh_initscript_run"):
         assert hasattr(L, name)
 
 
@@ -224,3 +227,37 @@ def test_pic_multicoll_run_matches_oracle():
         assert np.abs(out[c][:k] - p[c][alive]).max() <= 1e-4 * vscale, c
     for c in ("x", "z"):
         assert np.abs(out[c][:k] - p[c][alive]).max() <= 4 * cfg.steps * np.spacing(np.float32(cfg.length)), c
+
+
+@pytest.mark.gpu
+def test_initscript_samplers_match_oracle(tmp_path):
+    """add_particles_bessel / add_particles_everywhere lines of an initscript
+    (pic.cpp:241-326) through Species: each call draws from its own stream
+    (species stream ^ call << 16), as the oracle's Philox samplers do."""
+    script = tmp_path / "initscript.txt"
+    script.write_text("# samplers\nadd_particles_bessel ION 3000 0.004 0.01 0.005\n"
+                      "add_particles_everywhere ION 2000\n")
+    n_out = 5000
+    out = {k: np.zeros(n_out, np.float32) for k in ("x", "z", "vx", "vy", "vz")}
+    live = C.c_int64(0)
+    msg = C.create_string_buffer(512)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    mass, T, L, grid = 3 * 1.66053906660e-27, 1160.45, 2e-2, 64
+    rc = host().m2dh_initscript_run(str(script).encode(), grid, L, mass, T, 1234, fp(out["x"]), fp(out["z"]),
+                                    fp(out["vx"]), fp(out["vy"]), fp(out["vz"]), n_out, C.byref(live), msg, 512)
+    assert rc == 0, msg.value.decode()
+    dx = L / (grid - 1)
+    of = O.make_fields(O.grid2(grid, grid, dx, dx), np.full(grid * grid, O.FREE, np.uint8), np.zeros(grid * grid),
+                       x_max=L, z_max=L, geometry_empty=1)
+    vth = math.sqrt(abi.KB * T / mass)
+    pb = O.particles(3000)
+    O.lib().o_add_particles_bessel_philox(O.dptr(pb), 3000, 0.004, 0.01, 0.005, vth, C.byref(of), 1234, 0)
+    pe = O.particles(2000)
+    O.lib().o_add_particles_everywhere_philox(O.dptr(pe), 2000, vth, float("inf"), C.byref(of), 1234, 1 << 16)
+    want = np.concatenate([pb[pb["empty"] == 0], pe])
+    assert live.value == len(want) and live.value < 5000
+    k = live.value
+    for c in ("x", "z"):
+        np.testing.assert_allclose(out[c][:k], want[c], rtol=1e-7)
+    for c in ("vx", "vy", "vz"):
+        np.testing.assert_allclose(out[c][:k], want[c], rtol=1e-6, atol=1e-6 * vth)
