@@ -201,6 +201,21 @@ int m2d_fields2d_solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *in
 /* Fields::E at host-given points (fields.hpp:124-150), FP64 inputs, for tests */
 int m2d_fields2d_E(m2d_fields2d *f, const double *x, const double *z, int64_t n, double t,
                    double *ex, double *ez);
+/* Fields::load_magnetic_field (fields.cpp:870-959) as data, and Fields::B
+ * with magnetic_field_const = 0 (fields.hpp:171-176): Br and Bz on their own
+ * grid g (host arrays, row-major [nx*nz]), Bt = 0.  The particle passes then
+ * rotate every particle by the field interpolated at its position
+ * (Field2D::interpolate, Field2D.hpp:64-78), and the params' constant B is
+ * not used; a position outside the B grid is clamped onto it and raises
+ * M2D_FLAG_OUT_OF_RANGE (the reference throws).  NaN values are rejected
+ * ("garbage loaded").  Br == NULL goes back to the constant B.  The batched,
+ * adjoint and multicoll paths reject fields with a B grid. */
+int m2d_fields2d_set_magnetic_field(m2d_fields2d *f, const m2d_grid2d *g, const double *Br,
+                                    const double *Bz);
+/* Fields::B at n points (fields.hpp:152-177), device arrays, through the
+ * tables the passes use (FP32 values of a B grid) */
+int m2d_fields2d_B(m2d_fields2d *f, const double *x, const double *z, int64_t n, double *Br,
+                   double *Bz, double *Bt);
 
 /* ======================= Field2D kernels ================================= */
 

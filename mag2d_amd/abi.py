@@ -167,6 +167,8 @@ _SIGS = {
     "m2d_fields2d_reset": (C.c_int, [_P]),
     "m2d_fields2d_solve": (C.c_int, [_P, C.c_int, C.c_double, C.POINTER(SolveInfo)]),
     "m2d_fields2d_E": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, _P, _P]),
+    "m2d_fields2d_set_magnetic_field": (C.c_int, [_P, C.POINTER(Grid2D), _PD, _PD]),
+    "m2d_fields2d_B": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P]),
     "m2d_accumulate_2d": (C.c_int, [_P, C.POINTER(Grid2D), C.POINTER(Particles2D), C.c_double, _P]),
     "m2d_interpolate_2d": (C.c_int, [_P, C.POINTER(Grid2D), _P, _P, _P, C.c_int64, _P]),
     "m2d_grad_2d": (C.c_int, [_P, C.POINTER(Grid2D), _P, _P, _P, C.c_int64, _P, _P]),
@@ -456,6 +458,25 @@ class Fields2D:
         info = SolveInfo()
         check(lib().m2d_fields2d_solve(self.h, mode, tol, C.byref(info)))
         return info
+
+    def set_magnetic_field(self, g, Br, Bz):
+        """Fields::load_magnetic_field as data: Br, Bz (row-major) on grid g; None clears."""
+        if Br is None:
+            check(lib().m2d_fields2d_set_magnetic_field(self.h, None, None, None))
+            return
+        br = np.ascontiguousarray(Br, np.float64)
+        bz = np.ascontiguousarray(Bz, np.float64)
+        check(lib().m2d_fields2d_set_magnetic_field(self.h, C.byref(g), _ptr(br, np.float64), _ptr(bz, np.float64)))
+
+    def B(self, x, z):
+        x = np.ascontiguousarray(x, np.float64)
+        z = np.ascontiguousarray(z, np.float64)
+        n = len(x)
+        dx, dz = self.ctx.to_device(x), self.ctx.to_device(z)
+        out = [DeviceArray(self.ctx, (n,), np.float64) for _ in range(3)]
+        check(lib().m2d_fields2d_B(self.h, C.c_void_p(dx.ptr), C.c_void_p(dz.ptr), n, C.c_void_p(out[0].ptr),
+                                   C.c_void_p(out[1].ptr), C.c_void_p(out[2].ptr)))
+        return tuple(o.get() for o in out)
 
     def E(self, x, z, t=0.0):
         x = np.ascontiguousarray(x, np.float64)

@@ -347,6 +347,7 @@ int m2d_batch_create(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, cons
               "m2d_batch_create: NULL argument");
     *out = nullptr;
     M2D_CHECK(f->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_batch_create: fields belong to another context");
+    M2D_CHECK(!f->bt, M2D_ERR_UNSUPPORTED, "m2d_batch_create: gridded B is not supported by the batched pass");
     M2D_CHECK(!f->p.selfconsistent, M2D_ERR_UNSUPPORTED,
               "m2d_batch_create: runs of a batch carry no space charge (selfconsistent = 0)");
     M2D_CHECK(d->n_runs >= 1, M2D_ERR_BAD_ARG, "m2d_batch_create: no runs");

@@ -1389,11 +1389,37 @@ int m2d_fields2d_destroy(m2d_fields2d *f)
     if (!f) return M2D_OK;
     if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
     for (void *p : {(void *)f->mask, (void *)f->cellfree, (void *)f->winfree, (void *)f->voltage, (void *)f->u,
-                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz})
+                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz, (void *)f->bt})
         if (p) (void)hipFree(p);
     mg_free(f->mg);
     if (f->ctx) ctx_release(f->ctx);
     delete f;
+    return M2D_OK;
+}
+
+int m2d_fields2d_set_magnetic_field(m2d_fields2d *f, const m2d_grid2d *g, const double *Br, const double *Bz)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields2d_set_magnetic_field: NULL fields");
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    if (f->bt) {
+        M2D_HIP(hipFree(f->bt));
+        f->bt = nullptr;
+    }
+    if (!Br) return M2D_OK;  // back to the constant B of the params
+    M2D_CHECK(g && Bz, M2D_ERR_BAD_ARG, "m2d_fields2d_set_magnetic_field: NULL argument");
+    M2D_CHECK(g->nx >= 2 && g->nz >= 2 && g->dx > 0 && g->dz > 0 && (int64_t)g->nx * g->nz < (1ll << 30),
+              M2D_ERR_BAD_ARG, "m2d_fields2d_set_magnetic_field: bad grid %d x %d", g->nx, g->nz);
+    const size_t n = (size_t)g->nx * g->nz;
+    std::vector<float2> h(n);
+    for (size_t k = 0; k < n; k++) {
+        // Fields::load_magnetic_field (fields.cpp:954-958)
+        M2D_CHECK(!std::isnan(Br[k]) && !std::isnan(Bz[k]), M2D_ERR_BAD_ARG,
+                  "Fields::load_magnetic_field() garbage loaded");
+        h[k] = make_float2((float)Br[k], (float)Bz[k]);
+    }
+    M2D_TRY(dmalloc((void **)&f->bt, n * sizeof(float2)));
+    M2D_HIP(hipMemcpy(f->bt, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    f->bgrid = *g;
     return M2D_OK;
 }
 

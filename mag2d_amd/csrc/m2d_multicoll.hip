@@ -107,7 +107,7 @@ int m2d_advance_multicoll_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d 
     M2D_CHECK(!mcc || mcc->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_advance_multicoll_2d: mcc belongs to another context");
     const m2d_fields2d_params &P = f->p;
     // the reference's own restrictions (particles.cpp:818-823)
-    M2D_CHECK(P.Br == 0.0 && P.Bz == 0.0 && P.Bt == 0.0, M2D_ERR_UNSUPPORTED,
+    M2D_CHECK(!f->bt && P.Br == 0.0 && P.Bz == 0.0 && P.Bt == 0.0, M2D_ERR_UNSUPPORTED,
               "Species<CARTESIAN>::advance_multicoll() implemented for zero B only");
     M2D_CHECK(!P.selfconsistent && P.geometry_empty, M2D_ERR_UNSUPPORTED,
               "Species<CARTESIAN>::advance_multicoll() implemented for const extern field only: "

@@ -82,6 +82,10 @@ struct m2d_fields2d {
     double *voltage = nullptr, *u = nullptr, *urf = nullptr, *rho = nullptr;
     float2 *gx = nullptr;         // (nx+1)*nz edge x-gradients {u, uRF}, padded rows 0 and nx
     float2 *gz = nullptr;         // nx*(nz+1) edge z-gradients {u, uRF}, padded cols 0 and nz
+    // gridded B (Fields::Br, Bz with magnetic_field_const = 0): {Br, Bz} per
+    // node of its own grid, null for the constant B of the params
+    float2 *bt = nullptr;
+    m2d_grid2d bgrid{};
     m2d::Mg mg;
     std::vector<uint8_t> h_mask;
 };

@@ -48,7 +48,50 @@ struct PushArgs {
     const int4 *sched;             // chunk -> (tile, first slot, end slot)
     const unsigned int *n_chunks;
     int ntz;                       // sort tiles along z
+    // gridded B (Fields::B with magnetic_field_const = 0): {Br, Bz} per node
+    // of its own grid, null for the constant rotation above
+    const float2 *bt;
+    int bnx, bnz;
+    double bidx, bidz, bxmin, bzmin;
+    double rscale;                 // q dt / (2 m) times the rotation's scale
 };
+
+// Fields::B with magnetic_field_const = 0 (fields.hpp:171-176): Br, Bz by
+// Field2D::interpolate (Field2D.hpp:64-78) of the B grid, Bt = 0.  Index
+// arithmetic in FP64, weights FP32.  Returns true for a point outside the B
+// grid (a NaN included): it is clamped onto the grid, where the reference
+// throws.
+__device__ __forceinline__ bool gather_B(const PushArgs &a, double xd, double zd, float &br, float &bz)
+{
+    double X = (xd - a.bxmin) * a.bidx, Z = (zd - a.bzmin) * a.bidz;
+    const bool bad = !(X >= 0.0 && X <= a.bnx - 1.0 && Z >= 0.0 && Z <= a.bnz - 1.0);
+    if (bad) {  // fmax of a NaN and 0 is 0
+        X = fmin(fmax(X, 0.0), (double)(a.bnx - 1));
+        Z = fmin(fmax(Z, 0.0), (double)(a.bnz - 1));
+    }
+    const int i = min((int)X, a.bnx - 2), j = min((int)Z, a.bnz - 2);
+    const float u = (float)(X - i), v = (float)(Z - j);
+    const float2 *r0 = a.bt + ((size_t)i * a.bnz + j), *r1 = r0 + a.bnz;
+    const float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
+    const float w00 = (1.f - u) * (1.f - v), w10 = u * (1.f - v), w01 = (1.f - u) * v, w11 = u * v;
+    br = w00 * p00.x + w10 * p10.x + w01 * p01.x + w11 * p11.x;
+    bz = w00 * p00.y + w10 * p10.y + w01 * p01.y + w11 * p11.y;
+    return bad;
+}
+
+// Boris rotation vectors t = B q dt/(2m), s = 2t/(1 + t^2) (particles.cpp:957-975)
+// of the local field, with (Bx, By, Bz) = (Br, Bt, Bz) and Bt = 0
+__device__ __forceinline__ bool local_rotation(const PushArgs &a, float x, float z, float &tx, float &ty, float &tz,
+                                               float &sx, float &sy, float &sz)
+{
+    float br, bz;
+    const bool bad = gather_B(a, (double)x, (double)z, br, bz);
+    const double t1 = (double)br * a.rscale, t3 = (double)bz * a.rscale;
+    const double sc = 2.0 / (1.0 + t1 * t1 + t3 * t3);
+    tx = (float)t1; ty = 0.f; tz = (float)t3;
+    sx = (float)(t1 * sc); sy = 0.f; sz = (float)(t3 * sc);
+    return bad;
+}
 
 // ------------------------------------------------------------------ deposit
 // CIC weights (Field2D.hpp:45-62) as 2^-31 fixed point.  The cell offsets
@@ -165,8 +208,9 @@ __device__ __forceinline__ void wave_deposit(DepRun &d, unsigned long long *acc,
 
 // FAST: the common configuration fixed at compile time (B = 0, no periodic
 // wrap, no constant field, boundary and free-cell tests on, origin at 0),
-// which frees the scalar registers and branches the generic flags cost
-template <bool DEPOSIT, bool MCC, bool TILED, bool FAST>
+// which frees the scalar registers and branches the generic flags cost.
+// BGRID (generic pass only): B from the gridded field at each particle.
+template <bool DEPOSIT, bool MCC, bool TILED, bool FAST, bool BGRID = false>
 // four wavefronts per SIMD (128 VGPRs): a thread holds its four particles and
 // two gathers in registers (with fewer VGPRs the scalar and vector spills cost
 // more than the occupancy gains)
@@ -266,12 +310,15 @@ k_advance2d(PushArgs a)
                     vx = fmaf(ex, a.qmdt2, vx);
                     vz = fmaf(ez, a.qmdt2, vz);
                     if (!FAST && a.rotate) {  // with B = 0 the rotation is the identity, bit for bit
-                        float vpr = vx - vy * a.tz + vz * a.ty;
-                        float vpz = vz - vx * a.ty + vy * a.tx;
-                        float vpt = vy - vz * a.tx + vx * a.tz;
-                        vx = vx - vpt * a.sz + vpz * a.sy;
-                        vy = vy - vpz * a.sx + vpr * a.sz;
-                        vz = vz - vpr * a.sy + vpt * a.sx;
+                        float tx = a.tx, ty = a.ty, tz = a.tz, sx = a.sx, sy = a.sy, sz = a.sz;
+                        // B at the position of the E gather, before the drift
+                        if (BGRID) gbad |= local_rotation(a, x, z, tx, ty, tz, sx, sy, sz);
+                        float vpr = vx - vy * tz + vz * ty;
+                        float vpz = vz - vx * ty + vy * tx;
+                        float vpt = vy - vz * tx + vx * tz;
+                        vx = vx - vpt * sz + vpz * sy;
+                        vy = vy - vpz * sx + vpr * sz;
+                        vz = vz - vpr * sy + vpt * sx;
                     }
                     vx = fmaf(ex, a.qmdt2, vx);
                     vz = fmaf(ez, a.qmdt2, vz);
@@ -456,15 +503,41 @@ __global__ void __launch_bounds__(kBlock) k_advance_init2d(PushArgs a)
     float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
     float ex, ez;
     gather_E(a, (double)x, (double)z, a.phase, ex, ez);
-    float vpr = vx - vy * a.tz + vz * a.ty;
-    float vpz = vz - vx * a.ty + vy * a.tx;
-    float vpt = vy - vz * a.tx + vx * a.tz;
-    vx = vx - vpt * a.sz + vpz * a.sy;
-    vy = vy - vpz * a.sx + vpr * a.sz;
-    vz = vz - vpr * a.sy + vpt * a.sx;
+    float tx = a.tx, ty = a.ty, tz = a.tz, sx = a.sx, sy = a.sy, sz = a.sz;
+    if (a.bt && local_rotation(a, x, z, tx, ty, tz, sx, sy, sz)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+    float vpr = vx - vy * tz + vz * ty;
+    float vpz = vz - vx * ty + vy * tx;
+    float vpt = vy - vz * tx + vx * tz;
+    vx = vx - vpt * sz + vpz * sy;
+    vy = vy - vpz * sx + vpr * sz;
+    vz = vz - vpr * sy + vpt * sx;
     vx = fmaf(ex, a.qmdt2, vx);
     vz = fmaf(ez, a.qmdt2, vz);
     a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+}
+
+// Fields::B at FP64 points (magnetic_field_const = 0: the B grid's tables)
+__global__ void k_fields_B(PushArgs a, const double *xs, const double *zs, int64_t n, double *br, double *bz)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float r, z;
+    if (gather_B(a, xs[k], zs[k], r, z)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+    br[k] = r;
+    bz[k] = z;
+}
+
+__global__ void k_fill_f64(double *d, double v, int64_t n)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) d[k] = v;
+}
+
+static int fill_f64(m2d_ctx *ctx, double *d, double v, size_t n)
+{
+    k_fill_f64<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(d, v, (int64_t)n);
+    M2D_HIP(hipGetLastError());
+    return M2D_OK;
 }
 
 // Fields::E at FP64 points through the same tables (test entry point)
@@ -907,6 +980,18 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.ntz = 0;
     a.acc = nullptr;
     a.counters = f->ctx->counters;
+    a.bt = f->bt;
+    a.bnx = a.bnz = 0;
+    a.bidx = a.bidz = a.bxmin = a.bzmin = 0.0;
+    a.rscale = tmp;
+    if (f->bt) {
+        // Fields::B with magnetic_field_const = 0: the params' B is not used
+        a.bnx = f->bgrid.nx; a.bnz = f->bgrid.nz;
+        a.bidx = 1.0 / f->bgrid.dx; a.bidz = 1.0 / f->bgrid.dz;
+        a.bxmin = f->bgrid.xmin; a.bzmin = f->bgrid.zmin;
+        a.tx = a.ty = a.tz = a.sx = a.sy = a.sz = 0.f;
+        a.rotate = 1;
+    }
 }
 
 }  // namespace m2d
@@ -967,9 +1052,12 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
                           a.xmin == 0.0 && a.zmin == 0.0;
         void (*kern)(PushArgs) = nullptr;
         int g = 0;
+        const bool bgrid = f->bt != nullptr;  // B per particle (generic pass)
         if (T) {
             g = T->max_chunks;
             if (fast) kern = use_mcc ? k_advance2d<true, true, true, true> : k_advance2d<true, false, true, true>;
+            else if (bgrid)
+                kern = use_mcc ? k_advance2d<true, true, true, false, true> : k_advance2d<true, false, true, false, true>;
             else kern = use_mcc ? k_advance2d<true, true, true, false> : k_advance2d<true, false, true, false>;
         } else {
             g = (int)((p->n + kChunk - 1) / kChunk);
@@ -978,6 +1066,11 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
                 else if (deposit) kern = k_advance2d<true, false, false, true>;
                 else if (use_mcc) kern = k_advance2d<false, true, false, true>;
                 else kern = k_advance2d<false, false, false, true>;
+            } else if (bgrid) {
+                if (deposit && use_mcc) kern = k_advance2d<true, true, false, false, true>;
+                else if (deposit) kern = k_advance2d<true, false, false, false, true>;
+                else if (use_mcc) kern = k_advance2d<false, true, false, false, true>;
+                else kern = k_advance2d<false, false, false, false, true>;
             } else {
                 if (deposit && use_mcc) kern = k_advance2d<true, true, false, false>;
                 else if (deposit) kern = k_advance2d<true, false, false, false>;
@@ -1035,6 +1128,30 @@ int m2d_fields2d_E(m2d_fields2d *f, const double *x, const double *z, int64_t n,
     if (n > 0) {
         k_fields_E<<<(unsigned)((n + 255) / 256), 256, 0, f->ctx->stream>>>(a, x, z, n, ex, ez);
         M2D_HIP(hipGetLastError());
+    }
+    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
+    return M2D_OK;
+}
+
+int m2d_fields2d_B(m2d_fields2d *f, const double *x, const double *z, int64_t n, double *Br, double *Bz,
+                   double *Bt)
+{
+    M2D_CHECK(f && (n == 0 || (x && z && Br && Bz && Bt)), M2D_ERR_BAD_ARG, "m2d_fields2d_B: NULL argument");
+    if (n == 0) return M2D_OK;
+    if (!f->bt) {
+        // magnetic_field_const (fields.hpp:154-169)
+        M2D_TRY(fill_f64(f->ctx, Br, f->p.Br, (size_t)n));
+        M2D_TRY(fill_f64(f->ctx, Bz, f->p.Bz, (size_t)n));
+        M2D_TRY(fill_f64(f->ctx, Bt, f->p.Bt, (size_t)n));
+    } else {
+        m2d_species2d s{};
+        s.charge = 1; s.mass = 1; s.dt = 1;
+        PushArgs a;
+        m2d_particles2d none{};
+        fill_push_args(a, f, &s, &none, 1.0, 1.0);
+        k_fields_B<<<(unsigned)((n + 255) / 256), 256, 0, f->ctx->stream>>>(a, x, z, n, Br, Bz);
+        M2D_HIP(hipGetLastError());
+        M2D_TRY(fill_f64(f->ctx, Bt, 0.0, (size_t)n));
     }
     M2D_HIP(hipStreamSynchronize(f->ctx->stream));
     return M2D_OK;

@@ -199,6 +199,9 @@ struct Param {
     int rf = 0;
     double rf_amplitude = 0.0, rf_omega = 0.0, rf_U0 = 0.0;
     double Br = 0.0, Bz = 0.0, Bt = 0.0;
+    // 0: B from magnetic_field_file, loaded by Pic (param.cpp:41-42, pic.cpp:148-149)
+    int magnetic_field_const = 1;
+    std::string magnetic_field_file;
     double n_particles_total = 1e5, density_total = 1e11;
     uint64_t seed = 1234;
     double solve_tolerance = 1e-10;  // FP64 relative residual of every solve
@@ -253,6 +256,9 @@ public:
     // Fields::E / B at one point (fields.hpp:124-177), t in s
     void E(double x, double y, double &gx, double &gy, double t);
     void B(double x, double y, double &Br, double &Bz, double &Bt) const;
+    // fields.cpp:870-959: lines "r z Br Bz" of a regular grid in any order of
+    // rows; the passes then take B from it (magnetic_field_const = 0)
+    void load_magnetic_field(const char *fname);
     // send host edits of u / uRF and rebuild the gather tables
     void update();
     m2d_fields2d *handle();

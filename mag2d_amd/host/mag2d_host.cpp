@@ -501,12 +501,83 @@ void Fields::E(double x, double y, double &gx, double &gy, double t)
     gy = out[1];
 }
 
-void Fields::B(double, double, double &Br, double &Bz, double &Bt) const
+void Fields::B(double x, double y, double &Br, double &Bz, double &Bt) const
 {
-    // magnetic_field_const (fields.hpp:154-169)
-    Br = p_->Br;
-    Bz = p_->Bz;
-    Bt = p_->Bt;
+    if (p_->magnetic_field_const) {
+        // fields.hpp:154-169
+        Br = p_->Br;
+        Bz = p_->Bz;
+        Bt = p_->Bt;
+        return;
+    }
+    // fields.hpp:171-176, through the device tables of the passes
+    double *s = dev_->staging(5);
+    double pos[2] = {x, y}, out[3];
+    check(m2d_memcpy_h2d(dev_->ctx(), s, pos, sizeof pos));
+    check(m2d_fields2d_B(f_, s, s + 1, 1, s + 2, s + 3, s + 4));
+    check(m2d_memcpy_d2h(dev_->ctx(), out, s + 2, sizeof out));
+    Br = out[0];
+    Bz = out[1];
+    Bt = out[2];
+}
+
+// double2int (util.cpp:22-28): the nearest integer, which must be within 1e-2
+static int nearest_int(double x)
+{
+    const int r = (int)(x + 0.5);
+    if (std::fabs(r - x) > 1e-2)
+        throw std::runtime_error("double2int() " + std::to_string(x) + " is not integer\n");
+    return r;
+}
+
+void Fields::load_magnetic_field(const char *fname)
+{
+    std::ifstream fr(fname);
+    if (fr.fail()) throw std::runtime_error("Fields::load_magnetic_field(): failed opening file\n");
+    std::vector<double> rv, zv, brv, bzv;
+    std::string line;
+    while (std::getline(fr, line)) {
+        std::istringstream ss(line);
+        double a, b, c, d;
+        if (ss >> a >> b >> c >> d) {
+            rv.push_back(a);
+            zv.push_back(b);
+            brv.push_back(c);
+            bzv.push_back(d);
+        }
+    }
+    const size_t n = rv.size();
+    // spacing, first and last sample of one axis (fields.cpp:902-928): the
+    // first change between consecutive rows; a negative one swaps the ends
+    auto axis = [&](const std::vector<double> &v, double &h, double &lo, double &hi) {
+        size_t i = 1;
+        while (i < n && v[i] - v[i - 1] == 0.0) i++;
+        if (i >= n) throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
+        h = v[i] - v[i - 1];
+        lo = v[0];
+        hi = v[n - 1];
+        if (h < 0) {
+            h = -h;
+            std::swap(lo, hi);
+        }
+        return nearest_int((hi - lo) / h + 1);
+    };
+    if (n < 4) throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
+    double dr, rmin, rmax, dz, zmin, zmax;
+    const int rs = axis(rv, dr, rmin, rmax), zs = axis(zv, dz, zmin, zmax);
+    if ((size_t)rs * (size_t)zs != n)
+        throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
+    std::vector<double> br((size_t)rs * zs, NAN), bz((size_t)rs * zs, NAN);
+    for (size_t k = 0; k < n; k++) {
+        const int ri = nearest_int((rv[k] - rmin) / dr), zi = nearest_int((zv[k] - zmin) / dz);
+        if (ri < 0 || ri >= rs || zi < 0 || zi >= zs)
+            throw std::runtime_error("Fields::load_magnetic_field() garbage loaded");
+        br[(size_t)ri * zs + zi] = brv[k];
+        bz[(size_t)ri * zs + zi] = bzv[k];
+    }
+    // a sample left unset is NaN: the library rejects it ("garbage loaded")
+    m2d_grid2d g{rs, zs, dr, dz, rmin, zmin};
+    check(m2d_fields2d_set_magnetic_field(f_, &g, br.data(), bz.data()));
 }
 
 // ================================================================ Species
@@ -914,6 +985,8 @@ void Species::sort()
 Pic::Pic(Param &p, const std::vector<SpeciesParams> &species, const t_grid *grid, int device)
     : param(p), dev(std::make_shared<Device>(device, p.seed)), field(p, dev, grid ? *grid : t_grid(p))
 {
+    // pic.cpp:148-149
+    if (!p.magnetic_field_const) field.load_magnetic_field(p.magnetic_field_file.c_str());
     for (size_t i = 0; i < species.size(); i++)
         speclist.emplace_back(new Species(species[i], p, field, (uint32_t)i));
     std::vector<Species *> all;
@@ -1088,8 +1161,8 @@ typedef struct {
     uint64_t seed;
 } m2dh_trap_cfg;
 
-int m2dh_trap_run(const m2dh_trap_cfg *c, float *out_x, float *out_z, float *out_vx, float *out_vy,
-                  float *out_vz, int64_t n_out, double *metrics, char *msg, int len)
+static int trap_run(const m2dh_trap_cfg *c, const char *bfile, float *out_x, float *out_z, float *out_vx,
+                    float *out_vy, float *out_vz, int64_t n_out, double *metrics, char *msg, int len)
 {
     try {
         Param p;
@@ -1098,6 +1171,10 @@ int m2dh_trap_run(const m2dh_trap_cfg *c, float *out_x, float *out_z, float *out
         p.geometry = EMPTY;
         p.selfconsistent = c->selfconsistent;
         p.Bz = c->Bz;
+        if (bfile) {
+            p.magnetic_field_const = 0;
+            p.magnetic_field_file = bfile;
+        }
         p.n_particles_total = c->n_particles;
         p.density_total = c->density_total;
         p.seed = c->seed;
@@ -1193,6 +1270,20 @@ int m2dh_initscript_run(const char *script, int32_t grid, double length, double 
     }
     msg[0] = 0;
     return 0;
+}
+
+int m2dh_trap_run(const m2dh_trap_cfg *c, float *out_x, float *out_z, float *out_vx, float *out_vy,
+                  float *out_vz, int64_t n_out, double *metrics, char *msg, int len)
+{
+    return trap_run(c, nullptr, out_x, out_z, out_vx, out_vy, out_vz, n_out, metrics, msg, len);
+}
+
+// the same trap with magnetic_field_const = 0 and B from `bfile` (the
+// reference's "r z Br Bz" lines); cfg->Bz is not used
+int m2dh_trap_run_bfile(const m2dh_trap_cfg *c, const char *bfile, float *out_x, float *out_z, float *out_vx,
+                        float *out_vy, float *out_vz, int64_t n_out, double *metrics, char *msg, int len)
+{
+    return trap_run(c, bfile, out_x, out_z, out_vx, out_vy, out_vz, n_out, metrics, msg, len);
 }
 
 typedef struct {

@@ -772,18 +772,32 @@ static o_rng *species_rng(o_species *s, o_rng *local, long slot)
     return local;
 }
 
+/* Fields::B (fields.hpp:152-177) */
+int o_fields_B(const o_fields *f, double x, double z, double *Br, double *Bz, double *Bt)
+{
+    if (!f->b_grid) {
+        *Br = f->Br; *Bz = f->Bz; *Bt = f->Bt;
+        return 0;
+    }
+    *Bt = 0.0;
+    if (o_f2_interpolate(&f->gB, f->BgR, x, z, Br)) return -2;
+    return o_f2_interpolate(&f->gB, f->BgZ, x, z, Bz);
+}
+
 /* Species<CARTESIAN>::advance_boris (particles.cpp:924-995) */
 void o_advance_boris(const o_fields *f, o_species *s, o_particle *P, long n, long *n_coll)
 {
     const double qmdt = s->charge / s->mass * s->dt;
     const double prob = 1.0 - exp(-s->dt / s->lifetime);
-    const double Bx = f->Br, Bz = f->Bz, By = f->Bt;
     long nc = 0;
     for (long k = 0; k < n; k++) {
         o_particle *I = &P[k];
         if (I->empty) continue;
         double fx, fz;
         o_fields_E(f, I->x, I->z, s->niter * s->dt, &fx, &fz);
+        /* field->B(I->x, I->z, Bx, Bz, By) (particles.cpp:946-949) */
+        double Bx, Bz, By;
+        o_fields_B(f, I->x, I->z, &Bx, &Bz, &By);
         I->vx += fx * qmdt / 2.0;
         I->vz += fz * qmdt / 2.0;
         double tmp = s->charge * s->dt / (2.0 * s->mass);
@@ -816,12 +830,13 @@ void o_advance_boris(const o_fields *f, o_species *s, o_particle *P, long n, lon
 void o_advance_boris_init(const o_fields *f, o_species *s, o_particle *P, long n)
 {
     const double qmdt = -s->charge / s->mass * s->dt;
-    const double Bx = f->Br, Bz = f->Bz, By = f->Bt;
     for (long k = 0; k < n; k++) {
         o_particle *I = &P[k];
         if (I->empty) continue;
         double fx, fz;
         o_fields_E(f, I->x, I->z, s->niter * s->dt, &fx, &fz);
+        double Bx, Bz, By;
+        o_fields_B(f, I->x, I->z, &Bx, &Bz, &By);
         double tmp = -0.5 * s->charge * s->dt / (2.0 * s->mass);
         double tx = Bx * tmp, ty = By * tmp, tz = Bz * tmp;
         double vpr = I->vx - I->vy * tz + I->vz * ty;

@@ -191,6 +191,10 @@ typedef struct {
     int boundary;                /* O_BC_FREE / O_BC_PERIODIC */
     const double *u, *uRF;
     const uint8_t *mask;
+    /* magnetic_field_const = 0 (fields.hpp:171-176): Br, Bz on the grid gB */
+    int b_grid;
+    o_grid2 gB;
+    const double *BgR, *BgZ;
 } o_fields;
 
 typedef struct {
@@ -206,6 +210,8 @@ typedef struct {
 } o_species;
 
 void o_fields_E(const o_fields *f, double x, double z, double t, double *ex, double *ez); /* fields.hpp:124-150 */
+/* Fields::B (fields.hpp:152-177); -2 for a gridded B outside its grid (the reference throws) */
+int  o_fields_B(const o_fields *f, double x, double z, double *Br, double *Bz, double *Bt);
 void o_advance_boris(const o_fields *f, o_species *s, o_particle *p, long n, long *n_coll);
 void o_advance_boris_init(const o_fields *f, o_species *s, o_particle *p, long n);
 double o_rng_rexp(o_rng *r);   /* Exp(1) by inversion (the reference's ziggurat: statistical parity) */

@@ -98,7 +98,8 @@ class Fields(C.Structure):
                 ("electric_field_from_file", C.c_int), ("extern_field", C.c_double), ("rf", C.c_int),
                 ("rf_amplitude", C.c_double), ("rf_omega", C.c_double), ("rf_U0", C.c_double),
                 ("Br", C.c_double), ("Bz", C.c_double), ("Bt", C.c_double), ("boundary", C.c_int),
-                ("u", C.c_void_p), ("uRF", C.c_void_p), ("mask", C.c_void_p)]
+                ("u", C.c_void_p), ("uRF", C.c_void_p), ("mask", C.c_void_p),
+                ("b_grid", C.c_int), ("gB", Grid2), ("BgR", C.c_void_p), ("BgZ", C.c_void_p)]
 
 
 class Species(C.Structure):
@@ -143,6 +144,7 @@ _SIGS = {
     "o_svmax_find": (C.c_double, [C.POINTER(Inter), C.c_int, C.c_double, C.c_int]),
     "o_scatter": (C.c_int, [C.POINTER(Mcc), _P, C.POINTER(Rng)]),
     "o_fields_E": (None, [C.POINTER(Fields), C.c_double, C.c_double, C.c_double, _PD, _PD]),
+    "o_fields_B": (C.c_int, [C.POINTER(Fields), C.c_double, C.c_double, _PD, _PD, _PD]),
     "o_advance_boris": (None, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long,
                                C.POINTER(C.c_long)]),
     "o_advance_boris_init": (None, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long]),
@@ -239,6 +241,16 @@ def make_fields(g: Grid2, mask, u, urf=None, *, x_max, z_max, geometry_empty=0, 
                electric_field_from_file, extern_field, rf, rf_amplitude, rf_omega, rf_U0, Br, Bz, Bt,
                boundary, keep[1].ctypes.data, keep[2].ctypes.data, keep[0].ctypes.data)
     f._keep = keep
+    return f
+
+
+def set_b_grid(f, gB, Br, Bz):
+    """Fields::B with magnetic_field_const = 0: Br, Bz (row-major) on grid gB."""
+    keep = [np.ascontiguousarray(Br, np.float64), np.ascontiguousarray(Bz, np.float64)]
+    f.b_grid = 1
+    f.gB = gB
+    f.BgR, f.BgZ = keep[0].ctypes.data, keep[1].ctypes.data
+    f._keep_b = keep
     return f
 
 

@@ -48,6 +48,9 @@ def host():
     L.m2dh_trap_run.restype = C.c_int
     L.m2dh_trap_run.argtypes = [C.POINTER(TrapCfg), _f, _f, _f, _f, _f, C.c_int64, C.POINTER(C.c_double),
                                 C.c_char_p, C.c_int]
+    L.m2dh_trap_run_bfile.restype = C.c_int
+    L.m2dh_trap_run_bfile.argtypes = [C.POINTER(TrapCfg), C.c_char_p, _f, _f, _f, _f, _f, C.c_int64,
+                                      C.POINTER(C.c_double), C.c_char_p, C.c_int]
     L.m2dh_initscript_run.restype = C.c_int
     L.m2dh_initscript_run.argtypes = [C.c_char_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                       _f, _f, _f, _f, _f, C.c_int64, C.POINTER(C.c_int64), C.c_char_p, C.c_int]
@@ -59,7 +62,8 @@ def host():
 
 def test_host_library_loads_and_exports():
     L = host()
-    for name in ("m2dh_list_selftest", "m2dh_trap_run", "m2dh_mcc_run", "m2dh_initscript_run"):
+    for name in ("m2dh_list_selftest", "m2dh_trap_run", "m2dh_trap_run_bfile", "m2dh_mcc_run",
+                 "m2dh_initscript_run"):
         assert hasattr(L, name)
 
 
@@ -73,8 +77,9 @@ def test_particle_container_semantics_and_checkpoint(tmp_path):
     assert len(raw) == 8 + 64 * live and live <= size
 
 
-def oracle_trap(cfg):
-    """Pic<CARTESIAN> advance_init + advance (pic.cpp:330-384) in the oracle."""
+def oracle_trap(cfg, bgrid=None):
+    """Pic<CARTESIAN> advance_init + advance (pic.cpp:330-384) in the oracle;
+    bgrid = (Grid2, Br, Bz) for magnetic_field_const = 0."""
     n, L = cfg.grid, cfg.length
     dx = L / (n - 1)
     og = O.grid2(n, n, dx, dx)
@@ -90,6 +95,8 @@ def oracle_trap(cfg):
     of = O.make_fields(og, mask, u, x_max=L, z_max=L, geometry_empty=1, selfconsistent=cfg.selfconsistent,
                        Bz=cfg.Bz)
     u = of._keep[1]  # the array the oracle fields read
+    if bgrid is not None:
+        O.set_b_grid(of, *bgrid)
 
     def solve(rho_f):
         b = rho_f.copy()
@@ -261,3 +268,42 @@ def test_initscript_samplers_match_oracle(tmp_path):
         np.testing.assert_allclose(out[c][:k], want[c], rtol=1e-7)
     for c in ("vx", "vy", "vz"):
         np.testing.assert_allclose(out[c][:k], want[c], rtol=1e-6, atol=1e-6 * vth)
+
+
+@pytest.mark.gpu
+def test_pic_trap_with_magnetic_field_file_matches_oracle(tmp_path):
+    """magnetic_field_const = 0: Pic loads B from a file of "r z Br Bz" lines
+    (Fields::load_magnetic_field, fields.cpp:870-959; rows written with r
+    descending, which the reader turns round) and the push takes B per
+    particle (Fields::B, fields.hpp:171-176)."""
+    cfg = TrapCfg(grid=64, n_particles=3000, steps=25, selfconsistent=1, length=2e-2, dt=1e-11, Bz=0.0,
+                  end_cap_V=-10.0, temperature_K=11604.5, mass=9.11e-31, charge=-abi.QE, radius=5e-3,
+                  density_total=1e13, seed=1234)
+    nb, lo, hi = 26, -1e-3, 2.4e-2
+    h = (hi - lo) / (nb - 1)
+    r = lo + h * np.arange(nb)
+    R, Z = np.meshgrid(r, r, indexing="ij")
+    bz = 0.1 * (1 + ((Z - 0.01) / 8e-3) ** 2)
+    br = -0.1 * (R - 0.01) * (Z - 0.01) / 8e-3 ** 2
+    lines = [f"{R[i, j]:.17g} {Z[i, j]:.17g} {br[i, j]:.17g} {bz[i, j]:.17g}"
+             for i in reversed(range(nb)) for j in range(nb)]
+    bfile = tmp_path / "B.txt"
+    bfile.write_text("# r z Br Bz\n" + "\n".join(lines) + "\n")
+    n = cfg.n_particles
+    out = {k: np.zeros(n, np.float32) for k in ("x", "z", "vx", "vy", "vz")}
+    m = (C.c_double * 6)()
+    msg = C.create_string_buffer(512)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    rc = host().m2dh_trap_run_bfile(C.byref(cfg), str(bfile).encode(), fp(out["x"]), fp(out["z"]), fp(out["vx"]),
+                                    fp(out["vy"]), fp(out["vz"]), n, m, msg, 512)
+    assert rc == 0, msg.value.decode()
+    p, rho_f, u = oracle_trap(cfg, (O.grid2(nb, nb, h, h, lo, lo), br.ravel(), bz.ravel()))
+    alive = p["empty"] == 0
+    k = int(alive.sum())
+    assert int(m[1]) == k and k > 0.5 * n
+    pos_tol = 4 * cfg.steps * np.spacing(np.float32(cfg.length))
+    assert np.abs(out["x"][:k] - p["x"][alive]).max() <= pos_tol
+    assert np.abs(out["z"][:k] - p["z"][alive]).max() <= pos_tol
+    vscale = np.abs(p["vx"][alive]).max()
+    for c in ("vx", "vy", "vz"):
+        assert np.abs(out[c][:k] - p[c][alive]).max() <= 1e-4 * vscale, c
