@@ -184,6 +184,15 @@ int m2d_fields2d_download(m2d_fields2d *f, int which, double *host);
 /* Rebuild the gather tables after u or uRF changed outside a solve
  * (e.g. Field2D::load for electric_field_from_file, pic.cpp:154-158). */
 int m2d_fields2d_update(m2d_fields2d *f);
+/* Fields::u_smooth (fields.cpp:28-113) of u (which = M2D_FIELD_U) or uRF:
+ * with `symmetry` (square grids only, M2D_ERR_UNSUPPORTED otherwise) first
+ * the average of the 8 mirror images of every node; then the 2dx x 2dx
+ * square-particle filter (u + edge neighbours / 2 + diagonals / 4) / 4 at
+ * every interior node within `radius` (m) of the grid centre (<= 0: all),
+ * electrode nodes included as in the reference.  The reference's loop also
+ * writes the last column from reads past the row end; here it stays as it
+ * is.  Rebuilds the gather tables.  Bits equal to the FP64 formula. */
+int m2d_fields2d_u_smooth(m2d_fields2d *f, int which, int symmetry, double radius);
 /* Fields::reset (fields.cpp:355-360) */
 int m2d_fields2d_reset(m2d_fields2d *f);
 

@@ -165,6 +165,7 @@ _SIGS = {
     "m2d_fields2d_download": (C.c_int, [_P, C.c_int, _PD]),
     "m2d_fields2d_update": (C.c_int, [_P]),
     "m2d_fields2d_reset": (C.c_int, [_P]),
+    "m2d_fields2d_u_smooth": (C.c_int, [_P, C.c_int, C.c_int, C.c_double]),
     "m2d_fields2d_solve": (C.c_int, [_P, C.c_int, C.c_double, C.POINTER(SolveInfo)]),
     "m2d_fields2d_E": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, _P, _P]),
     "m2d_fields2d_set_magnetic_field": (C.c_int, [_P, C.POINTER(Grid2D), _PD, _PD]),
@@ -447,6 +448,9 @@ class Fields2D:
         out = np.empty(self.nx * self.nz, np.float64)
         check(lib().m2d_fields2d_download(self.h, which, _ptr(out, np.float64)))
         return out
+
+    def u_smooth(self, which=FIELD_U, symmetry=False, radius=-1.0):
+        check(lib().m2d_fields2d_u_smooth(self.h, which, int(symmetry), radius))
 
     def reset(self):
         check(lib().m2d_fields2d_reset(self.h))

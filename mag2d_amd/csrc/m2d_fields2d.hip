@@ -1389,7 +1389,8 @@ int m2d_fields2d_destroy(m2d_fields2d *f)
     if (!f) return M2D_OK;
     if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
     for (void *p : {(void *)f->mask, (void *)f->cellfree, (void *)f->winfree, (void *)f->voltage, (void *)f->u,
-                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz, (void *)f->bt})
+                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz, (void *)f->bt,
+                    (void *)f->smooth_tmp})
         if (p) (void)hipFree(p);
     mg_free(f->mg);
     if (f->ctx) ctx_release(f->ctx);
@@ -1421,6 +1422,46 @@ int m2d_fields2d_set_magnetic_field(m2d_fields2d *f, const m2d_grid2d *g, const 
     M2D_HIP(hipMemcpy(f->bt, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
     f->bgrid = *g;
     return M2D_OK;
+}
+
+// Fields::u_smooth, symmetry part (fields.cpp:38-72): thread (i, j) of the
+// triangle 0 <= j <= i <= ic/2 averages the 8 mirror images of the node on
+// the square grid, in the reference's order, and writes the average to all
+// of them.  The triangle is a fundamental domain of the square's symmetries,
+// so no two threads touch the same node.
+__global__ void k_smooth_sym(double *u, int lmax, int ic, int jc)
+{
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > ic / 2 || j > i) return;
+    double *n[8] = {u + (size_t)i * lmax + j,        u + (size_t)(ic - i) * lmax + j,
+                    u + (size_t)i * lmax + (jc - j), u + (size_t)(ic - i) * lmax + (jc - j),
+                    u + (size_t)(jc - j) * lmax + (ic - i), u + (size_t)j * lmax + i,
+                    u + (size_t)(jc - j) * lmax + i, u + (size_t)j * lmax + (ic - i)};
+    double sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) sum += *n[k];
+    sum /= 8.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) *n[k] = sum;
+}
+
+// Fields::u_smooth, filter part (fields.cpp:74-111): the 2dx x 2dx
+// square-particle stencil on a copy t of the field, at the interior nodes
+// within r2 (squared, in cells) of the centre (r2 <= 0: everywhere).  Every
+// product is by a power of two, so the sum has the reference's bits whether
+// or not it is fused.
+__global__ void k_smooth9(const double *t, double *u, int jmax, int lmax, double icf, double jcf, double r2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1, i = blockIdx.y + 1;
+    if (i > jmax - 2 || j > lmax - 2) return;
+    if (r2 > 0) {
+        const double r = (i - icf) * (i - icf) + (j - jcf) * (j - jcf);
+        if (r > r2) return;
+    }
+    const double *c = t + (size_t)i * lmax + j;
+    const double sum = c[0] + c[-lmax] * 0.5 + c[lmax] * 0.5 + c[-1] * 0.5 + c[1] * 0.5 + c[-lmax - 1] * 0.25 +
+                       c[lmax - 1] * 0.25 + c[-lmax + 1] * 0.25 + c[lmax + 1] * 0.25;
+    u[(size_t)i * lmax + j] = sum / 4.0;
 }
 
 static double *field_ptr(m2d_fields2d *f, int which)
@@ -1460,6 +1501,31 @@ int m2d_fields2d_download(m2d_fields2d *f, int which, double *host)
 int m2d_fields2d_update(m2d_fields2d *f)
 {
     M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields2d_update: NULL argument");
+    return fields_build_tables(f);
+}
+
+int m2d_fields2d_u_smooth(m2d_fields2d *f, int which, int symmetry, double radius)
+{
+    M2D_CHECK(f, M2D_ERR_BAD_ARG, "m2d_fields2d_u_smooth: NULL argument");
+    M2D_CHECK(which == M2D_FIELD_U || which == M2D_FIELD_URF, M2D_ERR_BAD_ARG,
+              "m2d_fields2d_u_smooth: smooths u or uRF");
+    double *u = field_ptr(f, which);
+    const int jmax = f->nx, lmax = f->nz, ic = jmax - 1, jc = lmax - 1;
+    hipStream_t st = f->ctx->stream;
+    if (symmetry) {
+        M2D_CHECK(ic == jc, M2D_ERR_UNSUPPORTED, "Fields::u_smooth: smoothing of non-square matrix not implemented");
+        const int h = ic / 2 + 1;
+        k_smooth_sym<<<dim3((unsigned)((h + 127) / 128), (unsigned)h), 128, 0, st>>>(u, lmax, ic, jc);
+        M2D_HIP(hipGetLastError());
+    }
+    if (!f->smooth_tmp) M2D_TRY(dmalloc((void **)&f->smooth_tmp, 8 * f->n));
+    M2D_HIP(hipMemcpyAsync(f->smooth_tmp, u, 8 * f->n, hipMemcpyDeviceToDevice, st));
+    const double r2 = radius > 0 ? (radius / f->p.grid.dx) * (radius / f->p.grid.dx) : radius;
+    if (jmax > 2 && lmax > 2) {
+        k_smooth9<<<dim3((unsigned)((lmax - 2 + 127) / 128), (unsigned)(jmax - 2)), 128, 0, st>>>(
+            f->smooth_tmp, u, jmax, lmax, ic / 2.0, jc / 2.0, r2);
+        M2D_HIP(hipGetLastError());
+    }
     return fields_build_tables(f);
 }
 

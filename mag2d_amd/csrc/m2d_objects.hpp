@@ -86,6 +86,7 @@ struct m2d_fields2d {
     // node of its own grid, null for the constant B of the params
     float2 *bt = nullptr;
     m2d_grid2d bgrid{};
+    double *smooth_tmp = nullptr;  // nx*nz copy of the field u_smooth filters
     m2d::Mg mg;
     std::vector<uint8_t> h_mask;
 };

@@ -205,6 +205,7 @@ struct Param {
     double n_particles_total = 1e5, density_total = 1e11;
     uint64_t seed = 1234;
     double solve_tolerance = 1e-10;  // FP64 relative residual of every solve
+    bool u_smooth = false;           // Fields::u_smooth after every solve (param.cpp:53, pic.cpp:336,372)
     // derived (param.cpp:126-142)
     double dx = 0, dz = 0, idx = 0, idz = 0, V = 0, dV = 0;
     void derive();
@@ -261,6 +262,8 @@ public:
     void load_magnetic_field(const char *fname);
     // send host edits of u / uRF and rebuild the gather tables
     void update();
+    // fields.cpp:28-113 on u (field == nullptr) or uRF
+    void u_smooth(bool symmetry = false, double radius = -1., Field2D *field = nullptr);
     m2d_fields2d *handle();
     const m2d_solve_info &last_solve() const { return info_; }
     Param &param() { return *p_; }

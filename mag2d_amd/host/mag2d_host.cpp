@@ -468,6 +468,16 @@ m2d_fields2d *Fields::handle()
     return f_;
 }
 
+void Fields::u_smooth(bool symmetry, double radius, Field2D *field)
+{
+    int which = M2D_FIELD_U;
+    if (field == &uRF) which = M2D_FIELD_URF;
+    else if (field && field != &u) throw std::runtime_error("Fields::u_smooth: smooths u or uRF");
+    (void)handle();  // host edits of u / uRF reach the device first
+    check(m2d_fields2d_u_smooth(f_, which, symmetry ? 1 : 0, radius));
+    (which == M2D_FIELD_U ? u : uRF).mark_device_written();
+}
+
 void Fields::update()
 {
     (void)u.device();
@@ -1052,6 +1062,7 @@ void Pic::advance()
 {
     if (param.selfconsistent) {
         field.boundary_solve();
+        if (param.u_smooth) field.u_smooth();
         field.reset();
         for (auto &s : speclist)
             if (s->particle && s->n_particles() > 0) s->rho.reset();
@@ -1071,6 +1082,7 @@ void Pic::advance_init()
                 field.rho.add(s->rho);
             }
         field.boundary_solve();
+        if (param.u_smooth) field.u_smooth();
         // boundary_solve builds its rhs in place on rho: rebuild rho
         field.reset();
         for (auto &s : speclist)
@@ -1159,6 +1171,7 @@ typedef struct {
     int32_t grid, n_particles, steps, selfconsistent;
     double length, dt, Bz, end_cap_V, temperature_K, mass, charge, radius, density_total;
     uint64_t seed;
+    int32_t u_smooth;  // Param::u_smooth
 } m2dh_trap_cfg;
 
 static int trap_run(const m2dh_trap_cfg *c, const char *bfile, float *out_x, float *out_z, float *out_vx,
@@ -1178,6 +1191,7 @@ static int trap_run(const m2dh_trap_cfg *c, const char *bfile, float *out_x, flo
         p.n_particles_total = c->n_particles;
         p.density_total = c->density_total;
         p.seed = c->seed;
+        p.u_smooth = c->u_smooth != 0;
         p.derive();
         SpeciesParams sp;
         sp.name = "particles";

@@ -784,6 +784,60 @@ int o_fields_B(const o_fields *f, double x, double z, double *Br, double *Bz, do
     return o_f2_interpolate(&f->gB, f->BgZ, x, z, Bz);
 }
 
+/* Fields::u_smooth (fields.cpp:28-113): the 8-fold mirror average on a
+ * square grid (symmetry), then the 2dx x 2dx filter at interior nodes within
+ * `radius` of the centre.  The reference's filter loop also runs over the
+ * last column, reading past the row end; here that column is left alone, as
+ * in the device library.  -1 for symmetry on a non-square grid (the
+ * reference throws). */
+int o_u_smooth(const o_grid2 *g, double *u, int symmetry, double radius)
+{
+    const int jmax = g->nx, lmax = g->nz, ic = jmax - 1, jc = lmax - 1;
+#define U(a, b) u[(size_t)(a) * lmax + (b)]
+    if (symmetry) {
+        if (ic != jc) return -1;
+        for (int i = 0; i <= ic / 2; i++)
+            for (int j = 0; j <= i; j++) {
+                double sum = 0;
+                sum += U(i, j);
+                sum += U(ic - i, j);
+                sum += U(i, jc - j);
+                sum += U(ic - i, jc - j);
+                sum += U(jc - j, ic - i);
+                sum += U(j, i);
+                sum += U(jc - j, i);
+                sum += U(j, ic - i);
+                sum /= 8.0;
+                U(i, j) = sum;
+                U(ic - i, j) = sum;
+                U(i, jc - j) = sum;
+                U(ic - i, jc - j) = sum;
+                U(jc - j, ic - i) = sum;
+                U(j, i) = sum;
+                U(jc - j, i) = sum;
+                U(j, ic - i) = sum;
+            }
+    }
+    if (radius > 0) radius = SQR(radius / g->dx);
+    double *t = malloc(sizeof(double) * (size_t)jmax * lmax);
+    memcpy(t, u, sizeof(double) * (size_t)jmax * lmax);
+#define T(a, b) t[(size_t)(a) * lmax + (b)]
+    const double icf = ic / 2.0, jcf = jc / 2.0;
+    for (int i = 1; i < jmax - 1; i++)
+        for (int j = 1; j < lmax - 1; j++) {
+            double r = SQR(i - icf) + SQR(j - jcf);
+            if (radius > 0 && r > radius) continue;
+            double sum = T(i, j) + T(i - 1, j) * 0.5 + T(i + 1, j) * 0.5 + T(i, j - 1) * 0.5 + T(i, j + 1) * 0.5 +
+                         T(i - 1, j - 1) * 0.25 + T(i + 1, j - 1) * 0.25 + T(i - 1, j + 1) * 0.25 +
+                         T(i + 1, j + 1) * 0.25;
+            U(i, j) = sum / 4.0;
+        }
+#undef T
+#undef U
+    free(t);
+    return 0;
+}
+
 /* Species<CARTESIAN>::advance_boris (particles.cpp:924-995) */
 void o_advance_boris(const o_fields *f, o_species *s, o_particle *P, long n, long *n_coll)
 {

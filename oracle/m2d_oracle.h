@@ -210,6 +210,8 @@ typedef struct {
 } o_species;
 
 void o_fields_E(const o_fields *f, double x, double z, double t, double *ex, double *ez); /* fields.hpp:124-150 */
+/* Fields::u_smooth (fields.cpp:28-113); -1 for symmetry on a non-square grid */
+int  o_u_smooth(const o_grid2 *g, double *u, int symmetry, double radius);
 /* Fields::B (fields.hpp:152-177); -2 for a gridded B outside its grid (the reference throws) */
 int  o_fields_B(const o_fields *f, double x, double z, double *Br, double *Bz, double *Bt);
 void o_advance_boris(const o_fields *f, o_species *s, o_particle *p, long n, long *n_coll);

@@ -145,6 +145,7 @@ _SIGS = {
     "o_scatter": (C.c_int, [C.POINTER(Mcc), _P, C.POINTER(Rng)]),
     "o_fields_E": (None, [C.POINTER(Fields), C.c_double, C.c_double, C.c_double, _PD, _PD]),
     "o_fields_B": (C.c_int, [C.POINTER(Fields), C.c_double, C.c_double, _PD, _PD, _PD]),
+    "o_u_smooth": (C.c_int, [C.POINTER(Grid2), _P, C.c_int, C.c_double]),
     "o_advance_boris": (None, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long,
                                C.POINTER(C.c_long)]),
     "o_advance_boris_init": (None, [C.POINTER(Fields), C.POINTER(Species), _P, C.c_long]),

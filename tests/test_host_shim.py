@@ -25,7 +25,7 @@ class TrapCfg(C.Structure):
                 ("selfconsistent", C.c_int32), ("length", C.c_double), ("dt", C.c_double),
                 ("Bz", C.c_double), ("end_cap_V", C.c_double), ("temperature_K", C.c_double),
                 ("mass", C.c_double), ("charge", C.c_double), ("radius", C.c_double),
-                ("density_total", C.c_double), ("seed", C.c_uint64)]
+                ("density_total", C.c_double), ("seed", C.c_uint64), ("u_smooth", C.c_int32)]
 
 
 class MccCfg(C.Structure):
@@ -103,6 +103,8 @@ def oracle_trap(cfg, bgrid=None):
         O.lib().o_poisson_rhs(C.byref(og), O.dptr(mask), O.dptr(volt), O.dptr(b), O.SOLVE_STATIC, dV)
         out = np.zeros_like(b)
         assert O.lib().o_poisson_solve(C.byref(og), O.dptr(mask), O.dptr(b), O.dptr(out)) == 0
+        if cfg.u_smooth:  # Fields::u_smooth after the solve (pic.cpp:336, 372)
+            assert O.lib().o_u_smooth(C.byref(og), O.dptr(out), 0, -1.0) == 0
         u[:] = out
 
     vth = math.sqrt(abi.KB * cfg.temperature_K / cfg.mass)
@@ -134,10 +136,12 @@ def oracle_trap(cfg, bgrid=None):
 
 
 @pytest.mark.gpu
-def test_pic_selfconsistent_trap_matches_oracle():
+@pytest.mark.parametrize("smooth", [0, 1])
+def test_pic_selfconsistent_trap_matches_oracle(smooth):
+    """smooth = 1: Param::u_smooth, the potential filtered after every solve."""
     cfg = TrapCfg(grid=64, n_particles=3000, steps=25, selfconsistent=1, length=2e-2, dt=1e-11, Bz=0.1,
                   end_cap_V=-10.0, temperature_K=11604.5, mass=9.11e-31, charge=-abi.QE, radius=5e-3,
-                  density_total=1e13, seed=1234)
+                  density_total=1e13, seed=1234, u_smooth=smooth)
     n = cfg.n_particles
     out = {k: np.zeros(n, np.float32) for k in ("x", "z", "vx", "vy", "vz")}
     m = (C.c_double * 6)()
