@@ -109,9 +109,20 @@ def cpu_baseline(fields, params, mask, n_grid, budget_s):
     steps = max(2, int(budget_s / (per_ps * n)))
     p = sample(n)
     t = O.lib().o_bench_steps(C.byref(of), C.byref(s), O.dptr(p), n, steps)
-    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} ions x {steps} steps of the same RF 22-pole scene (uRF from the GPU solve), "
-                      f"FP64 AoS, serial reference loop order, SHR3 stream, deposit on; {t:.1f} s"}
+    out = {"value": n * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{n} ions x {steps} steps of the same RF 22-pole scene (uRF from the GPU solve), "
+                     f"FP64 AoS, serial reference loop order, SHR3 stream, deposit on; {t:.1f} s"}
+    # the OpenMP form of the same loop (tests/test_particles_omp.cpp's idiom):
+    # particles split over the host's threads, a Philox stream per slot and a
+    # private rho per thread summed after each step
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1), 64)
+    steps_omp = max(2, int(0.3 * budget_s * threads / (per_ps * n)))
+    p = sample(n)
+    t = O.lib().o_bench_steps_omp(C.byref(of), C.byref(s), O.dptr(p), n, steps_omp, threads)
+    out["omp"] = {"value": n * steps_omp / t, "unit": "particle-steps/s", "threads": threads,
+                  "sample": f"{n} ions x {steps_omp} steps, same scene and loop, {threads} OpenMP threads; "
+                            f"{t:.1f} s"}
+    return out
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "k_advance2d_rf22pt_512x512_4M_traffic.md"
