@@ -634,8 +634,9 @@ def main():
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--sort-every", type=int, default=None,
                     help="cell-sort the particle store every K steps (inside the timed region); default "
-                         "40 for the 2D workloads, 20 for trap3d, whose ions cross the 4-cell deposit "
-                         "halo sooner")
+                         "60 for rf22pt (the 2D deposit windows reach 8 cells past their tile), 40 for "
+                         "the other 2D workloads, 20 for trap3d, whose ions leave the 2-cell 3D halo "
+                         "sooner")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -659,7 +660,7 @@ def main():
               "which this build does not implement", file=sys.stderr)
         sys.exit(2)
     if args.sort_every is None:
-        args.sort_every = 20 if args.workload == "trap3d" else 40
+        args.sort_every = {"trap3d": 20, "rf22pt": 60}.get(args.workload, 40)
     if args.workload == "penning2d":
         return run_penning2d(args)
     if args.workload == "trap3d":

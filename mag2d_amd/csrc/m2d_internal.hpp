@@ -31,7 +31,8 @@ constexpr uint32_t kPurposeTtd = 4;    // the first time_to_death of one slot
 
 constexpr int kTile = 8;               // cells per side of the sort tile
 constexpr int kTileShift = 6;          // key = tile << 6 | cell in tile
-constexpr int kHalo = 4;               // deposit window halo (cells) around a tile
+constexpr int kHalo = 8;               // deposit window halo (cells) around a tile: a particle
+                                       // stays in its window for many steps after a sort
 constexpr int kWin = kTile + 1 + 2 * kHalo;  // window side in nodes
 constexpr int kChunkCap = 960;         // particles per block of the tiled 2D pass: with its
                                        // start rounded down to 64 slots, two staged rounds of 512
