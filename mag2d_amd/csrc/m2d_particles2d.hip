@@ -145,7 +145,7 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 // over the lanes' last runs (lanes hold consecutive slots, so a wavefront of
 // a sorted store spans a few cells); only the last lane of each segment
 // issues the four 64-bit atomics.
-// Two-wavefront blocks with 2.3 KB of LDS (the deposit window): a launch of
+// Two-wavefront blocks with 5 KB of LDS (the deposit window): a launch of
 // a few thousand chunks is mostly resident at once.
 constexpr int kPass = 128;
 constexpr int kRound = kPass * 4;
