@@ -193,6 +193,28 @@ def summarize_pmc(fetch_dir, write_dir, copy_mib, kernel="k_advance2d"):
     return "\n".join(out)
 
 
+def summarize_counters(dirs, particles, kernel="k_advance2d"):
+    """Markdown table of the SQ counters of `kernel` (mean per dispatch) from
+    rocprofv3 --pmc runs, with the instruction counts per particle: a
+    wavefront instruction runs once for 64 lanes, and a lane carries one
+    particle per pass slot, so per particle = count * 64 / particles."""
+    import glob
+    vals = {}
+    for d in dirs:
+        for (k, c), vs in _counter_means(glob.glob(os.path.join(d, "*counter_collection.csv"))[0]).items():
+            if kernel in k and "<" in k:
+                vals[c] = sum(vs) / len(vs)
+    lines = ["| counter | mean per dispatch | per particle |", "|---|---|---|"]
+    for c in sorted(vals):
+        per = vals[c] * 64 / particles if c.startswith("SQ_INSTS") else None
+        lines.append(f"| {c} | {vals[c]:.4g} | {'' if per is None else f'{per:.1f}'} |")
+    if "SQ_ACTIVE_INST_VALU" in vals and "SQ_WAVE_CYCLES" in vals:
+        lines += ["", f"SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES: {vals['SQ_ACTIVE_INST_VALU'] / vals['SQ_WAVE_CYCLES']:.3f}"]
+    if "SQ_ACTIVE_INST_ANY" in vals and "SQ_WAVE_CYCLES" in vals:
+        lines += [f"SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES: {vals['SQ_ACTIVE_INST_ANY'] / vals['SQ_WAVE_CYCLES']:.3f}"]
+    return "\n".join(lines)
+
+
 def summarize_trace(trace_dir):
     """Markdown table of a rocprofv3 --kernel-trace --stats csv run."""
     import csv
@@ -643,6 +665,8 @@ def main():
                     help="after the timed steps, run plain copies of this many MiB (counter calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print a markdown table of a rocprofv3 --kernel-trace --stats csv output and exit")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="DIR",
+                    help="print the SQ counters of k_advance2d from rocprofv3 --pmc run directories")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the calibrated FETCH_SIZE/WRITE_SIZE summary of two rocprofv3 --pmc csv "
                          "outputs (with --copy-calibration-mib) and exit")
@@ -652,6 +676,9 @@ def main():
         return
     if args.summarize_pmc:
         print(summarize_pmc(*args.summarize_pmc, args.copy_calibration_mib))
+        return
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters, args.particles or 4194304))
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
