@@ -140,11 +140,14 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 // component moves as one aligned float4 per thread (a wavefront moves 1 KB
 // contiguous per component), and consecutive particles of a sorted store,
 // which mostly share a cell, meet in one thread.
-// Deposits are summed twice before they reach memory: in registers over a
-// thread's run of equal cells, then across the wavefront by a segmented scan
-// over the lanes' last runs (lanes hold consecutive slots, so a wavefront of
-// a sorted store spans a few cells); only the last lane of each segment
-// issues the four 64-bit atomics.
+// Deposits: the tiled pass adds every particle's four weights into its
+// block's LDS window (in a sorted store some lane of a wavefront changes cell
+// at almost every particle, so summing runs first would not save LDS
+// instructions, only add VALU work).  The untiled pass sums them twice before
+// they reach memory: in registers over a thread's run of equal cells, then
+// across the wavefront by a segmented scan over the lanes' last runs (lanes
+// hold consecutive slots, so a wavefront of a sorted store spans a few
+// cells); only the last lane of each segment issues the four 64-bit atomics.
 // Two-wavefront blocks with 5 KB of LDS (the deposit window): a launch of
 // a few thousand chunks is mostly resident at once.
 constexpr int kPass = 128;
@@ -167,21 +170,21 @@ struct DepRun {
     long long w[4];
 };
 
-// a finished run into the block's LDS window (tile + halo), or into the
-// global grid when it lies outside
-__device__ __forceinline__ void run_to_window(const DepRun &d, unsigned long long *win, int wi0, int wj0,
-                                              unsigned long long *acc, int nz)
+// one particle's weights into the block's LDS window (tile + halo), or
+// into the global grid when its cell lies outside the window
+__device__ __forceinline__ void window_deposit(int ci, int cj, const uint32_t w[4], unsigned long long *win,
+                                               int wi0, int wj0, unsigned long long *acc, int nz)
 {
-    if (d.ci < 0) return;
-    const int a = d.ci - wi0, b = d.cj - wj0;
+    const int a = ci - wi0, b = cj - wj0;
     if (a >= 0 && a + 1 < kWin && b >= 0 && b + 1 < kWin) {
         unsigned long long *s = win + a * kWin + b;
-        atomicAdd(s, (unsigned long long)d.w[0]);
-        atomicAdd(s + kWin, (unsigned long long)d.w[1]);
-        atomicAdd(s + 1, (unsigned long long)d.w[2]);
-        atomicAdd(s + kWin + 1, (unsigned long long)d.w[3]);
+        atomicAdd(s, (unsigned long long)w[0]);
+        atomicAdd(s + kWin, (unsigned long long)w[1]);
+        atomicAdd(s + 1, (unsigned long long)w[2]);
+        atomicAdd(s + kWin + 1, (unsigned long long)w[3]);
     } else {
-        deposit_global(acc, nz, d.ci, d.cj, d.w);
+        const long long ww[4] = {w[0], w[1], w[2], w[3]};
+        deposit_global(acc, nz, ci, cj, ww);
     }
 }
 
@@ -377,16 +380,18 @@ k_advance2d(PushArgs a)
                     n_removed++;
                     continue;
                 }
-                // ---- deposit (particles.hpp:408-409), summed per run of equal cells
+                // ---- deposit (particles.hpp:408-409): tiled, straight into the
+                // LDS window; untiled, summed per run of equal cells
                 if (DEPOSIT) {
                     int ci, cj;
                     uint32_t w[4];
                     if (cic_weights(X, Z, a.nx, a.nz, ci, cj, w)) {
-                        if (ci == run.ci && cj == run.cj) {
+                        if constexpr (TILED) {
+                            window_deposit(ci, cj, w, win, wi0, wj0, a.acc, a.nz);
+                        } else if (ci == run.ci && cj == run.cj) {
                             run.w[0] += w[0]; run.w[1] += w[1]; run.w[2] += w[2]; run.w[3] += w[3];
                         } else {
-                            if constexpr (TILED) run_to_window(run, win, wi0, wj0, a.acc, a.nz);
-                            else if (run.ci >= 0) deposit_global(a.acc, a.nz, run.ci, run.cj, run.w);
+                            if (run.ci >= 0) deposit_global(a.acc, a.nz, run.ci, run.cj, run.w);
                             run.ci = ci; run.cj = cj;
                             run.w[0] = w[0]; run.w[1] = w[1]; run.w[2] = w[2]; run.w[3] = w[3];
                         }
@@ -430,7 +435,6 @@ k_advance2d(PushArgs a)
         // the window's nonzero nodes go to the global grid as 64-bit atomics
         // (integer sums: the result does not depend on their order).  A
         // nonzero node belongs to a deposited cell, so it lies in the grid.
-        run_to_window(run, win, wi0, wj0, a.acc, a.nz);
         __syncthreads();
         for (int k = tid; k < kWin * kWin; k += kPass) {
             const unsigned long long v = win[k];
