@@ -97,9 +97,9 @@ __device__ __forceinline__ bool local_rotation(const PushArgs &a, float x, float
 // CIC weights (Field2D.hpp:45-62) as 2^-31 fixed point.  The cell offsets
 // u, v come from the FP64 products x*idx (exact subtraction of the cell
 // index), the four bilinear products are FP32 (relative error ~1e-7), and
-// the rounded weights are corrected on the largest so each particle deposits
-// exactly 2^31: charge is conserved bit for bit and node sums do not depend
-// on the order of the particles.
+// the rounded weights are corrected on the nearest node's (at least 1/4) so
+// each particle deposits exactly 2^31: charge is conserved bit for bit and
+// node sums do not depend on the order of the particles.
 __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
                                             uint32_t w[4])
 {
@@ -115,12 +115,13 @@ __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, 
     w[3] = __float2uint_rn(u * v * S);
     // each weight is at most 2^31, so the sum stays below 2^32
     const uint32_t d = 0x80000000u - (w[0] + w[1] + w[2] + w[3]);
-    int km = 0;
-    uint32_t wm = w[0];
-#pragma unroll
-    for (int q = 1; q < 4; q++)
-        if (w[q] > wm) { km = q; wm = w[q]; }
-    w[km] += d;  // modulo 2^32: the correction is a few units of either sign
+    // the correction goes to the weight of the node nearest the point, which
+    // is at least 1/4 (the largest, up to a rounding at u or v = 1/2)
+    const bool bu = u >= 0.5f, bv = v >= 0.5f;
+    w[0] += !bu && !bv ? d : 0u;  // modulo 2^32: a few units of either sign
+    w[1] += bu && !bv ? d : 0u;
+    w[2] += !bu && bv ? d : 0u;
+    w[3] += bu && bv ? d : 0u;
     return true;
 }
 
