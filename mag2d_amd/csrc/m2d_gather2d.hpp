@@ -13,10 +13,19 @@ namespace m2d {
 // idx, idz, xmin, zmin and the constant-field switch econst/ecx/ecz.
 // ORIGIN0: the caller guarantees xmin = zmin = 0 and no constant field
 // (x - 0 is x exactly, so the result is the same).  Returns true when the
-// point lies outside the grid (a NaN included): it is then clamped onto the
-// grid, and the caller raises M2D_FLAG_OUT_OF_RANGE.  Every table access
-// stays inside the tables for any input, so the gather may run ahead of the
-// test that discards its result.
+// point lies outside the grid (a NaN included): the caller raises
+// M2D_FLAG_OUT_OF_RANGE, and (generic path) the point is clamped onto the
+// grid.  Every table access stays inside the tables for any input, so the
+// gather may run ahead of the test that discards its result.
+// x truncated to an int as v_cvt_i32_f64 defines it: out-of-range values
+// saturate and a NaN gives 0 (a C++ conversion of those is undefined)
+__device__ __forceinline__ int cvt_i32_sat(double x)
+{
+    int r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 template <bool ORIGIN0 = false, class A>
 __device__ __forceinline__ bool gather_E_clamped(const A &a, double xd, double zd, float ph, float &ex,
                                                  float &ez)
@@ -26,20 +35,23 @@ __device__ __forceinline__ bool gather_E_clamped(const A &a, double xd, double z
     double X = ORIGIN0 ? xd * a.idx : (xd - a.xmin) * a.idx;
     double Z = ORIGIN0 ? zd * a.idz : (zd - a.zmin) * a.idz;
     const bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
-    if (bad) {  // fmax of a NaN and 0 is 0
+    // Inside that range every index below is already in the tables (x > -1
+    // truncates to 0).  Outside it the generic path moves the point onto the
+    // grid (fmax of a NaN and 0 is 0).  The ORIGIN0 pass does not: such a
+    // point is outside the box, so its particle is removed after the push;
+    // the unsigned clamps alone keep the table reads in bounds (a NaN
+    // converts to 0, a negative index wraps to a large one).
+    if (!ORIGIN0 && bad) {
         X = fmin(fmax(X, 0.0), (double)(nx - 1));
         Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
     }
+    auto trunc = [](double t) { return ORIGIN0 ? cvt_i32_sat(t) : (int)t; };
     // x component: i = (int)(x idx + .5), j = min((int)(z idz), lmax-2)
-    int ih = (int)(X + 0.5);
-    int jx = min((int)Z, nz - 2);
+    const int ih = (int)min((unsigned)trunc(X + 0.5), (unsigned)(nx - 1));
+    const int jx = (int)min((unsigned)trunc(Z), (unsigned)(nz - 2));
     // z component: i = min((int)(x idx), jmax-2), j = (int)(z idz + .5)
-    int i0 = min((int)X, nx - 2);
-    int jh = (int)(Z + 0.5);
-    ih = max(0, min(ih, nx - 1));
-    jx = max(jx, 0);
-    i0 = max(i0, 0);
-    jh = max(0, min(jh, nz - 1));
+    const int i0 = (int)min((unsigned)trunc(X), (unsigned)(nx - 2));
+    const int jh = (int)min((unsigned)trunc(Z + 0.5), (unsigned)(nz - 1));
     const float fx = (float)(X - ih + 0.5), fyx = (float)(Z - jx);
     const float fxz = (float)(X - i0), fz = (float)(Z - jh + 0.5);
 
