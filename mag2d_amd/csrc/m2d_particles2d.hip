@@ -100,19 +100,25 @@ __device__ __forceinline__ bool local_rotation(const PushArgs &a, float x, float
 // the rounded weights are corrected on the nearest node's (at least 1/4) so
 // each particle deposits exactly 2^31: charge is conserved bit for bit and
 // node sums do not depend on the order of the particles.
+// NONNEG: the caller guarantees X, Z >= 0 (a NaN included: it fails the
+// upper bounds).
+template <bool NONNEG = false>
 __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
                                             uint32_t w[4])
 {
     // range in FP64 before the conversion: cells [0, n-2]
-    if (!(X >= 0.0 && X < nx - 1.0 && Z >= 0.0 && Z < nz - 1.0)) return false;
+    if (!((NONNEG || X >= 0.0) && X < nx - 1.0 && (NONNEG || Z >= 0.0) && Z < nz - 1.0)) return false;
     i = (int)X;
     j = (int)Z;
     const float u = (float)(X - i), v = (float)(Z - j);
+    // scaling by 2^31 is exact, so (1 - u) * ((1 - v) 2^31) has the bits of
+    // ((1 - u)(1 - v)) 2^31
     const float S = 2147483648.0f;  // 2^31
-    w[0] = __float2uint_rn((1.f - u) * (1.f - v) * S);
-    w[1] = __float2uint_rn(u * (1.f - v) * S);
-    w[2] = __float2uint_rn((1.f - u) * v * S);
-    w[3] = __float2uint_rn(u * v * S);
+    const float a = (1.f - v) * S, b = v * S;
+    w[0] = __float2uint_rn((1.f - u) * a);
+    w[1] = __float2uint_rn(u * a);
+    w[2] = __float2uint_rn((1.f - u) * b);
+    w[3] = __float2uint_rn(u * b);
     // each weight is at most 2^31, so the sum stays below 2^32
     const uint32_t d = 0x80000000u - (w[0] + w[1] + w[2] + w[3]);
     // the correction goes to the weight of the node nearest the point, which
@@ -365,7 +371,8 @@ k_advance2d(PushArgs a)
                 const double Z = FAST ? (double)z * a.idz : ((double)z - a.zmin) * a.idz;
                 if (!dead && (FAST || (a.boundary && a.checkfree))) {
                     bool free_ = false;
-                    if (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz) {
+                    // the specialised pass's box test already gave x, z >= 0
+                    if ((FAST || X >= 0.0) && X < (double)a.nx && (FAST || Z >= 0.0) && Z < (double)a.nz) {
                         int ci = (int)X, cj = (int)Z;
                         // a cell of an all-free window needs no table lookup
                         if (wfree && (unsigned)(ci - wi0) < (unsigned)(kWin - 1) &&
@@ -386,7 +393,7 @@ k_advance2d(PushArgs a)
                 if (DEPOSIT) {
                     int ci, cj;
                     uint32_t w[4];
-                    if (cic_weights(X, Z, a.nx, a.nz, ci, cj, w)) {
+                    if (cic_weights<FAST>(X, Z, a.nx, a.nz, ci, cj, w)) {
                         if constexpr (TILED) {
                             window_deposit(ci, cj, w, win, wi0, wj0, a.acc, a.nz);
                         } else if (ci == run.ci && cj == run.cj) {
