@@ -318,12 +318,14 @@ def test_deposit_does_not_depend_on_particle_order(ctx):
     assert np.array_equal(rho.get(), grids[1])
 
 
-def test_tiled_and_untiled_passes_agree_bit_for_bit_over_many_steps(ctx):
+@pytest.mark.parametrize("density,steps", [(1e22, 40), (1e24, 8)])
+def test_tiled_and_untiled_passes_agree_bit_for_bit_over_many_steps(ctx, density, steps):
     """A sorted store (tiled deposit windows, fallback atomics once particles
     drift out of them) and a copy of it at another address (untiled pass)
-    stay identical over 40 RF + MCC steps: particles and every rho."""
+    stay identical over RF + MCC steps: particles and every rho.  Collision
+    probabilities per step: 1% and 63% (hundreds of queued hits a block)."""
     sc = Scene(ctx, n=128)
-    dev, orc = _mcc_pair(ctx, abi.ELASTIC, density=1e22)
+    dev, orc = _mcc_pair(ctx, abi.ELASTIC, density=density)
     rng = np.random.default_rng(17)
     n = 120_000
     x, z, vx, vy, vz = ions_on_disk(n, rng, T_eV=2.0)
@@ -332,14 +334,18 @@ def test_tiled_and_untiled_passes_agree_bit_for_bit_over_many_steps(ctx):
     abi.sort(ctx, sc.fields, a, s)
     c = abi.ParticlesSoA.from_host(ctx, *(s.download()[k] for k in abi.ParticlesSoA.NAMES))
     flags = abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT
-    for st in range(40):
+    coll = 0
+    for st in range(steps):
         rhos = []
         for P in (s, c):
             rho = ctx.to_device(np.zeros(128 * 128))
-            abi.advance(ctx, sc.fields, species(QE, M_ION, 1e-9, dev.lifetime, st), P, mcc=dev,
-                        rho_ptr=rho.ptr, flags=flags)
+            r = abi.advance(ctx, sc.fields, species(QE, M_ION, 1e-9, dev.lifetime, st), P, mcc=dev,
+                            rho_ptr=rho.ptr, flags=flags, want_stats=True)
             rhos.append(rho.get())
+            coll += r.collisions
         assert np.array_equal(rhos[0], rhos[1]), st
+    prob = 1 - math.exp(-1e-9 / orc.lifetime)
+    assert coll > 0.5 * prob * n * steps
     ds, dc = s.download(), c.download()
     for k in abi.ParticlesSoA.NAMES:
         assert np.array_equal(ds[k].view(np.uint32), dc[k].view(np.uint32)), k
