@@ -57,6 +57,38 @@ int set_error(int code, const char *fmt, ...);
         if (r_ != M2D_OK) return r_;                                                  \
     } while (0)
 
+// ---------------------------------------------------------------- XCD order
+// The dispatcher deals a launch's blocks round-robin to the chip's 8 XCDs
+// (block b to XCD b % 8), each with its own L2.  xcd_block turns that into
+// contiguous ranges: the blocks of XCD x take logical indices
+// [x G, (x + 1) G), G = ceil(n / 8), in dispatch order, so neighbouring
+// logical blocks (neighbouring tiles, or destination ranges of a nearly
+// sorted gather) share one L2.  A launch needs at least 8 G blocks (n
+// rounded up to a multiple of 8); indices >= n come back for the blocks
+// with nothing to do.
+constexpr unsigned kXcds = 8;
+inline unsigned xcd_grid(unsigned n) { return (n + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n)
+{
+    const unsigned G = (n + kXcds - 1) / kXcds;
+    return b < kXcds * G ? (b % kXcds) * G + b / kXcds : 0xFFFFFFFFu;
+}
+// The same in runs of kXcdRun consecutive logical blocks dealt round-robin
+// to the XCDs: XCD x takes runs x, x + 8, x + 16, ... in order.  Neighbours within
+// a run share an L2, and work that varies smoothly along the logical order
+// (dense and sparse regions of a sorted store) spreads over every XCD.  A
+// launch needs xcd_run_grid(n) blocks; indices >= n mean nothing to do.
+constexpr unsigned kXcdRun = 8;
+inline unsigned xcd_run_grid(unsigned n)
+{
+    return (n + kXcds * kXcdRun - 1) / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+}
+__device__ __forceinline__ unsigned xcd_run_block(unsigned b)
+{
+    const unsigned x = b % kXcds, j = b / kXcds;
+    return ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
+}
+
 // ---------------------------------------------------------------- device buffers
 // A growable device scratch buffer owned by a context.
 struct Scratch {

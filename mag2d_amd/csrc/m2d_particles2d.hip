@@ -236,8 +236,11 @@ k_advance2d(PushArgs a)
     int wi0 = 0, wj0 = 0;
     bool wfree = false;
     if constexpr (TILED) {
-        if (blockIdx.x >= *a.n_chunks) return;  // uniform: before any barrier
-        const int4 s = a.sched[blockIdx.x];
+        // chunks in runs per XCD: neighbouring tiles share an L2
+        const unsigned nch = *a.n_chunks;
+        const unsigned ch = xcd_run_block(blockIdx.x);
+        if (ch >= nch) return;  // uniform: before any barrier
+        const int4 s = a.sched[ch];
         lo = s.y;
         hi = s.z;
         co0 = lo & ~(int64_t)63;
@@ -1066,7 +1069,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         int g = 0;
         const bool bgrid = f->bt != nullptr;  // B per particle (generic pass)
         if (T) {
-            g = T->max_chunks;
+            g = (int)xcd_run_grid((unsigned)T->max_chunks);
             if (fast) kern = use_mcc ? k_advance2d<true, true, true, true> : k_advance2d<true, false, true, true>;
             else if (bgrid)
                 kern = use_mcc ? k_advance2d<true, true, true, false, true> : k_advance2d<true, false, true, false, true>;

@@ -44,7 +44,12 @@ __global__ void k_sort_keys(const float *xs, const float *zs, int64_t n, SortGeo
 __global__ void k_sort_gather(m2d_particles2d src, m2d_particles2d dst, const uint32_t *keys,
                               const uint32_t *vals, uint32_t dead_key, unsigned long long *n_live)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // destination ranges in XCD-contiguous order: a nearly sorted store's
+    // sources of neighbouring ranges share cache lines, and then one L2
+    const unsigned nb = (unsigned)((src.n + blockDim.x - 1) / blockDim.x);
+    const unsigned b = xcd_block(blockIdx.x, nb);
+    if (b >= nb) return;
+    int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
     if (i >= src.n) return;
     uint32_t k = keys[i];
     if (k == dead_key) {
@@ -234,7 +239,8 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     M2D_HIP(hipGetLastError());
     e = rocprim::radix_sort_pairs(tmp, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
-    k_sort_gather<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key, live);
+    k_sort_gather<<<xcd_grid((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key,
+                                                                              live);
     M2D_HIP(hipGetLastError());
     M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, g.dead_key, ntx, 1, g.ntz, kChunkCap));
     if (n_live) {
