@@ -55,7 +55,7 @@ def test_accumulate_matches_oracle_and_is_bit_reproducible(ctx):
     for a, b in zip(x.astype(np.float64), z.astype(np.float64)):
         O.lib().o_f2_accumulate(C.byref(og), O.dptr(ref), q, a, b)
     nz = np.abs(ref) > 0
-    assert np.max(np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])) < 1e-6  # north-star bound; ~1e-9 seen
+    assert np.max(np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])) < 1e-6  # the stated 1e-6 bound; about 1e-9 is seen
     # every particle deposits exactly 2^31 fixed-point units: total charge is N q to FP64 rounding
     assert got.sum() == pytest.approx(N * q, rel=1e-12)
     rho2 = ctx.to_device(np.zeros(n * n))
@@ -132,7 +132,7 @@ def test_push_rf_trap_100_steps_matches_oracle(ctx):
 
 
 def test_push_magnetised_static_trap_matches_oracle(ctx):
-    # c1-like: grounded box, end caps at -10 V, Bz = 0.1 T, electrons
+    # a Penning-like trap: grounded box, end caps at -10 V, Bz = 0.1 T, electrons
     def caps(g):
         mask, volt = abi.grid_build(g, abi.GEO_EMPTY, 2e-2, 2e-2, extern_field=0.0)
         abi.square_electrode(g, mask, volt, 2e-3, 1.8e-2, -1, 2e-3, -10.0)
@@ -385,7 +385,7 @@ def test_poisson_static_with_space_charge_matches_oracle(ctx):
 
 
 def test_poisson_512_rf_residual(ctx):
-    """Full c2 size: FP64 residual of the GPU solution with the oracle's rows."""
+    """The 512 x 512 grid of the RF 22-pole workload: FP64 residual of the GPU solution with the oracle's rows."""
     sc = Scene(ctx, n=512, solve_oracle=False)
     info = sc.fields.solve(abi.SOLVE_RF, 1e-10)
     urf = sc.fields.download(abi.FIELD_URF)
