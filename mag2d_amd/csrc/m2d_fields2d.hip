@@ -53,6 +53,7 @@ int fields_build_tables(m2d_fields2d *f)
 {
     hipStream_t s = f->ctx->stream;
     dim3 b(256);
+    f->ph_valid = false;
     k_build_gx<<<dim3((f->nz + 255) / 256, f->nx + 1), b, 0, s>>>(f->u, f->urf, f->nx, f->nz, f->idx, f->gx);
     k_build_gz<<<dim3((f->nz + 1 + 255) / 256, f->nx), b, 0, s>>>(f->u, f->urf, f->nx, f->nz, f->idz, f->gz);
     M2D_HIP(hipGetLastError());
@@ -1346,10 +1347,12 @@ int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_
         (r = dmalloc((void **)&f->voltage, 8 * n)) || (r = dmalloc((void **)&f->u, 8 * n)) ||
         (r = dmalloc((void **)&f->urf, 8 * n)) || (r = dmalloc((void **)&f->rho, 8 * n)) ||
         (r = dmalloc((void **)&f->gx, sizeof(float2) * (size_t)(g.nx + 1) * g.nz)) ||
-        (r = dmalloc((void **)&f->gz, sizeof(float2) * (size_t)g.nx * (g.nz + 1))))
+        (r = dmalloc((void **)&f->gz, sizeof(float2) * (size_t)g.nx * (g.nz + 1))) ||
+        (r = dmalloc((void **)&f->phx, sizeof(float) * (size_t)(g.nx + 1) * g.nz)) ||
+        (r = dmalloc((void **)&f->phz, sizeof(float) * (size_t)g.nx * (g.nz + 1))))
         return fail(r);
     // sort tiles of 8 x 8 cells; the deposit window of tile (ti, tj) spans
-    // cells [8 ti - 4, 8 ti + 12) x [8 tj - 4, 8 tj + 12)
+    // cells [8 ti - 8, 8 ti + 16) x [8 tj - 8, 8 tj + 16)
     const int ntx = (g.nx - 2) / kTile + 1, ntz = (g.nz - 2) / kTile + 1;
     if ((r = dmalloc((void **)&f->winfree, (size_t)ntx * ntz))) return fail(r);
     std::vector<uint8_t> cf(n, 0);
@@ -1389,7 +1392,7 @@ int m2d_fields2d_destroy(m2d_fields2d *f)
     if (!f) return M2D_OK;
     if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
     for (void *p : {(void *)f->mask, (void *)f->cellfree, (void *)f->winfree, (void *)f->voltage, (void *)f->u,
-                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz, (void *)f->bt,
+                    (void *)f->urf, (void *)f->rho, (void *)f->gx, (void *)f->gz, (void *)f->phx, (void *)f->phz, (void *)f->bt,
                     (void *)f->smooth_tmp})
         if (p) (void)hipFree(p);
     mg_free(f->mg);

@@ -26,7 +26,9 @@ __device__ __forceinline__ int cvt_i32_sat(double x)
     return r;
 }
 
-template <bool ORIGIN0 = false, class A>
+// PHT: read the tables with the phase already applied (a.phx, a.phz, one
+// float per edge: the same fmaf values, so the same bits) instead of gx, gz.
+template <bool ORIGIN0 = false, bool PHT = false, class A>
 __device__ __forceinline__ bool gather_E_clamped(const A &a, double xd, double zd, float ph, float &ex,
                                                  float &ez)
 {
@@ -57,18 +59,24 @@ __device__ __forceinline__ bool gather_E_clamped(const A &a, double xd, double z
 
     // indices are clamped non-negative and the tables hold fewer than 2^24
     // rows: 24-bit products are exact (full-rate multiplies)
-    const float2 *r0 = a.gx + (__umul24(ih, nz) + jx);
-    const float2 *r1 = r0 + nz;
-    float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
-    const float2 *q0 = a.gz + (__umul24(i0, nz + 1) + jh);
-    const float2 *q1 = q0 + (nz + 1);
-    float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
-
-    float g00 = fmaf(ph, p00.y, p00.x), g01 = fmaf(ph, p01.y, p01.x);
-    float g10 = fmaf(ph, p10.y, p10.x), g11 = fmaf(ph, p11.y, p11.x);
+    const int ox = __umul24(ih, nz) + jx, oz = __umul24(i0, nz + 1) + jh;
+    float g00, g01, g10, g11, h00, h01, h10, h11;
+    if constexpr (PHT) {
+        const float *r0 = a.phx + ox, *r1 = r0 + nz;
+        g00 = r0[0]; g01 = r0[1]; g10 = r1[0]; g11 = r1[1];
+        const float *q0 = a.phz + oz, *q1 = q0 + (nz + 1);
+        h00 = q0[0]; h01 = q0[1]; h10 = q1[0]; h11 = q1[1];
+    } else {
+        const float2 *r0 = a.gx + ox, *r1 = r0 + nz;
+        const float2 p00 = r0[0], p01 = r0[1], p10 = r1[0], p11 = r1[1];
+        const float2 *q0 = a.gz + oz, *q1 = q0 + (nz + 1);
+        const float2 s00 = q0[0], s01 = q0[1], s10 = q1[0], s11 = q1[1];
+        g00 = fmaf(ph, p00.y, p00.x); g01 = fmaf(ph, p01.y, p01.x);
+        g10 = fmaf(ph, p10.y, p10.x); g11 = fmaf(ph, p11.y, p11.x);
+        h00 = fmaf(ph, s00.y, s00.x); h01 = fmaf(ph, s01.y, s01.x);
+        h10 = fmaf(ph, s10.y, s10.x); h11 = fmaf(ph, s11.y, s11.x);
+    }
     float gxv = (1.f - fx) * ((1.f - fyx) * g00 + fyx * g01) + fx * ((1.f - fyx) * g10 + fyx * g11);
-    float h00 = fmaf(ph, s00.y, s00.x), h01 = fmaf(ph, s01.y, s01.x);
-    float h10 = fmaf(ph, s10.y, s10.x), h11 = fmaf(ph, s11.y, s11.x);
     float gzv = (1.f - fxz) * ((1.f - fz) * h00 + fz * h01) + fxz * ((1.f - fz) * h10 + fz * h11);
     ex = -gxv;
     ez = -gzv;

@@ -82,6 +82,14 @@ struct m2d_fields2d {
     double *voltage = nullptr, *u = nullptr, *urf = nullptr, *rho = nullptr;
     float2 *gx = nullptr;         // (nx+1)*nz edge x-gradients {u, uRF}, padded rows 0 and nx
     float2 *gz = nullptr;         // nx*(nz+1) edge z-gradients {u, uRF}, padded cols 0 and nz
+    // gx, gz with one step's RF phase applied, fmaf(phase, uRF part, u part)
+    // per edge: written by the flush after a pass for the next step's phase,
+    // read by the specialised tiled pass (one float per edge sample instead
+    // of two).  Valid while ph_valid, for the FP32 phase ph_phase;
+    // fields_build_tables invalidates them.
+    float *phx = nullptr, *phz = nullptr;
+    float ph_phase = 0.f;
+    bool ph_valid = false;
     // gridded B (Fields::Br, Bz with magnetic_field_const = 0): {Br, Bz} per
     // node of its own grid, null for the constant B of the params
     float2 *bt = nullptr;

@@ -18,6 +18,7 @@ struct PushArgs {
     float *x, *z, *vx, *vy, *vz;
     int64_t n;
     const float2 *gx, *gz;
+    const float *phx, *phz;        // gx, gz with this step's phase applied (PHT pass)
     const uint8_t *cellfree;
     const uint8_t *winfree;        // per sort tile: whole deposit window free
     int nx, nz;
@@ -220,7 +221,8 @@ __device__ __forceinline__ void wave_deposit(DepRun &d, unsigned long long *acc,
 // wrap, no constant field, boundary and free-cell tests on, origin at 0),
 // which frees the scalar registers and branches the generic flags cost.
 // BGRID (generic pass only): B from the gridded field at each particle.
-template <bool DEPOSIT, bool MCC, bool TILED, bool FAST, bool BGRID = false>
+// PHT (FAST only): the gather reads the phase-applied tables phx, phz.
+template <bool DEPOSIT, bool MCC, bool TILED, bool FAST, bool BGRID = false, bool PHT = false>
 // four wavefronts per SIMD (128 VGPRs): a thread holds its four particles and
 // two gathers in registers (with fewer VGPRs the scalar and vector spills cost
 // more than the occupancy gains)
@@ -305,7 +307,7 @@ k_advance2d(PushArgs a)
 #pragma unroll
                 for (int h = 0; h < kGather; h++) {
                     // a dead slot gathers at a clamped point and is discarded
-                    const bool b = gather_E_clamped<FAST>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
+                    const bool b = gather_E_clamped<FAST, PHT>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
                                                           gez[h]);
                     gbad |= b && !is_dead(v[0][h]);
                 }
@@ -485,20 +487,48 @@ __device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long l
     }
 }
 
-// After the pass: rho += charge * acc * 2^-31 and acc = 0 over the grid
-// (blocks [0, nflush)); with MCC, the blocks after them apply the collision
-// queue meanwhile.
+// The next step's phase-applied tables: phx[e] = fmaf(ph, gx[e].y, gx[e].x),
+// the value the gather forms from gx at phase ph, and the same for phz
+struct PhBuild {
+    const float2 *gx, *gz;
+    float *phx, *phz;
+    size_t nxe, nze;  // entries of gx and gz
+    float ph;
+};
+constexpr int kPhPerThread = 4;
+
+// After the pass, in dispatch order: with MCC, kScatterBlocks blocks apply
+// the collision queue (a long FP64 chain that sets the launch's length, so
+// it starts first); ntab blocks build the next step's phase-applied tables;
+// the rest add the deposit, rho += charge * acc * 2^-31, and clear acc.
+constexpr unsigned kScatterBlocks = 64;
 template <bool MCC>
 __global__ void __launch_bounds__(kBlock) k_flush_scatter(unsigned long long *acc, double *rho, double charge,
-                                                         size_t n, unsigned nflush, PushArgs a,
+                                                         size_t n, PhBuild ph, unsigned ntab, PushArgs a,
                                                          unsigned long long *next_len)
 {
-    if (MCC && blockIdx.x >= nflush) {
-        scatter_queue(a, (unsigned long long)(blockIdx.x - nflush) * kBlock + threadIdx.x,
-                      (unsigned long long)(gridDim.x - nflush) * kBlock, next_len);
+    const unsigned ns = MCC ? kScatterBlocks : 0u;
+    if (MCC && blockIdx.x < ns) {
+        scatter_queue(a, (unsigned long long)blockIdx.x * kBlock + threadIdx.x,
+                      (unsigned long long)kScatterBlocks * kBlock, next_len);
         return;
     }
-    const size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (blockIdx.x < ns + ntab) {
+        const size_t base = (size_t)(blockIdx.x - ns) * (kBlock * kPhPerThread) + threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < kPhPerThread; r++) {
+            const size_t e = base + (size_t)r * kBlock;
+            if (e < ph.nxe) {
+                const float2 g = ph.gx[e];
+                ph.phx[e] = fmaf(ph.ph, g.y, g.x);
+            } else if (e < ph.nxe + ph.nze) {
+                const float2 g = ph.gz[e - ph.nxe];
+                ph.phz[e - ph.nxe] = fmaf(ph.ph, g.y, g.x);
+            }
+        }
+        return;
+    }
+    const size_t k = (size_t)(blockIdx.x - ns - ntab) * kBlock + threadIdx.x;
     if (k >= n) return;
     const long long v = (long long)acc[k];
     if (v) {
@@ -962,6 +992,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     const m2d_fields2d_params &P = f->p;
     a.x = p->x; a.z = p->z; a.vx = p->vx; a.vy = p->vy; a.vz = p->vz; a.n = p->n;
     a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree; a.winfree = f->winfree;
+    a.phx = f->phx; a.phz = f->phz;
     a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
     a.xmin = P.grid.xmin; a.zmin = P.grid.zmin;
     a.fx0 = f32_at_least(P.grid.xmin); a.fx1 = f32_at_most(P.x_max);
@@ -1061,6 +1092,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         a.n_chunks = T->n_chunks;
         a.ntz = T->ntz;
     }
+    bool fast_tiled = false;  // this call ran the specialised tiled pass
     if (p->n > 0) {
         // the common configuration runs the compile-time specialised pass
         const bool fast = a.push && a.boundary && a.checkfree && !a.rotate && !a.periodic && !a.econst &&
@@ -1070,7 +1102,13 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         const bool bgrid = f->bt != nullptr;  // B per particle (generic pass)
         if (T) {
             g = (int)xcd_run_grid((unsigned)T->max_chunks);
-            if (fast) kern = use_mcc ? k_advance2d<true, true, true, true> : k_advance2d<true, false, true, true>;
+            // the flush after the previous pass left the tables for this phase
+            const bool pht = f->ph_valid && f->ph_phase == a.phase;
+            fast_tiled = fast;
+            if (fast && pht)
+                kern = use_mcc ? k_advance2d<true, true, true, true, false, true>
+                               : k_advance2d<true, false, true, true, false, true>;
+            else if (fast) kern = use_mcc ? k_advance2d<true, true, true, true> : k_advance2d<true, false, true, true>;
             else if (bgrid)
                 kern = use_mcc ? k_advance2d<true, true, true, false, true> : k_advance2d<true, false, true, false, true>;
             else kern = use_mcc ? k_advance2d<true, true, true, false> : k_advance2d<true, false, true, false>;
@@ -1097,14 +1135,35 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         M2D_HIP(hipGetLastError());
     }
     if (deposit || use_mcc) {
-        // the grid flush and the collision queue in one launch
+        // the grid flush, the next step's phase-applied tables (after a
+        // specialised tiled pass, when they do not hold that phase yet) and
+        // the collision queue in one launch.  The next phase is the one the
+        // next call computes when it steps niter by one; any other call finds
+        // the tables stale and gathers from gx, gz.
         const unsigned nfl = deposit ? (unsigned)((f->n + kBlock - 1) / kBlock) : 0u;
-        const unsigned g = nfl + (use_mcc ? 64u : 0u);
+        double phn = 0.0;
+        if (f->p.rf) phn = f->p.rf_amplitude * rf_cos(s->niter + 1, s->dt, f->p.rf_omega) + f->p.rf_U0;
+        PhBuild pb;
+        pb.gx = f->gx; pb.gz = f->gz; pb.phx = f->phx; pb.phz = f->phz;
+        pb.nxe = (size_t)(f->nx + 1) * f->nz;
+        pb.nze = (size_t)f->nx * (f->nz + 1);
+        pb.ph = (float)phn;
+        const bool build_ph = fast_tiled && !(f->ph_valid && f->ph_phase == pb.ph);
+        const unsigned ntab =
+            build_ph ? (unsigned)((pb.nxe + pb.nze + kBlock * kPhPerThread - 1) / (kBlock * kPhPerThread)) : 0u;
+        const unsigned g = nfl + ntab + (use_mcc ? kScatterBlocks : 0u);
+        // without a deposit no flush block runs: n = 0 ends them at once
+        const size_t nodes = deposit ? f->n : 0;
         if (use_mcc)
-            k_flush_scatter<true><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n, nfl, a, next_len);
+            k_flush_scatter<true><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, nodes, pb, ntab, a, next_len);
         else
-            k_flush_scatter<false><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n, nfl, a, next_len);
+            k_flush_scatter<false><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, nodes, pb, ntab, a,
+                                                                  next_len);
         M2D_HIP(hipGetLastError());
+        if (build_ph) {
+            f->ph_valid = true;
+            f->ph_phase = pb.ph;
+        }
     }
     if (stats) {
         M2D_TRY(read_counters(ctx));

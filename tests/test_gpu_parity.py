@@ -351,6 +351,37 @@ def test_tiled_and_untiled_passes_agree_bit_for_bit_over_many_steps(ctx, density
         assert np.array_equal(ds[k].view(np.uint32), dc[k].view(np.uint32)), k
 
 
+def test_phase_tables_follow_the_step(ctx):
+    """The specialised tiled pass gathers from tables that hold its step's RF
+    phase, built by the previous step's flush for niter + 1.  A niter that
+    does not follow by one, or new potentials (which rebuild the gradient
+    tables), sends the gather back to the two-component tables.  Every path
+    gives the same bits as the untiled pass on a copy of the store."""
+    sc = Scene(ctx, n=128)
+    dev, orc = _mcc_pair(ctx, abi.ELASTIC, density=1e22)
+    rng = np.random.default_rng(19)
+    n = 60_000
+    x, z, vx, vy, vz = ions_on_disk(n, rng, T_eV=2.0)
+    a = dev_particles(ctx, x, z, vx, vy, vz)
+    s = abi.ParticlesSoA(ctx, n)
+    abi.sort(ctx, sc.fields, a, s)
+    c = abi.ParticlesSoA.from_host(ctx, *(s.download()[k] for k in abi.ParticlesSoA.NAMES))
+    flags = abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT
+    for st in (0, 1, 2, 3, 9, 10, 11, 12, 40, 41, 42):
+        if st == 11:
+            sc.fields.upload(abi.FIELD_URF, sc.urf)  # new potentials: the tables are rebuilt
+        rhos = []
+        for P in (s, c):
+            rho = ctx.to_device(np.zeros(128 * 128))
+            abi.advance(ctx, sc.fields, species(QE, M_ION, 1e-9, dev.lifetime, st), P, mcc=dev,
+                        rho_ptr=rho.ptr, flags=flags)
+            rhos.append(rho.get())
+        assert np.array_equal(rhos[0], rhos[1]), st
+    ds, dc = s.download(), c.download()
+    for k in abi.ParticlesSoA.NAMES:
+        assert np.array_equal(ds[k].view(np.uint32), dc[k].view(np.uint32)), k
+
+
 # ------------------------------------------------------------------ Poisson
 
 def test_poisson_rf_basis_matches_fp64_oracle(ctx):
