@@ -1148,7 +1148,9 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         pb.nxe = (size_t)(f->nx + 1) * f->nz;
         pb.nze = (size_t)f->nx * (f->nz + 1);
         pb.ph = (float)phn;
-        const bool build_ph = fast_tiled && !(f->ph_valid && f->ph_phase == pb.ph);
+        // (self-consistent fields are solved again before the next step, which
+        // rebuilds gx, gz and drops tables built now)
+        const bool build_ph = fast_tiled && !f->p.selfconsistent && !(f->ph_valid && f->ph_phase == pb.ph);
         const unsigned ntab =
             build_ph ? (unsigned)((pb.nxe + pb.nze + kBlock * kPhPerThread - 1) / (kBlock * kPhPerThread)) : 0u;
         const unsigned g = nfl + ntab + (use_mcc ? kScatterBlocks : 0u);
