@@ -39,6 +39,18 @@ M_ION = 3 * 1.66053906660e-27
 M_HE = 6.69e-27
 
 
+def emit(out):
+    """Print the result line; a run whose kernels raised a device flag (NaN
+    positions, gathers or deposits off the grid) is marked invalid and exits
+    non-zero."""
+    flags = out.get("device_flags", out.get("config", {}).get("device_flags", 0))
+    if flags:
+        out["invalid"] = f"device flags 0x{flags:x} raised during the run"
+    print(json.dumps(out), flush=True)
+    if flags:
+        sys.exit(4)
+
+
 def build_scene(ctx, n_grid, n_particles):
     L = 2e-2
     dx = L / (n_grid - 1)
@@ -329,7 +341,7 @@ def run_penning2d(args):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_penning(fields, mask, volt, n, L, dV, N, vth, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def cpu_baseline_penning(fields, mask, volt, n, L, dV, N, vth, budget_s):
@@ -436,7 +448,7 @@ def run_adjoint2d(args):
         "J_eV": J1, "J_forward_only_eV": J0, "grad_eV_per_V": list(grad),
         "device_flags": ctx.status(clear=True),
     }
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def run_batch2d(args):
@@ -520,7 +532,7 @@ def run_batch2d(args):
         "live_particles_end": int(live.sum()), "mean_energy_eV_range": [float(mean.min()), float(mean.max())],
         "device_flags": ctx.status(clear=True),
     }
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def quad_trap_3d(n, L, V_rod=2.0, V_end=5.0, r0=0.3, a=0.08):
@@ -609,7 +621,7 @@ def run_trap3d(args):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_3d(fields, mask, n, L, h, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 def cpu_baseline_3d(fields, mask, n, L, h, budget_s):
@@ -786,7 +798,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(fields, params, mask, args.grid, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 if __name__ == "__main__":

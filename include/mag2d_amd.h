@@ -85,6 +85,16 @@ int m2d_ctx_status(m2d_ctx *ctx, uint32_t *flags, int clear);
  * time in ms and the launch count, and clears them. */
 int m2d_ctx_profile(m2d_ctx *ctx, int enable);
 int m2d_ctx_profile_read(m2d_ctx *ctx, double *ms, int64_t *launches);
+/* Which compile-time form of the 2D particle pass the last m2d_advance_2d
+ * launched (0 if none): an OR of the bits below.  A diagnostic for tests
+ * that must reach a given form; no counterpart in the reference. */
+#define M2D_PASS_DEPOSIT 1u   /* deposit fused */
+#define M2D_PASS_MCC     2u   /* MCC trial and collision queue */
+#define M2D_PASS_TILED   4u   /* sorted store: per-tile LDS deposit windows */
+#define M2D_PASS_FAST    8u   /* B = 0, no wrap, origin 0, box and free-cell tests */
+#define M2D_PASS_BGRID  16u   /* B gathered from a B grid per particle */
+#define M2D_PASS_PHT    32u   /* gather from the phase-applied tables */
+int m2d_ctx_last_pass(m2d_ctx *ctx, uint32_t *variant);
 
 int m2d_malloc(m2d_ctx *ctx, size_t bytes, void **out);
 int m2d_free(m2d_ctx *ctx, void *p);

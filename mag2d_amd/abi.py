@@ -28,6 +28,7 @@ FIELD_U, FIELD_URF, FIELD_RHO, FIELD_VOLTAGE = 0, 1, 2, 3
 SOLVE_STATIC, SOLVE_RF, SOLVE_RAW = 0, 1, 2
 ELASTIC, LANGEVIN, CX, COULOMB, SUPERELASTIC = range(5)
 STEP_PUSH, STEP_BOUNDARY, STEP_DEPOSIT = 1, 2, 4
+PASS_DEPOSIT, PASS_MCC, PASS_TILED, PASS_FAST, PASS_BGRID, PASS_PHT = 1, 2, 4, 8, 16, 32
 
 EPS0 = 8.854187817e-12
 KB = 1.380662e-23
@@ -144,6 +145,7 @@ _SIGS = {
     "m2d_ctx_status": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_int]),
     "m2d_ctx_profile": (C.c_int, [_P, C.c_int]),
     "m2d_ctx_profile_read": (C.c_int, [_P, _PD, C.POINTER(C.c_int64)]),
+    "m2d_ctx_last_pass": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "m2d_malloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "m2d_free": (C.c_int, [_P, _P]),
     "m2d_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -299,6 +301,12 @@ class Context:
         ms, n = C.c_double(), C.c_int64()
         check(lib().m2d_ctx_profile_read(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def last_pass(self) -> int:
+        """PASS_* bits of the particle pass the last advance launched"""
+        v = C.c_uint32()
+        check(lib().m2d_ctx_last_pass(self.h, C.byref(v)))
+        return v.value
 
     # device memory -------------------------------------------------------
     def malloc(self, nbytes: int) -> int:

@@ -13,6 +13,7 @@ libraries travel with the source tree.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import hashlib
 import os
 import shutil
 import subprocess
@@ -45,6 +46,58 @@ HIP_FLAGS = [
 ]
 
 
+def _sources_hip():
+    return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.hpp")) + sorted((ROOT / "include").glob("*.h"))
+
+
+def _sources_host():
+    return sorted((PKG / "host").glob("*.cpp")) + sorted((PKG / "host").glob("*.hpp")) + \
+        sorted((ROOT / "include").glob("*.h"))
+
+
+def _sources_oracle():
+    return sorted((ROOT / "oracle").glob("*.c")) + sorted((ROOT / "oracle").glob("*.h")) + \
+        [ROOT / "oracle" / "Makefile"]
+
+
+def source_hash(files) -> str:
+    """sha256 over the names and contents of a library's sources"""
+    h = hashlib.sha256()
+    for f in files:
+        h.update(str(f.relative_to(ROOT)).encode())
+        h.update(b"\0")
+        h.update(f.read_bytes())
+    return h.hexdigest()
+
+
+def _stamp(lib: Path) -> Path:
+    return lib.with_name(lib.name + ".srchash")
+
+
+# library -> its sources; build() stamps each library with their hash
+LIBRARIES = {
+    "hip": (LIB, _sources_hip),
+    "host": (HOST_LIB, _sources_host),
+    "oracle": (ROOT / "oracle" / "liboracle.so", _sources_oracle),
+}
+
+
+def stale_libraries() -> list:
+    """names of the libraries that are missing or older than their sources
+    (by content hash, not file times)"""
+    out = []
+    for name, (lib, srcs) in LIBRARIES.items():
+        st = _stamp(lib)
+        if not lib.exists() or not st.exists() or st.read_text().strip() != source_hash(srcs()):
+            out.append(name)
+    return out
+
+
+def _write_stamp(name: str) -> None:
+    lib, srcs = LIBRARIES[name]
+    _stamp(lib).write_text(source_hash(srcs()) + "\n")
+
+
 def _run(cmd, cwd=None):
     r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
@@ -68,6 +121,7 @@ def build_hip(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
+    _write_stamp("hip")
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -82,6 +136,7 @@ def build_host(verbose: bool = False) -> Path:
           f"-I{PKG / 'host'}", str(src), "-o", str(tmp), f"-L{PKG}", "-lmag2d_amd",
           "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, HOST_LIB)
+    _write_stamp("host")
     if verbose:
         print(f"built {HOST_LIB}")
     return HOST_LIB
@@ -90,7 +145,8 @@ def build_host(verbose: bool = False) -> Path:
 def build_oracle(verbose: bool = False) -> None:
     """Test infrastructure: the CPU oracle, and oracle/_ref when the
     reference sources exist in this container."""
-    _run(["make", "-C", str(ROOT / "oracle"), "-s", "all"])
+    _run(["make", "-B", "-C", str(ROOT / "oracle"), "-s", "all"])
+    _write_stamp("oracle")
     if REFERENCE.exists():
         _run(["make", "-C", str(ROOT / "oracle"), "-s", "ref", f"REF={REFERENCE}"])
     if verbose:

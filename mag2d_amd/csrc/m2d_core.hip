@@ -154,6 +154,13 @@ int m2d_ctx_profile_read(m2d_ctx *c, double *ms, int64_t *launches)
     return M2D_OK;
 }
 
+int m2d_ctx_last_pass(m2d_ctx *c, uint32_t *variant)
+{
+    M2D_CHECK(c && variant, M2D_ERR_BAD_ARG, "m2d_ctx_last_pass: NULL argument");
+    *variant = c->last_pass;
+    return M2D_OK;
+}
+
 const char *m2d_last_error(void) { return g_last_error.c_str(); }
 int m2d_abi_version(void) { return M2D_ABI_VERSION; }
 

@@ -132,6 +132,7 @@ struct m2d_ctx {
     m2d::Scratch cq;        // MCC collision queue (slot indices)
     m2d::Scratch partials;  // per-block deposit windows of the tiled 3D pass
     uint64_t launch_seq = 0;  // selects the collision-queue counter of a launch
+    uint32_t last_pass = 0;   // M2D_PASS_* bits of the last 2D particle pass launched
     std::vector<struct m2d_tiling *> tilings;  // schedules of recently sorted stores
     // kernel timing: event pairs around particle-pass launches
     bool profile = false;

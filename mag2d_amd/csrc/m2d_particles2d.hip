@@ -1133,6 +1133,15 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         }
         launch_timed(ctx, kern, dim3(g), dim3(kPass), a);
         M2D_HIP(hipGetLastError());
+        uint32_t v = 0;
+        if (T) {
+            v = M2D_PASS_DEPOSIT | M2D_PASS_TILED;
+            if (fast) v |= M2D_PASS_FAST | (f->ph_valid && f->ph_phase == a.phase ? M2D_PASS_PHT : 0u);
+            else if (bgrid) v |= M2D_PASS_BGRID;
+        } else {
+            v = (deposit ? M2D_PASS_DEPOSIT : 0u) | (fast ? M2D_PASS_FAST : bgrid ? M2D_PASS_BGRID : 0u);
+        }
+        ctx->last_pass = v | (use_mcc ? M2D_PASS_MCC : 0u);
     }
     if (deposit || use_mcc) {
         // the grid flush, the next step's phase-applied tables (after a

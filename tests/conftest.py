@@ -12,16 +12,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the C ABI on cuda:0)")
 
 
-def _built():
-    return all((ROOT / f).exists() for f in ("mag2d_amd/libmag2d_amd.so", "mag2d_amd/host/libmag2d_host.so",
-                                              "oracle/liboracle.so"))
-
-
 @pytest.fixture(scope="session", autouse=True)
 def _libraries():
-    # load what is built; build only when nothing is
-    if not _built():
-        from mag2d_amd import build
+    # A library whose sources changed since it was built (content hash
+    # recorded by the build) is rebuilt where there is no GPU; on a machine
+    # with a GPU the build belongs before the test run, so the run stops
+    # instead of testing old kernels.
+    from mag2d_amd import build
+    stale = build.stale_libraries()
+    if stale:
+        if os.path.exists("/dev/kfd"):
+            pytest.exit(f"stale or missing native libraries {stale}: build them first "
+                        "(python -c 'import __graft_entry__ as g; g.build()')", returncode=3)
         build.build_all()
     yield
 
