@@ -243,6 +243,15 @@ int m2d_fields2d_B(m2d_fields2d *f, const double *x, const double *z, int64_t n,
  * weights are summed as 64-bit fixed point (2^-31) and converted once. */
 int m2d_accumulate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_particles2d *p,
                       double charge, double *rho);
+/* No counterpart in the reference: m2d_accumulate_2d for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and the same bits; the
+ * fixed-point grid lives in a static per-GPU device array of 2^20 nodes
+ * (8 MiB), so nx * nz must be at most 1048576 (M2D_ERR_BAD_ARG otherwise).
+ * No allocation of any kind.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int m2d_accumulate_2d_noalloc(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_particles2d *p,
+                              double charge, double *rho);
 /* Field2D::interpolate / grad (Field2D.hpp:64-168) at n FP64 points, device arrays */
 int m2d_interpolate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const double *data, const double *x,
                        const double *z, int64_t n, double *out);

@@ -173,6 +173,7 @@ _SIGS = {
     "m2d_fields2d_set_magnetic_field": (C.c_int, [_P, C.POINTER(Grid2D), _PD, _PD]),
     "m2d_fields2d_B": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P]),
     "m2d_accumulate_2d": (C.c_int, [_P, C.POINTER(Grid2D), C.POINTER(Particles2D), C.c_double, _P]),
+    "m2d_accumulate_2d_noalloc": (C.c_int, [_P, C.POINTER(Grid2D), C.POINTER(Particles2D), C.c_double, _P]),
     "m2d_interpolate_2d": (C.c_int, [_P, C.POINTER(Grid2D), _P, _P, _P, C.c_int64, _P]),
     "m2d_grad_2d": (C.c_int, [_P, C.POINTER(Grid2D), _P, _P, _P, C.c_int64, _P, _P]),
     "m2d_mcc_create": (C.c_int, [_P, C.POINTER(MccDesc), C.POINTER(_P)]),

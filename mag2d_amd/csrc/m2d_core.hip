@@ -190,6 +190,7 @@ int m2d_ctx_create(int device, uint64_t seed, m2d_ctx **out)
     }
     int r = zero_words(c, c->counters, CNT_N);
     if (r == M2D_OK) e = hipStreamSynchronize(c->stream);
+    if (r == M2D_OK && e == hipSuccess) r = noalloc_init_device(device);
     if (r != M2D_OK || e != hipSuccess) {
         (void)hipFree(c->counters);
         (void)hipHostFree(c->h_counters);

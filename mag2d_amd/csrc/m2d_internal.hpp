@@ -176,6 +176,13 @@ int ensure(m2d_ctx *ctx, Scratch &s, size_t bytes);
 __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old, size_t n,
                          int first);
 __global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n);
+// Species::accumulate into a fixed-point grid (m2d_particles2d.hip)
+__global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, int64_t n, int nx, int nz,
+                                                         double idx, double idz, double xmin, double zmin,
+                                                         unsigned long long *acc, unsigned long long *counters);
+// device address of the no-allocation entry point's static scratch on the
+// current device, resolved once per device (m2d_noalloc.hip)
+int noalloc_init_device(int device);
 __global__ void k_dot_fin(const float *a, const float *b, size_t n, double *partial, unsigned int *ticket,
                           double *out, int fin);
 __global__ void k_cg_update_dot(float *e, float *r, const float *p, const float *q, const double *rz,
