@@ -117,16 +117,19 @@ def test_rf22pt_scene_on_the_bench_pass_matches_oracle_step_by_step(ctx):
         ti, tj = tile_of_slot[a] // ntz, tile_of_slot[a] % ntz
         out = (ci < 8 * ti - 8) | (ci + 1 > 8 * ti + 16) | (cj < 8 * tj - 8) | (cj + 1 > 8 * tj + 16)
         outside_window = max(outside_window, int(out.sum()))
+    d = bufs[cur].download()
+    alive = bufs[cur].alive()
+    oalive = op["empty"] == 0
+    mismatch = int(np.sum(alive != oalive))
+    print(f"scene: worst rho rel {worst_rho:.3g}, passes {sorted(set(hex(p) for p in passes))}, "
+          f"outside window {outside_window}, removal mismatch {mismatch}, collisions {coll_dev} / {coll_or}",
+          flush=True)
     assert worst_rho < 1e-6, worst_rho
     # the form of the pass that the benchmark times ran at every step but the
     # first, whose phase-applied tables the flush of a previous step had not built
     assert passes[0] == BENCH_PASS & ~abi.PASS_PHT
     assert all(p == BENCH_PASS for p in passes[1:]), [hex(p) for p in passes]
     assert outside_window > 0  # the window-overflow path of the deposit ran
-    d = bufs[cur].download()
-    alive = bufs[cur].alive()
-    oalive = op["empty"] == 0
-    mismatch = int(np.sum(alive != oalive))
     assert mismatch <= 3, mismatch
     # the same trials fire on both sides (same Philox words, same slots)
     assert abs(coll_dev - coll_or) <= mismatch, (coll_dev, coll_or)

@@ -143,3 +143,65 @@ def test_field2d_grad_live_against_reference_random_grids():
             a, b = C.c_double(0), C.c_double(0)
             O.lib().o_f2_grad(C.byref(g), O.dptr(data), xs[k], zs[k], C.byref(a), C.byref(b))
             assert a.value == gx[k] and b.value == gz[k]
+
+
+def _sv_interp(E, S, v, m1, m2, mul=1.0):
+    """sigma(E_rel[eV]) * v with the table read by linear interpolation,
+    clamped at both ends (numpy's interp), E_rel = mu v^2 / 2 in eV
+    (particles.hpp:61-70,415-420)."""
+    mu = m1 * m2 / (m1 + m2)
+    e = 0.5 * mu * v * v / O.QE
+    return mul * np.interp(e, E, S) * v
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.37])
+def test_svmax_with_tables_brackets_the_true_maximum(scale):
+    """BaseSpecies::svmax_find (particles.cpp:190-206) over two tabulated
+    interactions (the e-He elastic table of species_conf_MCC.txt and a second,
+    scaled copy on a shifted energy axis): the result equals the largest of
+    the 1000 sampled sums (the accumulated v += dv samples, v < vmax), and it
+    lies within the sampling error below the maximum of the piecewise-linear
+    sigma(E) v found by brute force on a grid 1000 times finer."""
+    d = np.load(G / "vecinterp_ref.npz")
+    E, S = np.ascontiguousarray(d["E"]), np.ascontiguousarray(d["S"])
+    E2 = np.ascontiguousarray(E * 3.0 + 0.5)
+    S2 = np.ascontiguousarray(S * scale)
+    m1, m2 = 9.11e-31, 6.69e-27
+    its = (O.Inter * 2)(O.inter(O.ELASTIC, m1, m2, tabE=E, tabS=S), O.inter(O.ELASTIC, m1, m2, tabE=E2, tabS=S2))
+    for E_max in (0.3, 5.0, 40.0):
+        vmax = np.sqrt(2.0 * E_max * O.QE / m1)
+        got = O.lib().o_svmax_find(its, 2, vmax, 1000)
+        dv = vmax / 1000
+        v = np.add.accumulate(np.concatenate([[0.0], np.full(1100, dv)]))
+        v = v[v < vmax]
+        sv = _sv_interp(E, S, v, m1, m2) + _sv_interp(E2, S2, v, m1, m2)
+        assert got == pytest.approx(sv.max(), rel=1e-13)
+        fine = np.linspace(0, vmax, 1_000_001)[:-1]
+        f = _sv_interp(E, S, fine, m1, m2) + _sv_interp(E2, S2, fine, m1, m2)
+        true = f.max()
+        # between grid points the sum changes by at most its slope times the spacing
+        slope = np.abs(np.diff(f)).max() / fine[1]
+        assert got <= true + slope * fine[1]
+        assert true - got <= slope * dv
+
+
+def test_lifetime_with_tables_and_constant_rate():
+    """lifetime = 1 / sum over targets of density * svmax (particles.cpp:142-170)
+    with a tabulated and a constant-rate interaction on one target and a
+    second target of another density."""
+    d = np.load(G / "vecinterp_ref.npz")
+    E, S = np.ascontiguousarray(d["E"]), np.ascontiguousarray(d["S"])
+    m1, m2, m3 = 9.11e-31, 6.69e-27, 3.3e-26
+    E_max = 8.0
+    mcc = O.make_mcc(m1, E_max, [
+        dict(mass=m2, density=2e20, temperature=300.0,
+             inters=[O.inter(O.ELASTIC, m1, m2, tabE=E, tabS=S), O.inter(O.CX, m1, m2, rate=3e-16)]),
+        dict(mass=m3, density=5e19, temperature=300.0, inters=[O.inter(O.ELASTIC, m1, m3, rate=1e-15)]),
+    ])
+    vmax = np.sqrt(2.0 * E_max * O.QE / m1)
+    v = np.add.accumulate(np.concatenate([[0.0], np.full(1100, vmax / 1000)]))
+    v = v[v < vmax]
+    r1 = 2e20 * (_sv_interp(E, S, v, m1, m2) + 3e-16).max()
+    r2 = 5e19 * 1e-15
+    assert mcc.lifetime == pytest.approx(1.0 / (r1 + r2), rel=1e-13)
+    assert mcc.targets[0].rate == pytest.approx(r1, rel=1e-13)

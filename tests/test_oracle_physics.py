@@ -153,7 +153,12 @@ def test_rf22pt_geometry_invariants():
     assert rods.sum() > 0
     v = volt.reshape(n, n)
     assert set(np.unique(v[rods])) == {-1.0, 1.0}
-    # 22 rods alternate in sign: equal node counts within discretisation
+    # exactly 22 rods (connected pieces), alternating in sign: equal node
+    # counts within discretisation; rod radius, ring radius and angles are
+    # pinned in tests/test_oracle_geometry.py
+    from scipy import ndimage
+    assert ndimage.label(rods)[1] == 22
+    assert ndimage.label(rods & (v > 0))[1] == 11 and ndimage.label(rods & (v < 0))[1] == 11
     assert abs((v[rods] > 0).sum() - (v[rods] < 0).sum()) <= 0.2 * rods.sum()
     # only FIXED 4-neighbours inside [2, M-2) create BOUNDARY (fields.cpp:507-518):
     # the frame is two nodes away and the rods are FIXED_RF, so there is none
