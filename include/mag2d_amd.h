@@ -145,11 +145,17 @@ int m2d_grid_is_free_2d(const m2d_grid2d *g, const uint8_t *mask, double x, doub
 
 /* ======================= particles ====================================== */
 
-/* Species particle store as SoA (replaces vector<t_particle>, particles.hpp:26-33,113) */
+/* Species particle store as SoA (replaces vector<t_particle>, particles.hpp:26-33,113).
+ * Long-time positions: with xl and zl set, a position is the exact FP64 sum
+ * x + xl (z + zl) of an FP32 value and its FP32 remainder, about 48
+ * significant bits, and every pass, sampler, sort and compaction reads and
+ * writes it as such (the reference's FP64 t_particle::x, z drift,
+ * particles.cpp:986-987).  Without them (NULL) positions are FP32. */
 typedef struct {
     float *x, *z;          /* in-plane position (reference x, z) */
     float *vx, *vy, *vz;   /* vy is the out-of-plane component */
     int64_t n;             /* slots, live + empty */
+    float *xl, *zl;        /* FP32 remainders of x and z, or both NULL */
 } m2d_particles2d;
 
 /* per-species constants of one step (BaseSpecies members, particles.hpp:118-131) */

@@ -591,9 +591,12 @@ __device__ __forceinline__ bool cic3(double X, double Y, double Z, const Grid3 &
     // range in FP64 before any conversion: cells [0, n-2] on every axis
     if (!(X >= 0.0 && X < g.nx - 1.0 && Y >= 0.0 && Y < g.ny - 1.0 && Z >= 0.0 && Z < g.nz - 1.0)) return false;
     const int i = (int)X, j = (int)Y, k = (int)Z;
-    const float u = (float)(X - i), v = (float)(Y - j), ww = (float)(Z - k);
+    // offsets and their complements from FP64, each rounded once (see the
+    // 2D cic_weights)
+    const double du = X - i, dv = Y - j, dw = Z - k;
     const float S = 2147483648.0f;
-    const float a[2] = {1.f - u, u}, b[2] = {1.f - v, v}, c[2] = {1.f - ww, ww};
+    const float a[2] = {(float)(1.0 - du), (float)du}, b[2] = {(float)(1.0 - dv), (float)dv},
+                c[2] = {(float)(1.0 - dw), (float)dw};
     uint32_t s = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++) {

@@ -45,6 +45,7 @@ struct BatchArgs {
     const uint8_t *cellfree;
     int nx, nz;
     double idx, idz, xmin, zmin;
+    float idxh, idxl, idzh, idzl;  // idx, idz as FP32 pairs (the origin-0 gather)
     int econst;
     float ecx, ecz;
     float fx0, fx1, fz0, fz1;
@@ -79,8 +80,11 @@ struct BP {
 };
 
 // segment of `nsteps` steps from step niter0 for PPT particles per thread;
-// cosv[s] = cos of the RF phase at step niter0 + s
-template <int PPT>
+// cosv[s] = cos of the RF phase at step niter0 + s.  FASTG: the fields and
+// species are those for which m2d_advance_2d runs its specialised pass (B =
+// 0, no wrap, no constant field, origin 0, free-cell test on), whose gather
+// is the FP32 origin-0 one; the same gather here keeps the runs bit for bit.
+template <int PPT, bool FASTG>
 __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, int nsteps, const double *cosv)
 {
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -113,7 +117,11 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
             const float ph = (float)__dadd_rn(__dmul_rn(R.amp, cs), a.U0);
             float x = q[k].x, z = q[k].z, vx = q[k].vx, vy = q[k].vy, vz = q[k].vz;
             float ex, ez;
-            gather_E(a, (double)x, (double)z, ph, ex, ez);
+            if constexpr (FASTG) {
+                if (gather_E_f(a, x, 0.f, z, 0.f, ph, ex, ez)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+            } else {
+                gather_E(a, (double)x, (double)z, ph, ex, ez);
+            }
             vx = fmaf(ex, a.qmdt2, vx);
             vz = fmaf(ez, a.qmdt2, vz);
             if (a.rotate) {
@@ -154,10 +162,20 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
                 }
             }
             if (!dead && a.checkfree) {
-                const double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
                 bool fr = false;
-                if (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz)
-                    fr = a.cellfree[(int)X * a.nz + (int)Z] != 0;
+                if constexpr (FASTG) {
+                    // the cell of the specialised single-run pass (the box test gave x, z >= 0)
+                    int ci, cj;
+                    float u, ou;
+                    cell_f(x, 0.f, a.idxh, a.idxl, ci, u, ou);
+                    cell_f(z, 0.f, a.idzh, a.idzl, cj, u, ou);
+                    if ((unsigned)ci <= (unsigned)(a.nx - 1) && (unsigned)cj <= (unsigned)(a.nz - 1))
+                        fr = a.cellfree[ci * a.nz + cj] != 0;
+                } else {
+                    const double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
+                    if (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz)
+                        fr = a.cellfree[(int)X * a.nz + (int)Z] != 0;
+                }
                 dead = !fr;
             }
             if (dead) { q[k].live = false; n_removed++; }
@@ -265,6 +283,8 @@ static void batch_args(const m2d_batch *b, const m2d_particles2d &p, BatchArgs &
     a.x = p.x; a.z = p.z; a.vx = p.vx; a.vy = p.vy; a.vz = p.vz; a.n = p.n;
     a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree;
     a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
+    split_f32(f->idx, a.idxh, a.idxl);
+    split_f32(f->idz, a.idzh, a.idzl);
     a.xmin = P.grid.xmin; a.zmin = P.grid.zmin;
     a.econst = P.geometry_empty && !P.selfconsistent;
     a.ecx = 0.f;
@@ -315,10 +335,19 @@ static int launch_segment(m2d_batch *b, const m2d_particles2d &p, const m2d_batc
     const int64_t threads = (p.n + sh.ppt - 1) / sh.ppt;
     const unsigned grid = (unsigned)std::max<int64_t>(1, (threads + sh.block - 1) / sh.block);
     hipStream_t st = b->ctx->stream;
-    switch (sh.ppt) {
-    case 1: k_batch2d<1><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-    case 2: k_batch2d<2><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-    default: k_batch2d<4><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+    const bool fastg = a.checkfree && !a.rotate && !a.periodic && !a.econst && a.xmin == 0.0 && a.zmin == 0.0;
+    if (fastg) {
+        switch (sh.ppt) {
+        case 1: k_batch2d<1, true><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        case 2: k_batch2d<2, true><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        default: k_batch2d<4, true><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        }
+    } else {
+        switch (sh.ppt) {
+        case 1: k_batch2d<1, false><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        case 2: k_batch2d<2, false><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        default: k_batch2d<4, false><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        }
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;

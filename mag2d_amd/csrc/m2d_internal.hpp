@@ -34,7 +34,7 @@ constexpr int kTileShift = 6;          // key = tile << 6 | cell in tile
 constexpr int kHalo = 8;               // deposit window halo (cells) around a tile: a particle
                                        // stays in its window for many steps after a sort
 constexpr int kWin = kTile + 1 + 2 * kHalo;  // window side in nodes
-constexpr int kChunkCap = 960;         // particles per block of the tiled 2D pass: with its
+constexpr int kChunkCap = 1984;        // particles per block of the tiled 2D pass: with its
                                        // start rounded down to 64 slots, two staged rounds of 512
 constexpr int kChunkCap3 = 1984;       // particles per block of the tiled 3D pass
 
@@ -288,6 +288,13 @@ inline double rf_cos(uint64_t niter, double dt, double omega)
     const double y = 10000000 * M_PI;
     if (!(ph >= 0.0 && ph <= y)) ph = ph - y * (int)(ph / y) + (ph < 0 ? y : 0);
     return cos(ph);
+}
+
+// v as an FP32 pair: hi = v rounded, lo = the rounded remainder
+__host__ __device__ inline void split_f32(double v, float &hi, float &lo)
+{
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
 }
 
 inline int grid_for(int64_t n, int per_thread, int block = kBlock)

@@ -23,6 +23,7 @@ struct PushArgs {
     const uint8_t *winfree;        // per sort tile: whole deposit window free
     int nx, nz;
     double idx, idz, xmin, zmin;
+    float idxh, idxl, idzh, idzl;  // idx, idz as FP32 pairs (the origin-0 gather)
     // box [x_min,x_max] x [z_min,z_max] as the FP32 bounds that give the same
     // answer for FP32 positions; x_max, z_max in FP64 for the periodic wrap
     float fx0, fx1, fz0, fz1;
@@ -96,29 +97,26 @@ __device__ __forceinline__ bool local_rotation(const PushArgs &a, float x, float
 
 // ------------------------------------------------------------------ deposit
 // CIC weights (Field2D.hpp:45-62) as 2^-31 fixed point.  The cell offsets
-// u, v come from the FP64 products x*idx (exact subtraction of the cell
-// index), the four bilinear products are FP32 (relative error ~1e-7), and
-// the rounded weights are corrected on the nearest node's (at least 1/4) so
-// each particle deposits exactly 2^31: charge is conserved bit for bit and
-// node sums do not depend on the order of the particles.
+// u, v and their complements 1 - u, 1 - v are formed in FP64 from the
+// products x*idx (exact subtraction of the cell index) and rounded to FP32
+// each, so every factor carries a relative error of at most 2^-24 (a
+// complement formed in FP32 from a rounded u would lose its relative
+// accuracy as u nears 1); the four bilinear products are FP32 (relative
+// error ~2e-7 in all), and the rounded weights are corrected on the nearest
+// node's (at least 1/4) so each particle deposits exactly 2^31: charge is
+// conserved bit for bit and node sums do not depend on the order of the
+// particles.
 // NONNEG: the caller guarantees X, Z >= 0 (a NaN included: it fails the
 // upper bounds).
-template <bool NONNEG = false>
-__device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
-                                            uint32_t w[4])
+// The four weights from the offsets u, v and their complements ou, ov.
+__device__ __forceinline__ void cic_from_offsets(float u, float ou, float v, float ov, uint32_t w[4])
 {
-    // range in FP64 before the conversion: cells [0, n-2]
-    if (!((NONNEG || X >= 0.0) && X < nx - 1.0 && (NONNEG || Z >= 0.0) && Z < nz - 1.0)) return false;
-    i = (int)X;
-    j = (int)Z;
-    const float u = (float)(X - i), v = (float)(Z - j);
-    // scaling by 2^31 is exact, so (1 - u) * ((1 - v) 2^31) has the bits of
-    // ((1 - u)(1 - v)) 2^31
+    // scaling by 2^31 is exact, so ou * (ov 2^31) has the bits of (ou ov) 2^31
     const float S = 2147483648.0f;  // 2^31
-    const float a = (1.f - v) * S, b = v * S;
-    w[0] = __float2uint_rn((1.f - u) * a);
+    const float a = ov * S, b = v * S;
+    w[0] = __float2uint_rn(ou * a);
     w[1] = __float2uint_rn(u * a);
-    w[2] = __float2uint_rn((1.f - u) * b);
+    w[2] = __float2uint_rn(ou * b);
     w[3] = __float2uint_rn(u * b);
     // each weight is at most 2^31, so the sum stays below 2^32
     const uint32_t d = 0x80000000u - (w[0] + w[1] + w[2] + w[3]);
@@ -129,8 +127,21 @@ __device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, 
     w[1] += bu && !bv ? d : 0u;
     w[2] += !bu && bv ? d : 0u;
     w[3] += bu && bv ? d : 0u;
+}
+
+template <bool NONNEG = false>
+__device__ __forceinline__ bool cic_weights(double X, double Z, int nx, int nz, int &i, int &j,
+                                            uint32_t w[4])
+{
+    // range in FP64 before the conversion: cells [0, n-2]
+    if (!((NONNEG || X >= 0.0) && X < nx - 1.0 && (NONNEG || Z >= 0.0) && Z < nz - 1.0)) return false;
+    i = (int)X;
+    j = (int)Z;
+    const double du = X - i, dv = Z - j;
+    cic_from_offsets((float)du, (float)(1.0 - du), (float)dv, (float)(1.0 - dv), w);
     return true;
 }
+
 
 __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, int i, int j,
                                                const long long w[4])
@@ -158,7 +169,7 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 // cells); only the last lane of each segment issues the four 64-bit atomics.
 // Two-wavefront blocks with 5 KB of LDS (the deposit window): a launch of
 // a few thousand chunks is mostly resident at once.
-constexpr int kPass = 128;
+constexpr int kPass = 256;
 constexpr int kRound = kPass * 4;
 constexpr int kGather = 2;                  // particles whose gathers are in flight together
 constexpr int kChunk = 2048;                // slots per block of the untiled pass
@@ -222,11 +233,23 @@ __device__ __forceinline__ void wave_deposit(DepRun &d, unsigned long long *acc,
 // which frees the scalar registers and branches the generic flags cost.
 // BGRID (generic pass only): B from the gridded field at each particle.
 // PHT (FAST only): the gather reads the phase-applied tables phx, phz.
+// Wavefronts per SIMD each form is built for: a thread holds its four
+// particles and two gathers in registers.  The specialised tiled pass's FP32
+// index arithmetic (gather, cell and CIC offsets) fits six wavefronts (80
+// VGPRs) when the gather reads the one-float phase-applied tables, five (96
+// VGPRs) with the two-component tables; the untiled forms carry the deposit
+// runs and the FP64 generic forms their wider arithmetic, and keep four (128
+// VGPRs), three for the untiled deposit with a gridded B.  Each is the most
+// that builds without spilling to scratch memory, which costs far more than
+// the occupancy gains.
+template <bool DEPOSIT, bool TILED, bool FAST, bool BGRID, bool PHT>
+constexpr int pass_waves()
+{
+    return TILED && FAST ? (PHT ? 6 : 5) : (BGRID && DEPOSIT && !TILED ? 3 : 4);
+}
+
 template <bool DEPOSIT, bool MCC, bool TILED, bool FAST, bool BGRID = false, bool PHT = false>
-// four wavefronts per SIMD (128 VGPRs): a thread holds its four particles and
-// two gathers in registers (with fewer VGPRs the scalar and vector spills cost
-// more than the occupancy gains)
-__global__ void __launch_bounds__(kPass) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kPass) __attribute__((amdgpu_waves_per_eu(pass_waves<DEPOSIT, TILED, FAST, BGRID, PHT>())))
 k_advance2d(PushArgs a)
 {
     __shared__ unsigned long long win[TILED ? kWin * kWin : 1];
@@ -307,8 +330,12 @@ k_advance2d(PushArgs a)
 #pragma unroll
                 for (int h = 0; h < kGather; h++) {
                     // a dead slot gathers at a clamped point and is discarded
-                    const bool b = gather_E_clamped<FAST, PHT>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
-                                                          gez[h]);
+                    bool b;
+                    if constexpr (FAST)
+                        b = gather_E_f<PHT>(a, v[0][h], 0.f, v[1][h], 0.f, a.phase, gex[h], gez[h]);
+                    else
+                        b = gather_E_clamped<false, PHT>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
+                                                         gez[h]);
                     gbad |= b && !is_dead(v[0][h]);
                 }
             }
@@ -372,18 +399,27 @@ k_advance2d(PushArgs a)
                         }
                     }
                 }
-                const double X = FAST ? (double)x * a.idx : ((double)x - a.xmin) * a.idx;
-                const double Z = FAST ? (double)z * a.idz : ((double)z - a.zmin) * a.idz;
+                // FAST: cell and offsets in FP32 pair arithmetic (the box
+                // test above gave 0 <= x, z); otherwise FP64 cell coordinates
+                int fci = 0, fcj = 0;
+                float fu = 0.f, fou = 0.f, fv = 0.f, fov = 0.f;
+                double X = 0.0, Z = 0.0;
+                if constexpr (FAST) {
+                    cell_f(x, 0.f, a.idxh, a.idxl, fci, fu, fou);
+                    cell_f(z, 0.f, a.idzh, a.idzl, fcj, fv, fov);
+                } else {
+                    X = ((double)x - a.xmin) * a.idx;
+                    Z = ((double)z - a.zmin) * a.idz;
+                }
                 if (!dead && (FAST || (a.boundary && a.checkfree))) {
                     bool free_ = false;
-                    // the specialised pass's box test already gave x, z >= 0
-                    if ((FAST || X >= 0.0) && X < (double)a.nx && (FAST || Z >= 0.0) && Z < (double)a.nz) {
-                        int ci = (int)X, cj = (int)Z;
+                    if (FAST || (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz)) {
+                        const int ci = FAST ? fci : (int)X, cj = FAST ? fcj : (int)Z;
                         // a cell of an all-free window needs no table lookup
                         if (wfree && (unsigned)(ci - wi0) < (unsigned)(kWin - 1) &&
                             (unsigned)(cj - wj0) < (unsigned)(kWin - 1))
                             free_ = true;
-                        else if (ci <= a.nx - 1 && cj <= a.nz - 1)
+                        else if ((unsigned)ci <= (unsigned)(a.nx - 1) && (unsigned)cj <= (unsigned)(a.nz - 1))
                             free_ = a.cellfree[ci * a.nz + cj] != 0;
                     }
                     dead = !free_;
@@ -398,7 +434,18 @@ k_advance2d(PushArgs a)
                 if (DEPOSIT) {
                     int ci, cj;
                     uint32_t w[4];
-                    if (cic_weights<FAST>(X, Z, a.nx, a.nz, ci, cj, w)) {
+                    bool in_grid;
+                    if constexpr (FAST) {
+                        // cells [0, n-2]; the free-cell test above already
+                        // removed the last row and column (not free)
+                        ci = fci;
+                        cj = fcj;
+                        in_grid = (unsigned)ci <= (unsigned)(a.nx - 2) && (unsigned)cj <= (unsigned)(a.nz - 2);
+                        if (in_grid) cic_from_offsets(fu, fou, fv, fov, w);
+                    } else {
+                        in_grid = cic_weights(X, Z, a.nx, a.nz, ci, cj, w);
+                    }
+                    if (in_grid) {
                         if constexpr (TILED) {
                             window_deposit(ci, cj, w, win, wi0, wj0, a.acc, a.nz);
                         } else if (ci == run.ci && cj == run.cj) {
@@ -610,7 +657,25 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
     if (is_dead(x)) return;
     int i, j;
     uint32_t w[4];
-    if (cic_weights(((double)x - xmin) * idx, ((double)zs[p] - zmin) * idz, nx, nz, i, j, w)) {
+    bool in_grid;
+    if (xmin == 0.0 && zmin == 0.0) {
+        // the FP32 pair arithmetic of the specialised pass, so a grid of
+        // origin 0 gets the bits of the fused deposit (a NaN fails x >= 0)
+        const float z = zs[p];
+        float sxh, sxl, szh, szl, u, ou, v, ov;
+        split_f32(idx, sxh, sxl);
+        split_f32(idz, szh, szl);
+        in_grid = x >= 0.f && z >= 0.f;
+        if (in_grid) {
+            cell_f(x, 0.f, sxh, sxl, i, u, ou);
+            cell_f(z, 0.f, szh, szl, j, v, ov);
+            in_grid = (unsigned)i <= (unsigned)(nx - 2) && (unsigned)j <= (unsigned)(nz - 2);
+            if (in_grid) cic_from_offsets(u, ou, v, ov, w);
+        }
+    } else {
+        in_grid = cic_weights(((double)x - xmin) * idx, ((double)zs[p] - zmin) * idz, nx, nz, i, j, w);
+    }
+    if (in_grid) {
         const long long ww[4] = {w[0], w[1], w[2], w[3]};
         deposit_global(acc, nz, i, j, ww);
     }
@@ -994,6 +1059,8 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree; a.winfree = f->winfree;
     a.phx = f->phx; a.phz = f->phz;
     a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
+    split_f32(f->idx, a.idxh, a.idxl);
+    split_f32(f->idz, a.idzh, a.idzl);
     a.xmin = P.grid.xmin; a.zmin = P.grid.zmin;
     a.fx0 = f32_at_least(P.grid.xmin); a.fx1 = f32_at_most(P.x_max);
     a.fz0 = f32_at_least(P.grid.zmin); a.fz1 = f32_at_most(P.z_max);
