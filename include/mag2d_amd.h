@@ -94,6 +94,7 @@ int m2d_ctx_profile_read(m2d_ctx *ctx, double *ms, int64_t *launches);
 #define M2D_PASS_FAST    8u   /* B = 0, no wrap, origin 0, box and free-cell tests */
 #define M2D_PASS_BGRID  16u   /* B gathered from a B grid per particle */
 #define M2D_PASS_PHT    32u   /* gather from the phase-applied tables */
+#define M2D_PASS_HIPOS  64u   /* long-time positions (FP32 pairs) */
 int m2d_ctx_last_pass(m2d_ctx *ctx, uint32_t *variant);
 
 int m2d_malloc(m2d_ctx *ctx, size_t bytes, void **out);

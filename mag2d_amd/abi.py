@@ -28,7 +28,7 @@ FIELD_U, FIELD_URF, FIELD_RHO, FIELD_VOLTAGE = 0, 1, 2, 3
 SOLVE_STATIC, SOLVE_RF, SOLVE_RAW = 0, 1, 2
 ELASTIC, LANGEVIN, CX, COULOMB, SUPERELASTIC = range(5)
 STEP_PUSH, STEP_BOUNDARY, STEP_DEPOSIT = 1, 2, 4
-PASS_DEPOSIT, PASS_MCC, PASS_TILED, PASS_FAST, PASS_BGRID, PASS_PHT = 1, 2, 4, 8, 16, 32
+PASS_DEPOSIT, PASS_MCC, PASS_TILED, PASS_FAST, PASS_BGRID, PASS_PHT, PASS_HIPOS = 1, 2, 4, 8, 16, 32, 64
 
 EPS0 = 8.854187817e-12
 KB = 1.380662e-23
@@ -53,7 +53,7 @@ class Geom2D(C.Structure):
 
 class Particles2D(C.Structure):
     _fields_ = [("x", C.c_void_p), ("z", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p),
-                ("vz", C.c_void_p), ("n", C.c_int64)]
+                ("vz", C.c_void_p), ("n", C.c_int64), ("xl", C.c_void_p), ("zl", C.c_void_p)]
 
 
 class Species2D(C.Structure):
@@ -361,22 +361,32 @@ class DeviceArray:
 
 
 class ParticlesSoA:
-    """Device SoA x, z, vx, vy, vz (FP32) of n slots."""
+    """Device SoA x, z, vx, vy, vz (FP32) of n slots.  With ``hipos`` the
+    store also keeps the FP32 remainders xl, zl of the positions (long-time
+    positions: x + xl is the position in FP64)."""
 
     NAMES = ("x", "z", "vx", "vy", "vz")
 
-    def __init__(self, ctx: Context, n: int):
-        self.ctx, self.n = ctx, int(n)
-        self.arr = {k: DeviceArray(ctx, (max(self.n, 1),), np.float32) for k in self.NAMES}
+    def __init__(self, ctx: Context, n: int, hipos: bool = False):
+        self.ctx, self.n, self.hipos = ctx, int(n), bool(hipos)
+        names = self.NAMES + (("xl", "zl") if self.hipos else ())
+        self.arr = {k: DeviceArray(ctx, (max(self.n, 1),), np.float32) for k in names}
 
     @classmethod
-    def from_host(cls, ctx, x, z, vx, vy, vz):
-        p = cls(ctx, len(x))
+    def from_host(cls, ctx, x, z, vx, vy, vz, hipos: bool = False):
+        p = cls(ctx, len(x), hipos=hipos)
         p.upload(x, z, vx, vy, vz)
         return p
 
     def upload(self, x, z, vx, vy, vz):
-        for k, a in zip(self.NAMES, (x, z, vx, vy, vz)):
+        """Positions may be FP64: a store with remainders keeps them to
+        about 2^-48 (the FP32 value and its rounded remainder)."""
+        x64, z64 = np.asarray(x, np.float64), np.asarray(z, np.float64)
+        cols = {"x": x64.astype(np.float32), "z": z64.astype(np.float32), "vx": vx, "vy": vy, "vz": vz}
+        if self.hipos:
+            cols["xl"] = (x64 - cols["x"].astype(np.float64)).astype(np.float32)
+            cols["zl"] = (z64 - cols["z"].astype(np.float64)).astype(np.float32)
+        for k, a in cols.items():
             a = np.asarray(a, dtype=np.float32)
             assert a.shape == (self.n,)
             if self.n:
@@ -384,10 +394,17 @@ class ParticlesSoA:
         sort_forget(self.ctx, self)
 
     def download(self):
-        return {k: self.arr[k].get()[: self.n] for k in self.NAMES}
+        """The arrays; with remainders also x64, z64, the FP64 positions."""
+        d = {k: self.arr[k].get()[: self.n] for k in self.arr}
+        if self.hipos:
+            d["x64"] = d["x"].astype(np.float64) + d["xl"].astype(np.float64)
+            d["z64"] = d["z"].astype(np.float64) + d["zl"].astype(np.float64)
+        return d
 
     def struct(self, n: int | None = None) -> Particles2D:
-        return Particles2D(*(self.arr[k].ptr for k in self.NAMES), self.n if n is None else n)
+        xl = self.arr["xl"].ptr if self.hipos else None
+        zl = self.arr["zl"].ptr if self.hipos else None
+        return Particles2D(*(self.arr[k].ptr for k in self.NAMES), self.n if n is None else n, xl, zl)
 
     def alive(self):
         x = self.arr["x"].get()[: self.n]

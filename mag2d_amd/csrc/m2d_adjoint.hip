@@ -286,6 +286,7 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
     M2D_CHECK(ctx && f && s && p && cfg && J, M2D_ERR_BAD_ARG, "m2d_adjoint_energy_2d: NULL argument");
     M2D_CHECK(f->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_adjoint_energy_2d: fields belong to another context");
     M2D_CHECK(!f->bt, M2D_ERR_UNSUPPORTED, "m2d_adjoint_energy_2d: gridded B is not supported");
+    M2D_CHECK(!p->xl, M2D_ERR_UNSUPPORTED, "m2d_adjoint_energy_2d: long-time positions are not supported");
     M2D_CHECK(f->p.selfconsistent && !f->p.rf && !f->p.electric_field_from_file, M2D_ERR_UNSUPPORTED,
               "m2d_adjoint_energy_2d: needs a self-consistent field without RF and without a field file");
     M2D_CHECK(cfg->steps >= 1 && cfg->checkpoint_every >= 1, M2D_ERR_BAD_ARG,

@@ -41,6 +41,7 @@ struct BatchRun {
 struct BatchArgs {
     float *x, *z, *vx, *vy, *vz;
     int64_t n;
+    float *xl, *zl;          // position remainders (long-time positions), or null
     const float2 *gx, *gz;
     const uint8_t *cellfree;
     int nx, nz;
@@ -52,6 +53,7 @@ struct BatchArgs {
     double xper, zper;
     int periodic, checkfree;
     float qmdt2, tx, ty, tz, sx, sy, sz, dt;
+    float dtl;               // dt - (double)dt as FP32
     int rotate;
     double U0;
     uint64_t seed;
@@ -74,6 +76,7 @@ __device__ __forceinline__ int run_of(const int64_t *off, int n_runs, int64_t p)
 // one particle state carried across the steps of a segment
 struct BP {
     float x, z, vx, vy, vz;
+    float xl, zl;   // position remainders (HIPOS)
     bool live;
     int r;
     uint32_t slot;  // slot within its run
@@ -84,7 +87,8 @@ struct BP {
 // species are those for which m2d_advance_2d runs its specialised pass (B =
 // 0, no wrap, no constant field, origin 0, free-cell test on), whose gather
 // is the FP32 origin-0 one; the same gather here keeps the runs bit for bit.
-template <int PPT, bool FASTG>
+// HIPOS: long-time positions (FP32 pairs), as in m2d_advance_2d.
+template <int PPT, bool FASTG, bool HIPOS>
 __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, int nsteps, const double *cosv)
 {
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -101,6 +105,8 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
             q[k].live = !is_dead(q[k].x);
             if (q[k].live) {
                 q[k].z = a.z[p]; q[k].vx = a.vx[p]; q[k].vy = a.vy[p]; q[k].vz = a.vz[p];
+                q[k].xl = HIPOS ? a.xl[p] : 0.f;
+                q[k].zl = HIPOS ? a.zl[p] : 0.f;
                 q[k].r = run_of(a.offsets, a.n_runs, p);
                 q[k].slot = (uint32_t)(p - a.runs[q[k].r].off);
             }
@@ -116,11 +122,12 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
             // the per-step phase of m2d_advance_2d, rounded the same way (no contraction)
             const float ph = (float)__dadd_rn(__dmul_rn(R.amp, cs), a.U0);
             float x = q[k].x, z = q[k].z, vx = q[k].vx, vy = q[k].vy, vz = q[k].vz;
+            float xl = q[k].xl, zl = q[k].zl;
             float ex, ez;
             if constexpr (FASTG) {
-                if (gather_E_f(a, x, 0.f, z, 0.f, ph, ex, ez)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+                if (gather_E_f(a, x, xl, z, zl, ph, ex, ez)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
             } else {
-                gather_E(a, (double)x, (double)z, ph, ex, ez);
+                gather_E(a, HIPOS ? pos64(x, xl) : (double)x, HIPOS ? pos64(z, zl) : (double)z, ph, ex, ez);
             }
             vx = fmaf(ex, a.qmdt2, vx);
             vz = fmaf(ez, a.qmdt2, vz);
@@ -134,8 +141,13 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
             }
             vx = fmaf(ex, a.qmdt2, vx);
             vz = fmaf(ez, a.qmdt2, vz);
-            x = fmaf(vx, a.dt, x);
-            z = fmaf(vz, a.dt, z);
+            if constexpr (HIPOS) {
+                drift_pair(x, xl, vx, a.dt, a.dtl);
+                drift_pair(z, zl, vz, a.dt, a.dtl);
+            } else {
+                x = fmaf(vx, a.dt, x);
+                z = fmaf(vz, a.dt, z);
+            }
             if (R.trial_thr) {
                 // the trial word of slot s: word s%4 of the Philox block (s/4, step, 0, purpose 2)
                 uint32_t c0 = q[k].slot >> 2, c1 = (uint32_t)step, c2 = 0u;
@@ -151,14 +163,22 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
             }
             // advance_boundary (particles.hpp:376-406), no deposit
             bool dead = false;
-            if (!(x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1)) {
+            const double xd = HIPOS ? pos64(x, xl) : (double)x, zd = HIPOS ? pos64(z, zl) : (double)z;
+            const bool inside = HIPOS ? xd >= a.xmin && xd <= a.xper && zd >= a.zmin && zd <= a.zper
+                                      : x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1;
+            if (!inside) {
                 if (x != x || z != z) { dead = true; nan_any = true; }
                 else if (!a.periodic) dead = true;
                 else {
-                    double xm = fmod((double)x, a.xper), zm = fmod((double)z, a.zper);
+                    double xm = fmod(xd, a.xper), zm = fmod(zd, a.zper);
                     if (xm < 0) xm += a.xper;
                     if (zm < 0) zm += a.zper;
-                    x = (float)xm; z = (float)zm;
+                    if constexpr (HIPOS) {
+                        pos_split(xm, x, xl);
+                        pos_split(zm, z, zl);
+                    } else {
+                        x = (float)xm; z = (float)zm;
+                    }
                 }
             }
             if (!dead && a.checkfree) {
@@ -167,12 +187,13 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
                     // the cell of the specialised single-run pass (the box test gave x, z >= 0)
                     int ci, cj;
                     float u, ou;
-                    cell_f(x, 0.f, a.idxh, a.idxl, ci, u, ou);
-                    cell_f(z, 0.f, a.idzh, a.idzl, cj, u, ou);
+                    cell_f(x, xl, a.idxh, a.idxl, ci, u, ou);
+                    cell_f(z, zl, a.idzh, a.idzl, cj, u, ou);
                     if ((unsigned)ci <= (unsigned)(a.nx - 1) && (unsigned)cj <= (unsigned)(a.nz - 1))
                         fr = a.cellfree[ci * a.nz + cj] != 0;
                 } else {
-                    const double X = ((double)x - a.xmin) * a.idx, Z = ((double)z - a.zmin) * a.idz;
+                    const double X = ((HIPOS ? pos64(x, xl) : (double)x) - a.xmin) * a.idx;
+                    const double Z = ((HIPOS ? pos64(z, zl) : (double)z) - a.zmin) * a.idz;
                     if (X >= 0.0 && X < (double)a.nx && Z >= 0.0 && Z < (double)a.nz)
                         fr = a.cellfree[(int)X * a.nz + (int)Z] != 0;
                 }
@@ -180,6 +201,7 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
             }
             if (dead) { q[k].live = false; n_removed++; }
             q[k].x = x; q[k].z = z; q[k].vx = vx; q[k].vy = vy; q[k].vz = vz;
+            q[k].xl = xl; q[k].zl = zl;
         }
     }
 #pragma unroll
@@ -188,6 +210,7 @@ __global__ void __launch_bounds__(256) k_batch2d(BatchArgs a, uint64_t niter0, i
         if (p >= a.n || is_dead(a.x[p])) continue;
         if (!q[k].live) { a.x[p] = dead_value(); continue; }
         a.x[p] = q[k].x; a.z[p] = q[k].z; a.vx[p] = q[k].vx; a.vy[p] = q[k].vy; a.vz[p] = q[k].vz;
+        if (HIPOS) { a.xl[p] = q[k].xl; a.zl[p] = q[k].zl; }
     }
     int sum = n_removed;
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
@@ -281,6 +304,7 @@ static void batch_args(const m2d_batch *b, const m2d_particles2d &p, BatchArgs &
     const m2d_fields2d_params &P = f->p;
     const m2d_species2d *s = &b->s;
     a.x = p.x; a.z = p.z; a.vx = p.vx; a.vy = p.vy; a.vz = p.vz; a.n = p.n;
+    a.xl = p.xl; a.zl = p.zl;
     a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree;
     a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
     split_f32(f->idx, a.idxh, a.idxl);
@@ -302,6 +326,7 @@ static void batch_args(const m2d_batch *b, const m2d_particles2d &p, BatchArgs &
     a.rotate = tx != 0.0 || ty != 0.0 || tz != 0.0;
     a.qmdt2 = (float)(s->charge / s->mass * s->dt / 2.0);
     a.dt = (float)s->dt;
+    a.dtl = (float)(s->dt - (double)a.dt);
     a.U0 = P.rf_U0;
     a.seed = b->ctx->seed;
     a.runs = b->d_runs;
@@ -327,6 +352,25 @@ static int ensure_cos(m2d_batch *b, uint64_t niter0, int steps)
     return M2D_OK;
 }
 
+template <bool HIPOS>
+static void launch_batch(const BatchArgs &a, const m2d_batch_shape &sh, unsigned grid, bool fastg, uint64_t niter0,
+                         int nsteps, const double *cosv, hipStream_t st)
+{
+    if (fastg) {
+        switch (sh.ppt) {
+        case 1: k_batch2d<1, true, HIPOS><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        case 2: k_batch2d<2, true, HIPOS><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        default: k_batch2d<4, true, HIPOS><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        }
+    } else {
+        switch (sh.ppt) {
+        case 1: k_batch2d<1, false, HIPOS><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        case 2: k_batch2d<2, false, HIPOS><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        default: k_batch2d<4, false, HIPOS><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
+        }
+    }
+}
+
 static int launch_segment(m2d_batch *b, const m2d_particles2d &p, const m2d_batch_shape &sh, uint64_t niter0,
                           int nsteps, const double *cosv)
 {
@@ -336,19 +380,8 @@ static int launch_segment(m2d_batch *b, const m2d_particles2d &p, const m2d_batc
     const unsigned grid = (unsigned)std::max<int64_t>(1, (threads + sh.block - 1) / sh.block);
     hipStream_t st = b->ctx->stream;
     const bool fastg = a.checkfree && !a.rotate && !a.periodic && !a.econst && a.xmin == 0.0 && a.zmin == 0.0;
-    if (fastg) {
-        switch (sh.ppt) {
-        case 1: k_batch2d<1, true><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-        case 2: k_batch2d<2, true><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-        default: k_batch2d<4, true><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-        }
-    } else {
-        switch (sh.ppt) {
-        case 1: k_batch2d<1, false><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-        case 2: k_batch2d<2, false><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-        default: k_batch2d<4, false><<<grid, sh.block, 0, st>>>(a, niter0, nsteps, cosv); break;
-        }
-    }
+    if (a.xl) launch_batch<true>(a, sh, grid, fastg, niter0, nsteps, cosv, st);
+    else launch_batch<false>(a, sh, grid, fastg, niter0, nsteps, cosv, st);
     M2D_HIP(hipGetLastError());
     return M2D_OK;
 }
@@ -387,6 +420,8 @@ int m2d_batch_create(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, cons
         M2D_CHECK(d->offset[r + 1] >= d->offset[r], M2D_ERR_BAD_ARG, "m2d_batch_create: offsets decrease at %d", r);
     M2D_CHECK(p->n < (1ll << 31) && (p->n == 0 || (p->x && p->z && p->vx && p->vy && p->vz)), M2D_ERR_BAD_ARG,
               "m2d_batch_create: particle store");
+    M2D_CHECK((p->xl == nullptr) == (p->zl == nullptr), M2D_ERR_BAD_ARG,
+              "m2d_batch_create: position remainders xl and zl must both be set or both be NULL");
     m2d_batch *b = new (std::nothrow) m2d_batch();
     M2D_CHECK(b, M2D_ERR_NO_MEMORY, "m2d_batch_create: host allocation");
     b->ctx = ctx;
@@ -454,11 +489,14 @@ int m2d_batch_tune(m2d_batch *b, uint64_t niter0, int32_t probe_steps, int32_t s
     hipStream_t st = ctx->stream;
     const size_t np = (size_t)b->p.n;
     // candidates run on a copy of the store: the batch's own state is not touched
-    M2D_TRY(ensure(ctx, ctx->tmp, sizeof(float) * 5 * std::max<size_t>(np, 1) + 256));
+    const int nc = b->p.xl ? 7 : 5;  // state components (with position remainders: 7)
+    const size_t stride = (std::max<size_t>(np, 1) + 3) / 4 * 4;  // 16-byte aligned arrays
+    M2D_TRY(ensure(ctx, ctx->tmp, sizeof(float) * (size_t)nc * stride + 256));
     float *t = (float *)ctx->tmp.ptr;
-    const m2d_particles2d copy{t, t + np, t + 2 * np, t + 3 * np, t + 4 * np, b->p.n};
-    float *const src[5] = {b->p.x, b->p.z, b->p.vx, b->p.vy, b->p.vz};
-    float *const dst[5] = {copy.x, copy.z, copy.vx, copy.vy, copy.vz};
+    m2d_particles2d copy{t, t + stride, t + 2 * stride, t + 3 * stride, t + 4 * stride, b->p.n};
+    if (b->p.xl) { copy.xl = t + 5 * stride; copy.zl = t + 6 * stride; }
+    float *const src[7] = {b->p.x, b->p.z, b->p.vx, b->p.vy, b->p.vz, b->p.xl, b->p.zl};
+    float *const dst[7] = {copy.x, copy.z, copy.vx, copy.vy, copy.vz, copy.xl, copy.zl};
     M2D_TRY(ensure_cos(b, niter0, probe_steps));
     hipEvent_t e0, e1;
     M2D_HIP(hipEventCreate(&e0));
@@ -468,7 +506,7 @@ int m2d_batch_tune(m2d_batch *b, uint64_t niter0, int32_t probe_steps, int32_t s
     for (int block : {64, 128, 256})
         for (int ppt : {1, 2, 4}) {
             const m2d_batch_shape sh{block, ppt, seg_steps};
-            for (int c = 0; c < 5; c++)
+            for (int c = 0; c < nc; c++)
                 M2D_HIP(hipMemcpyAsync(dst[c], src[c], sizeof(float) * np, hipMemcpyDeviceToDevice, st));
             M2D_HIP(hipEventRecord(e0, st));
             M2D_TRY(launch_segment(b, copy, sh, niter0, probe_steps, b->cosv));

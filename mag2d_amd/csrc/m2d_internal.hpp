@@ -177,8 +177,9 @@ __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const d
                          int first);
 __global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n);
 // Species::accumulate into a fixed-point grid (m2d_particles2d.hip)
-__global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, int64_t n, int nx, int nz,
-                                                         double idx, double idz, double xmin, double zmin,
+__global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, const float *xls,
+                                                         const float *zls, int64_t n, int nx, int nz, double idx,
+                                                         double idz, double xmin, double zmin,
                                                          unsigned long long *acc, unsigned long long *counters);
 // device address of the no-allocation entry point's static scratch on the
 // current device, resolved once per device (m2d_noalloc.hip)
@@ -262,6 +263,32 @@ struct Rng {
     // distribution by the ziggurat
     __device__ __forceinline__ double exponential() { return -log(uni()); }
 };
+
+// ---------------------------------------------------------------- long-time positions
+// A store with position remainders (m2d_particles2d::xl, zl) holds each
+// position as an FP32 pair whose exact FP64 sum is the position.
+__device__ __forceinline__ double pos64(float hi, float lo) { return (double)hi + (double)lo; }
+__device__ __forceinline__ void pos_split(double v, float &hi, float &lo)
+{
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
+}
+// hi + lo += v dt with dt = dth + dtl an FP32 pair: the product v dth is
+// d plus its rounding error (an fma), plus v dtl; Knuth's two-sum of hi and
+// d is exact, and its error joins the product's error and lo in the new
+// remainder; a fast two-sum renormalises the pair.  The pair then carries
+// the position to about 2^-48 of its size, where FP32 alone keeps 2^-24.
+__device__ __forceinline__ void drift_pair(float &hi, float &lo, float v, float dth, float dtl)
+{
+    const float d = v * dth;
+    const float dl = fmaf(v, dth, -d) + v * dtl;
+    const float s = hi + d;
+    const float bp = s - hi;
+    const float e = (hi - (s - bp)) + (d - bp);
+    const float l = lo + (e + dl);
+    hi = s + l;
+    lo = l - (hi - s);
+}
 
 // wave-aggregated counter increment (one atomic per wavefront)
 __device__ __forceinline__ void wave_count(unsigned long long *ctr, bool pred)

@@ -18,6 +18,7 @@ constexpr double kMaxEventRate = 256.0; // dt / lifetime accepted by the mover
 
 struct MulticollArgs {
     float *x, *z, *vx, *vy, *vz;
+    float *xl, *zl;          // position remainders (long-time positions), or null
     double *ttd;
     int64_t n;
     double fx, fz, qm;       // Fields::E(0, 0) (un-negated constant field) and charge / mass
@@ -46,7 +47,8 @@ __global__ void __launch_bounds__(kBlock) k_multicoll2d(MulticollArgs a)
     int events = 0;
     bool capped = false;
     if (p < a.n && !is_dead(a.x[p])) {
-        double x = a.x[p], z = a.z[p], vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+        double x = a.xl ? pos64(a.x[p], a.xl[p]) : a.x[p], z = a.zl ? pos64(a.z[p], a.zl[p]) : a.z[p];
+        double vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
         double ttd = a.ttd[p];
         double local = 0.0;
         Rng rg;
@@ -67,7 +69,12 @@ __global__ void __launch_bounds__(kBlock) k_multicoll2d(MulticollArgs a)
         const double rest = a.dt - local;
         fly(a, rest, x, z, vx, vz);
         ttd -= rest;
-        a.x[p] = (float)x; a.z[p] = (float)z;
+        if (a.xl) {
+            pos_split(x, a.x[p], a.xl[p]);
+            pos_split(z, a.z[p], a.zl[p]);
+        } else {
+            a.x[p] = (float)x; a.z[p] = (float)z;
+        }
         a.vx[p] = (float)vx; a.vy[p] = (float)vy; a.vz[p] = (float)vz;
         a.ttd[p] = ttd;
     }
@@ -121,6 +128,7 @@ int m2d_advance_multicoll_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d 
     if (p->n > 0) {
         MulticollArgs a;
         a.x = p->x; a.z = p->z; a.vx = p->vx; a.vy = p->vy; a.vz = p->vz;
+        a.xl = p->xl; a.zl = p->zl;
         a.ttd = time_to_death;
         a.n = p->n;
         // Fields::E(0, 0): EMPTY and not selfconsistent (fields.hpp:126-131)

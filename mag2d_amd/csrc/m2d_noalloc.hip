@@ -63,9 +63,9 @@ int m2d_accumulate_2d_noalloc(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_parti
     // (a fill kernel, which graph capture records as a kernel node)
     M2D_TRY(zero_words(ctx, acc, n));
     if (p->n > 0) {
-        k_accumulate2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(p->x, p->z, p->n, g->nx, g->nz, 1.0 / g->dx,
-                                                                     1.0 / g->dz, g->xmin, g->zmin, acc,
-                                                                     ctx->counters);
+        k_accumulate2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(p->x, p->z, p->xl, p->zl, p->n, g->nx, g->nz,
+                                                                     1.0 / g->dx, 1.0 / g->dz, g->xmin, g->zmin,
+                                                                     acc, ctx->counters);
         M2D_HIP(hipGetLastError());
     }
     k_flush<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(acc, rho, charge, n);

@@ -17,6 +17,7 @@ namespace m2d {
 struct PushArgs {
     float *x, *z, *vx, *vy, *vz;
     int64_t n;
+    float *xl, *zl;                // position remainders (long-time positions), or null
     const float2 *gx, *gz;
     const float *phx, *phz;        // gx, gz with this step's phase applied (PHT pass)
     const uint8_t *cellfree;
@@ -31,6 +32,7 @@ struct PushArgs {
     float qmdt2;                   // q dt / (2 m)
     float tx, ty, tz, sx, sy, sz;  // Boris rotation vectors
     float dt;
+    float dtl;                     // dt - (double)dt as FP32: the drift of long-time positions
     float phase;                   // E = -(grad u + phase grad uRF)
     float ecx, ecz;                // constant field (EMPTY geometry, fields.hpp:126-131)
     int econst;
@@ -175,12 +177,13 @@ constexpr int kGather = 2;                  // particles whose gathers are in fl
 constexpr int kChunk = 2048;                // slots per block of the untiled pass
 constexpr int kPPT = kChunk / kBlock;       // slots per thread of the helper kernels
 
-// component c of the state (x, z, vx, vy, vz); c is a constant after
-// unrolling, so the pointer stays in scalar registers (an array of the five
-// pointers would live in scratch memory)
+// component c of the state (x, z, vx, vy, vz, and the position remainders
+// xl, zl of long-time positions); c is a constant after unrolling, so the
+// pointer stays in scalar registers (an array of the pointers would live in
+// scratch memory)
 __device__ __forceinline__ float *comp(const PushArgs &a, int c)
 {
-    return c == 0 ? a.x : c == 1 ? a.z : c == 2 ? a.vx : c == 3 ? a.vy : a.vz;
+    return c == 0 ? a.x : c == 1 ? a.z : c == 2 ? a.vx : c == 3 ? a.vy : c == 4 ? a.vz : c == 5 ? a.xl : a.zl;
 }
 
 // running sum of one thread's deposits while consecutive particles share a cell
@@ -239,17 +242,21 @@ __device__ __forceinline__ void wave_deposit(DepRun &d, unsigned long long *acc,
 // VGPRs) when the gather reads the one-float phase-applied tables, five (96
 // VGPRs) with the two-component tables; the untiled forms carry the deposit
 // runs and the FP64 generic forms their wider arithmetic, and keep four (128
-// VGPRs), three for the untiled deposit with a gridded B.  Each is the most
-// that builds without spilling to scratch memory, which costs far more than
-// the occupancy gains.
-template <bool DEPOSIT, bool TILED, bool FAST, bool BGRID, bool PHT>
+// VGPRs), three for the untiled deposit with a gridded B; with long-time
+// positions (two more state components) the forms keep four, three for the
+// untiled generic deposit.  Each is the most that builds without spilling to
+// scratch memory, which costs far more than the occupancy gains.
+template <bool DEPOSIT, bool TILED, bool FAST, bool BGRID, bool PHT, bool HIPOS>
 constexpr int pass_waves()
 {
+    if (HIPOS) return !TILED && DEPOSIT && !FAST ? 3 : 4;  // two more state components
     return TILED && FAST ? (PHT ? 6 : 5) : (BGRID && DEPOSIT && !TILED ? 3 : 4);
 }
 
-template <bool DEPOSIT, bool MCC, bool TILED, bool FAST, bool BGRID = false, bool PHT = false>
-__global__ void __launch_bounds__(kPass) __attribute__((amdgpu_waves_per_eu(pass_waves<DEPOSIT, TILED, FAST, BGRID, PHT>())))
+// HIPOS: long-time positions, the FP32 pairs (x, xl), (z, zl).
+template <bool DEPOSIT, bool MCC, bool TILED, bool FAST, bool BGRID = false, bool PHT = false, bool HIPOS = false>
+__global__ void __launch_bounds__(kPass)
+__attribute__((amdgpu_waves_per_eu(pass_waves<DEPOSIT, TILED, FAST, BGRID, PHT, HIPOS>())))
 k_advance2d(PushArgs a)
 {
     __shared__ unsigned long long win[TILED ? kWin * kWin : 1];
@@ -289,10 +296,11 @@ k_advance2d(PushArgs a)
         // other blocks: they are neither computed nor written
         const int64_t pb = co + 4 * tid;
         const bool full = pb >= lo && pb + 4 <= hi;
-        float v[5][4];
+        constexpr int NC = HIPOS ? 7 : 5;  // state components held
+        float v[7][4];
         if (full) {
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
+            for (int c = 0; c < NC; c++) {
                 const float4 t = *reinterpret_cast<const float4 *>(comp(a, c) + pb);
                 v[c][0] = t.x; v[c][1] = t.y; v[c][2] = t.z; v[c][3] = t.w;
             }
@@ -301,11 +309,11 @@ k_advance2d(PushArgs a)
             for (int k = 0; k < 4; k++) {
                 if (pb + k >= lo && pb + k < hi) {
 #pragma unroll
-                    for (int c = 0; c < 5; c++) v[c][k] = comp(a, c)[pb + k];
+                    for (int c = 0; c < NC; c++) v[c][k] = comp(a, c)[pb + k];
                 } else {
                     v[0][k] = dead_value();
 #pragma unroll
-                    for (int c = 1; c < 5; c++) v[c][k] = 0.f;
+                    for (int c = 1; c < NC; c++) v[c][k] = 0.f;
                 }
             }
         }
@@ -332,10 +340,12 @@ k_advance2d(PushArgs a)
                     // a dead slot gathers at a clamped point and is discarded
                     bool b;
                     if constexpr (FAST)
-                        b = gather_E_f<PHT>(a, v[0][h], 0.f, v[1][h], 0.f, a.phase, gex[h], gez[h]);
+                        b = gather_E_f<PHT>(a, v[0][h], HIPOS ? v[5][h] : 0.f, v[1][h], HIPOS ? v[6][h] : 0.f,
+                                            a.phase, gex[h], gez[h]);
                     else
-                        b = gather_E_clamped<false, PHT>(a, (double)v[0][h], (double)v[1][h], a.phase, gex[h],
-                                                         gez[h]);
+                        b = gather_E_clamped<false, PHT>(a, HIPOS ? pos64(v[0][h], v[5][h]) : (double)v[0][h],
+                                                         HIPOS ? pos64(v[1][h], v[6][h]) : (double)v[1][h], a.phase,
+                                                         gex[h], gez[h]);
                     gbad |= b && !is_dead(v[0][h]);
                 }
             }
@@ -345,6 +355,7 @@ k_advance2d(PushArgs a)
                 float x = v[0][q];
                 if (is_dead(x)) continue;
                 float z = v[1][q], vx = v[2][q], vy = v[3][q], vz = v[4][q];
+                float xl = HIPOS ? v[5][q] : 0.f, zl = HIPOS ? v[6][q] : 0.f;
 
                 if (FAST || a.push) {
                     // ---- advance_boris (particles.cpp:945-988)
@@ -364,8 +375,13 @@ k_advance2d(PushArgs a)
                     }
                     vx = fmaf(ex, a.qmdt2, vx);
                     vz = fmaf(ez, a.qmdt2, vz);
-                    x = fmaf(vx, a.dt, x);
-                    z = fmaf(vz, a.dt, z);
+                    if constexpr (HIPOS) {
+                        drift_pair(x, xl, vx, a.dt, a.dtl);
+                        drift_pair(z, zl, vz, a.dt, a.dtl);
+                    } else {
+                        x = fmaf(vx, a.dt, x);
+                        z = fmaf(vz, a.dt, z);
+                    }
 
                     // ---- MCC trial (particles.cpp:990-993).  A hit only changes the
                     // velocity after the drift, so it cannot change the boundary test
@@ -383,19 +399,32 @@ k_advance2d(PushArgs a)
                 // ---- advance_boundary (particles.hpp:376-406)
                 bool dead = false;
                 if (FAST) {
-                    // a NaN fails every comparison, so it lands outside
-                    dead = !(x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1);
+                    // a NaN fails every comparison, so it lands outside; a
+                    // long-time position is tested in FP64 against the box
+                    if constexpr (HIPOS) {
+                        const double xd = pos64(x, xl), zd = pos64(z, zl);
+                        dead = !(xd >= 0.0 && xd <= a.xper && zd >= 0.0 && zd <= a.zper);
+                    } else {
+                        dead = !(x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1);
+                    }
                     nan_any |= dead && (x != x || z != z);
                 } else if (a.boundary) {
-                    bool inside = x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1;
+                    const double xd = HIPOS ? pos64(x, xl) : (double)x, zd = HIPOS ? pos64(z, zl) : (double)z;
+                    bool inside = HIPOS ? xd >= a.xmin && xd <= a.xper && zd >= a.zmin && zd <= a.zper
+                                        : x >= a.fx0 && x <= a.fx1 && z >= a.fz0 && z <= a.fz1;
                     if (!inside) {
                         if (x != x || z != z) { dead = true; nan_any = true; }
                         else if (!a.periodic) dead = true;
                         else {
-                            double xm = fmod((double)x, a.xper), zm = fmod((double)z, a.zper);
+                            double xm = fmod(xd, a.xper), zm = fmod(zd, a.zper);
                             if (xm < 0) xm += a.xper;
                             if (zm < 0) zm += a.zper;
-                            x = (float)xm; z = (float)zm;
+                            if constexpr (HIPOS) {
+                                pos_split(xm, x, xl);
+                                pos_split(zm, z, zl);
+                            } else {
+                                x = (float)xm; z = (float)zm;
+                            }
                         }
                     }
                 }
@@ -405,11 +434,11 @@ k_advance2d(PushArgs a)
                 float fu = 0.f, fou = 0.f, fv = 0.f, fov = 0.f;
                 double X = 0.0, Z = 0.0;
                 if constexpr (FAST) {
-                    cell_f(x, 0.f, a.idxh, a.idxl, fci, fu, fou);
-                    cell_f(z, 0.f, a.idzh, a.idzl, fcj, fv, fov);
+                    cell_f(x, xl, a.idxh, a.idxl, fci, fu, fou);
+                    cell_f(z, zl, a.idzh, a.idzl, fcj, fv, fov);
                 } else {
-                    X = ((double)x - a.xmin) * a.idx;
-                    Z = ((double)z - a.zmin) * a.idz;
+                    X = ((HIPOS ? pos64(x, xl) : (double)x) - a.xmin) * a.idx;
+                    Z = ((HIPOS ? pos64(z, zl) : (double)z) - a.zmin) * a.idz;
                 }
                 if (!dead && (FAST || (a.boundary && a.checkfree))) {
                     bool free_ = false;
@@ -460,9 +489,10 @@ k_advance2d(PushArgs a)
                     }
                 }
                 v[0][q] = x; v[1][q] = z; v[2][q] = vx; v[3][q] = vy; v[4][q] = vz;
+                if constexpr (HIPOS) { v[5][q] = xl; v[6][q] = zl; }
             }
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
+            for (int c = 0; c < NC; c++) {
                 float t[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) t[q] = v[c][(q + kGather) & 3];
@@ -479,14 +509,14 @@ k_advance2d(PushArgs a)
         // ---- store
         if (full) {
 #pragma unroll
-            for (int c = 0; c < 5; c++)
+            for (int c = 0; c < NC; c++)
                 *reinterpret_cast<float4 *>(comp(a, c) + pb) = make_float4(v[c][0], v[c][1], v[c][2], v[c][3]);
         } else {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (pb + k >= lo && pb + k < hi) {
 #pragma unroll
-                    for (int c = 0; c < 5; c++) comp(a, c)[pb + k] = v[c][k];
+                    for (int c = 0; c < NC; c++) comp(a, c)[pb + k] = v[c][k];
                 }
             }
         }
@@ -594,7 +624,8 @@ __global__ void __launch_bounds__(kBlock) k_advance_init2d(PushArgs a)
     float z = a.z[p];
     float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
     float ex, ez;
-    gather_E(a, (double)x, (double)z, a.phase, ex, ez);
+    if (a.xl) gather_E(a, pos64(x, a.xl[p]), pos64(z, a.zl[p]), a.phase, ex, ez);
+    else gather_E(a, (double)x, (double)z, a.phase, ex, ez);
     float tx = a.tx, ty = a.ty, tz = a.tz, sx = a.sx, sy = a.sy, sz = a.sz;
     if (a.bt && local_rotation(a, x, z, tx, ty, tz, sx, sy, sz)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
     float vpr = vx - vy * tz + vz * ty;
@@ -645,9 +676,9 @@ __global__ void k_fields_E(PushArgs a, const double *xs, const double *zs, int64
 }
 
 // Species::accumulate (particles.hpp:358-367) into the fixed-point grid
-__global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, int64_t n,
-                                                         int nx, int nz, double idx, double idz,
-                                                         double xmin, double zmin,
+__global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, const float *xls,
+                                                         const float *zls, int64_t n, int nx, int nz, double idx,
+                                                         double idz, double xmin, double zmin,
                                                          unsigned long long *acc,
                                                          unsigned long long *counters)
 {
@@ -655,6 +686,7 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
     if (p >= n) return;
     float x = xs[p];
     if (is_dead(x)) return;
+    const float xl = xls ? xls[p] : 0.f, zl = zls ? zls[p] : 0.f;
     int i, j;
     uint32_t w[4];
     bool in_grid;
@@ -665,15 +697,15 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
         float sxh, sxl, szh, szl, u, ou, v, ov;
         split_f32(idx, sxh, sxl);
         split_f32(idz, szh, szl);
-        in_grid = x >= 0.f && z >= 0.f;
+        in_grid = pos64(x, xl) >= 0.0 && pos64(z, zl) >= 0.0;
         if (in_grid) {
-            cell_f(x, 0.f, sxh, sxl, i, u, ou);
-            cell_f(z, 0.f, szh, szl, j, v, ov);
+            cell_f(x, xl, sxh, sxl, i, u, ou);
+            cell_f(z, zl, szh, szl, j, v, ov);
             in_grid = (unsigned)i <= (unsigned)(nx - 2) && (unsigned)j <= (unsigned)(nz - 2);
             if (in_grid) cic_from_offsets(u, ou, v, ov, w);
         }
     } else {
-        in_grid = cic_weights(((double)x - xmin) * idx, ((double)zs[p] - zmin) * idz, nx, nz, i, j, w);
+        in_grid = cic_weights((pos64(x, xl) - xmin) * idx, (pos64(zs[p], zl) - zmin) * idz, nx, nz, i, j, w);
     }
     if (in_grid) {
         const long long ww[4] = {w[0], w[1], w[2], w[3]};
@@ -893,6 +925,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_scatter(m2d_particles2d src,
             unsigned long long d = run + woff + before;
             dst.x[d] = src.x[p]; dst.z[d] = src.z[p];
             dst.vx[d] = src.vx[p]; dst.vy[d] = src.vy[p]; dst.vz[d] = src.vz[p];
+            if (src.xl) { dst.xl[d] = src.xl[p]; dst.zl[d] = src.zl[p]; }
         }
         run += tot;
         __syncthreads();
@@ -906,6 +939,19 @@ __global__ void k_fill_dead(float *x, int64_t n)
 }
 
 // ------------------------------------------------------------------ sampler
+// a sampled FP64 position: rounded to FP32, with its remainder in a store
+// that keeps long-time positions
+__device__ __forceinline__ void store_pos(const m2d_particles2d &P, int64_t p, double x, double z)
+{
+    if (P.xl) {
+        pos_split(x, P.x[p], P.xl[p]);
+        pos_split(z, P.z[p], P.zl[p]);
+    } else {
+        P.x[p] = (float)x;
+        P.z[p] = (float)z;
+    }
+}
+
 // add_particles_on_disk (particles.cpp:728-749), one Philox stream per slot
 __global__ void __launch_bounds__(kBlock) k_disk2d(m2d_particles2d P, double cx, double cz,
                                                    double R, double vth, double bx0, double bx1,
@@ -928,7 +974,7 @@ __global__ void __launch_bounds__(kBlock) k_disk2d(m2d_particles2d P, double cx,
         return;
     }
     double a = r.normal() * vth, b = r.normal() * vth, c = r.normal() * vth;
-    P.x[p] = (float)x; P.z[p] = (float)y;
+    store_pos(P, p, x, y);
     P.vx[p] = (float)a; P.vz[p] = (float)b; P.vy[p] = (float)c;
 }
 
@@ -974,7 +1020,7 @@ __global__ void __launch_bounds__(kBlock) k_bessel2d(m2d_particles2d P, double c
     }
     // rndv(vx, vz, vy) (particles.hpp:188-193)
     double a = r.normal() * vth, b = r.normal() * vth, c = r.normal() * vth;
-    P.x[p] = (float)x; P.z[p] = (float)y;
+    store_pos(P, p, x, y);
     P.vx[p] = (float)a; P.vz[p] = (float)b; P.vy[p] = (float)c;
 }
 
@@ -992,7 +1038,7 @@ __global__ void __launch_bounds__(kBlock) k_everywhere2d(m2d_particles2d P, doub
     (void)r.uni();  // y: the reference draws it; the Cartesian store has no y
     const double z = (z1 - z0) * r.uni() + z0;
     const double a = r.normal() * vth, b = r.normal() * vth, c = r.normal() * vth;
-    P.x[p] = (float)x; P.z[p] = (float)z;
+    store_pos(P, p, x, z);
     P.vx[p] = (float)a; P.vy[p] = (float)b; P.vz[p] = (float)c;
     if (ttd) ttd[p] = lifetime * r.exponential();
 }
@@ -1008,6 +1054,10 @@ int check_parts(const m2d_particles2d *p)
     uintptr_t al = (uintptr_t)p->x | (uintptr_t)p->z | (uintptr_t)p->vx | (uintptr_t)p->vy |
                    (uintptr_t)p->vz;
     M2D_CHECK((al & 15) == 0, M2D_ERR_BAD_ARG, "particle arrays must be 16-byte aligned");
+    M2D_CHECK((p->xl == nullptr) == (p->zl == nullptr), M2D_ERR_BAD_ARG,
+              "position remainders xl and zl must both be set or both be NULL");
+    M2D_CHECK((((uintptr_t)p->xl | (uintptr_t)p->zl) & 15) == 0, M2D_ERR_BAD_ARG,
+              "position remainder arrays must be 16-byte aligned");
     return M2D_OK;
 }
 
@@ -1056,6 +1106,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
 {
     const m2d_fields2d_params &P = f->p;
     a.x = p->x; a.z = p->z; a.vx = p->vx; a.vy = p->vy; a.vz = p->vz; a.n = p->n;
+    a.xl = p->xl; a.zl = p->zl;
     a.gx = f->gx; a.gz = f->gz; a.cellfree = f->cellfree; a.winfree = f->winfree;
     a.phx = f->phx; a.phz = f->phz;
     a.nx = f->nx; a.nz = f->nz; a.idx = f->idx; a.idz = f->idz;
@@ -1075,6 +1126,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.push = 1;
     a.qmdt2 = (float)(kick_scale * s->charge / s->mass * s->dt / 2.0);
     a.dt = (float)s->dt;
+    a.dtl = (float)(s->dt - (double)a.dt);
     // Fields::E (fields.hpp:124-150); phase hoisted: t = niter dt for every particle
     a.econst = P.geometry_empty && !P.selfconsistent;
     a.ecx = 0.f;
@@ -1105,6 +1157,46 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
         a.tx = a.ty = a.tz = a.sx = a.sy = a.sz = 0.f;
         a.rotate = 1;
     }
+}
+
+// The compiled form of the pass for one call's configuration: the store is
+// sorted (tiled), the configuration is the specialised one (fast), the
+// phase-applied tables hold this step's phase (pht), B comes from a B grid,
+// the deposit and the MCC trial are on.  Long-time positions (HIPOS) have no
+// generic tiled form: the caller runs those untiled.
+template <bool HIPOS>
+static void (*pick_pass(bool tiled, bool fast, bool pht, bool bgrid, bool deposit, bool mcc))(PushArgs)
+{
+    if (tiled) {
+        if (fast && pht)
+            return mcc ? k_advance2d<true, true, true, true, false, true, HIPOS>
+                       : k_advance2d<true, false, true, true, false, true, HIPOS>;
+        if (fast)
+            return mcc ? k_advance2d<true, true, true, true, false, false, HIPOS>
+                       : k_advance2d<true, false, true, true, false, false, HIPOS>;
+        if constexpr (HIPOS) {
+            return nullptr;
+        } else {
+            if (bgrid) return mcc ? k_advance2d<true, true, true, false, true> : k_advance2d<true, false, true, false, true>;
+            return mcc ? k_advance2d<true, true, true, false> : k_advance2d<true, false, true, false>;
+        }
+    }
+    if (fast) {
+        if (deposit) return mcc ? k_advance2d<true, true, false, true, false, false, HIPOS>
+                                : k_advance2d<true, false, false, true, false, false, HIPOS>;
+        return mcc ? k_advance2d<false, true, false, true, false, false, HIPOS>
+                   : k_advance2d<false, false, false, true, false, false, HIPOS>;
+    }
+    if (bgrid) {
+        if (deposit) return mcc ? k_advance2d<true, true, false, false, true, false, HIPOS>
+                                : k_advance2d<true, false, false, false, true, false, HIPOS>;
+        return mcc ? k_advance2d<false, true, false, false, true, false, HIPOS>
+                   : k_advance2d<false, false, false, false, true, false, HIPOS>;
+    }
+    if (deposit) return mcc ? k_advance2d<true, true, false, false, false, false, HIPOS>
+                            : k_advance2d<true, false, false, false, false, false, HIPOS>;
+    return mcc ? k_advance2d<false, true, false, false, false, false, HIPOS>
+               : k_advance2d<false, false, false, false, false, false, HIPOS>;
 }
 
 }  // namespace m2d
@@ -1150,6 +1242,12 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     // per-chunk LDS windows
     const m2d_tiling *T = deposit ? find_tiling(ctx, p->x, p->n) : nullptr;
     if (T && (T->nty != 1 || T->ntx != (f->nx - 2) / kTile + 1 || T->ntz != (f->nz - 2) / kTile + 1)) T = nullptr;
+    // the common configuration runs the compile-time specialised pass
+    const bool fast = a.push && a.boundary && a.checkfree && !a.rotate && !a.periodic && !a.econst &&
+                      a.xmin == 0.0 && a.zmin == 0.0;
+    // long-time positions: the generic forms are built untiled only
+    const bool hipos = p->xl != nullptr;
+    if (hipos && !fast) T = nullptr;
     if (deposit) {
         M2D_TRY(ensure_acc(ctx, f->n));
         a.acc = (unsigned long long *)ctx->acc.ptr;
@@ -1161,54 +1259,19 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     }
     bool fast_tiled = false;  // this call ran the specialised tiled pass
     if (p->n > 0) {
-        // the common configuration runs the compile-time specialised pass
-        const bool fast = a.push && a.boundary && a.checkfree && !a.rotate && !a.periodic && !a.econst &&
-                          a.xmin == 0.0 && a.zmin == 0.0;
-        void (*kern)(PushArgs) = nullptr;
-        int g = 0;
         const bool bgrid = f->bt != nullptr;  // B per particle (generic pass)
-        if (T) {
-            g = (int)xcd_run_grid((unsigned)T->max_chunks);
-            // the flush after the previous pass left the tables for this phase
-            const bool pht = f->ph_valid && f->ph_phase == a.phase;
-            fast_tiled = fast;
-            if (fast && pht)
-                kern = use_mcc ? k_advance2d<true, true, true, true, false, true>
-                               : k_advance2d<true, false, true, true, false, true>;
-            else if (fast) kern = use_mcc ? k_advance2d<true, true, true, true> : k_advance2d<true, false, true, true>;
-            else if (bgrid)
-                kern = use_mcc ? k_advance2d<true, true, true, false, true> : k_advance2d<true, false, true, false, true>;
-            else kern = use_mcc ? k_advance2d<true, true, true, false> : k_advance2d<true, false, true, false>;
-        } else {
-            g = (int)((p->n + kChunk - 1) / kChunk);
-            if (fast) {
-                if (deposit && use_mcc) kern = k_advance2d<true, true, false, true>;
-                else if (deposit) kern = k_advance2d<true, false, false, true>;
-                else if (use_mcc) kern = k_advance2d<false, true, false, true>;
-                else kern = k_advance2d<false, false, false, true>;
-            } else if (bgrid) {
-                if (deposit && use_mcc) kern = k_advance2d<true, true, false, false, true>;
-                else if (deposit) kern = k_advance2d<true, false, false, false, true>;
-                else if (use_mcc) kern = k_advance2d<false, true, false, false, true>;
-                else kern = k_advance2d<false, false, false, false, true>;
-            } else {
-                if (deposit && use_mcc) kern = k_advance2d<true, true, false, false>;
-                else if (deposit) kern = k_advance2d<true, false, false, false>;
-                else if (use_mcc) kern = k_advance2d<false, true, false, false>;
-                else kern = k_advance2d<false, false, false, false>;
-            }
-        }
+        // the flush after the previous pass left the tables for this phase
+        const bool pht = T && fast && f->ph_valid && f->ph_phase == a.phase;
+        void (*kern)(PushArgs) = hipos ? pick_pass<true>(T != nullptr, fast, pht, bgrid, deposit, use_mcc)
+                                       : pick_pass<false>(T != nullptr, fast, pht, bgrid, deposit, use_mcc);
+        M2D_CHECK(kern, M2D_ERR_UNSUPPORTED, "m2d_advance_2d: no compiled form of the pass for this configuration");
+        const int g = T ? (int)xcd_run_grid((unsigned)T->max_chunks) : (int)((p->n + kChunk - 1) / kChunk);
+        fast_tiled = T && fast;
         launch_timed(ctx, kern, dim3(g), dim3(kPass), a);
         M2D_HIP(hipGetLastError());
-        uint32_t v = 0;
-        if (T) {
-            v = M2D_PASS_DEPOSIT | M2D_PASS_TILED;
-            if (fast) v |= M2D_PASS_FAST | (f->ph_valid && f->ph_phase == a.phase ? M2D_PASS_PHT : 0u);
-            else if (bgrid) v |= M2D_PASS_BGRID;
-        } else {
-            v = (deposit ? M2D_PASS_DEPOSIT : 0u) | (fast ? M2D_PASS_FAST : bgrid ? M2D_PASS_BGRID : 0u);
-        }
-        ctx->last_pass = v | (use_mcc ? M2D_PASS_MCC : 0u);
+        ctx->last_pass = (deposit ? M2D_PASS_DEPOSIT : 0u) | (use_mcc ? M2D_PASS_MCC : 0u) |
+                         (T ? M2D_PASS_TILED : 0u) | (fast ? M2D_PASS_FAST : bgrid ? M2D_PASS_BGRID : 0u) |
+                         (pht ? M2D_PASS_PHT : 0u) | (hipos ? M2D_PASS_HIPOS : 0u);
     }
     if (deposit || use_mcc) {
         // the grid flush, the next step's phase-applied tables (after a
@@ -1320,7 +1383,7 @@ int m2d_accumulate_2d(m2d_ctx *ctx, const m2d_grid2d *g, const m2d_particles2d *
     M2D_TRY(ensure_acc(ctx, n));
     if (p->n > 0) {
         k_accumulate2d<<<grid_for(p->n, 1), kBlock, 0, ctx->stream>>>(
-            p->x, p->z, p->n, g->nx, g->nz, 1.0 / g->dx, 1.0 / g->dz, g->xmin, g->zmin,
+            p->x, p->z, p->xl, p->zl, p->n, g->nx, g->nz, 1.0 / g->dx, 1.0 / g->dz, g->xmin, g->zmin,
             (unsigned long long *)ctx->acc.ptr, ctx->counters);
         M2D_HIP(hipGetLastError());
     }
@@ -1407,6 +1470,8 @@ int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles
     M2D_CHECK(dst->n >= src->n, M2D_ERR_BAD_ARG, "m2d_compact_2d: dst holds %lld < %lld slots",
               (long long)dst->n, (long long)src->n);
     M2D_CHECK(dst->x != src->x, M2D_ERR_BAD_ARG, "m2d_compact_2d: src and dst must differ");
+    M2D_CHECK((src->xl == nullptr) == (dst->xl == nullptr), M2D_ERR_BAD_ARG,
+              "m2d_compact_2d: src and dst must both hold position remainders or neither");
     int nb = grid_for(src->n, kPPT);
     M2D_TRY(ensure(ctx, ctx->tmp, (size_t)nb * 12 + 64));
     unsigned int *counts = (unsigned int *)ctx->tmp.ptr;

@@ -22,22 +22,24 @@ struct SortGeom {
     uint32_t dead_key;
 };
 
-__device__ __forceinline__ uint32_t cell_key(const SortGeom &g, float x, float z)
+__device__ __forceinline__ uint32_t cell_key(const SortGeom &g, double x, double z)
 {
-    double X = ((double)x - g.xmin) * g.idx, Z = ((double)z - g.zmin) * g.idz;
+    double X = (x - g.xmin) * g.idx, Z = (z - g.zmin) * g.idz;
     int i = X >= 0.0 ? (int)fmin(X, (double)(g.nx - 2)) : 0;
     int j = Z >= 0.0 ? (int)fmin(Z, (double)(g.nz - 2)) : 0;
     uint32_t tile = (uint32_t)((i / kTile) * g.ntz + (j / kTile));
     return (tile << kTileShift) | ((uint32_t)(i % kTile) << 3) | (uint32_t)(j % kTile);
 }
 
-__global__ void k_sort_keys(const float *xs, const float *zs, int64_t n, SortGeom g, uint32_t *keys,
-                            uint32_t *vals)
+// the key of a long-time position (xls, zls not null) is that of the FP64 sum
+__global__ void k_sort_keys(const float *xs, const float *zs, const float *xls, const float *zls, int64_t n,
+                            SortGeom g, uint32_t *keys, uint32_t *vals)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float x = xs[p];
-    keys[p] = is_dead(x) ? g.dead_key : cell_key(g, x, zs[p]);
+    keys[p] = is_dead(x) ? g.dead_key
+                         : xls ? cell_key(g, pos64(x, xls[p]), pos64(zs[p], zls[p])) : cell_key(g, x, zs[p]);
     vals[p] = (uint32_t)p;
 }
 
@@ -63,6 +65,10 @@ __global__ void k_sort_gather(m2d_particles2d src, m2d_particles2d dst, const ui
     dst.vx[i] = src.vx[s];
     dst.vy[i] = src.vy[s];
     dst.vz[i] = src.vz[s];
+    if (src.xl) {
+        dst.xl[i] = src.xl[s];
+        dst.zl[i] = src.zl[s];
+    }
 }
 
 // [start, end) of every non-empty tile in the sorted keys
@@ -204,6 +210,9 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
                               dst->vx && dst->vy && dst->vz),
               M2D_ERR_BAD_ARG, "m2d_sort_2d: NULL particle array");
     M2D_CHECK(src->x != dst->x, M2D_ERR_BAD_ARG, "m2d_sort_2d: src and dst must differ");
+    M2D_CHECK((src->xl == nullptr) == (src->zl == nullptr) && (dst->xl == nullptr) == (dst->zl == nullptr) &&
+                  (src->xl == nullptr) == (dst->xl == nullptr),
+              M2D_ERR_BAD_ARG, "m2d_sort_2d: src and dst must both hold position remainders or neither");
     const size_t n = (size_t)src->n;
     SortGeom g;
     g.nx = f->nx; g.nz = f->nz;
@@ -235,7 +244,8 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     void *tmp = b + 4 * arr;
     unsigned long long *live = ctx->counters + CNT_LIVE;
     M2D_TRY(zero_words(ctx, live, 1));
-    k_sort_keys<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(src->x, src->z, src->n, g, k0, v0);
+    k_sort_keys<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(src->x, src->z, src->xl, src->zl, src->n, g,
+                                                                      k0, v0);
     M2D_HIP(hipGetLastError());
     e = rocprim::radix_sort_pairs(tmp, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
