@@ -304,7 +304,14 @@ typedef struct {
 } m2d_mcc_desc;
 
 /* BaseSpecies::lifetime_init + svmax_find (particles.cpp:142-206) and the
- * device tables for BaseSpecies::scatter (particles.cpp:208-365). */
+ * device tables for BaseSpecies::scatter (particles.cpp:208-365).  COULOMB
+ * (Interaction::coulomb_sigma, particles.cpp:19-26) needs the primary's
+ * charge, temperature and density and the target's charge; its partner
+ * update (particles.cpp:309-314) writes the partner store's velocities, so
+ * m2d_advance_2d applies a COULOMB table's hits one after another in slot
+ * order, as the reference's loop does (one device thread, after a host read
+ * of the hit count), and the batched, 3D and multicoll passes, which scatter
+ * their hits in parallel, refuse such a table (M2D_ERR_UNSUPPORTED). */
 int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out);
 int m2d_mcc_destroy(m2d_mcc *m);
 double m2d_mcc_lifetime(const m2d_mcc *m);

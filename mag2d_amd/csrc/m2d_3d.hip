@@ -1547,6 +1547,8 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
               "m2d_advance_3d: unknown flags 0x%x", flags);
     const bool deposit = (flags & M2D_STEP_DEPOSIT) != 0;
     M2D_CHECK(!deposit || rho, M2D_ERR_BAD_ARG, "m2d_advance_3d: DEPOSIT needs rho");
+    M2D_CHECK(!mcc || !mcc->h.partner_update, M2D_ERR_UNSUPPORTED,
+              "m2d_advance_3d: COULOMB (partner update) runs in m2d_advance_2d only");
     if (stats) M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
     PushArgs3 a;
     fill_args3(a, f, s, p, 1.0, 1.0);
@@ -1560,6 +1562,10 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
         a.mcc = mcc->d;
         M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
         a.cq = (uint32_t *)ctx->cq.ptr;
+        // a call that failed between its pass and its scatter left the
+        // alternating counters unknown: clear both first
+        if (ctx->cq_dirty) M2D_TRY(zero_words(ctx, ctx->counters + CNT_QLEN, 2));
+        ctx->cq_dirty = true;
         const int slot = (int)(ctx->launch_seq++ & 1);
         a.cq_len = ctx->counters + CNT_QLEN + slot;
         next_len = ctx->counters + CNT_QLEN + (slot ^ 1);
@@ -1590,6 +1596,7 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
         if (use_mcc) {
             k_scatter3d<<<64, 256, 0, ctx->stream>>>(a, next_len);
             M2D_HIP(hipGetLastError());
+            ctx->cq_dirty = false;
         }
     }
     if (T) {

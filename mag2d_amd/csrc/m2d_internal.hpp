@@ -132,6 +132,7 @@ struct m2d_ctx {
     m2d::Scratch cq;        // MCC collision queue (slot indices)
     m2d::Scratch partials;  // per-block deposit windows of the tiled 3D pass
     uint64_t launch_seq = 0;  // selects the collision-queue counter of a launch
+    bool cq_dirty = false;    // a call failed between appending to a queue counter and its scatter
     uint32_t last_pass = 0;   // M2D_PASS_* bits of the last 2D particle pass launched
     std::vector<struct m2d_tiling *> tilings;  // schedules of recently sorted stores
     // kernel timing: event pairs around particle-pass launches
@@ -143,6 +144,7 @@ struct m2d_ctx {
 namespace m2d {
 // tile schedules of sorted stores (m2d_sort.hip)
 m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n);
+int sort_keys_u32(m2d_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n);
 int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
                  int nty, int ntz, int cap);
 void forget_tiling(m2d_ctx *ctx, const float *x);

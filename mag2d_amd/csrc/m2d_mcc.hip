@@ -2,6 +2,7 @@
 // (particles.cpp:142-206) and the Interaction constants (particles.hpp:49-92),
 // packed into the device table read by BaseSpecies::scatter's kernel form.
 #include "m2d_objects.hpp"
+#include "m2d_scatter.hpp"
 #include <cmath>
 #include <new>
 
@@ -10,6 +11,7 @@ namespace m2d {
 // Interaction::sigma_v (particles.hpp:61-70) for table / constant rate
 static double host_sigma_v(const McDev &m, const McInter &it, const std::vector<double> &tab, double v)
 {
+    if (it.type == M2D_COULOMB) return coulomb_sigma(it, 0.5 * it.mu * v * v) * v;
     if (it.tab_len > 0) {
         const double *xs = tab.data() + it.tab_off;
         const double *ys = tab.data() + (tab.size() / 2) + it.tab_off;
@@ -79,10 +81,10 @@ int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out)
         for (int k = 0; k < d->n_inter; k++) {
             const m2d_interaction &I = d->inter[k];
             if (I.target != t) continue;
-            if (I.type == M2D_COULOMB) {
+            if (I.type == M2D_COULOMB && !(d->temperature > 0 && d->density > 0 && d->charge != 0 && T.charge != 0)) {
                 m2d_mcc_destroy(m);
-                return set_error(M2D_ERR_UNSUPPORTED, "m2d_mcc_create: COULOMB (partner update) is not "
-                                 "implemented on the device");
+                return set_error(M2D_ERR_BAD_ARG, "m2d_mcc_create: COULOMB needs the primary's temperature, "
+                                 "density and charge and the target's charge");
             }
             if (I.type < 0 || I.type > M2D_SUPERELASTIC) {
                 m2d_mcc_destroy(m);
@@ -101,6 +103,14 @@ int m2d_mcc_create(m2d_ctx *ctx, const m2d_mcc_desc *d, m2d_mcc **out)
             J.mu = d->mass * T.mass / (d->mass + T.mass);
             J.tab_off = I.tab_len > 0 ? I.tab_offset : 0;
             J.tab_len = I.tab_len > 0 ? I.tab_len : 0;
+            J.lambda_D = J.qq = 0.0;
+            if (I.type == M2D_COULOMB) {
+                // particles.cpp:19-26
+                J.lambda_D = sqrt(kEps0 * kKB * d->temperature / (d->density * d->charge * d->charge));
+                J.qq = d->charge * T.charge / (4 * M_PI * kEps0);
+                J.tab_len = 0;
+                h.partner_update = 1;
+            }
             D.n++;
         }
     }

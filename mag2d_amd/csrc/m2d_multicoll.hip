@@ -112,6 +112,8 @@ int m2d_advance_multicoll_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d 
     M2D_CHECK(p->n == 0 || time_to_death, M2D_ERR_BAD_ARG, "m2d_advance_multicoll_2d: time_to_death is NULL");
     M2D_CHECK(s->mass > 0 && s->dt > 0, M2D_ERR_BAD_ARG, "m2d_advance_multicoll_2d: mass and dt must be > 0");
     M2D_CHECK(!mcc || mcc->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_advance_multicoll_2d: mcc belongs to another context");
+    M2D_CHECK(!mcc || !mcc->h.partner_update, M2D_ERR_UNSUPPORTED,
+              "m2d_advance_multicoll_2d: COULOMB (partner update) runs in m2d_advance_2d only");
     const m2d_fields2d_params &P = f->p;
     // the reference's own restrictions (particles.cpp:818-823)
     M2D_CHECK(!f->bt && P.Br == 0.0 && P.Bz == 0.0 && P.Bt == 0.0, M2D_ERR_UNSUPPORTED,

@@ -18,6 +18,9 @@ struct McInter {
     double rate;    // m^3/s, LANGEVIN already scaled by cutoff^2 (particles.hpp:81-82)
     double cutoff;
     double mu;      // reduced mass of primary and target
+    // COULOMB (Interaction::coulomb_sigma, particles.cpp:19-26): the primary's
+    // Debye length and q1 q2 / (4 pi eps0)
+    double lambda_D, qq;
 };
 
 struct McTarget {
@@ -26,13 +29,14 @@ struct McTarget {
     double mass;
     double vth;     // sqrt(k_B T / m) = v_max/sqrt(2) (particles.hpp:180,190-192)
     int first, n;   // interactions [first, first+n)
-    const float *px, *pz, *pvx, *pvy, *pvz;  // partner ensemble (px == nullptr: continuum)
+    const float *px, *pz;                    // partner ensemble (px == nullptr: continuum)
+    float *pvx, *pvy, *pvz;                  // its velocities (COULOMB writes the partner's)
     int64_t pn;
 };
 
 struct McDev {
     int n_targets;
-    int pad;
+    int partner_update;  // some interaction (COULOMB) changes the partner's velocity
     double lifetime;
     double mass;    // primary
     McTarget t[kMaxTargets];

@@ -454,7 +454,9 @@ k_advance2d(PushArgs a)
                     dead = !free_;
                 }
                 if (dead) {
-                    v[0][q] = dead_value();
+                    // the slot keeps the step's velocity: a COULOMB partner
+                    // update of its collision still reads it
+                    v[0][q] = dead_value(); v[2][q] = vx; v[3][q] = vy; v[4][q] = vz;
                     n_removed++;
                     continue;
                 }
@@ -556,6 +558,28 @@ __device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long l
     for (unsigned long long k = start; k < len; k += stride) {
         const uint32_t p = a.cq[k];
         if (is_dead(a.x[p])) continue;  // removed by the boundary after its trial
+        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+        Rng rg;
+        rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
+        scatter(*a.mcc, rg, vx, vy, vz, a.counters);
+        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+    }
+}
+
+// BaseSpecies::scatter of the queued slots one after another in slot order,
+// by one thread: an interaction that turns its partner too (COULOMB,
+// particles.cpp:309-314) makes the order matter, and the reference scatters
+// in slot order inside its particle loop.  Slots removed by the boundary
+// after their trial scatter as well (the reference scatters before it
+// removes), so their partners turn as there.  The thread also clears the
+// other queue counter for the next launch.
+__global__ void k_scatter_serial(PushArgs a, const uint32_t *sorted, unsigned long long len,
+                                 unsigned long long *next_len)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    *next_len = 0ull;
+    for (unsigned long long k = 0; k < len; k++) {
+        const uint32_t p = sorted[k];
         float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
         Rng rg;
         rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
@@ -1233,7 +1257,11 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
         a.cq = (uint32_t *)ctx->cq.ptr;
         // two queue counters alternate between launches: a launch's scatter
-        // kernel clears the one the next launch appends to
+        // kernel clears the one the next launch appends to.  A call that
+        // failed between its pass and its scatter left them unknown: clear
+        // both first.
+        if (ctx->cq_dirty) M2D_TRY(zero_words(ctx, ctx->counters + CNT_QLEN, 2));
+        ctx->cq_dirty = true;  // until this call's scatter is queued
         const int slot = (int)(ctx->launch_seq++ & 1);
         a.cq_len = ctx->counters + CNT_QLEN + slot;
         next_len = ctx->counters + CNT_QLEN + (slot ^ 1);
@@ -1292,12 +1320,15 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         const bool build_ph = fast_tiled && !f->p.selfconsistent && !(f->ph_valid && f->ph_phase == pb.ph);
         const unsigned ntab =
             build_ph ? (unsigned)((pb.nxe + pb.nze + kBlock * kPhPerThread - 1) / (kBlock * kPhPerThread)) : 0u;
-        const unsigned g = nfl + ntab + (use_mcc ? kScatterBlocks : 0u);
+        // an interaction that turns its partner (COULOMB) scatters the queue
+        // in slot order after the flush, by one thread
+        const bool serial = use_mcc && mcc->h.partner_update;
+        const unsigned g = nfl + ntab + (use_mcc && !serial ? kScatterBlocks : 0u);
         // without a deposit no flush block runs: n = 0 ends them at once
         const size_t nodes = deposit ? f->n : 0;
-        if (use_mcc)
+        if (use_mcc && !serial)
             k_flush_scatter<true><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, nodes, pb, ntab, a, next_len);
-        else
+        else if (g > 0)
             k_flush_scatter<false><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, nodes, pb, ntab, a,
                                                                   next_len);
         M2D_HIP(hipGetLastError());
@@ -1305,7 +1336,21 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
             f->ph_valid = true;
             f->ph_phase = pb.ph;
         }
+        if (serial) {
+            unsigned long long len = 0;
+            M2D_HIP(hipMemcpyAsync(&len, a.cq_len, sizeof(len), hipMemcpyDeviceToHost, ctx->stream));
+            M2D_HIP(hipStreamSynchronize(ctx->stream));
+            uint32_t *sorted = nullptr;
+            if (len > 0) {
+                M2D_TRY(ensure(ctx, ctx->tmp, (size_t)len * sizeof(uint32_t) + 256));
+                sorted = (uint32_t *)ctx->tmp.ptr;
+                M2D_TRY(sort_keys_u32(ctx, a.cq, sorted, (size_t)len));
+            }
+            k_scatter_serial<<<1, 64, 0, ctx->stream>>>(a, sorted, len, next_len);
+            M2D_HIP(hipGetLastError());
+        }
     }
+    if (use_mcc) ctx->cq_dirty = false;
     if (stats) {
         M2D_TRY(read_counters(ctx));
         stats->removed = (int64_t)ctx->h_counters[CNT_REMOVED];

@@ -21,9 +21,18 @@ __device__ __forceinline__ double vec_interp(const double *xs, const double *ys,
     return ys[j1] * (1 - w) + ys[j2] * w;
 }
 
+// Interaction::coulomb_sigma (particles.cpp:19-26) at the relative energy
+// E (J): pi L^2 ln(lambda_D / L), L = q1 q2 / (4 pi eps0 E)
+__host__ __device__ inline double coulomb_sigma(const McInter &it, double E)
+{
+    const double L = it.qq / E;
+    return M_PI * L * L * log(it.lambda_D / L);
+}
+
 __device__ __forceinline__ double sigma_v(const McDev &m, const McInter &it, double v)
 {
-    // Interaction::sigma_v (particles.hpp:61-70); COULOMB is rejected at creation
+    // Interaction::sigma_v (particles.hpp:61-70)
+    if (it.type == M2D_COULOMB) return coulomb_sigma(it, 0.5 * it.mu * v * v) * v;
     if (it.tab_len > 0)
         return vec_interp(m.tabE + it.tab_off, m.tabS + it.tab_off, it.tab_len,
                           0.5 * it.mu * v * v / kQE) * v;
@@ -90,6 +99,7 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
     }
     const McTarget &T = m.t[sid];
     double vr2, vz2, vt2;
+    int64_t partner = -1;  // slot of the live partner (random_particle, particles.hpp:201-211)
     if (T.px == nullptr || T.pn == 0) {
         vr2 = r.normal() * T.vth;
         vz2 = r.normal() * T.vth;
@@ -100,6 +110,7 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
         while (is_dead(T.px[i]) && tries++ < 1000) i = (int64_t)(r.u32() % (uint32_t)T.pn);
         if (is_dead(T.px[i])) { raise_flag(counters, M2D_FLAG_PARTNER); return; }
         vr2 = T.pvx[i]; vz2 = T.pvz[i]; vt2 = T.pvy[i];
+        partner = i;
     }
     double dvx = vx - vr2, dvz = vz - vz2, dvy = vy - vt2;
     double v_rel = sqrt(dvx * dvx + dvz * dvz + dvy * dvy);
@@ -127,7 +138,9 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
         vz = nvz; vy = nvy;
         break;
     }
+    case M2D_COULOMB:
     case M2D_ELASTIC: {
+        // isotropic in the centre of mass (particles.cpp:290-316)
         double cx = (vx * mass + vr2 * m2) * tmp;
         double cz = (vz * mass + vz2 * m2) * tmp;
         double cy = (vy * mass + vt2 * m2) * tmp;
@@ -136,6 +149,13 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
         vx = rx * m2 * tmp + cx;
         vz = rz * m2 * tmp + cz;
         vy = ry * m2 * tmp + cy;
+        // COULOMB also turns the partner (particles.cpp:309-314); a partner
+        // drawn from the continuum has no state to change
+        if (it.type == M2D_COULOMB && partner >= 0) {
+            T.pvx[partner] = (float)(-rx * mass * tmp + cx);
+            T.pvz[partner] = (float)(-rz * mass * tmp + cz);
+            T.pvy[partner] = (float)(-ry * mass * tmp + cy);
+        }
         break;
     }
     case M2D_CX:

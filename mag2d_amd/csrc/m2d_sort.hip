@@ -192,6 +192,20 @@ int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, 
     return M2D_OK;
 }
 
+// Sort n 32-bit keys in ascending order from `in` into `out` (stream-ordered,
+// temporaries in the context's sort buffer).
+int sort_keys_u32(m2d_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n)
+{
+    if (n == 0) return M2D_OK;
+    size_t temp = 0;
+    hipError_t e = rocprim::radix_sort_keys(nullptr, temp, in, out, n, 0, 32, ctx->stream);
+    M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_keys size query: %s", hipGetErrorString(e));
+    M2D_TRY(ensure(ctx, ctx->sortbuf, temp + 256));
+    e = rocprim::radix_sort_keys(ctx->sortbuf.ptr, temp, in, out, n, 0, 32, ctx->stream);
+    M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_keys: %s", hipGetErrorString(e));
+    return M2D_OK;
+}
+
 }  // namespace m2d
 
 using namespace m2d;
