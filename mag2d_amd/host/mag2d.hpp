@@ -108,8 +108,8 @@ public:
     void rebuild_free_list();
     // checkpoint (particles.cpp:31-93): int size, int n_live, n_live x t_particle
     void save(const std::string &filename) const;
-    // load keeps the reference's layout: live particles first, then empty slots.
-    // The reference's re-draw of positions outside the box is left to the caller.
+    // load keeps the reference's layout: live particles first, then empty
+    // slots.  Species::load adds the re-draw of positions outside the box.
     void load(const std::string &filename);
 
 protected:
@@ -154,7 +154,9 @@ public:
     // node, a blank line after each x row
     void print(std::ostream &out, double factor = 1.0);
     void print(const char *filename, double factor = 1.0);
-    void load(const char *filename);    // the file must have this field's shape
+    // the file must hold this field's grid: its shape, and its spacing and
+    // origin to the 6 significant digits print writes
+    void load(const char *filename);
 
     // explicit mirror control
     double *device();                   // device data, host writes sent first
@@ -173,6 +175,20 @@ private:
     mutable State state_ = SYNCED;
     void pull() const;
 };
+
+// The text format of Field2D::print / load (Field2D.cpp:30-130) on host data
+// (field_text_read: Field2D::load's reader, any order of nodes)
+struct FieldText {
+    m2d_grid2d g;                 // jmax, lmax, dx, dy, xmin, ymin as found in the file
+    std::vector<double> data;     // data[i * lmax + j]
+};
+void field_text_write(std::ostream &out, const m2d_grid2d &g, const double *data, double factor = 1.0);
+FieldText field_text_read(const char *filename);
+
+// the re-draw of BaseSpecies::load for slots [0, live) of a loaded list
+// (particles.cpp:77-83), with the Philox stream described at Species::load
+void reload_remap(std::vector<t_particle> &particles, int live, double x_min, double x_max, double z_min,
+                  double z_max, uint64_t seed, uint32_t stream);
 
 // ------------------------------------------------------------------ Param
 enum geometry_type { EMPTY = M2D_GEO_EMPTY, PROBE = M2D_GEO_PROBE, RF_22PT = M2D_GEO_RF_22PT,
@@ -194,7 +210,16 @@ struct Param {
     double x_max = 1e-2, z_max = 1e-2, x_min = 0.0, z_min = 0.0;
     geometry_type geometry = EMPTY;
     double probe_radius = 1e-3, u_probe = 0.0, extern_field = 0.0;
-    int selfconsistent = 0, electric_field_from_file = 0;
+    int selfconsistent = 0;
+    // electric_field_from_file: Pic loads u (and uRF when rf) from text files
+    // in the format of Field2D::print instead of solving them (param.cpp:35-37,
+    // pic.cpp:154-175); the files must hold this grid
+    int electric_field_from_file = 0;
+    std::string electric_field_static_file, electric_field_rf_file;
+    // particle_reload: Pic loads every particle species from
+    // particle_reload_dir/particles_<name>.dat (param.cpp:73-74, pic.cpp:138-146)
+    int particle_reload = 0;
+    std::string particle_reload_dir = ".";
     int boundary = FREE;
     int rf = 0;
     double rf_amplitude = 0.0, rf_omega = 0.0, rf_U0 = 0.0;
@@ -209,7 +234,7 @@ struct Param {
     // derived (param.cpp:126-142)
     double dx = 0, dz = 0, idx = 0, idz = 0, V = 0, dV = 0;
     void derive();
-    // Pic::check_params and param.cpp:60-61: selfconsistent excludes rf
+    // param.cpp:60-64: selfconsistent excludes rf and electric_field_from_file
     void check() const;
 };
 
@@ -324,6 +349,12 @@ public:
     // re-order the device store by cell (drops holes; host mirror follows).
     // Boris only: the multicoll mover keeps time_to_death per slot.
     void sort();
+    // BaseSpecies::load (particles.cpp:57-93): the checkpoint of save(); a
+    // live particle whose x (z) lies outside [x_min, x_max] ([z_min, z_max])
+    // is put at a uniform position along that axis.  The uniforms are the
+    // first (x) and second (z) word of the Philox stream (seed, species
+    // stream, slot, step 0, purpose 5), so a reload is reproducible.
+    void load(const std::string &filename);
     // the container calls of BaseSpecies on the host mirror
     int insert();
     void remove(int n);

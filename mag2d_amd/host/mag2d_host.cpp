@@ -19,12 +19,13 @@ void check(int status)
     if (status != M2D_OK) throw std::runtime_error(m2d_last_error());
 }
 
-// nearest integer of x, which must lie within tol of it (Field2D.cpp:87,121)
-static int double2int(double x, double tol)
+// double2int (util.cpp:22-28): (int)(x + 1/2), which must lie within eps of x
+static int double2int(double x, double eps = 1e-2)
 {
-    double r = std::nearbyint(x);
-    if (!(std::fabs(x - r) <= tol)) throw std::runtime_error("double2int: value is not close to an integer");
-    return (int)r;
+    const int r = (int)(x + 0.5);
+    if (!(std::fabs(r - x) <= eps))
+        throw std::runtime_error("double2int() " + std::to_string(x) + " is not integer\n");
+    return r;
 }
 
 static inline float dead_float()
@@ -173,6 +174,47 @@ void ParticleList::load(const std::string &filename)
     for (int i = live; i < size; i++) empty_.push_back(i);
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): the library's draw
+// stream (seed, stream, slot, step, purpose) on the host, first block
+static void philox_block(uint64_t seed, uint32_t stream, uint32_t slot, uint64_t step, uint32_t purpose,
+                         uint32_t out[4])
+{
+    uint32_t c0 = slot, c1 = (uint32_t)step, c2 = 0, c3 = (purpose << 24) | ((uint32_t)(step >> 32) & 0xFFFFFFu);
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32) ^ stream;
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+constexpr uint32_t kPurposeReload = 5;
+
+void reload_remap(std::vector<t_particle> &particles, int live, double x_min, double x_max, double z_min,
+                  double z_max, uint64_t seed, uint32_t stream)
+{
+    // uniform in (0,1): (w + 1/2) 2^-32, as the library draws them
+    auto uni = [](uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; };
+    for (int i = 0; i < live && i < (int)particles.size(); i++) {
+        t_particle &p = particles[(size_t)i];
+        const bool ox = p.x < x_min || p.x > x_max, oz = p.z < z_min || p.z > z_max;
+        if (!ox && !oz) continue;
+        uint32_t w[4];
+        philox_block(seed, stream, (uint32_t)i, 0, kPurposeReload, w);
+        if (ox) p.x = (x_max - x_min) * uni(w[0]) + x_min;
+        if (oz) p.z = (z_max - z_min) * uni(w[1]) + z_min;
+    }
+}
+
 // ================================================================ Field2D
 Field2D::Field2D(std::shared_ptr<Device> dev, int x_sampl, int y_sampl, double dx, double dy, double xmin,
                  double ymin)
@@ -303,12 +345,7 @@ void Field2D::grad(double x, double y, double &gx, double &gy) { grad(&x, &y, 1,
 void Field2D::print(std::ostream &out, double factor)
 {
     pull();
-    for (int j = 0; j < jmax; j++) {
-        for (int l = 0; l < lmax; l++)
-            out << j * g_.dx + g_.xmin << "\t" << l * g_.dz + g_.zmin << "\t" << h_[(size_t)j * lmax + l] * factor
-                << std::endl;
-        out << std::endl;
-    }
+    field_text_write(out, g_, h_.data(), factor);
 }
 
 void Field2D::print(const char *filename, double factor)
@@ -318,6 +355,40 @@ void Field2D::print(const char *filename, double factor)
 }
 
 void Field2D::load(const char *filename)
+{
+    FieldText t = field_text_read(filename);
+    if (t.g.nx != jmax || t.g.nz != lmax) throw std::runtime_error("Field2D::load(): file shape differs from the field");
+    // print writes each coordinate to 6 significant digits (relative error
+    // 5e-6), so the origin, and the spacing found from the first difference
+    // of two coordinates, agree with the field's to 1e-5 of the largest
+    // coordinate.  The field keeps its own exact grid (the reference takes
+    // the file's rounded spacing; the two agree to that rounding).
+    auto span = [](double lo, double h, int n) { return std::fabs(lo) + std::fabs(lo + (n - 1) * h) + h; };
+    const double tx = 1e-5 * span(g_.xmin, g_.dx, jmax), tz = 1e-5 * span(g_.zmin, g_.dz, lmax);
+    if (std::fabs(t.g.dx - g_.dx) > tx || std::fabs(t.g.dz - g_.dz) > tz || std::fabs(t.g.xmin - g_.xmin) > tx ||
+        std::fabs(t.g.zmin - g_.zmin) > tz)
+        throw std::runtime_error("Field2D::load(): file grid differs from the field");
+    h_ = std::move(t.data);
+    state_ = HOST_NEWER;
+}
+
+// the text format of Field2D::print (Field2D.cpp:30-44): "x<TAB>z<TAB>value"
+// per node in the stream's default notation (6 significant digits), a blank
+// line after each x row
+void field_text_write(std::ostream &out, const m2d_grid2d &g, const double *data, double factor)
+{
+    for (int j = 0; j < g.nx; j++) {
+        for (int l = 0; l < g.nz; l++)
+            out << j * g.dx + g.xmin << "\t" << l * g.dz + g.zmin << "\t" << data[(size_t)j * g.nz + l] * factor
+                << std::endl;
+        out << std::endl;
+    }
+}
+
+// Field2D::load (Field2D.cpp:46-130): lines of three numbers in any order of
+// nodes; spacing, origin and extent from the first change of each coordinate
+// (a negative step means the file runs backwards); every node exactly once
+FieldText field_text_read(const char *filename)
 {
     std::ifstream fr(filename);
     if (fr.fail()) throw std::runtime_error("Field2D::load(): failed opening file\n");
@@ -333,7 +404,6 @@ void Field2D::load(const char *filename)
         }
     }
     if (xv.size() < 2) throw std::runtime_error("Field2D::load(): no data");
-    // grid spacing and extent from the first change of each coordinate
     auto axis = [](const std::vector<double> &v, double &d, double &vmin, double &vmax) {
         vmin = v.front();
         vmax = v.back();
@@ -352,16 +422,20 @@ void Field2D::load(const char *filename)
     const int xs = double2int((xmax - xmin) / dx + 1, 1e-1), ys = double2int((ymax - ymin) / dy + 1, 1e-1);
     if ((size_t)xs * ys != xv.size())
         throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
-    if (xs != jmax || ys != lmax) throw std::runtime_error("Field2D::load(): file shape differs from the field");
-    std::vector<double> d((size_t)xs * ys, std::nan(""));
+    FieldText t;
+    t.g = m2d_grid2d{xs, ys, dx, dy, xmin, ymin};
+    t.data.assign((size_t)xs * ys, std::nan(""));
     for (size_t i = 0; i < xv.size(); i++) {
         int xi = double2int((xv[i] - xmin) / dx, 1e-1), yi = double2int((yv[i] - ymin) / dy, 1e-1);
-        d[(size_t)xi * ys + yi] = fv[i];
+        // a node beyond the extent found from the ends (a file whose nodes do
+        // not run monotonically): the reference writes outside its array there
+        if (xi < 0 || xi >= xs || yi < 0 || yi >= ys)
+            throw std::runtime_error("Field2D::load(): node outside the grid");
+        t.data[(size_t)xi * ys + yi] = fv[i];
     }
-    for (double v : d)
+    for (double v : t.data)
         if (std::isnan(v)) throw std::runtime_error("Fields::load_magnetic_field() garbage loaded");
-    h_ = std::move(d);
-    state_ = HOST_NEWER;
+    return t;
 }
 
 // ================================================================ Param
@@ -378,6 +452,8 @@ void Param::derive()
 void Param::check() const
 {
     if (selfconsistent && rf) throw std::runtime_error("Param: selfconsistent RF simulation not implemented");
+    if (selfconsistent && electric_field_from_file)
+        throw std::runtime_error("Param: selfconsistent with electric_field_from_file not implemented");
     if (x_sampl < 3 || z_sampl < 3) throw std::runtime_error("Param: need at least 3 samples per axis");
 }
 
@@ -531,15 +607,6 @@ void Fields::B(double x, double y, double &Br, double &Bz, double &Bt) const
     Bt = out[2];
 }
 
-// double2int (util.cpp:22-28): the nearest integer, which must be within 1e-2
-static int nearest_int(double x)
-{
-    const int r = (int)(x + 0.5);
-    if (std::fabs(r - x) > 1e-2)
-        throw std::runtime_error("double2int() " + std::to_string(x) + " is not integer\n");
-    return r;
-}
-
 void Fields::load_magnetic_field(const char *fname)
 {
     std::ifstream fr(fname);
@@ -570,7 +637,7 @@ void Fields::load_magnetic_field(const char *fname)
             h = -h;
             std::swap(lo, hi);
         }
-        return nearest_int((hi - lo) / h + 1);
+        return double2int((hi - lo) / h + 1);
     };
     if (n < 4) throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
     double dr, rmin, rmax, dz, zmin, zmax;
@@ -579,7 +646,7 @@ void Fields::load_magnetic_field(const char *fname)
         throw std::runtime_error("Fields::load_magnetic_field() wrong size of input vector");
     std::vector<double> br((size_t)rs * zs, NAN), bz((size_t)rs * zs, NAN);
     for (size_t k = 0; k < n; k++) {
-        const int ri = nearest_int((rv[k] - rmin) / dr), zi = nearest_int((zv[k] - zmin) / dz);
+        const int ri = double2int((rv[k] - rmin) / dr), zi = double2int((zv[k] - zmin) / dz);
         if (ri < 0 || ri >= rs || zi < 0 || zi >= zs)
             throw std::runtime_error("Fields::load_magnetic_field() garbage loaded");
         br[(size_t)ri * zs + zi] = brv[k];
@@ -685,6 +752,16 @@ void Species::ensure_device()
 }
 
 void Species::ensure_host() { download(); }
+
+void Species::load(const std::string &filename)
+{
+    ParticleList::load(filename);
+    reload_remap(particles, (int)(particles.size() - empty_.size()), p_->x_min, p_->x_max, p_->z_min, p_->z_max,
+                 p_->seed, stream_);
+    // the host list is the store now: the next pass uploads it
+    dev_newer_ = false;
+    host_newer_ = true;
+}
 
 int Species::insert()
 {
@@ -995,14 +1072,33 @@ void Species::sort()
 Pic::Pic(Param &p, const std::vector<SpeciesParams> &species, const t_grid *grid, int device)
     : param(p), dev(std::make_shared<Device>(device, p.seed)), field(p, dev, grid ? *grid : t_grid(p))
 {
-    // pic.cpp:148-149
-    if (!p.magnetic_field_const) field.load_magnetic_field(p.magnetic_field_file.c_str());
+    p.check();
     for (size_t i = 0; i < species.size(); i++)
         speclist.emplace_back(new Species(species[i], p, field, (uint32_t)i));
     std::vector<Species *> all;
     for (auto &s : speclist) all.push_back(s.get());
     for (auto &s : speclist)
         if (s->particle) s->lifetime_init(all);
+    // the constructor body of pic.cpp:130-188
+    field.reset();
+    if (p.particle_reload)
+        for (auto &s : speclist)
+            if (s->particle) s->load(p.particle_reload_dir + "/particles_" + s->name + ".dat");
+    if (!p.magnetic_field_const) field.load_magnetic_field(p.magnetic_field_file.c_str());
+    if (p.electric_field_from_file) {
+        // the grid stays the Param's: Field2D::load checks that the files hold it
+        field.u.load(p.electric_field_static_file.c_str());
+        if (p.rf) field.uRF.load(p.electric_field_rf_file.c_str());
+        field.update();
+    } else {
+        // the RF basis is solved for self-consistent runs too: it is not
+        // recomputed afterwards
+        field.boundary_solve_rf();
+        if (!p.selfconsistent) {
+            field.boundary_solve();
+            field.reset();
+        }
+    }
 }
 
 Species *Pic::find(const std::string &name)
@@ -1368,6 +1464,152 @@ int m2dh_mcc_run(const m2dh_mcc_cfg *c, float *out_x, float *out_z, float *out_v
             if (out_ttd) out_ttd[k] = q.time_to_death;
             k++;
         }
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+// Field2D::print's text (Field2D.cpp:30-44) of host data (nx x nz,
+// row-major) to a file
+int m2dh_field_text_write(int32_t nx, int32_t nz, double dx, double dz, double xmin, double zmin,
+                          const double *data, double factor, const char *path, char *msg, int len)
+{
+    try {
+        std::ofstream out(path);
+        if (!out.is_open()) throw std::runtime_error("Field2D::print(): failed opening file");
+        field_text_write(out, m2d_grid2d{nx, nz, dx, dz, xmin, zmin}, data, factor);
+        if (!out.good()) throw std::runtime_error("Field2D::print(): write error");
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+// Field2D::load's reader (Field2D.cpp:46-130): meta = {jmax, lmax, dx, dy,
+// xmin, ymin}; the first min(cap, jmax lmax) values go to data.  -1 where the
+// reference throws.
+int m2dh_field_text_read(const char *path, double *meta, double *data, int64_t cap, char *msg, int len)
+{
+    try {
+        FieldText t = field_text_read(path);
+        meta[0] = t.g.nx;
+        meta[1] = t.g.nz;
+        meta[2] = t.g.dx;
+        meta[3] = t.g.dz;
+        meta[4] = t.g.xmin;
+        meta[5] = t.g.zmin;
+        const size_t n = std::min<size_t>(t.data.size(), cap > 0 ? (size_t)cap : 0);
+        if (n) std::memcpy(data, t.data.data(), n * sizeof(double));
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+// BaseSpecies::load (particles.cpp:57-93) on the host: the checkpoint's
+// slots (live ones first) as 64-byte t_particle records into out (up to cap
+// slots), after the re-draw of positions outside the box
+int m2dh_particles_load(const char *path, double x_min, double x_max, double z_min, double z_max, uint64_t seed,
+                        uint32_t stream, void *out, int64_t cap, int64_t *n_slots, int64_t *n_live, char *msg,
+                        int len)
+{
+    try {
+        ParticleList L;
+        L.load(path);
+        const int live = L.n_particles();
+        reload_remap(L.particles, live, x_min, x_max, z_min, z_max, seed, stream);
+        *n_slots = (int64_t)L.particles.size();
+        *n_live = live;
+        const size_t n = std::min<size_t>(L.particles.size(), cap > 0 ? (size_t)cap : 0);
+        if (n) std::memcpy(out, L.particles.data(), n * sizeof(t_particle));
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+// Param::check (param.cpp:60-64) of the given switches: 0, or -1 and the
+// reference's message
+int m2dh_param_check(int32_t selfconsistent, int32_t rf, int32_t electric_field_from_file, char *msg, int len)
+{
+    try {
+        Param p;
+        p.selfconsistent = selfconsistent;
+        p.rf = rf;
+        p.electric_field_from_file = electric_field_from_file;
+        p.check();
+    } catch (const std::exception &e) {
+        std::snprintf(msg, (size_t)len, "%s", e.what());
+        return -1;
+    }
+    msg[0] = 0;
+    return 0;
+}
+
+typedef struct {
+    int32_t grid, steps, geometry, rf;
+    double length, dt, rf_amplitude, rf_omega, mass, charge;
+    uint64_t seed;
+} m2dh_file_cfg;
+
+// A run started from files, as Pic's constructor reads them
+// (pic.cpp:138-175): one particle species "ION" reloaded from
+// reload_dir/particles_ION.dat, u (and uRF when cfg->rf) loaded from text
+// files in Field2D::print's format, no solve (electric_field_from_file,
+// not self-consistent); then Pic::advance_init and cfg->steps Pic::advance.
+// Writes the live particles in slot order (up to n_out) and their count.
+int m2dh_file_run(const m2dh_file_cfg *c, const char *u_file, const char *urf_file, const char *reload_dir,
+                  float *out_x, float *out_z, float *out_vx, float *out_vy, float *out_vz, int64_t n_out,
+                  int64_t *n_live, char *msg, int len)
+{
+    try {
+        Param p;
+        p.x_sampl = p.z_sampl = c->grid;
+        p.x_max = p.z_max = c->length;
+        p.geometry = (geometry_type)c->geometry;
+        p.selfconsistent = 0;
+        p.rf = c->rf;
+        p.rf_amplitude = c->rf_amplitude;
+        p.rf_omega = c->rf_omega;
+        p.electric_field_from_file = 1;
+        p.electric_field_static_file = u_file;
+        if (urf_file) p.electric_field_rf_file = urf_file;
+        p.particle_reload = 1;
+        p.particle_reload_dir = reload_dir;
+        p.seed = c->seed;
+        p.derive();
+        SpeciesParams ion;
+        ion.name = "ION";
+        ion.mass = c->mass;
+        ion.charge = c->charge;
+        ion.dt = c->dt;
+        Pic run(p, {ion});
+        Species &s = *run.speclist[0];
+        run.advance_init();
+        for (int k = 0; k < c->steps; k++) run.advance();
+        s.download();
+        int64_t k = 0;
+        for (const t_particle &q : s.particles) {
+            if (q.empty) continue;
+            if (k < n_out) {
+                out_x[k] = (float)q.x;
+                out_z[k] = (float)q.z;
+                out_vx[k] = (float)q.vx;
+                out_vy[k] = (float)q.vy;
+                out_vz[k] = (float)q.vz;
+            }
+            k++;
+        }
+        *n_live = k;
     } catch (const std::exception &e) {
         std::snprintf(msg, (size_t)len, "%s", e.what());
         return -1;

@@ -1,12 +1,13 @@
 // ref_harness.cpp -- TEST INFRASTRUCTURE ONLY.
 //
 // Exposes the parts of the rouckas/mag2d sources that compile on their own
-// (Field2D.hpp, Field3D.hpp, random.cpp, tabulate.cpp, mymath.cpp,
+// (Field2D.hpp/.cpp, Field3D.hpp, random.cpp, tabulate.cpp, mymath.cpp,
 // histogram.cpp) through extern "C" entry points, so the tests can check the
 // oracle's restatement against the reference's own code.  The reference
 // sources are compiled where they lie (include path set by oracle/Makefile);
 // nothing here stands in for a missing header.  Output goes to oracle/_ref/.
 #include "Field2D.hpp"
+#include "Field2D.cpp"
 #include "Field3D.hpp"
 #include "random.cpp"
 #include "tabulate.cpp"
@@ -116,6 +117,28 @@ void ref_vec_interpolate_many(int n, const double *xs, const double *ys, long m,
     std::vector<double> X(xs, xs + n), Y(ys, ys + n);
     vec_interpolate v(X, Y);
     for (long k = 0; k < m; k++) out[k] = v(q[k]);
+}
+
+// Field2D::print (Field2D.cpp:30-44) of a grid to a file
+void ref_f2_print(int nx, int nz, double dx, double dz, double xmin, double zmin, const double *data,
+                  double factor, const char *path)
+{
+    Field2D f(nx, nz, dx, dz, xmin, zmin);
+    std::memcpy(f.data[0], data, sizeof(double) * nx * nz);
+    f.print(path, factor);
+}
+
+// Field2D::load (Field2D.cpp:46-130): meta = {jmax, lmax, dx, dy, xmin, ymin};
+// data gets up to cap values; returns 0, or -1 where the reference throws
+int ref_f2_load(const char *path, double *meta, double *data, long cap)
+{
+    Field2D f;
+    try { f.load(path); } catch (...) { return -1; }
+    meta[0] = f.jmax; meta[1] = f.lmax; meta[2] = f.GetDx(); meta[3] = f.GetDy();
+    meta[4] = f.GetXMin(); meta[5] = f.GetYMin();
+    long n = (long)f.jmax * f.lmax;
+    std::memcpy(data, f.data[0], sizeof(double) * (n < cap ? n : cap));
+    return 0;
 }
 
 // mod (mymath.cpp:62-67)

@@ -13,6 +13,9 @@ Fixtures (data only: inputs and the reference's outputs):
   vecinterp_ref.npz  vec_interpolate (tabulate.cpp:89-141) on the e-He elastic
                      cross section of species_conf_MCC.txt
   misc_ref.npz       mod (mymath.cpp:62-67), Histogram::mean_tot (histogram.cpp)
+  field2d_print_ref.txt, field2d_load_[abcd].txt, field2d_text_ref.npz
+                     Field2D::print of field2d_text_ref's print_* data, and
+                     Field2D::load of the four files (Field2D.cpp:30-130)
 """
 from __future__ import annotations
 
@@ -145,7 +148,45 @@ def main():
     L.ref_hist_mean_tot.argtypes = [C.c_int, C.c_double, C.c_double, C.c_long, C.c_void_p]
     mt = L.ref_hist_mean_tot(200, 0.0, 1.0, len(vals), dp(vals))
     np.savez(HERE / "misc_ref.npz", mod_x=mx, mod_y=my, mod=mo, hist_vals=vals, hist_mean_tot=mt)
+
+    field_text(L)
     print("golden fixtures written to", HERE)
+
+
+def field_text(L):
+    """Field2D::print and Field2D::load (Field2D.cpp:30-130)."""
+    rng = np.random.default_rng(20241203)
+    # print: values over many decades, a negative zero and integers, a
+    # non-zero origin and a factor
+    nx, nz, dx, dz, x0, z0, fac = 5, 4, 1.25e-3, 7.5e-4, -2.5e-3, 1e-3, 2.5
+    data = rng.normal(size=nx * nz) * 10.0 ** rng.integers(-9, 10, nx * nz)
+    data[3], data[7], data[11] = -0.0, 4.0, 123456789.0
+    out = HERE / "field2d_print_ref.txt"
+    L.ref_f2_print(nx, nz, C.c_double(dx), C.c_double(dz), C.c_double(x0), C.c_double(z0), dp(data),
+                   C.c_double(fac), str(out).encode())
+    # load: a 6 x 5 grid written x-descending (a), z-major (b), with a node
+    # missing (c) and with one node twice and another missing (d)
+    mx, mz, hx, hz, ox, oz = 6, 5, 2e-4, 3e-4, 1e-3, -6e-4
+    vals = rng.normal(size=(mx, mz))
+    rows = {(i, j): f"{ox + i * hx:.6g}\t{oz + j * hz:.6g}\t{vals[i, j]:.17g}" for i in range(mx) for j in range(mz)}
+    a = [rows[i, j] for i in reversed(range(mx)) for j in range(mz)]
+    b = [rows[i, j] for j in range(mz) for i in range(mx)]
+    c = a[:7] + a[8:]
+    d = a[:7] + [a[9]] + a[8:]
+    meta = np.zeros((4, 6))
+    got = np.zeros((4, mx * mz))
+    rc = np.zeros(4, np.int32)
+    L.ref_f2_load.restype = C.c_int
+    for k, (name, lines) in enumerate(zip("abcd", (a, b, c, d))):
+        f = HERE / f"field2d_load_{name}.txt"
+        f.write_text("# x z value\n" + "\n".join(lines) + "\n\n")
+        m = np.zeros(6)
+        o = np.zeros(mx * mz)
+        rc[k] = L.ref_f2_load(str(f).encode(), dp(m), dp(o), C.c_long(mx * mz))
+        meta[k], got[k] = m, o
+    np.savez(HERE / "field2d_text_ref.npz", print_shape=np.array([nx, nz]),
+             print_grid=np.array([dx, dz, x0, z0]), print_factor=fac, print_data=data, load_rc=rc,
+             load_meta=meta, load_data=got)
 
 
 if __name__ == "__main__":
