@@ -373,9 +373,9 @@ int m2d_count_live_2d(m2d_ctx *ctx, const m2d_particles2d *p, int64_t *n_live);
  * the free list of BaseSpecies::remove/insert (particles.hpp:223-247). */
 int m2d_compact_2d(m2d_ctx *ctx, const m2d_particles2d *src, const m2d_particles2d *dst,
                    int64_t *n_live);
-/* Rebuild the store in tile-major cell order (8 x 8-cell tiles, row-major
- * cells inside a tile; equal cells keep their slot order) into dst, empty
- * slots dropped to the end; *n_live (may be NULL) gets the live count.
+/* Rebuild the store in tile order (8 x 8-cell tiles, row-major; inside a
+ * tile the particles keep their slot order) into dst, empty slots dropped
+ * to the end; *n_live (may be NULL) gets the live count.
  * Replaces the insertion-order vector<t_particle> with holes
  * (particles.hpp:223-249); the fused pass is fastest on a recently sorted
  * store.  Slot order keys the MCC random streams, so sorting changes which
@@ -540,7 +540,7 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
 /* leapfrog start of the 3D mover (the 2D advance_boris_init, particles.cpp:997-1050) */
 int m2d_advance_init_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s,
                         const m2d_particles3d *p);
-/* stable re-order by cell into dst (same slot count), empty slots dropped to the end */
+/* stable re-order by tile (4 x 4 x 4 cells) into dst (same slot count), empty slots dropped to the end */
 int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const m2d_particles3d *dst,
                 int64_t *n_live);
 /* drop the tile schedule of a 3D store written by other means than m2d_sort_3d */

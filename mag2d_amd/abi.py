@@ -626,7 +626,8 @@ def sort(ctx, fields, src: ParticlesSoA, dst: ParticlesSoA, want_count=True):
 
 
 def cell_keys(x, z, nx, nz, dx, dz, xmin=0.0, zmin=0.0):
-    """Host copy of the sort key (tile-major cells, 8 x 8 tiles) for tests."""
+    """Host copy of the sort key (tile << 6 | cell in tile, 8 x 8 tiles) for tests;
+    the sort orders by the tile bits (key >> 6) only."""
     X = (np.asarray(x, np.float64) - xmin) * (1.0 / dx)
     Z = (np.asarray(z, np.float64) - zmin) * (1.0 / dz)
     i = np.where(X >= 0, np.minimum(X, nx - 2), 0).astype(np.int64)

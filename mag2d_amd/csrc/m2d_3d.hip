@@ -1654,8 +1654,10 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
     M2D_CHECK(bits <= 32, M2D_ERR_BAD_ARG, "m2d_sort_3d: grid too large for 32-bit keys");
     const uint32_t dead_key = bits == 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
     size_t temp = 0;
+    // tile order, slot order inside a tile (the tile bits of the key only; see
+    // m2d_sort.hip for why cells are not ordered)
     hipError_t e = rocprim::radix_sort_pairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, bits,
+                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 6, bits,
                                              ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs size query: %s", hipGetErrorString(e));
     const size_t arr = ((n * 4 + 255) / 256) * 256;
@@ -1668,7 +1670,7 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
     k_sort_keys3d<<<blocks_for(n), 256, 0, ctx->stream>>>(src->x, src->y, src->z, src->n, g, nty3, ntz3, dead_key,
                                                           k0, v0);
     M2D_HIP(hipGetLastError());
-    e = rocprim::radix_sort_pairs(b + 4 * arr, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
+    e = rocprim::radix_sort_pairs(b + 4 * arr, temp, k0, k1, v0, v1, n, 6, bits, ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather3d<<<xcd_grid(blocks_for(n)), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, dead_key, live);
     M2D_HIP(hipGetLastError());

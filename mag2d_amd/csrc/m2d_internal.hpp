@@ -98,7 +98,7 @@ struct Scratch {
 
 }  // namespace m2d
 
-// Tile schedule of a cell-sorted store (built by m2d_sort_2d): the live
+// Tile schedule of a tile-sorted store (built by m2d_sort_2d): the live
 // particles of sort tile t are slots [tile_start[t], tile_end[t]); chunk c
 // of the tiled pass covers sched[c] = (tile, start, end), at most kChunkCap
 // (2D) or kChunkCap3 (3D) particles of one tile.  Valid for the store at `x` with `n` slots until

@@ -1,14 +1,17 @@
-// m2d_sort.hip -- stable reordering of a species' particles by cell, and the
+// m2d_sort.hip -- stable reordering of a species' particles by tile, and the
 // tile schedule of the sorted store.
 //
 // The reference keeps particles in insertion order with holes
-// (particles.hpp:223-249).  Here the store is periodically rebuilt in
-// tile-major cell order (8 x 8-cell tiles, row-major cells inside a tile),
-// dropping empty slots: the E gather of neighbouring threads then reads
-// neighbouring table entries, and each block of the particle pass deposits
-// into one small LDS window.  The order among equal keys is the old slot
-// order, so the result is deterministic.  The key/slot pairs are sorted with
-// rocPRIM's LSD radix sort.
+// (particles.hpp:223-249).  Here the store is periodically rebuilt in tile
+// order (8 x 8-cell tiles, row-major), dropping empty slots: each block of
+// the particle pass then gathers from and deposits into one small window of
+// the grid, held in LDS.  Inside a tile the particles keep their old slot
+// order, so the result is deterministic.  They are deliberately not ordered
+// by cell: the 64 lanes of a wave would then share one or two cells, and
+// their LDS atomics on the same four nodes would serialise (a cell-ordered
+// store runs the pass about 1.5x slower until the particles have mixed).
+// The key holds the cell too (tile << 6 | cell) and the radix sort ranks the
+// tile bits only (rocPRIM's LSD radix sort of the key/slot pairs).
 #include <cstring>
 #include <new>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -247,7 +250,7 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     }
     size_t temp = 0;
     hipError_t e = rocprim::radix_sort_pairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, bits,
+                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, kTileShift, bits,
                                              ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs size query: %s", hipGetErrorString(e));
     const size_t arr = ((n * 4 + 255) / 256) * 256;
@@ -261,7 +264,7 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     k_sort_keys<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(src->x, src->z, src->xl, src->zl, src->n, g,
                                                                       k0, v0);
     M2D_HIP(hipGetLastError());
-    e = rocprim::radix_sort_pairs(tmp, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
+    e = rocprim::radix_sort_pairs(tmp, temp, k0, k1, v0, v1, n, kTileShift, bits, ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather<<<xcd_grid((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key,
                                                                               live);

@@ -346,7 +346,7 @@ public:
     void lifetime_init(const std::vector<Species *> &speclist);
     // energy_dist.mean_tot() (histogram.cpp:68-71): mean kinetic energy in eV
     double mean_energy();
-    // re-order the device store by cell (drops holes; host mirror follows).
+    // re-order the device store by tile (drops holes; host mirror follows).
     // Boris only: the multicoll mover keeps time_to_death per slot.
     void sort();
     // BaseSpecies::load (particles.cpp:57-93): the checkpoint of save(); a

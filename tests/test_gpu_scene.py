@@ -24,15 +24,15 @@ BENCH_PASS = abi.PASS_DEPOSIT | abi.PASS_MCC | abi.PASS_TILED | abi.PASS_FAST | 
 
 
 def _apply_sort(op, x, z, alive, sc):
-    """The device sort's permutation (stable by cell key of the device
+    """The device sort's permutation (stable by the tile of the device
     positions, empty slots dropped to the end) applied to the oracle store."""
     idx = np.flatnonzero(alive)
     keys = abi.cell_keys(x[idx], z[idx], sc.n, sc.n, sc.g.dx, sc.g.dz)
-    order = idx[np.argsort(keys, kind="stable")]
+    perm = np.argsort(keys >> 6, kind="stable")
     out = O.particles(len(op))
-    out[: len(order)] = op[order]
-    out["empty"][len(order):] = 1
-    return out, keys[np.argsort(keys, kind="stable")]
+    out[: len(idx)] = op[idx[perm]]
+    out["empty"][len(idx):] = 1
+    return out, keys[perm]
 
 
 def _oracle_rho(sc, d, alive, q):
