@@ -419,6 +419,8 @@ typedef struct {
     const int32_t *group;        /* host, nx*nz: group of each FIXED node, -1 for none */
     double adjoint_tol;          /* FP64 relative residual of the adjoint solves; 0: 1e-10, the
                                     forward solves' (the forward run always solves to 1e-10) */
+    m2d_mcc *mcc;                /* NULL: no collisions; else the species' MCC (continuum targets
+                                    only), stepped with the species' lifetime */
 } m2d_adjoint_cfg;
 
 typedef struct {
@@ -429,14 +431,18 @@ typedef struct {
 
 /* J = energy_dist.mean_tot() (histogram.cpp:68-71) after cfg->steps iterations
  * of Pic::advance with a solve every step (pic.cpp:330-358; one species, no
- * collisions, no RF), starting from the store p already started by
- * advance_boris_init, whose charge is deposited first (pic.cpp:362-375).
+ * RF), starting from the store p already started by advance_boris_init,
+ * whose charge is deposited first (pic.cpp:362-375).  With cfg->mcc the
+ * steps collide as advance_boris does (particles.cpp:937-993,208-365).
  * grad (may be NULL: forward only) receives dJ/dV_g for each group of FIXED
  * nodes whose voltages move together, by reverse-mode differentiation of
  * the push, the deposit and the Poisson solve (self-adjoint on the free
- * nodes); particles removed by the boundary are frozen out.  The store,
- * Fields::u and rho end as after a plain forward run.  No counterpart in the
- * reference, which has no derivatives. */
+ * nodes); particles removed by the boundary are frozen out.  Collisions are
+ * differentiated under common random numbers: the same trials fire and each
+ * keeps its draws and its interaction, and the scatter's Jacobian in the
+ * particle's velocity enters the reverse sweep.  The store, Fields::u and
+ * rho end as after a plain forward run.  No counterpart in the reference,
+ * which has no derivatives. */
 int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const m2d_particles2d *p,
                           const m2d_adjoint_cfg *cfg, double *J, double *grad, m2d_adjoint_info *info);
 

@@ -99,7 +99,7 @@ class StepStats(C.Structure):
 
 class AdjointCfg(C.Structure):
     _fields_ = [("steps", C.c_int32), ("checkpoint_every", C.c_int32), ("n_groups", C.c_int32),
-                ("group", C.POINTER(C.c_int32)), ("adjoint_tol", C.c_double)]
+                ("group", C.POINTER(C.c_int32)), ("adjoint_tol", C.c_double), ("mcc", C.c_void_p)]
 
 
 class AdjointInfo(C.Structure):
@@ -837,13 +837,15 @@ def grad_3d(ctx, g, data_ptr, x, y, z):
 # --------------------------------------------------------------------- adjoint
 
 def adjoint_energy(ctx, fields: Fields2D, species: Species2D, parts: ParticlesSoA, steps, group=None,
-                   n_groups=0, checkpoint_every=100, adjoint_tol=0.0):
+                   n_groups=0, checkpoint_every=100, adjoint_tol=0.0, mcc=None):
     """J (mean kinetic energy in eV after `steps` self-consistent steps) and,
     when `group` (nx*nz int array, -1 for none) is given, dJ/dV per group;
-    adjoint_tol: relative residual of the adjoint solves (0: 1e-10)."""
+    adjoint_tol: relative residual of the adjoint solves (0: 1e-10); mcc: the
+    species' collisions (its lifetime is the species')."""
     g = None if group is None else np.ascontiguousarray(group, np.int32)
     cfg = AdjointCfg(steps, checkpoint_every, n_groups,
-                     g.ctypes.data_as(C.POINTER(C.c_int32)) if g is not None else None, adjoint_tol)
+                     g.ctypes.data_as(C.POINTER(C.c_int32)) if g is not None else None, adjoint_tol,
+                     mcc.h.value if mcc is not None else None)
     J = C.c_double()
     grad = np.zeros(max(n_groups, 1))
     info = AdjointInfo()

@@ -1,18 +1,30 @@
 // m2d_adjoint.hip -- reverse-mode gradient of the self-consistent 2D step
-// (Pic::advance with a solve every step, pic.cpp:330-358, one species, no
-// collisions) with respect to the voltages of groups of FIXED electrode
-// nodes, for the objective J = mean kinetic energy in eV of the live
-// particles after the last step (energy_dist.mean_tot, histogram.cpp:68-71).
+// (Pic::advance with a solve every step, pic.cpp:330-358, one species,
+// optionally with MCC collisions against continuum targets) with respect to
+// the voltages of groups of FIXED electrode nodes, for the objective J =
+// mean kinetic energy in eV of the live particles after the last step
+// (energy_dist.mean_tot, histogram.cpp:68-71).
 //
 // Forward step n (the library's own kernels):
 //   u^n = A^-1 b(rho^n, V)            Fields::boundary_solve (fields.cpp:278-312)
 //   a = (q dt / 2m) E(x^n; u^n)       E = -(staggered gradient gather)
 //   v^{n+1/2} = M (v^{n-1/2} + a) + a Boris with the constant rotation M
 //   x^{n+1} = x^n + v^{n+1/2} dt      (x and z)
+//   v^{n+1/2} <- S(v^{n+1/2}) if the slot's MCC trial fired (after the drift)
 //   boundary; rho^{n+1} = sum q W(x^{n+1})  CIC
+// Collisions are differentiated under common random numbers: the trial
+// word, the target's velocity and the scattering angles are functions of
+// (seed, stream, slot, step) only, so a run at other voltages draws the same
+// numbers; which collisions fire does not depend on the voltages (the trial
+// probability is fixed by the lifetime), and the interaction picked is taken
+// as fixed (it is a step function of the relative speed, whose derivative
+// is zero almost everywhere).  S's Jacobian comes from the device scatter
+// itself evaluated on forward-mode dual numbers.
 // Reverse step n, for the particles alive after the step (removals are
 // frozen):
-//   lx += q grad(interp lrho^{n+1})(x^{n+1});  lv += dt lx
+//   lx += q grad(interp lrho^{n+1})(x^{n+1})
+//   lv <- S'(v)^T lv for a slot whose trial fired (v: the velocity before
+//   the scatter, recomputed in FP64 from v^{n-1/2} and u^n);  lv += dt lx
 //   la = lv + M^T lv;  lv <- M^T lv
 //   lu += -(q dt/2m) G(x^n)^T la;  lx += -(q dt/2m) (dE/dx)^T la
 //   A_FF y = lu_F (same multigrid solver, zero Dirichlet values)
@@ -22,7 +34,7 @@
 // The forward stores u^n every step and the particle store every K steps;
 // the reverse sweep replays the pushes of one K-step segment at a time from
 // its checkpoint (bit for bit: same kernels, same u^n) to get x^n.
-#include "m2d_objects.hpp"
+#include "m2d_scatter.hpp"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -44,6 +56,11 @@ struct AdjArgs {
     double q, dt, c;             // charge, dt, q dt / 2m
     double M[9];                 // Boris rotation, row-major: v' = M v
     unsigned long long *counters;
+    // MCC (mcc non-null): v^{n-1/2} before the step and the draw keys
+    const float *v0x, *v0y, *v0z;
+    const McDev *mcc;
+    uint64_t thr, seed, step;    // trial threshold; Philox key; niter of this step
+    uint32_t stream;
 };
 
 __device__ __forceinline__ double uat(const double *u, int nz, int i, int j) { return u[(size_t)i * nz + j]; }
@@ -58,6 +75,76 @@ struct LuAdd {
 };
 
 constexpr int kAdjWin = 2304;  // doubles of the block's LDS window (18 KiB)
+
+// Fields::E at x^n in FP64 from u^n through the staggered edge gradients,
+// with the pieces its adjoint needs.  Returns true when the point lies
+// outside the grid (it is clamped onto it, as the forward gather does).
+struct AdjGather {
+    int jx, i0, er[2], ec[2];
+    double fx, fyx, fxz, fz;
+    double g[2][2], h[2][2];  // x and z edge gradients
+    double Ex, Ez;
+};
+
+__device__ __forceinline__ bool adj_gather(const AdjArgs &a, float xa, float za, AdjGather &G)
+{
+    const int nx = a.nx, nz = a.nz;
+    double X = ((double)xa - a.xmin) * a.idx, Z = ((double)za - a.zmin) * a.idz;
+    const bool bad = !(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5);
+    if (bad) {
+        X = fmin(fmax(X, 0.0), (double)(nx - 1));
+        Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
+    }
+    const int ih = max(0, min((int)(X + 0.5), nx - 1)), jx = max(0, min((int)Z, nz - 2));
+    const int i0 = max(0, min((int)X, nx - 2)), jh = max(0, min((int)(Z + 0.5), nz - 1));
+    G.jx = jx;
+    G.i0 = i0;
+    G.fx = X - ih + 0.5; G.fyx = Z - jx; G.fxz = X - i0; G.fz = Z - jh + 0.5;
+    // x component: edge e between rows ee-1 and ee, ee = clamp(e, 1, nx-1)
+    for (int c = 0; c < 2; c++) G.er[c] = min(max(ih + c, 1), nx - 1);
+    for (int c = 0; c < 2; c++)
+        for (int d = 0; d < 2; d++)
+            G.g[c][d] = (uat(a.u, nz, G.er[c], jx + d) - uat(a.u, nz, G.er[c] - 1, jx + d)) * a.idx;
+    // z component: edge e between columns ee-1 and ee, ee = clamp(e, 1, nz-1)
+    for (int d = 0; d < 2; d++) G.ec[d] = min(max(jh + d, 1), nz - 1);
+    for (int c = 0; c < 2; c++)
+        for (int d = 0; d < 2; d++)
+            G.h[c][d] = (uat(a.u, nz, i0 + c, G.ec[d]) - uat(a.u, nz, i0 + c, G.ec[d] - 1)) * a.idz;
+    const double fx = G.fx, fyx = G.fyx, fxz = G.fxz, fz = G.fz;
+    G.Ex = -((1 - fx) * ((1 - fyx) * G.g[0][0] + fyx * G.g[0][1]) + fx * ((1 - fyx) * G.g[1][0] + fyx * G.g[1][1]));
+    G.Ez = -((1 - fxz) * ((1 - fz) * G.h[0][0] + fz * G.h[0][1]) + fxz * ((1 - fz) * G.h[1][0] + fz * G.h[1][1]));
+    return bad;
+}
+
+// Adjoint of the MCC scatter of step n, before k_adj_step2d of the same
+// step: for each slot alive after the step whose trial fired, lv <- S'^T lv
+// with S' the Jacobian of BaseSpecies::scatter at the velocity it scattered
+// (the Boris push of v^{n-1/2} with E(x^n; u^n), in FP64) for the slot's own
+// draws.
+__global__ void __launch_bounds__(256) k_adj_scatter2d(AdjArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const float xa = a.x0[p];
+    if (is_dead(xa) || is_dead(a.x1[p])) return;  // not alive, or removed by the step: no scatter
+    if ((uint64_t)trial_word(a.seed, a.stream, (uint64_t)p, a.step) >= a.thr) return;
+    AdjGather G;
+    (void)adj_gather(a, xa, a.z0[p], G);
+    // v = M (v^{n-1/2} + c E) + c E
+    const double ax = a.c * G.Ex, az = a.c * G.Ez;
+    const double w0 = (double)a.v0x[p] + ax, w1 = (double)a.v0y[p], w2 = (double)a.v0z[p] + az;
+    const double *M = a.M;
+    Dual3 vx = mk(M[0] * w0 + M[1] * w1 + M[2] * w2 + ax, 1.0, 0.0, 0.0);
+    Dual3 vy = mk(M[3] * w0 + M[4] * w1 + M[5] * w2, 0.0, 1.0, 0.0);
+    Dual3 vz = mk(M[6] * w0 + M[7] * w1 + M[8] * w2 + az, 0.0, 0.0, 1.0);
+    Rng rg;
+    rg.begin(a.seed, a.stream, (uint32_t)p, a.step, kPurposeMcc);
+    scatter_d(*a.mcc, rg, vx, vy, vz, a.counters);
+    const double l0 = a.lvx[p], l1 = a.lvy[p], l2 = a.lvz[p];
+    a.lvx[p] = l0 * vx.d[0] + l1 * vy.d[0] + l2 * vz.d[0];
+    a.lvy[p] = l0 * vx.d[1] + l1 * vy.d[1] + l2 * vz.d[1];
+    a.lvz[p] = l0 * vx.d[2] + l1 * vy.d[2] + l2 * vz.d[2];
+}
 
 // one reverse step for every particle (see the header comment); the gather
 // transpose is summed in an LDS window over the block's node bounding box
@@ -104,29 +191,13 @@ __global__ void __launch_bounds__(256) k_adj_step2d(AdjArgs a)
             lvx = mtx; lvy = mty; lvz = mtz;
             // ---- gather adjoint at x^n: a = c E, E = -(staggered gradient of u)
             const double lEx = a.c * lax, lEz = a.c * laz;
-            double X = ((double)xa - a.xmin) * a.idx, Z = ((double)a.z0[p] - a.zmin) * a.idz;
-            if (!(X > -1.0 && X < nx - 0.5 && Z > -1.0 && Z < nz - 0.5)) {
-                raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
-                X = fmin(fmax(X, 0.0), (double)(nx - 1));
-                Z = fmin(fmax(Z, 0.0), (double)(nz - 1));
-            }
-            const int ih = max(0, min((int)(X + 0.5), nx - 1)), jx = max(0, min((int)Z, nz - 2));
-            const int i0 = max(0, min((int)X, nx - 2)), jh = max(0, min((int)(Z + 0.5), nz - 1));
-            const double fx = X - ih + 0.5, fyx = Z - jx, fxz = X - i0, fz = Z - jh + 0.5;
-            // x component: edge e between rows ee-1 and ee, ee = clamp(e, 1, nx-1)
-            double g[2][2];
-            int er[2];
-            for (int c = 0; c < 2; c++) er[c] = min(max(ih + c, 1), nx - 1);
-            for (int c = 0; c < 2; c++)
-                for (int d = 0; d < 2; d++)
-                    g[c][d] = (uat(a.u, nz, er[c], jx + d) - uat(a.u, nz, er[c] - 1, jx + d)) * a.idx;
-            // z component: edge e between columns ee-1 and ee, ee = clamp(e, 1, nz-1)
-            double h[2][2];
-            int ec[2];
-            for (int d = 0; d < 2; d++) ec[d] = min(max(jh + d, 1), nz - 1);
-            for (int c = 0; c < 2; c++)
-                for (int d = 0; d < 2; d++)
-                    h[c][d] = (uat(a.u, nz, i0 + c, ec[d]) - uat(a.u, nz, i0 + c, ec[d] - 1)) * a.idz;
+            AdjGather G;
+            if (adj_gather(a, xa, a.z0[p], G)) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
+            const int jx = G.jx, i0 = G.i0;
+            const int *er = G.er, *ec = G.ec;
+            const double fx = G.fx, fyx = G.fyx, fxz = G.fxz, fz = G.fz;
+            const double(&g)[2][2] = G.g;
+            const double(&h)[2][2] = G.h;
             // E_x = -sum wx_cd g_cd, E_z = -sum wz_cd h_cd
             const double wx[2][2] = {{(1 - fx) * (1 - fyx), (1 - fx) * fyx}, {fx * (1 - fyx), fx * fyx}};
             const double wz[2][2] = {{(1 - fxz) * (1 - fz), (1 - fxz) * fz}, {fxz * (1 - fz), fxz * fz}};
@@ -297,6 +368,16 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
               "m2d_adjoint_energy_2d: empty particle store");
     M2D_CHECK(cfg->adjoint_tol >= 0 && cfg->adjoint_tol < 1, M2D_ERR_BAD_ARG,
               "m2d_adjoint_energy_2d: adjoint_tol %g out of [0, 1)", cfg->adjoint_tol);
+    m2d_mcc *mcc = cfg->mcc;
+    if (mcc) {
+        M2D_CHECK(mcc->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_adjoint_energy_2d: MCC tables belong to another context");
+        M2D_CHECK(s->lifetime > 0 && std::isfinite(s->lifetime), M2D_ERR_BAD_ARG,
+                  "m2d_adjoint_energy_2d: MCC needs the species' finite lifetime");
+        for (int t = 0; t < mcc->h.n_targets; t++)
+            M2D_CHECK(mcc->h.t[t].px == nullptr, M2D_ERR_UNSUPPORTED,
+                      "m2d_adjoint_energy_2d: targets with a live partner ensemble are not differentiated");
+        M2D_TRY(mcc_sync(mcc));
+    }
     const double adj_tol = cfg->adjoint_tol > 0 ? cfg->adjoint_tol : 1e-10;
     const int N = cfg->steps, K = cfg->checkpoint_every;
     const size_t nc = f->n, np = (size_t)p->n;
@@ -310,7 +391,7 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
         }
     }
     m2d_species2d sp = *s;
-    sp.lifetime = INFINITY;
+    if (!mcc) sp.lifetime = INFINITY;
     const uint32_t fl_fwd = M2D_STEP_PUSH | M2D_STEP_BOUNDARY | M2D_STEP_DEPOSIT;
     // ---- forward: rho^0, then solve / reset / step, storing u^n and checkpoints
     const auto t0 = clock::now();
@@ -335,7 +416,7 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
                                    hipMemcpyDeviceToDevice, st));
         M2D_TRY(m2d_fields2d_reset(f));
         sp.niter = s->niter + (uint64_t)n;
-        M2D_TRY(m2d_advance_2d(ctx, f, &sp, p, nullptr, f->rho, fl_fwd, nullptr));
+        M2D_TRY(m2d_advance_2d(ctx, f, &sp, p, mcc, f->rho, fl_fwd, nullptr));
     }
     double mean = 0.0;
     int64_t live = 0;
@@ -355,7 +436,9 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
     M2D_TRY(lam.alloc(sizeof(double) * 5 * np));
     M2D_TRY(lu.alloc(sizeof(double) * nc));
     M2D_TRY(lrho.alloc(sizeof(double) * nc));
-    M2D_TRY(seg.alloc(sizeof(float) * 2 * np * (size_t)(K + 1)));
+    // per replayed step: x, z, and with MCC the velocities vx, vy, vz
+    const int SC = mcc ? 5 : 2;
+    M2D_TRY(seg.alloc(sizeof(float) * SC * np * (size_t)(K + 1)));
     M2D_TRY(rep.alloc(sizeof(float) * 5 * np));
     const unsigned gb = nblk(nc);
     M2D_TRY(gpart.alloc(sizeof(double) * (size_t)gb * cfg->n_groups));
@@ -402,6 +485,12 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
         }
     }
     a.counters = ctx->counters;
+    if (mcc) {
+        a.mcc = mcc->d;
+        a.thr = trial_threshold(1.0 - exp(-s->dt / s->lifetime));  // the pass's own threshold
+        a.seed = ctx->seed;
+        a.stream = s->stream;
+    }
     const double rscale = -(f->p.grid.dx * f->p.grid.dx) / kEps0 / f->p.dV;  // b_F = rscale * rho
     float *const rarr[5] = {rep.as<float>(), rep.as<float>() + np, rep.as<float>() + 2 * np,
                             rep.as<float>() + 3 * np, rep.as<float>() + 4 * np};
@@ -413,25 +502,36 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
         for (int c = 0; c < 5; c++)
             M2D_HIP(hipMemcpyAsync(rarr[c], ckpt.as<float>() + ((size_t)(s0 / K) * 5 + c) * np, sizeof(float) * np,
                                    hipMemcpyDeviceToDevice, st));
-        M2D_HIP(hipMemcpyAsync(segx, rarr[0], sizeof(float) * np, hipMemcpyDeviceToDevice, st));
-        M2D_HIP(hipMemcpyAsync(segx + np, rarr[1], sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+        // snapshot k of the segment: the components of the store before step s0 + k
+        auto snap = [&](int k) -> int {
+            float *o = segx + (size_t)k * SC * np;
+            for (int c = 0; c < SC; c++)
+                M2D_HIP(hipMemcpyAsync(o + (size_t)c * np, rarr[c], sizeof(float) * np,
+                                       hipMemcpyDeviceToDevice, st));
+            return M2D_OK;
+        };
+        M2D_TRY(snap(0));
         for (int n = s0; n < s1; n++) {
             M2D_HIP(hipMemcpyAsync(f->u, U.as<double>() + (size_t)n * nc, sizeof(double) * nc,
                                    hipMemcpyDeviceToDevice, st));
             M2D_TRY(fields_build_tables(f));
             sp.niter = s->niter + (uint64_t)n;
-            M2D_TRY(m2d_advance_2d(ctx, f, &sp, &rp, nullptr, nullptr, M2D_STEP_PUSH | M2D_STEP_BOUNDARY, nullptr));
-            float *o = segx + (size_t)(n - s0 + 1) * 2 * np;
-            M2D_HIP(hipMemcpyAsync(o, rarr[0], sizeof(float) * np, hipMemcpyDeviceToDevice, st));
-            M2D_HIP(hipMemcpyAsync(o + np, rarr[1], sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+            M2D_TRY(m2d_advance_2d(ctx, f, &sp, &rp, mcc, nullptr, M2D_STEP_PUSH | M2D_STEP_BOUNDARY, nullptr));
+            M2D_TRY(snap(n - s0 + 1));
         }
         // reverse steps of the segment
         for (int n = s1 - 1; n >= s0; n--) {
-            const float *xa = segx + (size_t)(n - s0) * 2 * np, *xb = segx + (size_t)(n - s0 + 1) * 2 * np;
+            const float *xa = segx + (size_t)(n - s0) * SC * np, *xb = segx + (size_t)(n - s0 + 1) * SC * np;
             a.x0 = xa; a.z0 = xa + np;
             a.x1 = xb; a.z1 = xb + np;
             a.u = U.as<double>() + (size_t)n * nc;
             M2D_TRY(fill_bytes(ctx, lu.p, 0, sizeof(double) * nc));
+            if (mcc) {
+                a.v0x = xa + 2 * np; a.v0y = xa + 3 * np; a.v0z = xa + 4 * np;
+                a.step = s->niter + (uint64_t)n;
+                k_adj_scatter2d<<<nblk(np), 256, 0, st>>>(a);
+                M2D_HIP(hipGetLastError());
+            }
             k_adj_step2d<<<nblk(np), 256, 0, st>>>(a);
             M2D_HIP(hipGetLastError());
             // A_FF y = lu_F

@@ -266,6 +266,26 @@ struct Rng {
     __device__ __forceinline__ double exponential() { return -log(uni()); }
 };
 
+// The MCC trial draw is uni() = (w + 1/2) 2^-32 for a 32-bit word w; since
+// scaling by 2^32 is exact, uni() < prob iff w < ceil(prob 2^32 - 1/2).
+inline uint64_t trial_threshold(double prob)
+{
+    if (!(prob > 0.0)) return 0;
+    if (prob >= 1.0) return 1ull << 32;
+    return (uint64_t)ceil(prob * 4294967296.0 - 0.5);
+}
+
+// The trial word of slot p at a step: slots 4m..4m+3 share the Philox block
+// (counter (m, step, 0, purpose 2), key (seed, seed >> 32 ^ stream)), word p & 3
+__device__ __forceinline__ uint32_t trial_word(uint64_t seed, uint32_t stream, uint64_t p, uint64_t step)
+{
+    uint32_t c0 = (uint32_t)(p >> 2), c1 = (uint32_t)step, c2 = 0u;
+    uint32_t c3 = (kPurposeTrial << 24) | ((uint32_t)(step >> 32) & 0xFFFFFFu);
+    philox4x32_10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32) ^ stream);
+    const int w = (int)(p & 3);
+    return w == 0 ? c0 : w == 1 ? c1 : w == 2 ? c2 : c3;
+}
+
 // ---------------------------------------------------------------- long-time positions
 // A store with position remainders (m2d_particles2d::xl, zl) holds each
 // position as an FP32 pair whose exact FP64 sum is the position.

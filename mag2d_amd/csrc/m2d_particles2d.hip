@@ -1116,15 +1116,6 @@ static float f32_at_most(double v)
     return f;
 }
 
-// The trial draw is uni() = (w + 1/2) 2^-32 for a 32-bit word w; since
-// scaling by 2^32 is exact, uni() < prob iff w < ceil(prob 2^32 - 1/2).
-static uint64_t trial_threshold(double prob)
-{
-    if (!(prob > 0.0)) return 0;
-    if (prob >= 1.0) return 1ull << 32;
-    return (uint64_t)ceil(prob * 4294967296.0 - 0.5);
-}
-
 static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
                            const m2d_particles2d *p, double rot_scale, double kick_scale)
 {

@@ -1,10 +1,64 @@
 // m2d_scatter.hpp -- BaseSpecies::scatter (particles.cpp:208-365) on the
-// device, shared by the 2D and 3D particle passes.
+// device, shared by the 2D and 3D particle passes and the adjoint.
 #pragma once
 #include "m2d_objects.hpp"
 #include <cmath>
 
 namespace m2d {
+
+// ------------------------------------------------------------------ derivatives
+// A value with its derivatives with respect to three inputs (forward mode).
+// The scatter below is written once over its velocity type: as double it is
+// the collision itself, as Dual3 seeded with the unit vectors it also gives
+// the 3 x 3 Jacobian of the new velocity with respect to the old one for the
+// same draws (the random numbers, and every branch taken on a value, are the
+// same in both).
+struct Dual3 {
+    double v, d[3];
+};
+__device__ __forceinline__ Dual3 mk(double v, double d0, double d1, double d2) { return Dual3{v, {d0, d1, d2}}; }
+__device__ __forceinline__ Dual3 operator+(Dual3 a, Dual3 b)
+{
+    return mk(a.v + b.v, a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]);
+}
+__device__ __forceinline__ Dual3 operator-(Dual3 a, Dual3 b)
+{
+    return mk(a.v - b.v, a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]);
+}
+__device__ __forceinline__ Dual3 operator-(Dual3 a) { return mk(-a.v, -a.d[0], -a.d[1], -a.d[2]); }
+__device__ __forceinline__ Dual3 operator+(Dual3 a, double b) { return mk(a.v + b, a.d[0], a.d[1], a.d[2]); }
+__device__ __forceinline__ Dual3 operator+(double a, Dual3 b) { return b + a; }
+__device__ __forceinline__ Dual3 operator-(Dual3 a, double b) { return mk(a.v - b, a.d[0], a.d[1], a.d[2]); }
+__device__ __forceinline__ Dual3 operator-(double a, Dual3 b) { return mk(a - b.v, -b.d[0], -b.d[1], -b.d[2]); }
+__device__ __forceinline__ Dual3 operator*(Dual3 a, Dual3 b)
+{
+    return mk(a.v * b.v, a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]);
+}
+__device__ __forceinline__ Dual3 operator*(Dual3 a, double b) { return mk(a.v * b, a.d[0] * b, a.d[1] * b, a.d[2] * b); }
+__device__ __forceinline__ Dual3 operator*(double a, Dual3 b) { return b * a; }
+__device__ __forceinline__ Dual3 operator/(Dual3 a, Dual3 b)
+{
+    const double q = a.v / b.v;
+    return mk(q, (a.d[0] - q * b.d[0]) / b.v, (a.d[1] - q * b.d[1]) / b.v, (a.d[2] - q * b.d[2]) / b.v);
+}
+__device__ __forceinline__ Dual3 operator/(Dual3 a, double b) { return mk(a.v / b, a.d[0] / b, a.d[1] / b, a.d[2] / b); }
+__device__ __forceinline__ Dual3 operator/(double a, Dual3 b)
+{
+    const double q = a / b.v;
+    return mk(q, -q * b.d[0] / b.v, -q * b.d[1] / b.v, -q * b.d[2] / b.v);
+}
+__device__ __forceinline__ double vsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ Dual3 vsqrt(Dual3 a)
+{
+    const double r = sqrt(a.v), h = 0.5 / r;
+    return mk(r, a.d[0] * h, a.d[1] * h, a.d[2] * h);
+}
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(const Dual3 &x) { return x.v; }
+// a constant of the velocity type
+template <class V> __device__ __forceinline__ V vconst(double x);
+template <> __device__ __forceinline__ double vconst<double>(double x) { return x; }
+template <> __device__ __forceinline__ Dual3 vconst<Dual3>(double x) { return mk(x, 0.0, 0.0, 0.0); }
 
 // ------------------------------------------------------------------ MCC
 __device__ __forceinline__ double vec_interp(const double *xs, const double *ys, int n, double x)
@@ -39,7 +93,8 @@ __device__ __forceinline__ double sigma_v(const McDev &m, const McInter &it, dou
     return it.rate;
 }
 
-__device__ __forceinline__ void rot_len(Rng &r, double len, double &x, double &y, double &z)
+template <class V>
+__device__ __forceinline__ void rot_len(Rng &r, V len, V &x, V &y, V &z)
 {
     // t_random::rot(len, x, y, z) (random.cpp:103-115)
     double cs = 1.0 - 2.0 * r.uni();
@@ -63,7 +118,8 @@ __device__ __forceinline__ double ellint_k(double k)
     return M_PI / (a + b);
 }
 
-__device__ inline void deflect(Rng &r, double angle, double &x, double &y, double &z)
+template <class V>
+__device__ inline void deflect(Rng &r, double angle, V &x, V &y, V &z)
 {
     // t_random::deflect (random.cpp:133-175)
     double len = tan(angle / 2.0);
@@ -73,22 +129,23 @@ __device__ inline void deflect(Rng &r, double angle, double &x, double &y, doubl
     double sp, cp;
     sincos(2.0 * M_PI * r.uni(), &sp, &cp);
     double y1 = len * t * sp, z1 = len * t * cp;
-    double tx = y * z1 - z * y1, ty = z * x1 - x * z1, tz = x * y1 - y * x1;
-    t = len / sqrt(tx * tx + ty * ty + tz * tz);
-    tx *= t; ty *= t; tz *= t;
-    double xp = x - y * tz + z * ty, yp = y - z * tx + x * tz, zp = z - x * ty + y * tx;
+    V tx = y * z1 - z * y1, ty = z * x1 - x * z1, tz = x * y1 - y * x1;
+    const V tn = len / vsqrt(tx * tx + ty * ty + tz * tz);
+    tx = tx * tn; ty = ty * tn; tz = tz * tn;
+    V xp = x - y * tz + z * ty, yp = y - z * tx + x * tz, zp = z - x * ty + y * tx;
     t = 2.0 / (1.0 + len * len);
-    x1 = tx * t; y1 = ty * t; z1 = tz * t;
-    x += -yp * z1 + zp * y1;
-    y += -zp * x1 + xp * z1;
-    z += -xp * y1 + yp * x1;
+    const V x2 = tx * t, y2 = ty * t, z2 = tz * t;
+    x = x + (-yp * z2 + zp * y2);
+    y = y + (-zp * x2 + xp * z2);
+    z = z + (-xp * y2 + yp * x2);
 }
 
 // BaseSpecies::scatter (particles.cpp:208-365), FP64, same draw order as
 // the reference: species pick, partner, interaction pick, angles.  The
-// velocity changes only when an interaction fires.
-__device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy, double &vz,
-                                 unsigned long long *counters)
+// velocity changes only when an interaction fires.  V is double, or Dual3
+// for the Jacobian (the picks compare values only).
+template <class V>
+__device__ inline void scatter_d(const McDev &m, Rng &r, V &vx, V &vy, V &vz, unsigned long long *counters)
 {
     double gamma = r.uni() / m.lifetime;
     double tmp = 0.0;
@@ -112,15 +169,15 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
         vr2 = T.pvx[i]; vz2 = T.pvz[i]; vt2 = T.pvy[i];
         partner = i;
     }
-    double dvx = vx - vr2, dvz = vz - vz2, dvy = vy - vt2;
-    double v_rel = sqrt(dvx * dvx + dvz * dvz + dvy * dvy);
+    V dvx = vx - vr2, dvz = vz - vz2, dvy = vy - vt2;
+    V v_rel = vsqrt(dvx * dvx + dvz * dvz + dvy * dvy);
     double m2 = T.mass, mass = m.mass;
 
     gamma = r.uni() * T.rate;
     tmp = 0.0;
     int k;
     for (k = 0; k < T.n; k++) {
-        tmp += sigma_v(m, m.in[T.first + k], v_rel) * T.density;
+        tmp += sigma_v(m, m.in[T.first + k], val(v_rel)) * T.density;
         if (tmp > gamma) break;
     }
     if (k == T.n) return;  // null collision
@@ -128,23 +185,23 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
     tmp = 1.0 / (mass + m2);
     switch (it.type) {
     case M2D_SUPERELASTIC: {
-        double E = 0.5 * it.mu * v_rel * v_rel + it.DE;
-        double vr = sqrt(2 * E / it.mu);
-        double rx, rz, ry;
+        V E = 0.5 * it.mu * v_rel * v_rel + it.DE;
+        V vr = vsqrt(2 * E / it.mu);
+        V rx, rz, ry;
         rot_len(r, vr, rx, rz, ry);
         vx = (rx * m2 + vx * mass + vr2 * m2) * tmp;
-        double nvz = (ry * m2 + vz * mass + vz2 * m2) * tmp;
-        double nvy = (rz * m2 + vy * mass + vt2 * m2) * tmp;
+        V nvz = (ry * m2 + vz * mass + vz2 * m2) * tmp;
+        V nvy = (rz * m2 + vy * mass + vt2 * m2) * tmp;
         vz = nvz; vy = nvy;
         break;
     }
     case M2D_COULOMB:
     case M2D_ELASTIC: {
         // isotropic in the centre of mass (particles.cpp:290-316)
-        double cx = (vx * mass + vr2 * m2) * tmp;
-        double cz = (vz * mass + vz2 * m2) * tmp;
-        double cy = (vy * mass + vt2 * m2) * tmp;
-        double rx, ry, rz;
+        V cx = (vx * mass + vr2 * m2) * tmp;
+        V cz = (vz * mass + vz2 * m2) * tmp;
+        V cy = (vy * mass + vt2 * m2) * tmp;
+        V rx, ry, rz;
         rot_len(r, v_rel, rx, ry, rz);
         vx = rx * m2 * tmp + cx;
         vz = rz * m2 * tmp + cz;
@@ -152,17 +209,17 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
         // COULOMB also turns the partner (particles.cpp:309-314); a partner
         // drawn from the continuum has no state to change
         if (it.type == M2D_COULOMB && partner >= 0) {
-            T.pvx[partner] = (float)(-rx * mass * tmp + cx);
-            T.pvz[partner] = (float)(-rz * mass * tmp + cz);
-            T.pvy[partner] = (float)(-ry * mass * tmp + cy);
+            T.pvx[partner] = (float)val(-rx * mass * tmp + cx);
+            T.pvz[partner] = (float)val(-rz * mass * tmp + cz);
+            T.pvy[partner] = (float)val(-ry * mass * tmp + cy);
         }
         break;
     }
     case M2D_CX:
-        vx = vr2; vz = vz2; vy = vt2;
+        vx = vconst<V>(vr2); vz = vconst<V>(vz2); vy = vconst<V>(vt2);
         break;
     case M2D_LANGEVIN: {
-        double ax = (vx - vr2) * m2 * tmp, ay = (vz - vz2) * m2 * tmp, az = (vy - vt2) * m2 * tmp;
+        V ax = (vx - vr2) * m2 * tmp, ay = (vz - vz2) * m2 * tmp, az = (vy - vt2) * m2 * tmp;
         double beta = sqrt(r.uni()) * it.cutoff;
         if (beta > 1.0) {
             // Nanbu & Kitatani, J. Phys. D 28 (1995) 324 (particles.cpp:337-345)
@@ -171,12 +228,12 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, double &vx, double &vy,
             double theta = ellint_k(xi0 / xi1) * M_SQRT2 * beta / xi1;
             deflect(r, M_PI - 2 * theta, ax, ay, az);
         } else {
-            double len = sqrt(ax * ax + ay * ay + az * az);
+            V len = vsqrt(ax * ax + ay * ay + az * az);
             rot_len(r, len, ax, ay, az);
         }
-        double nvx = ax + (vx * mass + vr2 * m2) * tmp;
-        double nvz = ay + (vz * mass + vz2 * m2) * tmp;
-        double nvy = az + (vy * mass + vt2 * m2) * tmp;
+        V nvx = ax + (vx * mass + vr2 * m2) * tmp;
+        V nvz = ay + (vz * mass + vz2 * m2) * tmp;
+        V nvy = az + (vy * mass + vt2 * m2) * tmp;
         vx = nvx; vz = nvz; vy = nvy;
         break;
     }

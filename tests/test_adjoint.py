@@ -1,6 +1,8 @@
 """Adjoint gradient of the self-consistent 2D run (mean kinetic energy after
 N steps of Pic::advance with a solve every step) with respect to electrode
-voltages, against FP64 central finite differences of the oracle's run."""
+voltages, against FP64 central finite differences of the oracle's run;
+without collisions, and with MCC collisions under common random numbers
+through the torch.autograd wrapper (mag2d_amd.autograd)."""
 import ctypes as C
 import math
 
@@ -37,13 +39,16 @@ def electrons(N, L, rng, T=11604.5, r=5e-3):
     return x, z, v
 
 
-def oracle_J(g_o, mask, volt, parts, dV, steps, Bz, dt):
+def oracle_J(g_o, mask, volt, parts, dV, steps, Bz, dt, mcc=None):
+    """The run in FP64; with mcc, the collisions draw the device's Philox
+    numbers (seed 1234, stream 0), so runs at nearby voltages share them."""
     n = g_o.nx
     u = np.zeros(n * n)
     of = O.make_fields(g_o, mask, u, x_max=(n - 1) * g_o.dx, z_max=(n - 1) * g_o.dz, geometry_empty=1,
                        selfconsistent=1, Bz=Bz)
     p = parts.copy()
-    s = O.make_species(-QE, M_E, dt)
+    s = (O.make_species(-QE, M_E, dt, lifetime=mcc.lifetime, mcc=mcc, seed=1234, stream=0) if mcc is not None
+         else O.make_species(-QE, M_E, dt))
     v = np.ascontiguousarray(volt, np.float64)
     return O.lib().o_selfconsistent_run(C.byref(of), C.byref(s), O.dptr(p), len(p), O.dptr(v), dV, steps)
 
@@ -121,3 +126,68 @@ def test_adjoint_gradient_matches_fp64_finite_differences(ctx, K):
     scale = np.abs(fd).max()
     assert scale > 0
     assert np.abs(grad - fd).max() <= 1e-4 * scale, (grad, fd)
+
+
+M_HE = 4.002602 * 1.66053906660e-27
+
+
+@pytest.mark.gpu
+def test_adjoint_with_collisions_through_autograd_matches_fp64_finite_differences(ctx):
+    """Electrons in a He background: ELASTIC and SUPERELASTIC (0.1 eV) at
+    constant rates, about 1% of the electrons colliding each step (45% at
+    least once over the run), so the interaction each trial picks does not
+    depend on the speed.  J and its gradient come from the autograd wrapper;
+    the oracle's runs at V +- h draw the same numbers, which makes its central
+    differences the derivative the adjoint computes."""
+    import torch
+    from mag2d_amd.autograd import ElectrodeRun, electrode_energy
+
+    n, L, N, steps, dt, Bz = 48, 2e-2, 2000, 60, 1e-11, 0.1
+    g, mask, volt, group, dx = trap(n, L)
+    rng = np.random.default_rng(2)
+    V0 = rng.uniform(-5, 5, 8)
+    x, z, v = electrons(N, L, rng)
+    dV = (N / 1e12) / ((n - 1) ** 2)
+    params = abi.FieldsParams(g, L, L, 1, 1, 0, abi.BC_FREE, 0.0, 0, 0.0, 0.0, 0.0, 0.0, Bz, 0.0, dV)
+    fields = abi.Fields2D(ctx, params, mask, volt)
+    dens, k_el, k_se, DE = 1e21, 7e-13, 3e-13, 0.1
+    mcc = abi.Mcc(ctx, M_E, [abi.Target(M_HE, dens, 300.0, 0.0, abi.Particles2D())],
+                  [abi.Interaction(abi.ELASTIC, 0, 0.0, k_el, 0.0, 0, 0),
+                   abi.Interaction(abi.SUPERELASTIC, 0, DE, k_se, 0.0, 0, 0)], 1.0)
+    omcc = O.make_mcc(M_E, 1.0, [dict(mass=M_HE, density=dens, temperature=300.0,
+                                      inters=[O.inter(O.ELASTIC, M_E, M_HE, rate=k_el),
+                                              O.inter(O.SUPERELASTIC, M_E, M_HE, rate=k_se, DE=DE)])])
+    assert mcc.lifetime == pytest.approx(omcc.lifetime, rel=1e-14)
+    assert 1 - math.exp(-dt / mcc.lifetime) == pytest.approx(0.00995, rel=1e-3)
+    sp = abi.Species2D(-QE, M_E, dt, mcc.lifetime, 0, 0, 0)
+    run = ElectrodeRun(ctx, fields, volt, group, 8, sp, (x, z, v[0], v[1], v[2]), steps, mcc=mcc,
+                       checkpoint_every=7, adjoint_tol=1e-8)
+    V = torch.tensor(V0, dtype=torch.float64, requires_grad=True)
+    J = electrode_energy(V, run)
+    J.backward()
+    assert ctx.status(clear=True) == 0
+    # the wrapper's gradient is the library's (the reverse sweep sums the
+    # gather transpose with FP64 atomics: equal to rounding, not bit for bit)
+    J2, g2 = run.evaluate(V0)
+    assert J2 == J.item()
+    np.testing.assert_allclose(g2, V.grad.numpy(), rtol=1e-7, atol=1e-9 * np.abs(g2).max())
+    # FP64 oracle with the same draws
+    og = O.grid2(n, n, dx, dx)
+    p = O.particles(N)
+    p["x"], p["z"], p["vx"], p["vy"], p["vz"] = x, z, v[0], v[1], v[2]
+    volt0 = volt + np.where(group >= 0, V0[np.maximum(group, 0)], 0.0)
+    J64 = oracle_J(og, mask, volt0, p, dV, steps, Bz, dt, omcc)
+    assert J.item() == pytest.approx(J64, rel=1e-5)
+    h = 1e-4
+    fd, fd0 = np.zeros(8), np.zeros(8)
+    for k in range(8):
+        e = np.where(group == k, h, 0.0)
+        fd[k] = (oracle_J(og, mask, volt0 + e, p, dV, steps, Bz, dt, omcc) -
+                 oracle_J(og, mask, volt0 - e, p, dV, steps, Bz, dt, omcc)) / (2 * h)
+        fd0[k] = (oracle_J(og, mask, volt0 + e, p, dV, steps, Bz, dt) -
+                  oracle_J(og, mask, volt0 - e, p, dV, steps, Bz, dt)) / (2 * h)
+    scale = np.abs(fd).max()
+    assert scale > 0
+    assert np.abs(V.grad.numpy() - fd).max() <= 1e-4 * scale, (V.grad.numpy(), fd)
+    # the collisions matter: without them the gradient is another one
+    assert np.abs(fd - fd0).max() > 0.05 * scale, (fd, fd0)
