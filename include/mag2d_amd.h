@@ -546,7 +546,8 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
 /* leapfrog start of the 3D mover (the 2D advance_boris_init, particles.cpp:997-1050) */
 int m2d_advance_init_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s,
                         const m2d_particles3d *p);
-/* stable re-order by tile (4 x 4 x 4 cells) into dst (same slot count), empty slots dropped to the end */
+/* stable re-order by cell, tile-major (4 x 4 x 4-cell tiles), into dst (same slot count), empty slots
+ * dropped to the end */
 int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const m2d_particles3d *dst,
                 int64_t *n_live);
 /* drop the tile schedule of a 3D store written by other means than m2d_sort_3d */

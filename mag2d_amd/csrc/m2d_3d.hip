@@ -776,13 +776,13 @@ __global__ void __launch_bounds__(256) k_advance3d(PushArgs3 a)
 }
 
 // tiled (store sorted by m2d_sort_3d): block b runs chunk sched[b] of one
-// sort tile and sums its deposits in an LDS window of the tile + halo,
-// written out with plain stores for k_reduce_tiles3; deposits that leave
-// the window go to the global grid
+// sort tile and sums its deposits in an LDS window of the tile + halo, then
+// adds the window's non-zero nodes to the 64-bit grid with atomics (integer
+// sums: the grid does not depend on their order); deposits that leave the
+// window go to the grid directly
 template <bool MCC>
 __global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4 *sched,
-                                                         const unsigned int *n_chunks, int nty3, int ntz3,
-                                                         unsigned long long *partials)
+                                                         const unsigned int *n_chunks, int nty3, int ntz3)
 {
     __shared__ unsigned long long win[kWinN3];
     // plain block order: grouping chunks per XCD (contiguous ranges or runs
@@ -821,8 +821,15 @@ __global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4
     }
     (void)removed_any; (void)coll_any;
     __syncthreads();
-    unsigned long long *out = partials + (size_t)ch * kWinN3;
-    for (int k = threadIdx.x; k < kWinN3; k += 256) out[k] = win[k];
+    for (int k = threadIdx.x; k < kWinN3; k += 256) {
+        const unsigned long long v = win[k];
+        if (!v) continue;
+        // window node (da, db, dc) of the tile's window: only nodes of the
+        // grid receive deposits, so a non-zero entry lies on the grid
+        const int i = wi0 + k / (kWin3 * kWin3), j = wj0 + (k / kWin3) % kWin3, l = wk0 + k % kWin3;
+        if ((unsigned)i < (unsigned)a.g.nx && (unsigned)j < (unsigned)a.g.ny && (unsigned)l < (unsigned)a.g.nz)
+            atomicAdd(a.acc + ((size_t)i * a.g.ny + j) * a.g.nz + l, v);
+    }
     int sum = n_removed;
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
     if (__lane_id() == 0 && sum) atomicAdd(&a.counters[CNT_REMOVED], (unsigned long long)sum);
@@ -833,41 +840,6 @@ __global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4
     }
     if (__ballot(nan_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_NAN);
     if (__ballot(bad_any) && __lane_id() == 0) raise_flag(a.counters, M2D_FLAG_OUT_OF_RANGE);
-}
-
-// node sums of the tiled pass: every node of a 4 x 4 x 4 output tile adds the
-// windows of the chunks of the 27 neighbouring sort tiles that cover it and
-// the global fallback; rho += charge * sum * 2^-31; the global grid is cleared
-__global__ void __launch_bounds__(64) k_reduce_tiles3(const unsigned long long *partials, const int *chunk_first,
-                                                      const int *chunk_count, int ntx3, int nty3, int ntz3,
-                                                      unsigned long long *acc, double *rho, double charge, Grid3 g)
-{
-    const int TI = blockIdx.z, TJ = blockIdx.y, TK = blockIdx.x;
-    const int i = TI * kTile3 + threadIdx.x / 16, j = TJ * kTile3 + (threadIdx.x / 4) % 4,
-              k = TK * kTile3 + threadIdx.x % 4;
-    if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-    const size_t m = ((size_t)i * g.ny + j) * g.nz + k;
-    long long s = (long long)acc[m];
-    for (int ai = max(TI - 1, 0); ai <= min(TI + 1, ntx3 - 1); ai++) {
-        const int da = i - (ai * kTile3 - kHalo3);
-        if (da < 0 || da >= kWin3) continue;
-        for (int aj = max(TJ - 1, 0); aj <= min(TJ + 1, nty3 - 1); aj++) {
-            const int db = j - (aj * kTile3 - kHalo3);
-            if (db < 0 || db >= kWin3) continue;
-            for (int ak = max(TK - 1, 0); ak <= min(TK + 1, ntz3 - 1); ak++) {
-                const int dc = k - (ak * kTile3 - kHalo3);
-                if (dc < 0 || dc >= kWin3) continue;
-                const int t = (ai * nty3 + aj) * ntz3 + ak;
-                const int c0 = chunk_first[t], c1 = c0 + chunk_count[t];
-                const int off = (da * kWin3 + db) * kWin3 + dc;
-                for (int c = c0; c < c1; c++) s += (long long)partials[(size_t)c * kWinN3 + off];
-            }
-        }
-    }
-    if (s) {
-        rho[m] += charge * ((double)s * kInvFix);
-        if (acc[m]) acc[m] = 0ull;
-    }
 }
 
 // BaseSpecies::scatter for the queued slots (velocity only, after the pass)
@@ -1578,15 +1550,10 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
     const int ntx3 = (f->nx - 2) / kTile3 + 1, nty3 = (f->ny - 2) / kTile3 + 1, ntz3 = (f->nz - 2) / kTile3 + 1;
     const m2d_tiling *T = deposit ? find_tiling(ctx, p->x, p->n) : nullptr;
     if (T && (T->ntx != ntx3 || T->nty != nty3 || T->ntz != ntz3)) T = nullptr;
-    unsigned long long *partials = nullptr;
-    if (T) {
-        M2D_TRY(ensure(ctx, ctx->partials, (size_t)T->max_chunks * kWinN3 * sizeof(unsigned long long)));
-        partials = (unsigned long long *)ctx->partials.ptr;
-    }
     if (p->n > 0) {
         if (T) {
             launch_timed(ctx, use_mcc ? k_advance3d_tiled<true> : k_advance3d_tiled<false>, dim3(T->max_chunks),
-                         dim3(256), a, T->sched, T->n_chunks, nty3, ntz3, partials);
+                         dim3(256), a, T->sched, T->n_chunks, nty3, ntz3);
         } else {
             void (*kern)(PushArgs3) = deposit ? (use_mcc ? k_advance3d<true, true> : k_advance3d<true, false>)
                                               : (use_mcc ? k_advance3d<false, true> : k_advance3d<false, false>);
@@ -1599,12 +1566,7 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
             ctx->cq_dirty = false;
         }
     }
-    if (T) {
-        dim3 gr((f->nz + kTile3 - 1) / kTile3, (f->ny + kTile3 - 1) / kTile3, (f->nx + kTile3 - 1) / kTile3);
-        k_reduce_tiles3<<<gr, 64, 0, ctx->stream>>>(partials, T->chunk_first, T->chunk_count, ntx3, nty3, ntz3,
-                                                    a.acc, rho, s->charge, a.g);
-        M2D_HIP(hipGetLastError());
-    } else if (deposit) {
+    if (deposit) {
         k_flush<<<blocks_for(f->n), 256, 0, ctx->stream>>>(a.acc, rho, s->charge, f->n);
         M2D_HIP(hipGetLastError());
     }
@@ -1654,10 +1616,13 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
     M2D_CHECK(bits <= 32, M2D_ERR_BAD_ARG, "m2d_sort_3d: grid too large for 32-bit keys");
     const uint32_t dead_key = bits == 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
     size_t temp = 0;
-    // tile order, slot order inside a tile (the tile bits of the key only; see
-    // m2d_sort.hip for why cells are not ordered)
+    // tile-major cell order.  Unlike the 2D store (ordered by tile only), a
+    // 3D cell holds few particles (8 on average at the 16M-ion, 128^3
+    // workload), so the lanes of a wave spread over several cells anyway, and
+    // the cell order's gather locality wins: the tiled pass runs 419 us on a
+    // cell-ordered store against 505 us on a tile-ordered one
     hipError_t e = rocprim::radix_sort_pairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 6, bits,
+                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, bits,
                                              ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs size query: %s", hipGetErrorString(e));
     const size_t arr = ((n * 4 + 255) / 256) * 256;
@@ -1670,7 +1635,7 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
     k_sort_keys3d<<<blocks_for(n), 256, 0, ctx->stream>>>(src->x, src->y, src->z, src->n, g, nty3, ntz3, dead_key,
                                                           k0, v0);
     M2D_HIP(hipGetLastError());
-    e = rocprim::radix_sort_pairs(b + 4 * arr, temp, k0, k1, v0, v1, n, 6, bits, ctx->stream);
+    e = rocprim::radix_sort_pairs(b + 4 * arr, temp, k0, k1, v0, v1, n, 0, bits, ctx->stream);
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather3d<<<xcd_grid(blocks_for(n)), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, dead_key, live);
     M2D_HIP(hipGetLastError());

@@ -220,7 +220,7 @@ void ctx_release(m2d_ctx *c)
     if (--c->refs > 0) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (Scratch *s : {&c->acc, &c->red, &c->tmp, &c->sortbuf, &c->cq, &c->partials})
+    for (Scratch *s : {&c->acc, &c->red, &c->tmp, &c->sortbuf, &c->cq})
         if (s->ptr) (void)hipFree(s->ptr);
     free_tilings(c);
     for (auto &pr : c->ev_pending) c->ev_pool.insert(c->ev_pool.end(), {pr.first, pr.second});

@@ -130,7 +130,6 @@ struct m2d_ctx {
     m2d::Scratch tmp;       // general temporaries (compaction offsets)
     m2d::Scratch sortbuf;   // keys, slots and radix-sort temporaries
     m2d::Scratch cq;        // MCC collision queue (slot indices)
-    m2d::Scratch partials;  // per-block deposit windows of the tiled 3D pass
     uint64_t launch_seq = 0;  // selects the collision-queue counter of a launch
     bool cq_dirty = false;    // a call failed between appending to a queue counter and its scatter
     uint32_t last_pass = 0;   // M2D_PASS_* bits of the last 2D particle pass launched

@@ -284,7 +284,7 @@ def test_fused_deposit_3d_matches_oracle(ctx):
     assert r.sum() == pytest.approx(-QE * alive.sum(), rel=1e-12)
 
 
-def test_sort_3d_orders_by_tile_stably_and_drops_empty(ctx):
+def test_sort_3d_orders_by_cell_stably_and_drops_empty(ctx):
     sc = Scene3(ctx)
     rng = np.random.default_rng(7)
     N = 30000
@@ -300,7 +300,7 @@ def test_sort_3d_orders_by_tile_stably_and_drops_empty(ctx):
                for c in range(3))
     nt = (sc.n - 2) // 4 + 1  # 4 x 4 x 4-cell tiles, tile-major
     key = ((((i // 4) * nt + j // 4) * nt + k // 4) << 6) | ((i % 4) * 16 + (j % 4) * 4 + k % 4)
-    order = np.argsort(key >> 6, kind="stable")  # tile order, slot order inside a tile
+    order = np.argsort(key, kind="stable")  # tile-major cell order
     d = Q.download()
     np.testing.assert_array_equal(d["x"][:live], pos[0][alive][order])
     np.testing.assert_array_equal(d["vy"][:live], vel[1][alive][order])
