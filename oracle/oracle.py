@@ -7,12 +7,15 @@ never loads it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# M2D_ORACLE_LIB: another build of the same source (the sanitizer build of
+# tests/test_sanitizers.py)
+LIB = Path(os.environ.get("M2D_ORACLE_LIB", HERE / "liboracle.so"))
 REF_LIB = HERE / "_ref" / "libm2d_ref.so"
 REF_LANGEVIN = HERE / "_ref" / "test_langevin"
 

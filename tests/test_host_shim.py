@@ -8,6 +8,7 @@ Pic::advance_init / Pic::advance matches the oracle's restatement of the same
 step ordering (pic.cpp:330-384)."""
 import ctypes as C
 import math
+import os
 from pathlib import Path
 
 import numpy as np
@@ -17,7 +18,9 @@ from mag2d_amd import abi
 from oracle import oracle as O
 
 ROOT = Path(__file__).resolve().parent.parent
-HOST_LIB = ROOT / "mag2d_amd" / "host" / "libmag2d_host.so"
+# M2D_HOST_LIB: another build of the same source (the sanitizer build of
+# tests/test_sanitizers.py)
+HOST_LIB = Path(os.environ.get("M2D_HOST_LIB", ROOT / "mag2d_amd" / "host" / "libmag2d_host.so"))
 
 
 class TrapCfg(C.Structure):
