@@ -310,14 +310,22 @@ __global__ void __launch_bounds__(256) k_adj_groups(const uint8_t *mask, const i
     for (int g = threadIdx.x; g < n_groups; g += blockDim.x) partial[(size_t)blockIdx.x * n_groups + g] = s[g];
 }
 
-// grad[g] += sum over blocks, in block order
-__global__ void k_adj_groups_sum(const double *partial, int nb, int n_groups, double *grad)
+// grad[g] += the sum over blocks of group g's partials, one workgroup per
+// group: thread t sums blocks t, t + 256, ... in order, then a fixed tree
+// over the threads (the same order every run)
+__global__ void __launch_bounds__(256) k_adj_groups_sum(const double *partial, int nb, int n_groups, double *grad)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_groups) return;
-    double s = 0.0;
-    for (int b = 0; b < nb; b++) s += partial[(size_t)b * n_groups + g];
-    grad[g] += s;
+    __shared__ double s[256];
+    const int g = blockIdx.x;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += partial[(size_t)b * n_groups + g];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[g] += s[0];
 }
 
 // device buffer freed at scope exit
@@ -540,7 +548,8 @@ int m2d_adjoint_energy_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
             M2D_TRY(m2d_fields2d_solve(af, M2D_SOLVE_RAW, adj_tol, &si));
             k_adj_groups<<<gb, 256, 0, st>>>(f->mask, dgroup.as<int>(), lu.as<double>(), af->u, f->nx, f->nz,
                                              cfg->n_groups, gpart.as<double>());
-            k_adj_groups_sum<<<1, 64, 0, st>>>(gpart.as<double>(), (int)gb, cfg->n_groups, dgrad.as<double>());
+            k_adj_groups_sum<<<cfg->n_groups, 256, 0, st>>>(gpart.as<double>(), (int)gb, cfg->n_groups,
+                                                             dgrad.as<double>());
             k_adj_lrho<<<gb, 256, 0, st>>>(f->mask, af->u, rscale, lrho.as<double>(), nc);
             M2D_HIP(hipGetLastError());
         }
