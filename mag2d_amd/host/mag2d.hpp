@@ -300,7 +300,8 @@ private:
 
 // ------------------------------------------------------------------ Species
 struct InteractionParams {         // one INTERACTION block (parser.cpp:97-143)
-    int type = M2D_ELASTIC;        // M2D_ELASTIC / LANGEVIN / CX / SUPERELASTIC
+    int type = M2D_ELASTIC;        // M2D_ELASTIC / LANGEVIN / CX / SUPERELASTIC / COULOMB (which
+                                   // needs the species' temperature and density and both charges)
     std::string target;            // species name
     double rate = 0.0;             // RATE (raw; LANGEVIN scales by cutoff^2)
     double DE = 0.0;               // eV

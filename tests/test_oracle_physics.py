@@ -3,6 +3,7 @@ boundary, MCC scatter, Poisson, geometry builders): closed forms, invariants
 and independent library routines, each chosen so that a plausible slip
 (dropped term, sign, swapped component) fails."""
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
@@ -475,3 +476,23 @@ def test_mean_energy_is_mean_kinetic_energy_in_eV_over_live_particles():
     live = p["empty"] == 0
     e = 0.5 * M_ION * (p["vx"] ** 2 + p["vy"] ** 2 + p["vz"] ** 2) / QE
     assert O.lib().o_mean_energy_eV(O.dptr(p), N, M_ION) == pytest.approx(e[live].mean(), rel=1e-12)
+
+
+def test_coulomb_cross_section_from_textbook_plasma_constants():
+    """Interaction::coulomb_sigma (particles.cpp:19-26) is pi L^2 ln(lambda_D / L)
+    with L = q1 q2 / (4 pi eps0 E) and lambda_D = sqrt(eps0 k T / (n q1^2)).
+    Pinned with the textbook values e^2 / (4 pi eps0) = 1.44 eV nm and
+    lambda_D = 7434 m sqrt(T[eV] / n[m^-3]) rather than the formula itself:
+    at E = 1 eV, T = 1 eV, n = 1e16 m^-3 that is L = 1.44e-9 m, lambda_D =
+    7.43e-5 m, sigma = 7.07e-17 m^2; sigma v is the rate, and E^-2 scaling
+    with the logarithm's change follows for E = 4 eV."""
+    m1, m2 = M_ION, 6.69e-27
+    mu = m1 * m2 / (m1 + m2)
+    it = O.inter(O.COULOMB, m1, m2)
+    it.t1, it.n1, it.q1, it.q2 = 11604.5, 1e16, QE, QE   # T = 1 eV in K
+    for E_eV in (1.0, 4.0):
+        v = math.sqrt(2 * E_eV * QE / mu)
+        L = 1.44e-9 / E_eV
+        lam = 7434.0 * math.sqrt(1.0 / 1e16)
+        want = math.pi * L * L * math.log(lam / L)
+        assert O.lib().o_sigma_v(C.byref(it), v) / v == pytest.approx(want, rel=2e-3), E_eV
