@@ -45,6 +45,7 @@ struct PushArgs {
     uint64_t step;
     const McDev *mcc;
     uint32_t *cq;                  // collision queue: slots whose MCC trial fired
+    uint32_t cq_cap;               // its capacity in entries
     unsigned long long *cq_len;    // its length (one of two counters, alternating per launch)
     unsigned long long *acc;       // nx*nz fixed-point deposit grid
     unsigned long long *counters;
@@ -549,14 +550,19 @@ k_advance2d(PushArgs a)
 // BaseSpecies::scatter for the slots queued by k_advance2d, threads
 // [start, start + stride, ...); each slot draws from its own Philox stream,
 // so the queue order does not matter.  Thread 0 clears the other queue
-// counter for the next launch.
+// counter for the next launch.  The scatter is a chain of dependent reads
+// and FP64 arithmetic that sets the flush launch's length, so the thread's
+// first queue entry is read beside the queue length, not after it (the
+// queue holds cq_cap entries: the read stays in bounds, and its value is
+// used only below the length).
 __device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long long start,
                                               unsigned long long stride, unsigned long long *next_len)
 {
+    const uint32_t p0 = start < a.cq_cap ? a.cq[start] : 0u;
     const unsigned long long len = *a.cq_len;
     if (start == 0) *next_len = 0ull;
     for (unsigned long long k = start; k < len; k += stride) {
-        const uint32_t p = a.cq[k];
+        const uint32_t p = k == start ? p0 : a.cq[k];
         if (is_dead(a.x[p])) continue;  // removed by the boundary after its trial
         float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
         Rng rg;
@@ -1154,6 +1160,7 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.trial_thr = 0; a.seed = f->ctx->seed; a.stream = s->stream; a.step = s->niter;
     a.mcc = nullptr;
     a.cq = nullptr;
+    a.cq_cap = 0;
     a.cq_len = nullptr;
     a.sched = nullptr;
     a.n_chunks = nullptr;
@@ -1246,6 +1253,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         a.trial_thr = trial_threshold(1.0 - exp(-s->dt / s->lifetime));  // particles.cpp:937
         a.mcc = mcc->d;
         M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
+        a.cq_cap = (uint32_t)std::min<size_t>(ctx->cq.bytes / sizeof(uint32_t), 0xFFFFFFFFu);
         a.cq = (uint32_t *)ctx->cq.ptr;
         // two queue counters alternate between launches: a launch's scatter
         // kernel clears the one the next launch appends to.  A call that
