@@ -180,9 +180,31 @@ __device__ __forceinline__ void node3_rbgs(const uint8_t *nbm, float *x, const f
     x[m] = s * (1.f / 6.f);
 }
 
+// The six neighbours of node (i, j, k) = m, read beside its mask byte: the
+// indices come from the coordinates, not from the mask (a face's missing
+// neighbour reads the node itself, which the mask never selects), so the
+// loads do not wait for the mask.  Order: -x, -y, -z, +z, +y, +x (mask bits
+// 1, 2, 4, 8, 16, 32).
+struct Nb7 {
+    float v[6];
+};
+__device__ __forceinline__ Nb7 neighbours7(const float *x, size_t m, int i, int j, int k, int nx, int ny, int nz)
+{
+    const size_t sx = (size_t)ny * nz;
+    Nb7 q;
+    q.v[0] = x[i > 0 ? m - sx : m];
+    q.v[1] = x[j > 0 ? m - nz : m];
+    q.v[2] = x[k > 0 ? m - 1 : m];
+    q.v[3] = x[k < nz - 1 ? m + 1 : m];
+    q.v[4] = x[j < ny - 1 ? m + nz : m];
+    q.v[5] = x[i < nx - 1 ? m + sx : m];
+    return q;
+}
+
 // red-black Gauss-Seidel half sweep, colour = (i+j+k) & 1.  Threads cover
 // only the nodes of the colour: thread (kk, j, i) updates k = 2 kk + parity,
-// blocks of 64 x 4 (kk, j) and a grid of (nz/2 / 64, ny / 4, nx).
+// blocks of 64 x 4 (kk, j) and a grid of (nz/2 / 64, ny / 4, nx).  The node
+// body is node3_rbgs's, in the same order, on neighbours read early.
 constexpr int kRbgsX = 64, kRbgsY = 4;
 __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, float *x, const float *b, int nx,
                                                            int ny, int nz, int colour)
@@ -191,7 +213,19 @@ __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, f
     const int i = blockIdx.z, j = blockIdx.y * kRbgsY + threadIdx.y;
     const int k = 2 * (blockIdx.x * kRbgsX + threadIdx.x) + ((i + j + colour) & 1);
     if (j >= ny || k >= nz) return;
-    node3_rbgs(nbm, x, b, (size_t)i * sx + (size_t)j * nz + k, sx, nz);
+    const size_t m = (size_t)i * sx + (size_t)j * nz + k;
+    const uint8_t nb = nbm[m];
+    const float bm = b[m];
+    const Nb7 q = neighbours7(x, m, i, j, k, nx, ny, nz);
+    if (nb & kNbSelf) return;
+    float s = bm;
+    if (nb & 1) s += q.v[0];
+    if (nb & 2) s += q.v[1];
+    if (nb & 4) s += q.v[2];
+    if (nb & 8) s += q.v[3];
+    if (nb & 16) s += q.v[4];
+    if (nb & 32) s += q.v[5];
+    x[m] = s * (1.f / 6.f);
 }
 
 __global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx, int ny, int nz)
@@ -199,8 +233,20 @@ __global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, flo
     const size_t n = (size_t)nx * ny * nz;
     size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
+    int i, j, k;
+    node3(m, ny, nz, i, j, k);
     const uint8_t nb = nbm[m];
-    r[m] = (nb & kNbSelf) ? 0.f : b[m] - apply7(nb, x, m, (size_t)ny * nz, nz);
+    const float bm = b[m], xm = x[m];
+    const Nb7 q = neighbours7(x, m, i, j, k, nx, ny, nz);
+    // apply7's body, in its order, on the neighbours read early
+    float s = 6.f * xm;
+    if (nb & 1) s -= q.v[0];
+    if (nb & 2) s -= q.v[1];
+    if (nb & 4) s -= q.v[2];
+    if (nb & 8) s -= q.v[3];
+    if (nb & 16) s -= q.v[4];
+    if (nb & 32) s -= q.v[5];
+    r[m] = (nb & kNbSelf) ? 0.f : bm - s;
 }
 
 
