@@ -45,7 +45,9 @@ struct PushArgs {
     uint64_t step;
     const McDev *mcc;
     uint32_t *cq;                  // collision queue: slots whose MCC trial fired
-    uint32_t cq_cap;               // its capacity in entries
+    float4 *cqv;                   // the queued slot's velocity after the push (vx, vy, vz, 0)
+    uint32_t cq_cap;               // capacity of both in entries
+    int cq_defer;                  // scatter results go back to cqv; k_scatter_apply stores them
     unsigned long long *cq_len;    // its length (one of two counters, alternating per launch)
     unsigned long long *acc;       // nx*nz fixed-point deposit grid
     unsigned long long *counters;
@@ -392,6 +394,7 @@ k_advance2d(PushArgs a)
                         if ((uint64_t)tw[q] < a.trial_thr) {
                             unsigned long long at = atomicAdd(a.cq_len, 1ull);
                             a.cq[at] = (uint32_t)(pb + q0 + q);
+                            a.cqv[at] = make_float4(vx, vy, vz, 0.f);
                             n_coll++;
                         }
                     }
@@ -551,24 +554,45 @@ k_advance2d(PushArgs a)
 // [start, start + stride, ...); each slot draws from its own Philox stream,
 // so the queue order does not matter.  Thread 0 clears the other queue
 // counter for the next launch.  The scatter is a chain of dependent reads
-// and FP64 arithmetic that sets the flush launch's length, so the thread's
-// first queue entry is read beside the queue length, not after it (the
-// queue holds cq_cap entries: the read stays in bounds, and its value is
-// used only below the length).
+// and FP64 arithmetic that sets the flush launch's length, so the queue
+// carries each slot's velocity (the particle's state is not read first), and
+// the thread's first entry is read beside the queue length, not after it
+// (the queue holds cq_cap entries: the read stays in bounds, and its value
+// is used only below the length).  A slot that the boundary removed after
+// its trial scatters too, as in the reference's loop (the scatter comes
+// before advance_boundary there); nothing reads a removed slot's velocity.
 __device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long long start,
                                               unsigned long long stride, unsigned long long *next_len)
 {
-    const uint32_t p0 = start < a.cq_cap ? a.cq[start] : 0u;
+    const bool in_cap = start < a.cq_cap;
+    const uint32_t p0 = in_cap ? a.cq[start] : 0u;
+    const float4 w0 = in_cap ? a.cqv[start] : make_float4(0.f, 0.f, 0.f, 0.f);
     const unsigned long long len = *a.cq_len;
     if (start == 0) *next_len = 0ull;
     for (unsigned long long k = start; k < len; k += stride) {
         const uint32_t p = k == start ? p0 : a.cq[k];
-        if (is_dead(a.x[p])) continue;  // removed by the boundary after its trial
-        float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+        const float4 w = k == start ? w0 : a.cqv[k];
+        float vx = w.x, vy = w.y, vz = w.z;
         Rng rg;
         rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
         scatter(*a.mcc, rg, vx, vy, vz, a.counters);
-        a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
+        if (a.cq_defer) a.cqv[k] = make_float4(vx, vy, vz, 0.f);
+        else { a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz; }
+    }
+}
+
+// The stores deferred by scatter_queue (cq_defer): when the partners come
+// from the colliding species' own store, every scatter of the step reads
+// its partner's velocity as the push left it, whatever the order in which
+// the threads run, and only then do the new velocities land.
+__global__ void k_scatter_apply(PushArgs a)
+{
+    const unsigned long long len = *a.cq_len;
+    for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < len;
+         k += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint32_t p = a.cq[k];
+        const float4 w = a.cqv[k];
+        a.vx[p] = w.x; a.vy[p] = w.y; a.vz[p] = w.z;
     }
 }
 
@@ -1160,7 +1184,9 @@ static void fill_push_args(PushArgs &a, m2d_fields2d *f, const m2d_species2d *s,
     a.trial_thr = 0; a.seed = f->ctx->seed; a.stream = s->stream; a.step = s->niter;
     a.mcc = nullptr;
     a.cq = nullptr;
+    a.cqv = nullptr;
     a.cq_cap = 0;
+    a.cq_defer = 0;
     a.cq_len = nullptr;
     a.sched = nullptr;
     a.n_chunks = nullptr;
@@ -1252,9 +1278,12 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
     if (use_mcc) {
         a.trial_thr = trial_threshold(1.0 - exp(-s->dt / s->lifetime));  // particles.cpp:937
         a.mcc = mcc->d;
-        M2D_TRY(ensure(ctx, ctx->cq, (size_t)p->n * sizeof(uint32_t)));
-        a.cq_cap = (uint32_t)std::min<size_t>(ctx->cq.bytes / sizeof(uint32_t), 0xFFFFFFFFu);
+        // slots (u32), then velocities (float4) from a 16-byte boundary
+        const size_t qn = (size_t)p->n, vo = (qn * sizeof(uint32_t) + 15) & ~(size_t)15;
+        M2D_TRY(ensure(ctx, ctx->cq, vo + qn * sizeof(float4)));
+        a.cq_cap = (uint32_t)std::min<size_t>(qn, 0xFFFFFFFFu);
         a.cq = (uint32_t *)ctx->cq.ptr;
+        a.cqv = (float4 *)((uint8_t *)ctx->cq.ptr + vo);
         // two queue counters alternate between launches: a launch's scatter
         // kernel clears the one the next launch appends to.  A call that
         // failed between its pass and its scatter left them unknown: clear
@@ -1322,6 +1351,14 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         // an interaction that turns its partner (COULOMB) scatters the queue
         // in slot order after the flush, by one thread
         const bool serial = use_mcc && mcc->h.partner_update;
+        // a target whose partner ensemble is this very store would read
+        // velocities that other collisions of the step are writing: the
+        // scatter then keeps its results in the queue, and k_scatter_apply
+        // stores them once every collision has read its partner
+        bool self_target = false;
+        if (use_mcc && !serial)
+            for (int t = 0; t < mcc->h.n_targets; t++) self_target |= mcc->h.t[t].px == p->x;
+        a.cq_defer = self_target ? 1 : 0;
         const unsigned g = nfl + ntab + (use_mcc && !serial ? kScatterBlocks : 0u);
         // without a deposit no flush block runs: n = 0 ends them at once
         const size_t nodes = deposit ? f->n : 0;
@@ -1331,6 +1368,10 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
             k_flush_scatter<false><<<g, kBlock, 0, ctx->stream>>>(a.acc, rho, s->charge, nodes, pb, ntab, a,
                                                                   next_len);
         M2D_HIP(hipGetLastError());
+        if (a.cq_defer) {
+            k_scatter_apply<<<kScatterBlocks, kBlock, 0, ctx->stream>>>(a);
+            M2D_HIP(hipGetLastError());
+        }
         if (build_ph) {
             f->ph_valid = true;
             f->ph_phase = pb.ph;

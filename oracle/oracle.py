@@ -121,6 +121,7 @@ _SIGS = {
                                   C.c_uint32]),
     "o_rng_u32": (C.c_uint32, [C.POINTER(Rng)]),
     "o_rng_uni": (C.c_double, [C.POINTER(Rng)]),
+    "o_trial_uniform": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64]),
     "o_rng_normal": (C.c_double, [C.POINTER(Rng)]),
     "o_rot_len": (None, [C.POINTER(Rng), C.c_double, _PD, _PD, _PD]),
     "o_rot": (None, [C.POINTER(Rng), _PD, _PD, _PD]),

@@ -194,3 +194,55 @@ def test_coulomb_is_rejected_where_hits_run_in_parallel(ctx):
     with pytest.raises(abi.M2DError):
         abi.Mcc(ctx, M_ION, [abi.Target(M_ION, 1e20, 300.0, QE, PB.struct())],
                 [abi.Interaction(abi.COULOMB, 0, 0.0, 0.0, 0.0, 0, 0)], 0.1)  # no primary temperature, density, charge
+
+
+def test_self_target_partners_are_read_as_the_push_left_them(ctx):
+    """A species whose collision partners come from its own store (ELASTIC,
+    random_particle over the same ensemble).  The reference scatters inside
+    its particle loop, so a partner later in slot order still has the
+    previous step's velocity there (particles.cpp:924-995); the device pushes
+    every particle first, and each scatter of the step reads its partner's
+    velocity as the push left it (the new velocities land after all of them,
+    so the result does not depend on thread timing).  With no field the push
+    leaves velocities as they are, so the expected velocity of every slot
+    whose trial fired is the oracle's scatter on the same Philox draws against
+    the step's starting velocities; two runs give the same bits."""
+    sc = _no_field_scene(ctx)
+    rng = np.random.default_rng(61)
+    n, vs = 3000, 800.0
+    st = _store(ctx, rng, n, vs)
+    dens, k_el, dt = 1e22, 2e-15, 1e-8
+    runs = []
+    for rep in range(2):
+        P = abi.ParticlesSoA.from_host(ctx, *st)
+        dev = abi.Mcc(ctx, M_ION, [abi.Target(M_ION, dens, 300.0, QE, P.struct())],
+                      [abi.Interaction(abi.ELASTIC, 0, 0.0, k_el, 0.0, 0, 0)], 1.0)
+        prob = 1 - math.exp(-dt / dev.lifetime)
+        assert 0.1 < prob < 0.3
+        for s in range(3):
+            before = P.download()
+            snap = to_oracle(before["x"], before["z"], before["vx"], before["vy"], before["vz"])
+            orc = O.make_mcc(M_ION, 1.0, [dict(mass=M_ION, density=dens, temperature=300.0,
+                                               inters=[O.inter(O.ELASTIC, M_ION, M_ION, rate=k_el)], parts=snap)])
+            abi.advance(ctx, sc.fields, species(QE, M_ION, dt, dev.lifetime, s), P, mcc=dev)
+            after = P.download()
+            assert P.alive().all()
+            hits = 0
+            for p in range(n):
+                u = O.lib().o_trial_uniform(1234, 0, p, s)
+                if u >= prob:
+                    for k in ("vx", "vy", "vz"):
+                        assert after[k][p] == before[k][p], (s, p, k)
+                    continue
+                hits += 1
+                one = snap[p:p + 1].copy()
+                r = O.Rng()
+                O.lib().o_rng_philox_begin(C.byref(r), 1234, 0, p, s, 0)
+                O.lib().o_scatter(C.byref(orc), O.dptr(one), C.byref(r))
+                for k in ("vx", "vy", "vz"):
+                    assert abs(after[k][p] - one[k][0]) <= 2e-6 * vs, (s, p, k, after[k][p], one[k][0])
+            assert hits > 0.5 * prob * n
+        runs.append(P.download())
+    for k in ("x", "z", "vx", "vy", "vz"):
+        assert np.array_equal(runs[0][k].view(np.uint32), runs[1][k].view(np.uint32)), k
+    assert ctx.status(clear=True) == 0
