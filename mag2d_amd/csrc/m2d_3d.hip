@@ -1567,6 +1567,8 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
     M2D_CHECK(!deposit || rho, M2D_ERR_BAD_ARG, "m2d_advance_3d: DEPOSIT needs rho");
     M2D_CHECK(!mcc || !mcc->h.partner_update, M2D_ERR_UNSUPPORTED,
               "m2d_advance_3d: COULOMB (partner update) runs in m2d_advance_2d only");
+    M2D_CHECK(!mcc || !mcc_partners_alias(mcc, p->x), M2D_ERR_UNSUPPORTED,
+              "m2d_advance_3d: partners from the colliding species' own store run in m2d_advance_2d only");
     if (stats) M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
     PushArgs3 a;
     fill_args3(a, f, s, p, 1.0, 1.0);

@@ -441,6 +441,11 @@ int m2d_batch_create(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, cons
             return set_error(M2D_ERR_UNSUPPORTED, "m2d_batch_create: COULOMB (partner update) runs in "
                              "m2d_advance_2d only");
         }
+        if (m && mcc_partners_alias(m, p->x)) {
+            m2d_batch_destroy(b);
+            return set_error(M2D_ERR_UNSUPPORTED, "m2d_batch_create: partners from the batch's own store "
+                             "run in m2d_advance_2d only");
+        }
         R.mcc = m ? m->d : nullptr;
         R.trial_thr = 0;
         if (m && std::isfinite(m->h.lifetime) && m->h.lifetime > 0) {

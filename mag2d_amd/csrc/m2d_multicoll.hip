@@ -114,6 +114,9 @@ int m2d_advance_multicoll_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d 
     M2D_CHECK(!mcc || mcc->ctx == ctx, M2D_ERR_BAD_ARG, "m2d_advance_multicoll_2d: mcc belongs to another context");
     M2D_CHECK(!mcc || !mcc->h.partner_update, M2D_ERR_UNSUPPORTED,
               "m2d_advance_multicoll_2d: COULOMB (partner update) runs in m2d_advance_2d only");
+    M2D_CHECK(!mcc || !mcc_partners_alias(mcc, p->x), M2D_ERR_UNSUPPORTED,
+              "m2d_advance_multicoll_2d: partners from the colliding species' own store run in "
+              "m2d_advance_2d only");
     const m2d_fields2d_params &P = f->p;
     // the reference's own restrictions (particles.cpp:818-823)
     M2D_CHECK(!f->bt && P.Br == 0.0 && P.Bz == 0.0 && P.Bt == 0.0, M2D_ERR_UNSUPPORTED,

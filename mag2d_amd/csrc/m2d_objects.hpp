@@ -114,4 +114,12 @@ struct m2d_mcc {
 namespace m2d {
 int fields_build_tables(m2d_fields2d *f);
 int mcc_sync(m2d_mcc *m);
+// some target draws its partners from the store whose first position array
+// is x (the colliding species itself)
+inline bool mcc_partners_alias(const m2d_mcc *m, const void *x)
+{
+    for (int t = 0; t < m->h.n_targets; t++)
+        if ((const void *)m->h.t[t].px == x) return true;
+    return false;
+}
 }  // namespace m2d

@@ -194,6 +194,15 @@ def test_coulomb_is_rejected_where_hits_run_in_parallel(ctx):
     with pytest.raises(abi.M2DError):
         abi.Mcc(ctx, M_ION, [abi.Target(M_ION, 1e20, 300.0, QE, PB.struct())],
                 [abi.Interaction(abi.COULOMB, 0, 0.0, 0.0, 0.0, 0, 0)], 0.1)  # no primary temperature, density, charge
+    # partners drawn from the colliding store itself: those passes refuse it too
+    own = abi.Mcc(ctx, M_ION, [abi.Target(M_ION, 1e20, 300.0, QE, P.struct())],
+                  [abi.Interaction(abi.ELASTIC, 0, 0.0, 1e-15, 0.0, 0, 0)], 1.0)
+    with pytest.raises(abi.M2DError) as e:
+        abi.Batch(ctx, sc.fields, species(QE, M_ION, 1e-9), P, np.array([0, 8], np.int64), [0.0], [own])
+    assert e.value.code == -6
+    with pytest.raises(abi.M2DError) as e:
+        abi.advance_multicoll(ctx, sc.fields, species(QE, M_ION, 1e-9, own.lifetime), P, ttd.ptr, mcc=own)
+    assert e.value.code == -6
 
 
 def test_self_target_partners_are_read_as_the_push_left_them(ctx):
