@@ -335,7 +335,12 @@ typedef struct {
  * E/B gather + Boris + drift + MCC trial/scatter, then box test, is_free and
  * (flag DEPOSIT) CIC deposit of the survivors into rho (C/node, added).
  * Does not increment niter (the caller owns Species::niter).  `mcc` may be
- * NULL.  `stats` may be NULL (no synchronisation then). */
+ * NULL.  `stats` may be NULL (no synchronisation then).  The trial's hits
+ * are queued and scattered after the pass (a hit changes only the velocity
+ * after the drift).  When a target draws its partners from this very store,
+ * every scatter reads its partner as the push left it and the new velocities
+ * are stored after all of them (the reference's loop would show a later
+ * partner its previous-step velocity). */
 int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s,
                    const m2d_particles2d *p, const m2d_mcc *mcc, double *rho,
                    uint32_t flags, m2d_step_stats *stats);
