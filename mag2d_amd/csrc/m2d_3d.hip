@@ -1266,9 +1266,9 @@ static int pcg3(m2d_fields3d *f, double rtol, int max_it, double r0, int *iters)
     M2D_TRY(vcycle3(f, f->pr, f->pz));
     int cur = 0;
     reduce_launch<1>(rb, f->partial, f->dots, s, [&](int fin) {
-        k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin);
+        k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin, nullptr);
     });
-    k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1);
+    k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1, nullptr);
     for (int it = 1; it <= max_it; it++) {
         reduce_launch<1>(rb, f->partial, f->dots + 1, s, [&](int fin) {
             k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.nbm, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
@@ -1276,7 +1276,7 @@ static int pcg3(m2d_fields3d *f, double rtol, int max_it, double r0, int *iters)
         });
         reduce_launch<1>(rb, f->partial, f->dots + 2, s, [&](int fin) {
             k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots + 4 * cur, f->dots + 1, n,
-                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin);
+                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin, nullptr);
         });
         M2D_HIP(hipGetLastError());
         *iters = it;
@@ -1284,9 +1284,10 @@ static int pcg3(m2d_fields3d *f, double rtol, int max_it, double r0, int *iters)
         if (sqrt(f->h_dots[2]) <= rtol * r0) break;
         M2D_TRY(vcycle3(f, f->pr, f->pz));
         reduce_launch<1>(rb, f->partial, f->dots + 4 * (cur ^ 1), s, [&](int fin) {
-            k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1), fin);
+            k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1), fin,
+                                               nullptr);
         });
-        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0);
+        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0, nullptr);
         cur ^= 1;
     }
     M2D_HIP(hipGetLastError());

@@ -110,8 +110,9 @@ __device__ __forceinline__ void node_residual64(const uint8_t *mask, const doubl
 }
 
 // u += e on free nodes
-__global__ void k_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
+__global__ void k_add_correction(const uint8_t *mask, double *u, const float *e, size_t n, const int *stop)
 {
+    if (stopped(stop)) return;
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n && !is_dir(mask[k])) u[k] += (double)e[k];
 }
@@ -288,8 +289,10 @@ __global__ void k_prolong(float *xf, const uint8_t *dirf, int nxf, int nzf, cons
 }
 
 // dense coarsest solve x = A^-1 b, one block
-__global__ void __launch_bounds__(256) k_coarse_solve(const float *inv, const float *b, float *x, int n)
+__global__ void __launch_bounds__(256) k_coarse_solve(const float *inv, const float *b, float *x, int n,
+                                                      const int *stop)
 {
+    if (stopped(stop)) return;
     extern __shared__ float sb[];
     for (int k = threadIdx.x; k < n; k += blockDim.x) sb[k] = b[k];
     __syncthreads();
@@ -303,8 +306,9 @@ __global__ void __launch_bounds__(256) k_coarse_solve(const float *inv, const fl
 
 // p = z + beta p, beta = rz_new / rz_old
 __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old,
-                         size_t n, int first)
+                         size_t n, int first, const int *stop)
 {
+    if (stopped(stop)) return;
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     if (first) { p[k] = z[k]; return; }
@@ -322,8 +326,9 @@ __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const d
 // every run.
 // out = a.b (FP64 accumulation)
 __global__ void __launch_bounds__(kRedBlock) k_dot_fin(const float *a, const float *b, size_t n, double *partial,
-                                                      unsigned int *ticket, double *out, int fin)
+                                                      unsigned int *ticket, double *out, int fin, const int *stop)
 {
+    if (stopped(stop)) return;
     double v[1] = {0};
     for (size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (size_t)gridDim.x * kRedBlock)
         v[0] += (double)a[k] * b[k];
@@ -333,8 +338,9 @@ __global__ void __launch_bounds__(kRedBlock) k_dot_fin(const float *a, const flo
 // q = A' p on the finest level, out = p.q
 __global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, const float *p, float *q, size_t n,
                                                          int nz, double *partial, unsigned int *ticket, double *out,
-                                                         int fin)
+                                                         int fin, const int *stop)
 {
+    if (stopped(stop)) return;
     double v[1] = {0};
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     if (k < n) {
@@ -350,8 +356,9 @@ __global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, co
 __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r, const float *p, const float *q,
                                                             const double *rz, const double *pq, size_t n, int first,
                                                             double *partial, unsigned int *ticket, double *out,
-                                                            int fin)
+                                                            int fin, const int *stop)
 {
+    if (stopped(stop)) return;
     double v[1] = {0};
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     if (k < n) {
@@ -369,8 +376,10 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r,
 // |r|^2, out[1] = |b_eff|^2, out[2] = |d|^2 of the FP32 copy d = -r
 __global__ void __launch_bounds__(kRedBlock) k_residual64_fin(const uint8_t *mask, const double *b, const double *u,
                                                              size_t n, int nz, double *r, float *d, double *partial,
-                                                             unsigned int *ticket, double *out, int fin)
+                                                             unsigned int *ticket, double *out, int fin,
+                                                             const int *stop)
 {
+    if (stopped(stop)) return;
     double v[3] = {0, 0, 0};
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     if (k < n) {
@@ -442,8 +451,10 @@ __device__ __forceinline__ void tile_rbgs(float *sx, const float *sb, const uint
 // writes the smoothed x of the tile and those coarse right-hand sides
 template <int kT0>
 __global__ void __launch_bounds__(kT0Threads) k_l0_down(const uint8_t *dir, const float *b, float *x, int nx, int nz,
-                                                      float *bc, const float *dinvc, int nxc, int nzc)
+                                                      float *bc, const float *dinvc, int nxc, int nzc,
+                                                      const int *stop)
 {
+    if (stopped(stop)) return;
     constexpr int S = kT0 + 2 * kH0d;
     __shared__ float sx[S * S], sb[S * S], sr[S * S];
     __shared__ uint8_t sd[S * S];
@@ -512,8 +523,9 @@ __global__ void __launch_bounds__(kT0Threads) k_l0_down(const uint8_t *dir, cons
 // ascent on the finest level: x += P x_c, then the post-smoothing sweeps
 template <int kT0>
 __global__ void __launch_bounds__(kT0Threads) k_l0_up(const uint8_t *dir, const float *b, float *x, int nx, int nz,
-                                                    const float *xc, int nxc, int nzc)
+                                                    const float *xc, int nxc, int nzc, const int *stop)
 {
+    if (stopped(stop)) return;
     constexpr int S = kT0 + 2 * kH0u;
     __shared__ float sx[S * S], sb[S * S];
     __shared__ uint8_t sd[S * S];
@@ -607,8 +619,9 @@ __device__ __forceinline__ void tile_load9(const float *st, const float *dinv, c
 // to the next level's right-hand side; writes the smoothed x of the tile
 __global__ void __launch_bounds__(kT0Threads) k_lm_down(const float *st, const float *dinv, const float *b, float *x,
                                                       int nx, int nz, float *bc, const float *dinvc, int nxc, int nzc,
-                                                      float w)
+                                                      float w, const int *stop)
 {
+    if (stopped(stop)) return;
     constexpr int S = kTm + 2 * kHmd;
     __shared__ float sst[9 * S * S], sdv[S * S], sb[S * S], sx[S * S], sx2[S * S];
     const int gi0 = blockIdx.y * kTm, gj0 = blockIdx.x * kTm;
@@ -668,8 +681,9 @@ __global__ void __launch_bounds__(kT0Threads) k_lm_down(const float *st, const f
 // ascent on a Galerkin level: x += P x_c on free nodes, then the sweeps
 __global__ void __launch_bounds__(kT0Threads) k_lm_up(const float *st, const float *dinv, const uint8_t *dir,
                                                     const float *b, float *x, int nx, int nz, const float *xc,
-                                                    int nxc, int nzc, float w)
+                                                    int nxc, int nzc, float w, const int *stop)
 {
+    if (stopped(stop)) return;
     constexpr int S = kTm + 2 * kHmu;
     __shared__ float sst[9 * S * S], sdv[S * S], sb[S * S], sx[S * S], sx2[S * S];
     const int gi0 = blockIdx.y * kTm, gj0 = blockIdx.x * kTm;
@@ -716,6 +730,7 @@ struct CoarseArgs {
     int coarse_n;
     float omega;
     int pre, post;
+    const int *stop;    // the solve's stop flag (see stopped())
 };
 
 __device__ void c_jacobi(const BLevel &A, float *x, int sweeps, float w)
@@ -735,6 +750,7 @@ __device__ void c_jacobi(const BLevel &A, float *x, int sweeps, float w)
 
 __global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse(CoarseArgs C)
 {
+    if (stopped(C.stop)) return;
     extern __shared__ float sb[];  // coarse_n right-hand side words, then kDenseParts partial rows
     for (int l = C.lc; l < C.L - 1; l++) {
         const BLevel &A = C.lv[l];
@@ -800,6 +816,7 @@ constexpr int kLdsLevelNodes = 1024, kLdsDenseNodes = 256;
 
 __global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse_lds(CoarseArgs C)
 {
+    if (stopped(C.stop)) return;
     __shared__ float s_st[9 * kLdsLevelNodes], s_dv[kLdsLevelNodes], s_b[kLdsLevelNodes];
     __shared__ float s_x[kLdsLevelNodes], s_x2[kLdsLevelNodes], s_r[kLdsLevelNodes];
     __shared__ uint8_t s_dir[kLdsLevelNodes];
@@ -1099,7 +1116,8 @@ static int mg_setup(m2d_fields2d *f)
     M2D_TRY(dmalloc((void **)&mg.pq, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.pe, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.r64, sizeof(double) * n));
-    M2D_TRY(dmalloc((void **)&mg.dots, sizeof(double) * 16));
+    // 16 device scalars, then the solve's control block (SolveCtl)
+    M2D_TRY(dmalloc((void **)&mg.dots, sizeof(double) * 32));
     M2D_TRY(dmalloc((void **)&mg.ticket, sizeof(unsigned int)));
     M2D_HIP(hipMemset(mg.ticket, 0, sizeof(unsigned int)));
     // the coarse part of the V-cycle runs in one workgroup from the first
@@ -1109,7 +1127,7 @@ static int mg_setup(m2d_fields2d *f)
         if (mg.lv[l].n <= kBlockLevelNodes) { mg.lc = l; break; }
     M2D_CHECK(mg.lv.size() <= (size_t)kMaxLevels && mg.coarse_n <= 3200, M2D_ERR_BAD_ARG,
               "Fields: multigrid hierarchy of %d levels, coarsest %d nodes", (int)mg.lv.size(), mg.coarse_n);
-    M2D_HIP(hipHostMalloc((void **)&mg.h_dots, sizeof(double) * 16));
+    M2D_HIP(hipHostMalloc((void **)&mg.h_dots, sizeof(double) * 32));
     mg.npartial = (int)std::min<size_t>((n + 255) / 256, 65536);
     // up to three sums per block of the fused reductions
     M2D_TRY(dmalloc((void **)&mg.partial, sizeof(double) * 3 * std::max<size_t>(mg.npartial, (n + 255) / 256)));
@@ -1135,14 +1153,14 @@ static void mg_free(Mg &mg)
 // one V-cycle: the finest level in grid kernels (first sweep from zero,
 // fused residual + restriction, fused prolongation + black sweep), middle
 // levels in grid kernels, and every level from mg.lc down in one workgroup
-static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
+static int vcycle(m2d_fields2d *f, const float *rhs, float *out, const int *stop)
 {
     Mg &mg = f->mg;
     hipStream_t s = f->ctx->stream;
     const int L = (int)mg.lv.size();
     MgLevel &A0 = mg.lv[0];
     if (L == 1) {
-        k_coarse_solve<<<1, 256, sizeof(float) * mg.coarse_n, s>>>(mg.coarse_inv, rhs, out, mg.coarse_n);
+        k_coarse_solve<<<1, 256, sizeof(float) * mg.coarse_n, s>>>(mg.coarse_inv, rhs, out, mg.coarse_n, stop);
         M2D_HIP(hipGetLastError());
         return M2D_OK;
     }
@@ -1152,15 +1170,16 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
     const dim3 gt((A0.nz + T0 - 1) / T0, (A0.nx + T0 - 1) / T0);
     {
         MgLevel &C = mg.lv[1];
-        if (small) k_l0_down<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz);
-        else k_l0_down<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz);
+        if (small) k_l0_down<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz, stop);
+        else k_l0_down<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz, stop);
     }
     // ---- middle levels [1, lc) in LDS tiles
     for (int l = 1; l < mg.lc; l++) {
         MgLevel &A = mg.lv[l];
         MgLevel &C = mg.lv[l + 1];
         const dim3 g((A.nz + kTm - 1) / kTm, (A.nx + kTm - 1) / kTm);
-        k_lm_down<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.b, A.x, A.nx, A.nz, C.b, C.dinv, C.nx, C.nz, kOmega);
+        k_lm_down<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.b, A.x, A.nx, A.nz, C.b, C.dinv, C.nx, C.nz, kOmega,
+                                           stop);
     }
     // ---- levels [lc, L) in one workgroup
     {
@@ -1176,6 +1195,7 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
         C.omega = kOmega;
         C.pre = kPre;
         C.post = kPost;
+        C.stop = stop;
         if (L - 1 - mg.lc == 1 && mg.lv[mg.lc].n <= (size_t)kLdsLevelNodes && mg.coarse_n <= kLdsDenseNodes)
             k_vcycle_coarse_lds<<<1, kCoarseThreads, 0, s>>>(C);
         else
@@ -1186,13 +1206,14 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
         MgLevel &A = mg.lv[l];
         MgLevel &Cc = mg.lv[l + 1];
         const dim3 g((A.nz + kTm - 1) / kTm, (A.nx + kTm - 1) / kTm);
-        k_lm_up<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.dir, A.b, A.x, A.nx, A.nz, Cc.x, Cc.nx, Cc.nz, kOmega);
+        k_lm_up<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.dir, A.b, A.x, A.nx, A.nz, Cc.x, Cc.nx, Cc.nz, kOmega,
+                                         stop);
     }
     // ---- level 0 ascent (LDS tiles: prolongation, smoothing)
     {
         MgLevel &C = mg.lv[1];
-        if (small) k_l0_up<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz);
-        else k_l0_up<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz);
+        if (small) k_l0_up<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz, stop);
+        else k_l0_up<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz, stop);
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -1201,63 +1222,166 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out)
 // fused-reduction launch shape over the finest level
 static unsigned red_blocks(size_t n) { return (unsigned)((n + kRedBlock - 1) / kRedBlock); }
 
-static int read_dots(m2d_fields2d *f)
+// The solve's control block, in device memory after the 16 scalars of
+// mg.dots.  Small kernels of one thread keep it between the launches, so the
+// refinement and its CG passes are queued whole and read on the host once: a
+// pass that has converged turns the launches queued after it into launches
+// that return at once (see stopped()).
+constexpr int kMaxOuter = 12;            // refinement passes at most
+constexpr int kMaxPasses = kMaxOuter + 1;
+struct SolveCtl {
+    int stop_outer;      // refinement done (converged, failed or out of passes): nothing more runs
+    int stop_pcg;        // no CG pass open, or the open one has converged
+    int no_add;          // the pass's correction is not to be added
+    int outer;           // corrections added
+    int iters;           // CG iterations run in all passes
+    int pass_iters;      // in the open pass
+    int max_pass_iters;  // the most in one pass
+    int nan;             // the residual was not finite
+    double rel;          // FP64 relative residual at the last evaluation
+    double thr;          // the open pass's stop: |r| <= rtol |d|
+    int pass_count[kMaxPasses];  // CG iterations of each pass
+};
+static_assert(sizeof(SolveCtl) <= 16 * sizeof(double), "SolveCtl fits its 16 words");
+
+// the FP32 correction is solved to 1e-5 of its right-hand side, or only as
+// far as the FP64 target needs (half of tol / rel, at most 0.1): the last
+// refinement pass then takes one or two CG iterations, not five
+__host__ __device__ inline double inner_rtol(double rel, double tol)
 {
-    M2D_HIP(hipMemcpyAsync(f->mg.h_dots, f->mg.dots, sizeof(double) * 16, hipMemcpyDeviceToHost,
-                           f->ctx->stream));
-    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
-    return M2D_OK;
+    return fmax(1e-5, fmin(0.1, 0.5 * tol / rel));
 }
 
-// FP32 PCG on A' e = d (d in mg.pr on entry, |d| = r0), e in mg.pe; stops
-// at ||r|| <= rtol r0 or max_it.  One host read per iteration (the
-// convergence test); the dot products are finished on the device.
+__global__ void k_ctl_init(SolveCtl *c)
+{
+    if (threadIdx.x) return;
+    SolveCtl z{};
+    z.stop_pcg = 1;
+    z.no_add = 1;
+    z.rel = INFINITY;
+    *c = z;
+}
+
+// after k_residual64_fin (dots[8] = |r|^2, dots[9] = |b_eff|^2): the
+// relative residual and whether the refinement is done
+__global__ void k_ctl_eval(SolveCtl *c, const double *dots, double tol, int max_outer)
+{
+    if (threadIdx.x || c->stop_outer) return;
+    const double rn = sqrt(dots[8]), bn = sqrt(dots[9]);
+    const double rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
+    c->rel = rel;
+    if (!isfinite(rn)) {
+        c->nan = 1;
+        c->stop_outer = 1;
+    } else if (rel <= tol || c->outer >= max_outer) {
+        c->stop_outer = 1;
+    }
+}
+
+// opens a CG pass on the FP32 residual d (|d|^2 = dots[10])
+__global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol)
+{
+    if (threadIdx.x) return;
+    c->pass_iters = 0;
+    const double r0 = sqrt(dots[10]);
+    if (c->stop_outer || r0 == 0.0) {
+        // nothing to solve: no iteration runs and nothing is added
+        c->stop_pcg = 1;
+        c->no_add = 1;
+        return;
+    }
+    c->thr = inner_rtol(c->rel, tol) * r0;
+    c->stop_pcg = 0;
+    c->no_add = 0;
+}
+
+// after k_cg_update_dot of an iteration (dots[2] = |r|^2)
+__global__ void k_ctl_check(SolveCtl *c, const double *dots)
+{
+    if (threadIdx.x || c->stop_pcg) return;
+    c->iters++;
+    c->pass_iters++;
+    if (c->pass_iters > c->max_pass_iters) c->max_pass_iters = c->pass_iters;
+    if (c->outer < kMaxPasses) c->pass_count[c->outer] = c->pass_iters;
+    if (sqrt(dots[2]) <= c->thr) c->stop_pcg = 1;
+}
+
+// after the pass's correction
+__global__ void k_ctl_close(SolveCtl *c)
+{
+    if (threadIdx.x || c->no_add) return;
+    c->outer++;
+    c->no_add = 1;
+}
+
+static SolveCtl *solve_ctl(Mg &mg) { return (SolveCtl *)(mg.dots + 16); }
+
+// One CG pass of at most K iterations on A' e = d (d in mg.pr, e in mg.pe),
+// queued whole; every launch stops with the pass (SolveCtl::stop_pcg).
 // dots: [0], [4] r.z of alternate iterations, [1] p.q, [2] r.r
-static int pcg(m2d_fields2d *f, double rtol, int max_it, double r0, int *iters)
+static int queue_pcg(m2d_fields2d *f, int K)
 {
     Mg &mg = f->mg;
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
     const unsigned gb = (unsigned)((n + 255) / 256), rb = red_blocks(n);
     const MgLevel &L0 = mg.lv[0];
-    *iters = 0;
-    if (r0 == 0.0) return fill_bytes(f->ctx, mg.pe, 0, sizeof(float) * n);  // e = 0
-    M2D_TRY(vcycle(f, mg.pr, mg.pz));
+    SolveCtl *ctl = solve_ctl(mg);
+    const int *stop = &ctl->stop_pcg;
+    M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
     int cur = 0;
     reduce_launch<1>(rb, mg.partial, mg.dots, s, [&](int fin) {
-        k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots, fin);
-    });
-    k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1);
-    for (int it = 1; it <= max_it; it++) {
+        k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots, fin, stop);
+    }, stop);
+    k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1, stop);
+    for (int it = 1; it <= K; it++) {
         reduce_launch<1>(rb, mg.partial, mg.dots + 1, s, [&](int fin) {
             k_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pp, mg.pq, n, L0.nz, mg.partial, mg.ticket, mg.dots + 1,
-                                                  fin);
-        });
+                                                  fin, stop);
+        }, stop);
         reduce_launch<1>(rb, mg.partial, mg.dots + 2, s, [&](int fin) {
             k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
-                                                     it == 1, mg.partial, mg.ticket, mg.dots + 2, fin);
-        });
-        M2D_HIP(hipGetLastError());
-        *iters = it;
-        M2D_TRY(read_dots(f));
-        if (sqrt(mg.h_dots[2]) <= rtol * r0) break;
-        M2D_TRY(vcycle(f, mg.pr, mg.pz));
+                                                     it == 1, mg.partial, mg.ticket, mg.dots + 2, fin, stop);
+        }, stop);
+        k_ctl_check<<<1, 64, 0, s>>>(ctl, mg.dots);
+        if (it == K) break;
+        M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
         reduce_launch<1>(rb, mg.partial, mg.dots + 4 * (cur ^ 1), s, [&](int fin) {
-            k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1), fin);
-        });
-        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0);
+            k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1), fin,
+                                               stop);
+        }, stop);
+        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0, stop);
         cur ^= 1;
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
 }
 
+// the FP64 residual of u, its norms, and the refinement's verdict on them
+static void queue_residual(m2d_fields2d *f, double *u, double tol, int max_outer)
+{
+    Mg &mg = f->mg;
+    hipStream_t s = f->ctx->stream;
+    const size_t n = f->n;
+    const unsigned rb = red_blocks(n);
+    SolveCtl *ctl = solve_ctl(mg);
+    const int *stop = &ctl->stop_outer;
+    reduce_launch<3>(rb, mg.partial, mg.dots + 8, s, [&](int fin) {
+        k_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, u, n, f->nz, mg.r64, mg.pr, mg.partial, mg.ticket,
+                                                  mg.dots + 8, fin, stop);
+    }, stop);
+    k_ctl_eval<<<1, 64, 0, s>>>(ctl, mg.dots, tol, max_outer);
+}
+
+// FP64 iterative refinement of u over FP32 multigrid-preconditioned CG
+// passes.  The passes are queued J at a time with at most K iterations each
+// (J and K from the last solve on these fields), and the control block is
+// read once after them; a solve that needs more passes or iterations than
+// queued gets another round.  The iterations that run are those of a solve
+// that tests convergence on the host after every one, so the result is the
+// same while the estimates suffice.
 static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
 {
-    // the FP32 correction is solved to 1e-5 of its right-hand side, or only
-    // as far as the FP64 target needs (half of tol / rel, at most 0.1): the
-    // last refinement pass then takes one or two CG iterations, not five
-    auto inner_rtol = [](double rel, double tol) { return std::max(1e-5, std::min(0.1, 0.5 * tol / rel)); };
     hipStream_t s = f->ctx->stream;
     Mg &mg = f->mg;
     const size_t n = f->n;
@@ -1265,38 +1389,43 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
     double *u = mode == M2D_SOLVE_RF ? f->urf : f->u;
     double scale = -(f->p.grid.dx * f->p.grid.dx) / kEps0 / f->p.dV;  // fields.cpp:307
     if (tol <= 0) tol = 1e-10;
+    const int max_outer = kMaxOuter;
+    SolveCtl *ctl = solve_ctl(mg);
+    const SolveCtl *h = (const SolveCtl *)(mg.h_dots + 16);
     k_rhs<<<gb, 256, 0, s>>>(f->mask, f->voltage, f->rho, u, n, mode, scale);
-    M2D_HIP(hipGetLastError());
-    int outer = 0, inner_total = 0;
-    double rel = 0;
-    const int max_outer = 12;
-    for (;;) {
-        const unsigned rb = red_blocks(n);
-        reduce_launch<3>(rb, mg.partial, mg.dots + 8, s, [&](int fin) {
-            k_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, u, n, f->nz, mg.r64, mg.pr, mg.partial, mg.ticket,
-                                                      mg.dots + 8, fin);
-        });
-        M2D_HIP(hipGetLastError());
-        M2D_TRY(read_dots(f));
-        double rn = sqrt(mg.h_dots[8]), bn = sqrt(mg.h_dots[9]);
-        rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
-        if (!std::isfinite(rn)) {
-            if (info) { info->outer_iters = outer; info->inner_iters = inner_total; info->rel_residual = rel; }
-            return set_error(M2D_ERR_NAN, "Fields solve: residual is not finite");
+    k_ctl_init<<<1, 64, 0, s>>>(ctl);
+    queue_residual(f, u, tol, max_outer);
+    // passes as many as the last solve took, each with one iteration more
+    // than its counterpart took then (the last pass of a refinement needs
+    // far fewer than the first)
+    int J = std::min(std::max(mg.est_passes, 1), max_outer), queued = 0, extra = 0;
+    for (int round = 0;; round++) {
+        for (int j = 0; j < J; j++, queued++) {
+            const int est = queued < kMaxPasses ? mg.est_pass_iters[queued] : mg.est_iters;
+            const int K = std::min(std::max(est + 1 + extra, 2), 60);
+            k_ctl_open<<<1, 64, 0, s>>>(ctl, mg.dots, tol);
+            M2D_TRY(queue_pcg(f, K));
+            k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n, &ctl->no_add);
+            k_ctl_close<<<1, 64, 0, s>>>(ctl);
+            queue_residual(f, u, tol, max_outer);
         }
-        if (rel <= tol || outer >= max_outer) break;
-        int it = 0;
-        M2D_TRY(pcg(f, inner_rtol(rel, tol), 60, sqrt(mg.h_dots[10]), &it));
-        inner_total += it;
-        k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n);
         M2D_HIP(hipGetLastError());
-        outer++;
+        M2D_HIP(hipMemcpyAsync(mg.h_dots + 16, mg.dots + 16, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+        M2D_HIP(hipStreamSynchronize(s));
+        if (h->nan || h->stop_outer || round > max_outer) break;
+        // more passes needed than queued: queue more, with room for longer passes
+        J = 1;
+        extra = std::min(2 * extra + 4, 60);
     }
-    if (info) { info->outer_iters = outer; info->inner_iters = inner_total; info->rel_residual = rel; }
+    mg.est_passes = std::max(h->outer, 1);
+    mg.est_iters = h->max_pass_iters;
+    for (int j = 0; j < kMaxPasses; j++) mg.est_pass_iters[j] = j < h->outer ? h->pass_count[j] : h->max_pass_iters;
+    if (info) { info->outer_iters = h->outer; info->inner_iters = h->iters; info->rel_residual = h->rel; }
+    if (h->nan) return set_error(M2D_ERR_NAN, "Fields solve: residual is not finite");
     M2D_TRY(fields_build_tables(f));
-    if (rel > tol)
+    if (h->rel > tol)
         return set_error(M2D_ERR_NOT_CONVERGED, "Fields solve: relative residual %.3e > %.3e after %d steps",
-                         rel, tol, outer);
+                         h->rel, tol, h->outer);
     return M2D_OK;
 }
 

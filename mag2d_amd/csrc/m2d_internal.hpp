@@ -174,8 +174,13 @@ int ensure(m2d_ctx *ctx, Scratch &s, size_t bytes);
 
 // vector kernels shared by the 2D and 3D solvers (m2d_fields2d.hip) and the
 // fixed-point flush (m2d_particles2d.hip)
+// The solver kernels take `stop`, a device flag that the solve's own control
+// kernels set once an iteration is not to run: a launch whose flag is set
+// returns at once (the flag is set before the launch starts and read the same
+// by every thread, so a whole launch runs or none of it).  Null: always run.
+__device__ __forceinline__ bool stopped(const int *stop) { return stop && *stop; }
 __global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old, size_t n,
-                         int first);
+                         int first, const int *stop);
 __global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n);
 // Species::accumulate into a fixed-point grid (m2d_particles2d.hip)
 __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, const float *xls,
@@ -186,10 +191,10 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
 // current device, resolved once per device (m2d_noalloc.hip)
 int noalloc_init_device(int device);
 __global__ void k_dot_fin(const float *a, const float *b, size_t n, double *partial, unsigned int *ticket,
-                          double *out, int fin);
+                          double *out, int fin, const int *stop);
 __global__ void k_cg_update_dot(float *e, float *r, const float *p, const float *q, const double *rz,
                                 const double *pq, size_t n, int first, double *partial, unsigned int *ticket,
-                                double *out, int fin);
+                                double *out, int fin, const int *stop);
 constexpr int kCoarseThreads = 1024;
 constexpr int kDenseParts = 4;  // threads per row of the dense coarsest solves
 __global__ void k_dense_solve(const float *invT, const float *b, float *x, int n);
@@ -394,8 +399,10 @@ __device__ __forceinline__ void sum_partials(const double *partial, unsigned nb,
 }
 
 template <int NV>
-__global__ void __launch_bounds__(kRedBlock) k_sum_partials(const double *partial, unsigned nb, double *out)
+__global__ void __launch_bounds__(kRedBlock) k_sum_partials(const double *partial, unsigned nb, double *out,
+                                                           const int *stop)
 {
+    if (stopped(stop)) return;
     sum_partials<NV>(partial, nb, out);
 }
 
@@ -435,11 +442,12 @@ __device__ __forceinline__ void finish_sums(double (&v)[NV], double *partial, un
 // launch a reduction kernel over nb blocks and finish its sums: in the
 // kernel (fin) when nb <= kFinBlocks, else with k_sum_partials
 template <int NV, class Launch>
-inline void reduce_launch(unsigned nb, double *partial, double *out, hipStream_t s, Launch launch)
+inline void reduce_launch(unsigned nb, double *partial, double *out, hipStream_t s, Launch launch,
+                          const int *stop = nullptr)
 {
     const int fin = nb <= kFinBlocks;
     launch(fin);
-    if (!fin) k_sum_partials<NV><<<1, kRedBlock, 0, s>>>(partial, nb, out);
+    if (!fin) k_sum_partials<NV><<<1, kRedBlock, 0, s>>>(partial, nb, out, stop);
 }
 
 }  // namespace m2d
