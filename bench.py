@@ -260,12 +260,14 @@ M_E = 9.11e-31
 M_40 = 40 * 1.66053906660e-27
 
 
-def _timed(ctx, one_step, steps, warmup):
+def _timed(ctx, one_step, steps, warmup, on_timed=None):
     """warmup untimed steps, then `steps` timed ones bracketed by device
     synchronisation; returns (elapsed s, particle-pass ms, launches)."""
     for k in range(warmup):
         one_step(k)
     ctx.sync()
+    if on_timed:
+        on_timed()
     ctx.profile(True)
     t0 = time.perf_counter()
     for k in range(warmup, warmup + steps):
@@ -585,19 +587,34 @@ def run_trap3d(args):
     bufs = [P, abi.ParticlesSoA3D(ctx, N)]
     cur = [0]
     infos = []
+    # Re-sort policy.  A deposit outside its tile's LDS window costs a
+    # contended global atomic per node: about 0.43 ns a particle against
+    # 0.8 ms for a sort of the 16.8M store, so once the spills since the last
+    # sort reach about a tenth of the store they have cost one sort (the
+    # rent-or-buy rule).  It needs the step statistics every step, whose read
+    # stalls the queue: here it measured 2% slower than sorting every 10 steps.
+    adaptive = args.resort_spill is not None
+    spilled = [0]
+    sorts = [0]
 
     def one_step(k):
-        if args.sort_every and k % args.sort_every == 0:
+        if (k == 0 or spilled[0] >= args.resort_spill * N) if adaptive else \
+                (args.sort_every and k % args.sort_every == 0):
             abi.sort_3d(ctx, fields, bufs[cur[0]], bufs[1 - cur[0]])
             cur[0] = 1 - cur[0]
+            spilled[0] = 0
+            sorts[0] += 1
         if k > 0:
             infos.append(fields.solve(1e-10))
             fields.reset()
         sp.niter = k
-        abi.advance_3d(ctx, fields, sp, bufs[cur[0]], rho_ptr=rho,
-                       flags=abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT)
+        st = abi.advance_3d(ctx, fields, sp, bufs[cur[0]], rho_ptr=rho,
+                            flags=abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT, want_stats=adaptive)
+        spilled[0] += st.spilled
 
-    elapsed, kern_ms, launches = _timed(ctx, one_step, steps, warmup)
+    sorts_warm = []
+    elapsed, kern_ms, launches = _timed(ctx, one_step, steps, warmup,
+                                        on_timed=lambda: sorts_warm.append(sorts[0]))
     live = abi.count_live_3d(ctx, bufs[cur[0]])
     kern_us = kern_ms / max(launches, 1) * 1e3
     bytes_launch = 48.0 * live + 12.0 * n ** 3  # 24 B state read+written, FP32 gradient tables once
@@ -608,7 +625,10 @@ def run_trap3d(args):
         "vs_baseline": None, "dtype": "f32 push/deposit, f64 Poisson residual", "data": "synthetic",
         "config": {"workload": "3D linear quadrupole + end caps, 128^3, Bz 0.5 T, 16M ions of 40 u, "
                                "Poisson every step", "particles": N, "live_particles_end": live,
-                   "grid": [n, n, n], "dt": 1e-8, "sort_every": args.sort_every},
+                   "grid": [n, n, n], "dt": 1e-8,
+                   "sort": (f"re-sort once the spilled deposits since the last sort reach {args.resort_spill} "
+                            "of the store" if adaptive else f"every {args.sort_every} steps"),
+                   "sorts_timed": sorts[0] - sorts_warm[0]},
         "roofline": {"kernel": "k_advance3d", "bound": "hbm", "achieved": bytes_launch / (kern_us * 1e-6) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_launch / (kern_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": None,
@@ -669,8 +689,12 @@ def main():
     ap.add_argument("--sort-every", type=int, default=None,
                     help="tile-sort the particle store every K steps (inside the timed region); default "
                          "60 for rf22pt (the 2D deposit windows reach 8 cells past their tile), 40 for "
-                         "the other 2D workloads, 20 for trap3d, whose ions leave the 2-cell 3D halo "
-                         "sooner")
+                         "the other 2D workloads, 10 for trap3d, whose ions leave the 2-cell 3D halo "
+                         "sooner (5 and 20 measured slower)")
+    ap.add_argument("--resort-spill", type=float, default=None,
+                    help="trap3d: instead of a fixed period, re-sort once the deposits that fell outside "
+                         "their tile's LDS window since the last sort add up to this fraction of the store "
+                         "(read from the step statistics: one more synchronisation a step)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -699,7 +723,7 @@ def main():
               "which this build does not implement", file=sys.stderr)
         sys.exit(2)
     if args.sort_every is None:
-        args.sort_every = {"trap3d": 20, "rf22pt": 60}.get(args.workload, 40)
+        args.sort_every = {"trap3d": 10, "rf22pt": 60}.get(args.workload, 40)
     if args.workload == "penning2d":
         return run_penning2d(args)
     if args.workload == "trap3d":

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define M2D_ABI_VERSION 1
+#define M2D_ABI_VERSION 2
 
 /* ---- status codes (the reference throws std::runtime_error instead) ---- */
 #define M2D_OK                 0
@@ -329,6 +329,15 @@ int m2d_mcc_set_partner(m2d_mcc *m, int32_t target, const m2d_particles2d *p);
 typedef struct {
     int64_t removed;          /* particles removed by the boundary pass */
     int64_t collisions;       /* MCC trials that called scatter */
+    int64_t spilled;          /* m2d_advance_3d over a store sorted by
+                                 m2d_sort_3d: deposits that fell outside their
+                                 tile's LDS window and went to the grid one by
+                                 one (particles that moved more than 2 cells
+                                 past their sort tile).  A rising count says
+                                 the store wants sorting again.  0 for an
+                                 unsorted store, whose pass has no windows,
+                                 and for the 2D passes, which do not count
+                                 (it cost the 2D pass 1.5% of its time) */
 } m2d_step_stats;
 
 /* Species<CARTESIAN>::advance (particles.hpp:342-349) fused into one pass:

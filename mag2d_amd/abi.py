@@ -94,7 +94,7 @@ class MccDesc(C.Structure):
 
 
 class StepStats(C.Structure):
-    _fields_ = [("removed", C.c_int64), ("collisions", C.c_int64)]
+    _fields_ = [("removed", C.c_int64), ("collisions", C.c_int64), ("spilled", C.c_int64)]
 
 
 class AdjointCfg(C.Structure):

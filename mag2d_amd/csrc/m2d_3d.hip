@@ -48,6 +48,7 @@ struct m2d_fields3d {
     float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
     double *r64 = nullptr, *dots = nullptr, *h_dots = nullptr, *partial = nullptr;
     int npartial = 0;
+    m2d::SolveEst est;  // the last solve's passes and iterations: what the next queues
 };
 
 namespace m2d {
@@ -90,8 +91,9 @@ __global__ void k3_rhs(const uint8_t *mask, const double *voltage, double *rho, 
 }
 
 // r = b - A u on free rows (matrix_init, fields3d.cpp:35-76); d = -r in FP32;
-__global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e, size_t n)
+__global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e, size_t n, const int *stop)
 {
+    if (stopped(stop)) return;
     size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m < n && !dir3(mask[m])) u[m] += (double)e[m];
 }
@@ -101,8 +103,9 @@ __global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e
 __global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *mask, const double *b, const double *u,
                                                               int nx, int ny, int nz, double *r, float *d,
                                                               double *partial, unsigned int *ticket, double *out,
-                                                              int fin)
+                                                              int fin, const int *stop)
 {
+    if (stopped(stop)) return;
     const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
     const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     double v[3] = {0, 0, 0};
@@ -150,8 +153,9 @@ __device__ __forceinline__ float apply7(uint8_t nb, const float *x, size_t m, si
 // y = A' x and out = x.y in one launch
 __global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *nbm, const float *x, float *y, int nx, int ny,
                                                          int nz, double *partial, unsigned int *ticket, double *out,
-                                                         int fin)
+                                                         int fin, const int *stop)
 {
+    if (stopped(stop)) return;
     const size_t n = (size_t)nx * ny * nz;
     const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     double v[1] = {0};
@@ -207,8 +211,9 @@ __device__ __forceinline__ Nb7 neighbours7(const float *x, size_t m, int i, int 
 // body is node3_rbgs's, in the same order, on neighbours read early.
 constexpr int kRbgsX = 64, kRbgsY = 4;
 __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, float *x, const float *b, int nx,
-                                                           int ny, int nz, int colour)
+                                                           int ny, int nz, int colour, const int *stop)
 {
+    if (stopped(stop)) return;
     const size_t sx = (size_t)ny * nz;
     const int i = blockIdx.z, j = blockIdx.y * kRbgsY + threadIdx.y;
     const int k = 2 * (blockIdx.x * kRbgsX + threadIdx.x) + ((i + j + colour) & 1);
@@ -228,8 +233,10 @@ __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, f
     x[m] = s * (1.f / 6.f);
 }
 
-__global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx, int ny, int nz)
+__global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx, int ny, int nz,
+                         const int *stop)
 {
+    if (stopped(stop)) return;
     const size_t n = (size_t)nx * ny * nz;
     size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
@@ -276,8 +283,9 @@ __device__ __forceinline__ float node3_restrict(const float *rf, int fx, int fy,
 }
 
 __global__ void k3_restrict(const float *rf, int fx, int fy, int fz, float *bc, const uint8_t *dirc, int cx,
-                            int cy, int cz)
+                            int cy, int cz, const int *stop)
 {
+    if (stopped(stop)) return;
     const size_t N = (size_t)cx * cy * cz;
     size_t M = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (M >= N) return;
@@ -321,8 +329,9 @@ __device__ __forceinline__ void node3_prolong(float *xf, const uint8_t *dirf, in
 
 // x_f += P x_c on free fine nodes
 __global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int fz, const float *xc, int cx,
-                           int cy, int cz)
+                           int cy, int cz, const int *stop)
 {
+    if (stopped(stop)) return;
     const size_t n = (size_t)fx * fy * fz;
     size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
@@ -347,6 +356,7 @@ struct Coarse3Args {
     int dense;          // dense coarsest solve (else sweeps symmetric RB-GS sweeps)
     const float *invT;  // the dense inverse, transposed
     int coarse_n, sweeps, pre, post;
+    const int *stop;    // the solve's stop flag (see stopped())
 };
 
 __device__ void c3_smooth(const BLevel3 &V, float *x, const float *b, int sweeps, bool forward)
@@ -367,6 +377,7 @@ __device__ void c3_smooth(const BLevel3 &V, float *x, const float *b, int sweeps
 
 __global__ void __launch_bounds__(kCoarseThreads) k3_vcycle_coarse(Coarse3Args C)
 {
+    if (stopped(C.stop)) return;
     extern __shared__ float sb[];  // dense solve: coarse_n words, then kDenseParts partial rows
     for (int l = C.lc; l < C.L - 1; l++) {
         const BLevel3 &V = C.lv[l];
@@ -438,6 +449,7 @@ constexpr int kLds3Nodes = 4096, kLds3Dense = 512, kLds3PerThread = kLds3Nodes /
 
 __global__ void __launch_bounds__(kCoarseThreads) k3_vcycle_coarse_lds(Coarse3Args C)
 {
+    if (stopped(C.stop)) return;
     __shared__ float s_x[kLds3Nodes], s_b[kLds3Nodes], s_r[kLds3Nodes];
     __shared__ uint8_t s_nb[kLds3Nodes];
     __shared__ float s_zb[kLds3Dense], s_zx[kLds3Dense];
@@ -649,12 +661,16 @@ __device__ __forceinline__ bool cic3(double X, double Y, double Z, const Grid3 &
         w[q] = __float2uint_rn(a[q & 1] * b[(q >> 1) & 1] * c[q >> 2] * S);
         s += w[q];
     }
+    // the first largest weight takes the rounding's remainder (selects, not
+    // an index: a computed index would put w in memory)
     int km = 0;
     uint32_t wm = w[0];
 #pragma unroll
     for (int q = 1; q < 8; q++)
         if (w[q] > wm) { km = q; wm = w[q]; }
-    w[km] += 0x80000000u - s;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (q == km) w[q] += 0x80000000u - s;
     cell = (int)(((size_t)i * g.ny + j) * g.nz + k);
     return true;
 }
@@ -825,9 +841,11 @@ __global__ void __launch_bounds__(256) k_advance3d(PushArgs3 a)
 // sort tile and sums its deposits in an LDS window of the tile + halo, then
 // adds the window's non-zero nodes to the 64-bit grid with atomics (integer
 // sums: the grid does not depend on their order); deposits that leave the
-// window go to the grid directly
+// window go to the grid directly.  Six waves per SIMD (at most 80 VGPRs):
+// the pass waits on its 24 gathers, and 4 waves (111 VGPRs unconstrained)
+// ran it 12% slower.
 template <bool MCC>
-__global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4 *sched,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_advance3d_tiled(PushArgs3 a, const int4 *sched,
                                                          const unsigned int *n_chunks, int nty3, int ntz3)
 {
     __shared__ unsigned long long win[kWinN3];
@@ -842,7 +860,7 @@ __global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4
     for (int k = threadIdx.x; k < kWinN3; k += 256) win[k] = 0ull;
     __syncthreads();
     bool removed_any = false, coll_any = false, nan_any = false, bad_any = false;
-    int n_removed = 0, n_coll = 0;
+    int n_removed = 0, n_coll = 0, n_spill = 0;
     for (int64_t p = s.y + threadIdx.x; p < s.z; p += 256) {
         Step3 o;
         step3<true, MCC>(a, p, o);
@@ -862,6 +880,7 @@ __global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4
                 }
             } else {
                 deposit8_global(a.acc, a.g, o.cell, o.w8);
+                n_spill++;
             }
         }
     }
@@ -879,6 +898,9 @@ __global__ void __launch_bounds__(256) k_advance3d_tiled(PushArgs3 a, const int4
     int sum = n_removed;
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
     if (__lane_id() == 0 && sum) atomicAdd(&a.counters[CNT_REMOVED], (unsigned long long)sum);
+    int sp = n_spill;
+    for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
+    if (__lane_id() == 0 && sp) atomicAdd(&a.counters[CNT_SPILL], (unsigned long long)sp);
     if (MCC) {
         int c = n_coll;
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
@@ -1161,8 +1183,9 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
     const size_t n = f->n;
     for (float **q : {&f->pr, &f->pz, &f->pp, &f->pq, &f->pe}) M2D_TRY(dmalloc3((void **)q, sizeof(float) * n));
     M2D_TRY(dmalloc3((void **)&f->r64, sizeof(double) * n));
-    M2D_TRY(dmalloc3((void **)&f->dots, sizeof(double) * 16));
-    M2D_HIP(hipHostMalloc((void **)&f->h_dots, sizeof(double) * 16));
+    // 16 scalars, then the solve's control block
+    M2D_TRY(dmalloc3((void **)&f->dots, sizeof(double) * 32));
+    M2D_HIP(hipHostMalloc((void **)&f->h_dots, sizeof(double) * 32));
     f->npartial = (int)std::min<size_t>((n + 255) / 256, 65536);
     // up to three sums per block of the fused reductions
     M2D_TRY(dmalloc3((void **)&f->partial, sizeof(double) * 3 * std::max<size_t>(f->npartial, (n + 255) / 256)));
@@ -1173,9 +1196,16 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
 
 constexpr int kPre3 = 2, kPost3 = 2;
 
+__global__ void k3_zero(float *x, size_t n, const int *stop)
+{
+    if (stopped(stop)) return;
+    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) x[m] = 0.f;
+}
+
 static unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
-static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
+static int vcycle3(m2d_fields3d *f, const float *rhs, float *out, const int *stop)
 {
     hipStream_t s = f->ctx->stream;
     const int L = (int)f->lv.size();
@@ -1184,8 +1214,8 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
                      (unsigned)V.nx);
         const dim3 blk(kRbgsX, kRbgsY);
         for (int it = 0; it < sweeps; it++) {
-            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1);
-            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0);
+            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1, stop);
+            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0, stop);
         }
     };
     // levels [lc, L) run in one workgroup (k3_vcycle_coarse) when lc > 0
@@ -1194,11 +1224,11 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         Mg3Level &V = f->lv[l];
         float *x = l == 0 ? out : V.x;
         const float *b = l == 0 ? rhs : V.b;
-        M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * V.n));
+        k3_zero<<<blocks_for(V.n), 256, 0, s>>>(x, V.n, stop);
         smooth(V, x, b, kPre3, true);
-        k3_resid<<<blocks_for(V.n), 256, 0, s>>>(V.nbm, x, b, V.r, V.nx, V.ny, V.nz);
+        k3_resid<<<blocks_for(V.n), 256, 0, s>>>(V.nbm, x, b, V.r, V.nx, V.ny, V.nz, stop);
         Mg3Level &C = f->lv[l + 1];
-        k3_restrict<<<blocks_for(C.n), 256, 0, s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz);
+        k3_restrict<<<blocks_for(C.n), 256, 0, s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz, stop);
     }
     if (lc < L) {
         Coarse3Args C;
@@ -1214,6 +1244,7 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         C.sweeps = kCoarseSweeps;
         C.pre = kPre3;
         C.post = kPost3;
+        C.stop = stop;
         if (L - 1 - lc == 1 && f->lv[lc].n <= (size_t)kLds3Nodes && C.dense && f->coarse_n <= kLds3Dense) {
             k3_vcycle_coarse_lds<<<1, kCoarseThreads, 0, s>>>(C);
         } else {
@@ -1226,9 +1257,9 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         const float *b = L == 1 ? rhs : C.b;
         if (f->coarse_dense) {
             k_dense_solve<<<1, kCoarseThreads, sizeof(float) * f->coarse_n * (1 + kDenseParts), s>>>(
-                f->coarse_invT, b, x, f->coarse_n);
+                f->coarse_invT, b, x, f->coarse_n, stop);
         } else {
-            M2D_TRY(fill_bytes(f->ctx, x, 0, sizeof(float) * C.n));
+            k3_zero<<<blocks_for(C.n), 256, 0, s>>>(x, C.n, stop);
             smooth(C, x, b, kCoarseSweeps / 2, true);
             smooth(C, x, b, kCoarseSweeps / 2, false);
         }
@@ -1238,60 +1269,67 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out)
         Mg3Level &C = f->lv[l + 1];
         float *x = l == 0 ? out : V.x;
         const float *b = l == 0 ? rhs : V.b;
-        k3_prolong<<<blocks_for(V.n), 256, 0, s>>>(x, V.dir, V.nx, V.ny, V.nz, C.x, C.nx, C.ny, C.nz);
+        k3_prolong<<<blocks_for(V.n), 256, 0, s>>>(x, V.dir, V.nx, V.ny, V.nz, C.x, C.nx, C.ny, C.nz, stop);
         smooth(V, x, b, kPost3, false);
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
 }
 
-static int read_dots3(m2d_fields3d *f)
-{
-    M2D_HIP(hipMemcpyAsync(f->h_dots, f->dots, sizeof(double) * 16, hipMemcpyDeviceToHost, f->ctx->stream));
-    M2D_HIP(hipStreamSynchronize(f->ctx->stream));
-    return M2D_OK;
-}
+static SolveCtl *solve_ctl3(m2d_fields3d *f) { return (SolveCtl *)(f->dots + 16); }
 
-// FP32 PCG on A' e = d (d in pr, |d| = r0), e in pe; the dot products finish
-// on the device, one host read per iteration (the convergence test).
+// One FP32 PCG pass of at most K iterations on A' e = d (d in pr, e in pe),
+// queued whole; every launch stops with the pass (SolveCtl::stop_pcg).
 // dots: [0], [4] r.z of alternate iterations, [1] p.q, [2] r.r
-static int pcg3(m2d_fields3d *f, double rtol, int max_it, double r0, int *iters)
+static int queue_pcg3(m2d_fields3d *f, int K)
 {
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
     const unsigned gb = blocks_for(n), rb = (unsigned)((n + kRedBlock - 1) / kRedBlock);
     const Mg3Level &V = f->lv[0];
-    *iters = 0;
-    if (r0 == 0.0) return fill_bytes(f->ctx, f->pe, 0, sizeof(float) * n);  // e = 0
-    M2D_TRY(vcycle3(f, f->pr, f->pz));
+    SolveCtl *ctl = solve_ctl3(f);
+    const int *stop = &ctl->stop_pcg;
+    M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
     int cur = 0;
     reduce_launch<1>(rb, f->partial, f->dots, s, [&](int fin) {
-        k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin, nullptr);
-    });
-    k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1, nullptr);
-    for (int it = 1; it <= max_it; it++) {
+        k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin, stop);
+    }, stop);
+    k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1, stop);
+    for (int it = 1; it <= K; it++) {
         reduce_launch<1>(rb, f->partial, f->dots + 1, s, [&](int fin) {
             k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.nbm, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
-                                                  f->dots + 1, fin);
-        });
+                                                  f->dots + 1, fin, stop);
+        }, stop);
         reduce_launch<1>(rb, f->partial, f->dots + 2, s, [&](int fin) {
             k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots + 4 * cur, f->dots + 1, n,
-                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin, nullptr);
-        });
-        M2D_HIP(hipGetLastError());
-        *iters = it;
-        M2D_TRY(read_dots3(f));
-        if (sqrt(f->h_dots[2]) <= rtol * r0) break;
-        M2D_TRY(vcycle3(f, f->pr, f->pz));
+                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin, stop);
+        }, stop);
+        k_ctl_check<<<1, 64, 0, s>>>(ctl, f->dots);
+        if (it == K) break;
+        M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
         reduce_launch<1>(rb, f->partial, f->dots + 4 * (cur ^ 1), s, [&](int fin) {
             k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1), fin,
-                                               nullptr);
-        });
-        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0, nullptr);
+                                               stop);
+        }, stop);
+        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0, stop);
         cur ^= 1;
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
+}
+
+// the FP64 residual of u, its norms, and the refinement's verdict on them
+static void queue_residual3(m2d_fields3d *f, double tol)
+{
+    hipStream_t s = f->ctx->stream;
+    const unsigned rb = (unsigned)((f->n + kRedBlock - 1) / kRedBlock);
+    SolveCtl *ctl = solve_ctl3(f);
+    const int *stop = &ctl->stop_outer;
+    reduce_launch<3>(rb, f->partial, f->dots + 8, s, [&](int fin) {
+        k3_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, f->u, f->nx, f->ny, f->nz, f->r64, f->pr,
+                                                   f->partial, f->ticket, f->dots + 8, fin, stop);
+    }, stop);
+    k_ctl_eval<<<1, 64, 0, s>>>(ctl, f->dots, tol, kMaxOuter);
 }
 
 static int build_tables3(m2d_fields3d *f)
@@ -1479,33 +1517,22 @@ int m2d_fields3d_solve(m2d_fields3d *f, double tol, m2d_solve_info *info)
     k3_rhs<<<gb, 256, 0, s>>>(f->mask, f->voltage, f->rho, f->u, n, f->p.rhs_scale);
     M2D_HIP(hipGetLastError());
     if (tol <= 0) tol = 1e-10;
-    int outer = 0, inner = 0;
-    double rel = 0;
-    const int max_outer = 12;
-    for (;;) {
-        const unsigned rb = (unsigned)((n + kRedBlock - 1) / kRedBlock);
-        reduce_launch<3>(rb, f->partial, f->dots + 8, s, [&](int fin) {
-            k3_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, f->u, f->nx, f->ny, f->nz, f->r64, f->pr,
-                                                       f->partial, f->ticket, f->dots + 8, fin);
-        });
-        M2D_HIP(hipGetLastError());
-        M2D_TRY(read_dots3(f));
-        const double rn = sqrt(f->h_dots[8]), bn = sqrt(f->h_dots[9]);
-        rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
-        if (!std::isfinite(rn)) {
-            if (info) { info->outer_iters = outer; info->inner_iters = inner; info->rel_residual = rel; }
-            return set_error(M2D_ERR_NAN, "Fields3D solve: residual is not finite");
-        }
-        if (rel <= tol || outer >= max_outer) break;
-        int it = 0;
-        // solved to 1e-5, or only as far as the FP64 target needs
-        M2D_TRY(pcg3(f, std::max(1e-5, std::min(0.1, 0.5 * tol / rel)), 80, sqrt(f->h_dots[10]), &it));
-        inner += it;
-        k3_add_correction<<<gb, 256, 0, s>>>(f->mask, f->u, f->pe, n);
-        M2D_HIP(hipGetLastError());
-        outer++;
-    }
-    if (info) { info->outer_iters = outer; info->inner_iters = inner; info->rel_residual = rel; }
+    SolveCtl *ctl = solve_ctl3(f);
+    SolveCtl *h = (SolveCtl *)(f->h_dots + 16);
+    k_ctl_init<<<1, 64, 0, s>>>(ctl);
+    queue_residual3(f, tol);
+    M2D_TRY(queue_refinement(
+        s, ctl, h, f->dots, f->est, tol, 80,
+        [&](int K) {
+            M2D_TRY(queue_pcg3(f, K));
+            k3_add_correction<<<gb, 256, 0, s>>>(f->mask, f->u, f->pe, n, &ctl->no_add);
+            return (int)M2D_OK;
+        },
+        [&] { queue_residual3(f, tol); }));
+    const int outer = h->outer;
+    const double rel = h->rel;
+    if (info) { info->outer_iters = outer; info->inner_iters = h->iters; info->rel_residual = rel; }
+    if (h->nan) return set_error(M2D_ERR_NAN, "Fields3D solve: residual is not finite");
     M2D_TRY(build_tables3(f));
     if (rel > tol)
         return set_error(M2D_ERR_NOT_CONVERGED, "Fields3D solve: relative residual %.3e > %.3e after %d steps",
@@ -1570,7 +1597,10 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
               "m2d_advance_3d: COULOMB (partner update) runs in m2d_advance_2d only");
     M2D_CHECK(!mcc || !mcc_partners_alias(mcc, p->x), M2D_ERR_UNSUPPORTED,
               "m2d_advance_3d: partners from the colliding species' own store run in m2d_advance_2d only");
-    if (stats) M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
+    if (stats) {
+        M2D_TRY(zero_words(ctx, ctx->counters + CNT_REMOVED, 2));
+        M2D_TRY(zero_words(ctx, ctx->counters + CNT_SPILL, 1));
+    }
     PushArgs3 a;
     fill_args3(a, f, s, p, 1.0, 1.0);
     a.push = (flags & M2D_STEP_PUSH) != 0;
@@ -1623,6 +1653,7 @@ int m2d_advance_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_species2d *s, const 
         M2D_TRY(read_counters(ctx));
         stats->removed = (int64_t)ctx->h_counters[CNT_REMOVED];
         stats->collisions = (int64_t)ctx->h_counters[CNT_COLL];
+        stats->spilled = (int64_t)ctx->h_counters[CNT_SPILL];
     }
     return M2D_OK;
 }

@@ -885,8 +885,10 @@ __global__ void __launch_bounds__(kCoarseThreads) k_vcycle_coarse_lds(CoarseArgs
 // dense solve x = A^-1 b with the transposed inverse, one workgroup of
 // kCoarseThreads: kDenseParts threads per row (the same sums as the dense
 // step of k_vcycle_coarse); dynamic LDS of (1 + kDenseParts) n words
-__global__ void __launch_bounds__(kCoarseThreads) k_dense_solve(const float *invT, const float *b, float *x, int n)
+__global__ void __launch_bounds__(kCoarseThreads) k_dense_solve(const float *invT, const float *b, float *x, int n,
+                                                                 const int *stop)
 {
+    if (stopped(stop)) return;
     extern __shared__ float sb[];
     float *part = sb + n;
     for (int k = threadIdx.x; k < n; k += kCoarseThreads) sb[k] = b[k];
@@ -1222,36 +1224,6 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out, const int *stop
 // fused-reduction launch shape over the finest level
 static unsigned red_blocks(size_t n) { return (unsigned)((n + kRedBlock - 1) / kRedBlock); }
 
-// The solve's control block, in device memory after the 16 scalars of
-// mg.dots.  Small kernels of one thread keep it between the launches, so the
-// refinement and its CG passes are queued whole and read on the host once: a
-// pass that has converged turns the launches queued after it into launches
-// that return at once (see stopped()).
-constexpr int kMaxOuter = 12;            // refinement passes at most
-constexpr int kMaxPasses = kMaxOuter + 1;
-struct SolveCtl {
-    int stop_outer;      // refinement done (converged, failed or out of passes): nothing more runs
-    int stop_pcg;        // no CG pass open, or the open one has converged
-    int no_add;          // the pass's correction is not to be added
-    int outer;           // corrections added
-    int iters;           // CG iterations run in all passes
-    int pass_iters;      // in the open pass
-    int max_pass_iters;  // the most in one pass
-    int nan;             // the residual was not finite
-    double rel;          // FP64 relative residual at the last evaluation
-    double thr;          // the open pass's stop: |r| <= rtol |d|
-    int pass_count[kMaxPasses];  // CG iterations of each pass
-};
-static_assert(sizeof(SolveCtl) <= 16 * sizeof(double), "SolveCtl fits its 16 words");
-
-// the FP32 correction is solved to 1e-5 of its right-hand side, or only as
-// far as the FP64 target needs (half of tol / rel, at most 0.1): the last
-// refinement pass then takes one or two CG iterations, not five
-__host__ __device__ inline double inner_rtol(double rel, double tol)
-{
-    return fmax(1e-5, fmin(0.1, 0.5 * tol / rel));
-}
-
 __global__ void k_ctl_init(SolveCtl *c)
 {
     if (threadIdx.x) return;
@@ -1374,12 +1346,7 @@ static void queue_residual(m2d_fields2d *f, double *u, double tol, int max_outer
 }
 
 // FP64 iterative refinement of u over FP32 multigrid-preconditioned CG
-// passes.  The passes are queued J at a time with at most K iterations each
-// (J and K from the last solve on these fields), and the control block is
-// read once after them; a solve that needs more passes or iterations than
-// queued gets another round.  The iterations that run are those of a solve
-// that tests convergence on the host after every one, so the result is the
-// same while the estimates suffice.
+// passes, queued whole (queue_refinement)
 static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
 {
     hipStream_t s = f->ctx->stream;
@@ -1389,37 +1356,19 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
     double *u = mode == M2D_SOLVE_RF ? f->urf : f->u;
     double scale = -(f->p.grid.dx * f->p.grid.dx) / kEps0 / f->p.dV;  // fields.cpp:307
     if (tol <= 0) tol = 1e-10;
-    const int max_outer = kMaxOuter;
     SolveCtl *ctl = solve_ctl(mg);
-    const SolveCtl *h = (const SolveCtl *)(mg.h_dots + 16);
+    SolveCtl *h = (SolveCtl *)(mg.h_dots + 16);
     k_rhs<<<gb, 256, 0, s>>>(f->mask, f->voltage, f->rho, u, n, mode, scale);
     k_ctl_init<<<1, 64, 0, s>>>(ctl);
-    queue_residual(f, u, tol, max_outer);
-    // passes as many as the last solve took, each with one iteration more
-    // than its counterpart took then (the last pass of a refinement needs
-    // far fewer than the first)
-    int J = std::min(std::max(mg.est_passes, 1), max_outer), queued = 0, extra = 0;
-    for (int round = 0;; round++) {
-        for (int j = 0; j < J; j++, queued++) {
-            const int est = queued < kMaxPasses ? mg.est_pass_iters[queued] : mg.est_iters;
-            const int K = std::min(std::max(est + 1 + extra, 2), 60);
-            k_ctl_open<<<1, 64, 0, s>>>(ctl, mg.dots, tol);
+    queue_residual(f, u, tol, kMaxOuter);
+    M2D_TRY(queue_refinement(
+        s, ctl, h, mg.dots, mg.est, tol, 60,
+        [&](int K) {
             M2D_TRY(queue_pcg(f, K));
             k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n, &ctl->no_add);
-            k_ctl_close<<<1, 64, 0, s>>>(ctl);
-            queue_residual(f, u, tol, max_outer);
-        }
-        M2D_HIP(hipGetLastError());
-        M2D_HIP(hipMemcpyAsync(mg.h_dots + 16, mg.dots + 16, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
-        M2D_HIP(hipStreamSynchronize(s));
-        if (h->nan || h->stop_outer || round > max_outer) break;
-        // more passes needed than queued: queue more, with room for longer passes
-        J = 1;
-        extra = std::min(2 * extra + 4, 60);
-    }
-    mg.est_passes = std::max(h->outer, 1);
-    mg.est_iters = h->max_pass_iters;
-    for (int j = 0; j < kMaxPasses; j++) mg.est_pass_iters[j] = j < h->outer ? h->pass_count[j] : h->max_pass_iters;
+            return (int)M2D_OK;
+        },
+        [&] { queue_residual(f, u, tol, kMaxOuter); }));
     if (info) { info->outer_iters = h->outer; info->inner_iters = h->iters; info->rel_residual = h->rel; }
     if (h->nan) return set_error(M2D_ERR_NAN, "Fields solve: residual is not finite");
     M2D_TRY(fields_build_tables(f));

@@ -115,7 +115,9 @@ struct m2d_tiling {
 };
 
 // counters written by kernels
-enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_QLEN = 4 /* and 5 */, CNT_N = 8 };
+// CNT_SPILL: deposits of a tiled pass that fell outside their tile's LDS
+// window (summed straight into the grid)
+enum { CNT_REMOVED = 0, CNT_COLL = 1, CNT_FLAGS = 2, CNT_LIVE = 3, CNT_QLEN = 4 /* and 5 */, CNT_SPILL = 6, CNT_N = 8 };
 
 struct m2d_ctx {
     int refs = 1;           // the creator's reference plus one per Fields / MCC object
@@ -197,11 +199,99 @@ __global__ void k_cg_update_dot(float *e, float *r, const float *p, const float 
                                 double *out, int fin, const int *stop);
 constexpr int kCoarseThreads = 1024;
 constexpr int kDenseParts = 4;  // threads per row of the dense coarsest solves
-__global__ void k_dense_solve(const float *invT, const float *b, float *x, int n);
+__global__ void k_dense_solve(const float *invT, const float *b, float *x, int n, const int *stop);
 int ensure_acc(m2d_ctx *ctx, size_t n);
 int fill_bytes(m2d_ctx *ctx, void *dst, int value, size_t bytes);
 int zero_words(m2d_ctx *ctx, unsigned long long *dst, size_t n);
 int read_counters(m2d_ctx *ctx);
+
+// ---------------------------------------------------------------- solve control
+// The 2D and 3D solves' control block, in device memory after the 16 scalars
+// of their dot array ([0], [4] r.z of alternate CG iterations, [1] p.q, [2]
+// r.r, [8] |r|^2 and [9] |b_eff|^2 of the FP64 residual, [10] |d|^2 of its
+// FP32 copy).  Small kernels of one thread keep it between the launches, so
+// the refinement and its CG passes are queued whole and read on the host
+// once: a pass that has converged turns the launches queued after it into
+// launches that return at once (see stopped()).
+constexpr int kMaxOuter = 12;            // refinement passes at most
+constexpr int kMaxPasses = kMaxOuter + 1;
+struct SolveCtl {
+    int stop_outer;      // refinement done (converged, failed or out of passes): nothing more runs
+    int stop_pcg;        // no CG pass open, or the open one has converged
+    int no_add;          // the pass's correction is not to be added
+    int outer;           // corrections added
+    int iters;           // CG iterations run in all passes
+    int pass_iters;      // in the open pass
+    int max_pass_iters;  // the most in one pass
+    int nan;             // the residual was not finite
+    double rel;          // FP64 relative residual at the last evaluation
+    double thr;          // the open pass's stop: |r| <= rtol |d|
+    int pass_count[kMaxPasses];  // CG iterations of each pass
+};
+static_assert(sizeof(SolveCtl) <= 16 * sizeof(double), "SolveCtl fits its 16 words");
+
+// the FP32 correction is solved to 1e-5 of its right-hand side, or only as
+// far as the FP64 target needs (half of tol / rel, at most 0.1): the last
+// refinement pass then takes one or two CG iterations, not five
+__host__ __device__ inline double inner_rtol(double rel, double tol)
+{
+    return fmax(1e-5, fmin(0.1, 0.5 * tol / rel));
+}
+
+
+// the last solve's shape on a set of fields: refinement passes, and the CG
+// iterations each pass took (the next solve queues as many, plus one)
+struct SolveEst {
+    int passes = 2, max_iters = 6;
+    int iters[kMaxPasses] = {6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6};
+};
+static_assert(kMaxPasses == 13, "SolveEst::iters initialiser");
+
+__global__ void k_ctl_init(SolveCtl *c);
+__global__ void k_ctl_eval(SolveCtl *c, const double *dots, double tol, int max_outer);
+__global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol);
+__global__ void k_ctl_check(SolveCtl *c, const double *dots);
+__global__ void k_ctl_close(SolveCtl *c);
+
+// FP64 iterative refinement over FP32 CG passes, queued under SolveCtl.  The
+// caller has queued the right-hand side, k_ctl_init and resid() (the FP64
+// residual, its norms and k_ctl_eval).  Passes are queued as many as the
+// last solve took, each with one CG iteration more than its counterpart took
+// then (the last pass of a refinement needs far fewer than the first); the
+// control block is read once after them, and a solve that needs more passes
+// or iterations gets another round of one longer pass at a time.
+// pass(K) queues one CG pass of at most K iterations and the correction's
+// addition (which skips itself when ctl->no_add is set).  The iterations that
+// run are those of a solve that tests convergence on the host after each,
+// so the result is the same while the estimates suffice.
+template <class Pass, class Resid>
+int queue_refinement(hipStream_t s, SolveCtl *ctl, SolveCtl *h, const double *dots, SolveEst &est, double tol,
+                     int max_iters, Pass pass, Resid resid)
+{
+    int J = est.passes < 1 ? 1 : (est.passes > kMaxOuter ? kMaxOuter : est.passes), queued = 0, extra = 0;
+    for (int round = 0;; round++) {
+        for (int j = 0; j < J; j++, queued++) {
+            const int e = queued < kMaxPasses ? est.iters[queued] : est.max_iters;
+            int K = e + 1 + extra;
+            K = K < 2 ? 2 : (K > max_iters ? max_iters : K);
+            k_ctl_open<<<1, 64, 0, s>>>(ctl, dots, tol);
+            if (int r = pass(K)) return r;
+            k_ctl_close<<<1, 64, 0, s>>>(ctl);
+            resid();
+        }
+        M2D_HIP(hipGetLastError());
+        M2D_HIP(hipMemcpyAsync(h, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+        M2D_HIP(hipStreamSynchronize(s));
+        if (h->nan || h->stop_outer || round > kMaxOuter) break;
+        J = 1;
+        extra = 2 * extra + 4 < max_iters ? 2 * extra + 4 : max_iters;
+    }
+    est.passes = h->outer > 1 ? h->outer : 1;
+    est.max_iters = h->max_pass_iters;
+    for (int j = 0; j < kMaxPasses; j++) est.iters[j] = j < h->outer ? h->pass_count[j] : h->max_pass_iters;
+    return M2D_OK;
+}
+
 // null, count, non-null and 16-byte aligned arrays of a 2D store
 int check_parts(const m2d_particles2d *p);
 

@@ -153,6 +153,7 @@ int m2d_advance_multicoll_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d 
     if (stats) {
         M2D_TRY(read_counters(ctx));
         stats->removed = 0;
+        stats->spilled = 0;
         stats->collisions = (int64_t)ctx->h_counters[CNT_COLL];
     }
     return M2D_OK;

@@ -70,10 +70,7 @@ struct Mg {
     int npartial = 0;
     unsigned int *ticket = nullptr;  // block counter of the fused reductions (0 between launches)
     int lc = 0;                   // first level of the one-workgroup coarse V-cycle
-    // refinement passes and CG iterations per pass to queue at once: the
-    // last solve's (one more iteration), so that one host read ends a solve
-    int est_passes = 2, est_iters = 6;
-    int est_pass_iters[13] = {6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6};
+    SolveEst est;  // the last solve's passes and iterations: what the next queues
 };
 
 }  // namespace m2d

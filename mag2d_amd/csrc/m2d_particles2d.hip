@@ -1395,6 +1395,7 @@ int m2d_advance_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_species2d *s, const 
         M2D_TRY(read_counters(ctx));
         stats->removed = (int64_t)ctx->h_counters[CNT_REMOVED];
         stats->collisions = (int64_t)ctx->h_counters[CNT_COLL];
+        stats->spilled = 0;  // not counted by the 2D pass
     }
     return M2D_OK;
 }

@@ -322,13 +322,30 @@ def test_tiled_and_untiled_3d_passes_agree_bit_for_bit(ctx):
     rs, ru = (abi.DeviceArray(ctx, (sc.n ** 3,), np.float64) for _ in range(2))
     sp_ = abi.Species2D(QE, M40, 2e-8, float("inf"), 0, 0, 0)
     fl = abi.STEP_PUSH | abi.STEP_BOUNDARY | abi.STEP_DEPOSIT
+    # each slot's sort tile (4 x 4 x 4 cells) and its deposit window: the
+    # tile's cells plus 2 on every side
+    idx = 1.0 / sc.h
+    tile = [np.clip(np.floor(d[c][:live].astype(np.float64) * idx), 0, sc.n - 2).astype(np.int64) // 4
+            for c in ("x", "y", "z")]
+    spilled = []
     for k in range(10):
         sp_.niter = k
         rs.zero()
         ru.zero()
-        abi.advance_3d(ctx, sc.fields, sp_, S, rho_ptr=rs.ptr, flags=fl)
-        abi.advance_3d(ctx, sc.fields, sp_, U, rho_ptr=ru.ptr, flags=fl)
+        st = abi.advance_3d(ctx, sc.fields, sp_, S, rho_ptr=rs.ptr, flags=fl, want_stats=True)
+        su = abi.advance_3d(ctx, sc.fields, sp_, U, rho_ptr=ru.ptr, flags=fl, want_stats=True)
         assert np.array_equal(rs.get(), ru.get()), k
+        assert su.spilled == 0 and st.removed == su.removed, k  # the untiled pass has no windows
+        # the spill count is the number of deposits outside their window
+        e = S.download()
+        alive = e["x"][:live].view(np.uint32) != abi.DEAD_BITS
+        out = np.zeros(live, bool)
+        for c, t in zip(("x", "y", "z"), tile):
+            cell = (np.where(alive, e[c][:live], 0.0).astype(np.float64) * idx).astype(np.int64)
+            out |= (cell < 4 * t - 2) | (cell > 4 * t + 5)
+        assert st.spilled == int((out & alive).sum()), k
+        spilled.append(st.spilled)
+    assert spilled[0] == 0 and spilled[-1] > 0  # ions drift out of their windows
     a, b = S.download(), U.download()
     for c in abi.ParticlesSoA3D.NAMES:
         assert np.array_equal(a[c].view(np.uint32), b[c].view(np.uint32)), c
