@@ -702,7 +702,10 @@ def main():
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print a markdown table of a rocprofv3 --kernel-trace --stats csv output and exit")
     ap.add_argument("--summarize-counters", nargs="+", metavar="DIR",
-                    help="print the SQ counters of k_advance2d from rocprofv3 --pmc run directories")
+                    help="print the SQ counters of a kernel (--counter-kernel) from rocprofv3 --pmc run "
+                         "directories")
+    ap.add_argument("--counter-kernel", default="k_advance2d",
+                    help="kernel whose counters --summarize-counters prints (default k_advance2d)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the calibrated FETCH_SIZE/WRITE_SIZE summary of two rocprofv3 --pmc csv "
                          "outputs (with --copy-calibration-mib) and exit")
@@ -714,7 +717,7 @@ def main():
         print(summarize_pmc(*args.summarize_pmc, args.copy_calibration_mib))
         return
     if args.summarize_counters:
-        print(summarize_counters(args.summarize_counters, args.particles or 4194304))
+        print(summarize_counters(args.summarize_counters, args.particles or 4194304, args.counter_kernel))
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -1278,10 +1278,11 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out, const int *sto
 
 static SolveCtl *solve_ctl3(m2d_fields3d *f) { return (SolveCtl *)(f->dots + 16); }
 
-// One FP32 PCG pass of at most K iterations on A' e = d (d in pr, e in pe),
-// queued whole; every launch stops with the pass (SolveCtl::stop_pcg).
-// dots: [0], [4] r.z of alternate iterations, [1] p.q, [2] r.r
-static int queue_pcg3(m2d_fields3d *f, int K)
+// Iterations from + 1 .. to of the open FP32 PCG pass on A' e = d (d in pr,
+// e in pe), from 0: the pass's start; every launch stops with the pass
+// (SolveCtl::stop_pcg).  cur: which of dots[0], dots[4] holds the current
+// r.z, (iterations run - 1) & 1.  dots: [1] p.q, [2] r.r
+static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
 {
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
@@ -1289,13 +1290,24 @@ static int queue_pcg3(m2d_fields3d *f, int K)
     const Mg3Level &V = f->lv[0];
     SolveCtl *ctl = solve_ctl3(f);
     const int *stop = &ctl->stop_pcg;
-    M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
-    int cur = 0;
-    reduce_launch<1>(rb, f->partial, f->dots, s, [&](int fin) {
-        k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin, stop);
-    }, stop);
-    k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1, stop);
-    for (int it = 1; it <= K; it++) {
+    if (from == 0) {
+        M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
+        reduce_launch<1>(rb, f->partial, f->dots, s, [&](int fin) {
+            k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin, stop);
+        }, stop);
+        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1, stop);
+        cur = 0;
+    }
+    for (int it = from + 1; it <= to; it++) {
+        if (it > 1) {
+            M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
+            reduce_launch<1>(rb, f->partial, f->dots + 4 * (cur ^ 1), s, [&](int fin) {
+                k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1),
+                                                   fin, stop);
+            }, stop);
+            k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0, stop);
+            cur ^= 1;
+        }
         reduce_launch<1>(rb, f->partial, f->dots + 1, s, [&](int fin) {
             k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.nbm, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
                                                   f->dots + 1, fin, stop);
@@ -1305,14 +1317,6 @@ static int queue_pcg3(m2d_fields3d *f, int K)
                                                      it == 1, f->partial, f->ticket, f->dots + 2, fin, stop);
         }, stop);
         k_ctl_check<<<1, 64, 0, s>>>(ctl, f->dots);
-        if (it == K) break;
-        M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
-        reduce_launch<1>(rb, f->partial, f->dots + 4 * (cur ^ 1), s, [&](int fin) {
-            k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1), fin,
-                                               stop);
-        }, stop);
-        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0, stop);
-        cur ^= 1;
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -1523,11 +1527,8 @@ int m2d_fields3d_solve(m2d_fields3d *f, double tol, m2d_solve_info *info)
     queue_residual3(f, tol);
     M2D_TRY(queue_refinement(
         s, ctl, h, f->dots, f->est, tol, 80,
-        [&](int K) {
-            M2D_TRY(queue_pcg3(f, K));
-            k3_add_correction<<<gb, 256, 0, s>>>(f->mask, f->u, f->pe, n, &ctl->no_add);
-            return (int)M2D_OK;
-        },
+        [&](int from, int to, int &cur) { return queue_pcg3(f, from, to, cur); },
+        [&] { k3_add_correction<<<gb, 256, 0, s>>>(f->mask, f->u, f->pe, n, &ctl->no_add); },
         [&] { queue_residual3(f, tol); }));
     const int outer = h->outer;
     const double rel = h->rel;

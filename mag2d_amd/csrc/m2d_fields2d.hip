@@ -1253,7 +1253,7 @@ __global__ void k_ctl_eval(SolveCtl *c, const double *dots, double tol, int max_
 // opens a CG pass on the FP32 residual d (|d|^2 = dots[10])
 __global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol)
 {
-    if (threadIdx.x) return;
+    if (threadIdx.x || c->halted) return;
     c->pass_iters = 0;
     const double r0 = sqrt(dots[10]);
     if (c->stop_outer || r0 == 0.0) {
@@ -1278,6 +1278,30 @@ __global__ void k_ctl_check(SolveCtl *c, const double *dots)
     if (sqrt(dots[2]) <= c->thr) c->stop_pcg = 1;
 }
 
+// after the last iteration queued for a pass: a pass that has neither
+// converged nor reached max_iters is suspended -- every launch queued after
+// it returns at once -- until the host queues more of its iterations
+// (k_ctl_resume); at max_iters it ends, as a loop with that cap would end it
+__global__ void k_ctl_end(SolveCtl *c, int max_iters)
+{
+    if (threadIdx.x || c->stop_pcg) return;
+    c->stop_pcg = 1;
+    if (c->pass_iters >= max_iters) return;
+    c->halted = 1;
+    c->stop_outer = 1;
+    c->no_add = 1;
+}
+
+// the suspended pass continues: its CG state is where it stopped
+__global__ void k_ctl_resume(SolveCtl *c)
+{
+    if (threadIdx.x || !c->halted) return;
+    c->halted = 0;
+    c->stop_outer = 0;
+    c->stop_pcg = 0;
+    c->no_add = 0;
+}
+
 // after the pass's correction
 __global__ void k_ctl_close(SolveCtl *c)
 {
@@ -1288,10 +1312,11 @@ __global__ void k_ctl_close(SolveCtl *c)
 
 static SolveCtl *solve_ctl(Mg &mg) { return (SolveCtl *)(mg.dots + 16); }
 
-// One CG pass of at most K iterations on A' e = d (d in mg.pr, e in mg.pe),
-// queued whole; every launch stops with the pass (SolveCtl::stop_pcg).
-// dots: [0], [4] r.z of alternate iterations, [1] p.q, [2] r.r
-static int queue_pcg(m2d_fields2d *f, int K)
+// Iterations from + 1 .. to of the open CG pass on A' e = d (d in mg.pr, e
+// in mg.pe), from 0: the pass's start; every launch stops with the pass
+// (SolveCtl::stop_pcg).  cur: which of dots[0], dots[4] holds the current
+// r.z, (iterations run - 1) & 1.  dots: [1] p.q, [2] r.r
+static int queue_pcg(m2d_fields2d *f, int from, int to, int &cur)
 {
     Mg &mg = f->mg;
     hipStream_t s = f->ctx->stream;
@@ -1300,13 +1325,24 @@ static int queue_pcg(m2d_fields2d *f, int K)
     const MgLevel &L0 = mg.lv[0];
     SolveCtl *ctl = solve_ctl(mg);
     const int *stop = &ctl->stop_pcg;
-    M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
-    int cur = 0;
-    reduce_launch<1>(rb, mg.partial, mg.dots, s, [&](int fin) {
-        k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots, fin, stop);
-    }, stop);
-    k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1, stop);
-    for (int it = 1; it <= K; it++) {
+    if (from == 0) {
+        M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
+        reduce_launch<1>(rb, mg.partial, mg.dots, s, [&](int fin) {
+            k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots, fin, stop);
+        }, stop);
+        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1, stop);
+        cur = 0;
+    }
+    for (int it = from + 1; it <= to; it++) {
+        if (it > 1) {
+            M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
+            reduce_launch<1>(rb, mg.partial, mg.dots + 4 * (cur ^ 1), s, [&](int fin) {
+                k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1),
+                                                   fin, stop);
+            }, stop);
+            k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0, stop);
+            cur ^= 1;
+        }
         reduce_launch<1>(rb, mg.partial, mg.dots + 1, s, [&](int fin) {
             k_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pp, mg.pq, n, L0.nz, mg.partial, mg.ticket, mg.dots + 1,
                                                   fin, stop);
@@ -1316,14 +1352,6 @@ static int queue_pcg(m2d_fields2d *f, int K)
                                                      it == 1, mg.partial, mg.ticket, mg.dots + 2, fin, stop);
         }, stop);
         k_ctl_check<<<1, 64, 0, s>>>(ctl, mg.dots);
-        if (it == K) break;
-        M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
-        reduce_launch<1>(rb, mg.partial, mg.dots + 4 * (cur ^ 1), s, [&](int fin) {
-            k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1), fin,
-                                               stop);
-        }, stop);
-        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0, stop);
-        cur ^= 1;
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -1362,12 +1390,8 @@ static int solve(m2d_fields2d *f, int mode, double tol, m2d_solve_info *info)
     k_ctl_init<<<1, 64, 0, s>>>(ctl);
     queue_residual(f, u, tol, kMaxOuter);
     M2D_TRY(queue_refinement(
-        s, ctl, h, mg.dots, mg.est, tol, 60,
-        [&](int K) {
-            M2D_TRY(queue_pcg(f, K));
-            k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n, &ctl->no_add);
-            return (int)M2D_OK;
-        },
+        s, ctl, h, mg.dots, mg.est, tol, 60, [&](int from, int to, int &cur) { return queue_pcg(f, from, to, cur); },
+        [&] { k_add_correction<<<gb, 256, 0, s>>>(f->mask, u, mg.pe, n, &ctl->no_add); },
         [&] { queue_residual(f, u, tol, kMaxOuter); }));
     if (info) { info->outer_iters = h->outer; info->inner_iters = h->iters; info->rel_residual = h->rel; }
     if (h->nan) return set_error(M2D_ERR_NAN, "Fields solve: residual is not finite");

@@ -224,6 +224,7 @@ struct SolveCtl {
     int pass_iters;      // in the open pass
     int max_pass_iters;  // the most in one pass
     int nan;             // the residual was not finite
+    int halted;          // a pass ran out of queued iterations: suspended until the host queues more
     double rel;          // FP64 relative residual at the last evaluation
     double thr;          // the open pass's stop: |r| <= rtol |d|
     int pass_count[kMaxPasses];  // CG iterations of each pass
@@ -251,40 +252,69 @@ __global__ void k_ctl_init(SolveCtl *c);
 __global__ void k_ctl_eval(SolveCtl *c, const double *dots, double tol, int max_outer);
 __global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol);
 __global__ void k_ctl_check(SolveCtl *c, const double *dots);
+__global__ void k_ctl_end(SolveCtl *c, int max_iters);
+__global__ void k_ctl_resume(SolveCtl *c);
 __global__ void k_ctl_close(SolveCtl *c);
 
 // FP64 iterative refinement over FP32 CG passes, queued under SolveCtl.  The
 // caller has queued the right-hand side, k_ctl_init and resid() (the FP64
 // residual, its norms and k_ctl_eval).  Passes are queued as many as the
-// last solve took, each with one CG iteration more than its counterpart took
-// then (the last pass of a refinement needs far fewer than the first); the
-// control block is read once after them, and a solve that needs more passes
-// or iterations gets another round of one longer pass at a time.
-// pass(K) queues one CG pass of at most K iterations and the correction's
-// addition (which skips itself when ctl->no_add is set).  The iterations that
-// run are those of a solve that tests convergence on the host after each,
-// so the result is the same while the estimates suffice.
-template <class Pass, class Resid>
+// last solve took, each with one CG iteration more than its counterpart
+// took then (the last pass of a refinement needs far fewer than the first);
+// the control block is read once after them.  A pass that needs more
+// iterations than queued is suspended on the device (k_ctl_end) and resumed
+// in the next round with twice as many; a refinement that needs more passes
+// gets one more a round.  So the iterations that run, and the result, are
+// those of a loop that tests on the host after each, whatever the estimates.
+// pcg(from, to, cur) queues iterations from + 1 .. to of the open pass,
+// add() the correction's addition (which skips itself while ctl->no_add).
+template <class Pcg, class Add, class Resid>
 int queue_refinement(hipStream_t s, SolveCtl *ctl, SolveCtl *h, const double *dots, SolveEst &est, double tol,
-                     int max_iters, Pass pass, Resid resid)
+                     int max_iters, Pcg pcg, Add add, Resid resid)
 {
-    int J = est.passes < 1 ? 1 : (est.passes > kMaxOuter ? kMaxOuter : est.passes), queued = 0, extra = 0;
-    for (int round = 0;; round++) {
+    auto finish_pass = [&] {
+        k_ctl_end<<<1, 64, 0, s>>>(ctl, max_iters);
+        add();
+        k_ctl_close<<<1, 64, 0, s>>>(ctl);
+        resid();
+    };
+    int J = est.passes < 1 ? 1 : (est.passes > kMaxOuter ? kMaxOuter : est.passes);
+    int queued = 0;      // passes opened so far
+    int resume = 0;      // iterations the suspended pass has run (0: none suspended)
+    // each round opens at least one pass or lengthens a suspended one by at
+    // least 4 of its max_iters iterations: a bound the rounds cannot reach
+    const int max_rounds = (kMaxOuter + 1) * (max_iters / 4 + 2);
+    for (int round = 0; round < max_rounds; round++) {
+        if (resume) {
+            int cur = (resume - 1) & 1;
+            const int to = resume + (resume > 4 ? resume : 4);
+            k_ctl_resume<<<1, 64, 0, s>>>(ctl);
+            if (int r = pcg(resume, to < max_iters ? to : max_iters, cur)) return r;
+            finish_pass();
+        }
         for (int j = 0; j < J; j++, queued++) {
             const int e = queued < kMaxPasses ? est.iters[queued] : est.max_iters;
-            int K = e + 1 + extra;
+            int K = e + 1;
             K = K < 2 ? 2 : (K > max_iters ? max_iters : K);
+            int cur = 0;
             k_ctl_open<<<1, 64, 0, s>>>(ctl, dots, tol);
-            if (int r = pass(K)) return r;
-            k_ctl_close<<<1, 64, 0, s>>>(ctl);
-            resid();
+            if (int r = pcg(0, K, cur)) return r;
+            finish_pass();
         }
         M2D_HIP(hipGetLastError());
         M2D_HIP(hipMemcpyAsync(h, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
         M2D_HIP(hipStreamSynchronize(s));
-        if (h->nan || h->stop_outer || round > kMaxOuter) break;
+        if (h->halted) {
+            // pass h->outer stopped after all its queued iterations: passes
+            // queued after it never opened
+            resume = h->pass_iters;
+            queued = h->outer + 1;
+            J = est.passes - queued > 0 ? est.passes - queued : 0;
+            continue;
+        }
+        if (h->nan || h->stop_outer) break;
+        resume = 0;
         J = 1;
-        extra = 2 * extra + 4 < max_iters ? 2 * extra + 4 : max_iters;
     }
     est.passes = h->outer > 1 ? h->outer : 1;
     est.max_iters = h->max_pass_iters;
