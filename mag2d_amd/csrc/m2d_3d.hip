@@ -107,9 +107,8 @@ __global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *ma
 {
     if (stopped(stop)) return;
     const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
-    const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     double v[3] = {0, 0, 0};
-    if (m < n) {
+    for (size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (size_t)gridDim.x * kRedBlock) {
         double rv = 0;
         if (!dir3(mask[m])) {
             // free rows never touch the grid faces (checked at creation)
@@ -121,13 +120,13 @@ __global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *ma
                 if (dir3(mask[nb[q]])) be -= u[nb[q]];
             }
             rv = b[m] - au;
-            v[1] = be * be;
+            v[1] += be * be;
         }
         r[m] = rv;
         const float dm = (float)(-rv);
         d[m] = dm;
-        v[0] = rv * rv;
-        v[2] = (double)dm * dm;
+        v[0] += rv * rv;
+        v[2] += (double)dm * dm;
     }
     finish_sums<3>(v, partial, ticket, out, fin);
 }
@@ -157,13 +156,12 @@ __global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *nbm, co
 {
     if (stopped(stop)) return;
     const size_t n = (size_t)nx * ny * nz;
-    const size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     double v[1] = {0};
-    if (m < n) {
+    for (size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (size_t)gridDim.x * kRedBlock) {
         const uint8_t nb = nbm[m];
         const float ym = (nb & kNbSelf) ? 0.f : apply7(nb, x, m, (size_t)ny * nz, nz);
         y[m] = ym;
-        v[0] = (double)x[m] * ym;
+        v[0] += (double)x[m] * ym;
     }
     finish_sums<1>(v, partial, ticket, out, fin);
 }
@@ -233,15 +231,36 @@ __global__ void __launch_bounds__(kRbgsX * kRbgsY) k3_rbgs(const uint8_t *nbm, f
     x[m] = s * (1.f / 6.f);
 }
 
-__global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx, int ny, int nz,
-                         const int *stop)
+// x = 0, then the colour-0 half sweep, in one pass over every node: from
+// zero, every neighbour a colour-0 node reads is +0, so its update is
+// (b + 0) / 6 -- the +0 once, which turns a -0 right-hand side into +0 as
+// the sweep's sums do, and not at all for a node with no free neighbour.
+// Blocks of 64 x 4 nodes (k, j) and a grid of (nz / 64, ny / 4, nx).
+__global__ void __launch_bounds__(256) k3_rbgs_from_zero(const uint8_t *nbm, float *x, const float *b, int nx,
+                                                         int ny, int nz, const int *stop)
 {
     if (stopped(stop)) return;
-    const size_t n = (size_t)nx * ny * nz;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    int i, j, k;
-    node3(m, ny, nz, i, j, k);
+    const int i = blockIdx.z, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.x * 64 + threadIdx.x;
+    if (j >= ny || k >= nz) return;
+    const size_t m = ((size_t)i * ny + j) * nz + k;
+    const uint8_t nb = nbm[m];
+    float v = 0.f;
+    if (((i + j + k) & 1) == 0 && !(nb & kNbSelf)) {
+        float t = b[m];
+        if (nb & 63) t += 0.f;
+        v = t * (1.f / 6.f);
+    }
+    x[m] = v;
+}
+
+// r = b - A' x; blocks of 64 x 4 nodes (k, j) and a grid of (nz / 64, ny / 4, nx)
+__global__ void __launch_bounds__(256) k3_resid(const uint8_t *nbm, const float *x, const float *b, float *r, int nx,
+                                                int ny, int nz, const int *stop)
+{
+    if (stopped(stop)) return;
+    const int i = blockIdx.z, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.x * 64 + threadIdx.x;
+    if (j >= ny || k >= nz) return;
+    const size_t m = ((size_t)i * ny + j) * nz + k;
     const uint8_t nb = nbm[m];
     const float bm = b[m], xm = x[m];
     const Nb7 q = neighbours7(x, m, i, j, k, nx, ny, nz);
@@ -260,44 +279,70 @@ __global__ void k3_resid(const uint8_t *nbm, const float *x, const float *b, flo
 // 1D weights of the trilinear prolongation: fine i from coarse I
 __device__ __forceinline__ float w1(int i, int I) { int d = i - 2 * I; return d == 0 ? 1.f : (d == 1 || d == -1 ? 0.5f : 0.f); }
 
-// b_c = P^T r_f / 2 at free coarse node M
+// b_c = P^T r_f / 2 at free coarse node M = (I, J, K).  The 27 fine nodes
+// are summed in (i, j, k) order; one outside the grid reads a node inside
+// with weight 0, which adds +0 to a sum that never holds -0: the same bits
+// as skipping it, with every load issued at once.
+__device__ __forceinline__ float node3_restrict_at(const float *rf, int fx, int fy, int fz, int I, int J, int K)
+{
+    float w[3][3];
+    int ix[3][3];
+    const int c[3] = {I, J, K}, nf[3] = {fx, fy, fz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int f = 2 * c[d] - 1 + t;
+            const bool in = f >= 0 && f < nf[d];
+            ix[d][t] = in ? f : 2 * c[d];
+            w[d][t] = in ? w1(f, c[d]) : 0.f;
+        }
+    }
+    float v[27];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int e = 0; e < 3; e++)
+                v[(a * 3 + b) * 3 + e] = rf[((size_t)ix[0][a] * fy + ix[1][b]) * fz + ix[2][e]];
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const float wij = w[0][a] * w[1][b];
+#pragma unroll
+            for (int e = 0; e < 3; e++) s = fmaf(wij * w[2][e], v[(a * 3 + b) * 3 + e], s);  // r is 0 on Dirichlet nodes
+        }
+    return 0.5f * s;
+}
+
 __device__ __forceinline__ float node3_restrict(const float *rf, int fx, int fy, int fz, const uint8_t *dirc, int cy,
                                                 int cz, size_t M)
 {
     if (dirc[M]) return 0.f;
     int I, J, K;
     node3(M, cy, cz, I, J, K);
-    float s = 0.f;
-    for (int i = 2 * I - 1; i <= 2 * I + 1; i++) {
-        if (i < 0 || i >= fx) continue;
-        for (int j = 2 * J - 1; j <= 2 * J + 1; j++) {
-            if (j < 0 || j >= fy) continue;
-            float wij = w1(i, I) * w1(j, J);
-            for (int k = 2 * K - 1; k <= 2 * K + 1; k++) {
-                if (k < 0 || k >= fz) continue;
-                s = fmaf(wij * w1(k, K), rf[((size_t)i * fy + j) * fz + k], s);  // r is 0 on Dirichlet nodes
-            }
-        }
-    }
-    return 0.5f * s;
+    return node3_restrict_at(rf, fx, fy, fz, I, J, K);
 }
 
-__global__ void k3_restrict(const float *rf, int fx, int fy, int fz, float *bc, const uint8_t *dirc, int cx,
-                            int cy, int cz, const int *stop)
+// blocks of 64 x 4 coarse nodes (K, J) and a grid of (cz / 64, cy / 4, cx):
+// the coordinates come from the launch, not from divisions
+__global__ void __launch_bounds__(256) k3_restrict(const float *rf, int fx, int fy, int fz, float *bc,
+                                                   const uint8_t *dirc, int cx, int cy, int cz, const int *stop)
 {
     if (stopped(stop)) return;
-    const size_t N = (size_t)cx * cy * cz;
-    size_t M = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (M >= N) return;
-    bc[M] = node3_restrict(rf, fx, fy, fz, dirc, cy, cz, M);
+    const int I = blockIdx.z, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.x * 64 + threadIdx.x;
+    if (J >= cy || K >= cz) return;
+    const size_t M = ((size_t)I * cy + J) * cz + K;
+    bc[M] = dirc[M] ? 0.f : node3_restrict_at(rf, fx, fy, fz, I, J, K);
 }
 
-// x_f += P x_c at fine node m (a free node)
-__device__ __forceinline__ void node3_prolong_free(float *xf, int fy, int fz, const float *xc, int cx, int cy, int cz,
-                                                   size_t m)
+// x_f += P x_c at free fine node m = (i, j, k)
+__device__ __forceinline__ void node3_prolong_at(float *xf, int i, int j, int k, const float *xc, int cx, int cy,
+                                                 int cz, size_t m)
 {
-    int i, j, k;
-    node3(m, fy, fz, i, j, k);
     int Ia[2], Ja[2], Ka[2];
     float wa[2], wb[2], wc[2];
     int ni = 0, nj = 0, nk = 0;
@@ -320,6 +365,15 @@ __device__ __forceinline__ void node3_prolong_free(float *xf, int fy, int fz, co
     xf[m] += s;
 }
 
+// x_f += P x_c at fine node m (a free node)
+__device__ __forceinline__ void node3_prolong_free(float *xf, int fy, int fz, const float *xc, int cx, int cy, int cz,
+                                                   size_t m)
+{
+    int i, j, k;
+    node3(m, fy, fz, i, j, k);
+    node3_prolong_at(xf, i, j, k, xc, cx, cy, cz, m);
+}
+
 // x_f += P x_c at fine node m when it is free
 __device__ __forceinline__ void node3_prolong(float *xf, const uint8_t *dirf, int fy, int fz, const float *xc, int cx,
                                               int cy, int cz, size_t m)
@@ -327,15 +381,16 @@ __device__ __forceinline__ void node3_prolong(float *xf, const uint8_t *dirf, in
     if (!dirf[m]) node3_prolong_free(xf, fy, fz, xc, cx, cy, cz, m);
 }
 
-// x_f += P x_c on free fine nodes
-__global__ void k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int fz, const float *xc, int cx,
-                           int cy, int cz, const int *stop)
+// x_f += P x_c on free fine nodes; blocks of 64 x 4 nodes (k, j) and a grid
+// of (fz / 64, fy / 4, fx)
+__global__ void __launch_bounds__(256) k3_prolong(float *xf, const uint8_t *dirf, int fx, int fy, int fz,
+                                                  const float *xc, int cx, int cy, int cz, const int *stop)
 {
     if (stopped(stop)) return;
-    const size_t n = (size_t)fx * fy * fz;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    node3_prolong(xf, dirf, fy, fz, xc, cx, cy, cz, m);
+    const int i = blockIdx.z, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.x * 64 + threadIdx.x;
+    if (j >= fy || k >= fz) return;
+    const size_t m = ((size_t)i * fy + j) * fz + k;
+    if (!dirf[m]) node3_prolong_at(xf, i, j, k, xc, cx, cy, cz, m);
 }
 
 // ---- the coarse levels [lc, L) of the V-cycle in one workgroup: the same
@@ -1209,12 +1264,18 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out, const int *sto
 {
     hipStream_t s = f->ctx->stream;
     const int L = (int)f->lv.size();
-    auto smooth = [&](Mg3Level &V, float *x, const float *b, int sweeps, bool forward) {
+    // from_zero: x starts at 0 (the first half sweep also clears x)
+    auto smooth = [&](Mg3Level &V, float *x, const float *b, int sweeps, bool forward, bool from_zero = false) {
         const dim3 g((unsigned)(((V.nz + 1) / 2 + kRbgsX - 1) / kRbgsX), (unsigned)((V.ny + kRbgsY - 1) / kRbgsY),
                      (unsigned)V.nx);
         const dim3 blk(kRbgsX, kRbgsY);
         for (int it = 0; it < sweeps; it++) {
-            k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1, stop);
+            if (it == 0 && from_zero && forward) {
+                k3_rbgs_from_zero<<<dim3((unsigned)((V.nz + 63) / 64), (unsigned)((V.ny + 3) / 4), (unsigned)V.nx),
+                                    dim3(64, 4), 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, stop);
+            } else {
+                k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 0 : 1, stop);
+            }
             k3_rbgs<<<g, blk, 0, s>>>(V.nbm, x, b, V.nx, V.ny, V.nz, forward ? 1 : 0, stop);
         }
     };
@@ -1224,11 +1285,12 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out, const int *sto
         Mg3Level &V = f->lv[l];
         float *x = l == 0 ? out : V.x;
         const float *b = l == 0 ? rhs : V.b;
-        k3_zero<<<blocks_for(V.n), 256, 0, s>>>(x, V.n, stop);
-        smooth(V, x, b, kPre3, true);
-        k3_resid<<<blocks_for(V.n), 256, 0, s>>>(V.nbm, x, b, V.r, V.nx, V.ny, V.nz, stop);
+        smooth(V, x, b, kPre3, true, true);
+        k3_resid<<<dim3((unsigned)((V.nz + 63) / 64), (unsigned)((V.ny + 3) / 4), (unsigned)V.nx), dim3(64, 4), 0,
+                   s>>>(V.nbm, x, b, V.r, V.nx, V.ny, V.nz, stop);
         Mg3Level &C = f->lv[l + 1];
-        k3_restrict<<<blocks_for(C.n), 256, 0, s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz, stop);
+        k3_restrict<<<dim3((unsigned)((C.nz + 63) / 64), (unsigned)((C.ny + 3) / 4), (unsigned)C.nx), dim3(64, 4), 0,
+                      s>>>(V.r, V.nx, V.ny, V.nz, C.b, C.dir, C.nx, C.ny, C.nz, stop);
     }
     if (lc < L) {
         Coarse3Args C;
@@ -1269,7 +1331,8 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out, const int *sto
         Mg3Level &C = f->lv[l + 1];
         float *x = l == 0 ? out : V.x;
         const float *b = l == 0 ? rhs : V.b;
-        k3_prolong<<<blocks_for(V.n), 256, 0, s>>>(x, V.dir, V.nx, V.ny, V.nz, C.x, C.nx, C.ny, C.nz, stop);
+        k3_prolong<<<dim3((unsigned)((V.nz + 63) / 64), (unsigned)((V.ny + 3) / 4), (unsigned)V.nx), dim3(64, 4), 0,
+                     s>>>(x, V.dir, V.nx, V.ny, V.nz, C.x, C.nx, C.ny, C.nz, stop);
         smooth(V, x, b, kPost3, false);
     }
     M2D_HIP(hipGetLastError());
@@ -1277,6 +1340,19 @@ static int vcycle3(m2d_fields3d *f, const float *rhs, float *out, const int *sto
 }
 
 static SolveCtl *solve_ctl3(m2d_fields3d *f) { return (SolveCtl *)(f->dots + 16); }
+
+// blocks of the fused reductions over the finest level: one node a thread up
+// to kRed3Blocks blocks (524288 nodes), then that many, whose threads stride
+// over several nodes (each block's reduction and partial sum then serve more
+// nodes; the FP64 sums' order changes with the count, and with it their last
+// bits, which the FP32 CG coefficients round away in practice).  At 128^3,
+// 2048 blocks made the step 3% faster than one node a thread, 1024 and 4096
+// less so.
+constexpr size_t kRed3Blocks = 2048;
+static unsigned red_blocks3(size_t n)
+{
+    return (unsigned)std::min<size_t>((n + kRedBlock - 1) / kRedBlock, kRed3Blocks);
+}
 
 // Iterations from + 1 .. to of the open FP32 PCG pass on A' e = d (d in pr,
 // e in pe), from 0: the pass's start; every launch stops with the pass
@@ -1286,7 +1362,7 @@ static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
 {
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
-    const unsigned gb = blocks_for(n), rb = (unsigned)((n + kRedBlock - 1) / kRedBlock);
+    const unsigned gb = blocks_for(n), rb = red_blocks3(n);
     const Mg3Level &V = f->lv[0];
     SolveCtl *ctl = solve_ctl3(f);
     const int *stop = &ctl->stop_pcg;
@@ -1326,7 +1402,7 @@ static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
 static void queue_residual3(m2d_fields3d *f, double tol)
 {
     hipStream_t s = f->ctx->stream;
-    const unsigned rb = (unsigned)((f->n + kRedBlock - 1) / kRedBlock);
+    const unsigned rb = red_blocks3(f->n);
     SolveCtl *ctl = solve_ctl3(f);
     const int *stop = &ctl->stop_outer;
     reduce_launch<3>(rb, f->partial, f->dots + 8, s, [&](int fin) {

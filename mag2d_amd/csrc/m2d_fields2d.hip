@@ -360,14 +360,13 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r,
 {
     if (stopped(stop)) return;
     double v[1] = {0};
-    const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
-    if (k < n) {
-        const double den = *pq;
-        const float alpha = den != 0.0 ? (float)(*rz / den) : 0.f;
+    const double den = *pq;
+    const float alpha = den != 0.0 ? (float)(*rz / den) : 0.f;
+    for (size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x; k < n; k += (size_t)gridDim.x * kRedBlock) {
         e[k] = fmaf(alpha, p[k], first ? 0.f : e[k]);
         const float rk = fmaf(-alpha, q[k], r[k]);
         r[k] = rk;
-        v[0] = (double)rk * rk;
+        v[0] += (double)rk * rk;
     }
     finish_sums<1>(v, partial, ticket, out, fin);
 }
