@@ -931,7 +931,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
                     const int off = ((q & 1) ? kWin3 * kWin3 : 0) + (((q >> 1) & 1) ? kWin3 : 0) + (q >> 2);
-                    if (o.w8[q]) atomicAdd(w + off, (unsigned long long)o.w8[q]);
+                    atomicAdd(w + off, (unsigned long long)o.w8[q]);  // a zero weight adds 0: no branch
                 }
             } else {
                 deposit8_global(a.acc, a.g, o.cell, o.w8);
