@@ -343,25 +343,38 @@ __global__ void __launch_bounds__(256) k3_restrict(const float *rf, int fx, int 
 __device__ __forceinline__ void node3_prolong_at(float *xf, int i, int j, int k, const float *xc, int cx, int cy,
                                                  int cz, size_t m)
 {
+    // per axis the coarse nodes of fine index f: f / 2 (weight 1) when f is
+    // even, else (f - 1) / 2 and (f + 1) / 2 (0.5 each; the second can lie
+    // past the coarse grid).  All 8 combinations are summed in (a, b, c)
+    // order; an absent node is read at the first with weight 0, which adds
+    // +0 to a sum that never holds -0 -- the same bits as skipping it, with
+    // the loads issued together.
     int Ia[2], Ja[2], Ka[2];
     float wa[2], wb[2], wc[2];
-    int ni = 0, nj = 0, nk = 0;
-    auto split = [](int f, int nc, int *I, float *w, int &c) {
-        c = 0;
-        if ((f & 1) == 0) { if (f / 2 < nc) { I[c] = f / 2; w[c++] = 1.f; } }
-        else {
-            if ((f - 1) / 2 < nc) { I[c] = (f - 1) / 2; w[c++] = 0.5f; }
-            if ((f + 1) / 2 < nc) { I[c] = (f + 1) / 2; w[c++] = 0.5f; }
-        }
+    auto split = [](int f, int nc, int *I, float *w) {
+        const bool odd = f & 1, second = odd && (f + 1) / 2 < nc;
+        I[0] = f >> 1;
+        w[0] = odd ? 0.5f : 1.f;
+        I[1] = second ? (f + 1) / 2 : I[0];
+        w[1] = second ? 0.5f : 0.f;
     };
-    split(i, cx, Ia, wa, ni);
-    split(j, cy, Ja, wb, nj);
-    split(k, cz, Ka, wc, nk);
+    split(i, cx, Ia, wa);
+    split(j, cy, Ja, wb);
+    split(k, cz, Ka, wc);
+    float v[8];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) v[(a * 2 + b) * 2 + c] = xc[((size_t)Ia[a] * cy + Ja[b]) * cz + Ka[c]];
     float s = 0.f;
-    for (int a = 0; a < ni; a++)
-        for (int b = 0; b < nj; b++)
-            for (int c = 0; c < nk; c++)
-                s = fmaf(wa[a] * wb[b] * wc[c], xc[((size_t)Ia[a] * cy + Ja[b]) * cz + Ka[c]], s);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) s = fmaf(wa[a] * wb[b] * wc[c], v[(a * 2 + b) * 2 + c], s);
     xf[m] += s;
 }
 
