@@ -568,12 +568,12 @@ __global__ void __launch_bounds__(kCoarseThreads) k3_vcycle_coarse_lds(Coarse3Ar
             const int c0 = p * len, c1 = min(cn, c0 + len);
             float s = 0.f;
             int c = c0;
-            for (; c + 8 <= c1; c += 8) {
-                float a[8];
+            for (; c + 32 <= c1; c += 32) {  // 32 loads in flight: the loop waits on L2
+                float a[32];
 #pragma unroll
-                for (int q = 0; q < 8; q++) a[q] = C.invT[(size_t)(c + q) * cn + row];
+                for (int q = 0; q < 32; q++) a[q] = C.invT[(size_t)(c + q) * cn + row];
 #pragma unroll
-                for (int q = 0; q < 8; q++) s = fmaf(a[q], s_zb[c + q], s);
+                for (int q = 0; q < 32; q++) s = fmaf(a[q], s_zb[c + q], s);
             }
             for (; c < c1; c++) s = fmaf(C.invT[(size_t)c * cn + row], s_zb[c], s);
             part[p * cn + row] = s;
