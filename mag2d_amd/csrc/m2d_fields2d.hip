@@ -418,7 +418,7 @@ __device__ __forceinline__ float prolong_at(const float *xc, int nxc, int nzc, i
 }
 
 // ---- the finest level in LDS tiles.  A block holds a tile of kT0 x kT0
-// nodes (32, or 16 on grids of at most 256^2 nodes, for more blocks) plus a halo and runs several red-black half sweeps on it with
+// nodes (16, 8 or 4 by the grid's size, see l0_tile) plus a halo and runs several red-black half sweeps on it with
 // workgroup barriers in between.  A node at distance d from the halo's edge
 // keeps the exact global value for d half sweeps after the first, so the
 // halo is as wide as the sweeps (and the residual stencil) need.  Updates
@@ -519,10 +519,14 @@ __global__ void __launch_bounds__(kT0Threads) k_l0_down(const uint8_t *dir, cons
     }
 }
 
-// ascent on the finest level: x += P x_c, then the post-smoothing sweeps
+// ascent on the finest level: x = xd + P x_c, then the post-smoothing
+// sweeps.  xd (the descent's x) and x are different arrays: a block reads
+// its halo from xd, and other blocks of the launch, not all resident at
+// once on a large grid, write x.
 template <int kT0>
-__global__ void __launch_bounds__(kT0Threads) k_l0_up(const uint8_t *dir, const float *b, float *x, int nx, int nz,
-                                                    const float *xc, int nxc, int nzc, const int *stop)
+__global__ void __launch_bounds__(kT0Threads) k_l0_up(const uint8_t *dir, const float *b, const float *xd, float *x,
+                                                    int nx, int nz, const float *xc, int nxc, int nzc,
+                                                    const int *stop)
 {
     if (stopped(stop)) return;
     constexpr int S = kT0 + 2 * kH0u;
@@ -535,7 +539,7 @@ __global__ void __launch_bounds__(kT0Threads) k_l0_up(const uint8_t *dir, const 
         const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
         const size_t k = (size_t)i * nz + j;
         const uint8_t d = in ? dir[k] : (uint8_t)1;
-        float xv = in ? x[k] : 0.f;
+        float xv = in ? xd[k] : 0.f;
         if (!d) xv += prolong_at(xc, nxc, nzc, i, j);  // node_prolong
         sd[t] = d;
         sb[t] = in ? b[k] : 0.f;
@@ -555,7 +559,11 @@ __global__ void __launch_bounds__(kT0Threads) k_l0_up(const uint8_t *dir, const 
 // ---- the Galerkin levels in LDS tiles (damped Jacobi, 9-point operator).
 // The first sweep starts from x = 0 and reads no neighbour; each later sweep,
 // the residual and the restriction widen the inexact border by one ring.
-constexpr int kTm = 16;
+// Galerkin levels' tile edge: 4 nodes, so that even the 64 x 64 level of a
+// 128^2 grid spreads over 256 blocks (16 blocks at an edge of 16); the halo
+// loads cost less than the idle CUs did (Penning trap step -17%, the
+// 512^2 RF basis solve -23%)
+constexpr int kTm = 4;
 constexpr int kHmd = kPre + 1;  // descent: kPre - 1 sweeps, residual, restriction
 constexpr int kHmu = kPost;     // ascent: kPost sweeps
 
@@ -677,10 +685,12 @@ __global__ void __launch_bounds__(kT0Threads) k_lm_down(const float *st, const f
     }
 }
 
-// ascent on a Galerkin level: x += P x_c on free nodes, then the sweeps
+// ascent on a Galerkin level: xo = x + P x_c on free nodes, then the
+// sweeps.  x (the descent's) and xo are different arrays: a block reads its
+// halo from x while other blocks of the launch write xo.
 __global__ void __launch_bounds__(kT0Threads) k_lm_up(const float *st, const float *dinv, const uint8_t *dir,
-                                                    const float *b, float *x, int nx, int nz, const float *xc,
-                                                    int nxc, int nzc, float w, const int *stop)
+                                                    const float *b, const float *x, float *xo, int nx, int nz,
+                                                    const float *xc, int nxc, int nzc, float w, const int *stop)
 {
     if (stopped(stop)) return;
     constexpr int S = kTm + 2 * kHmu;
@@ -705,7 +715,7 @@ __global__ void __launch_bounds__(kT0Threads) k_lm_up(const float *st, const flo
     }
     for (int t = threadIdx.x; t < kTm * kTm; t += kT0Threads) {
         const int i = gi0 + t / kTm, j = gj0 + t % kTm;
-        if (i < nx && j < nz) x[(size_t)i * nz + j] = xa[(i - i0) * S + (j - j0)];
+        if (i < nx && j < nz) xo[(size_t)i * nz + j] = xa[(i - i0) * S + (j - j0)];
     }
 }
 
@@ -1154,6 +1164,12 @@ static void mg_free(Mg &mg)
 // one V-cycle: the finest level in grid kernels (first sweep from zero,
 // fused residual + restriction, fused prolongation + black sweep), middle
 // levels in grid kernels, and every level from mg.lc down in one workgroup
+// edge of the finest level's LDS tiles: more, smaller tiles on smaller grids,
+// whose V-cycle is bound by latency rather than bytes (with edges of 16 and
+// 32 the 128^2 and 512^2 solves kept most CUs idle)
+constexpr int kT0Tiny = 4, kT0Small = 8, kT0Large = 16;
+static int l0_tile(size_t n) { return n <= 16384 ? kT0Tiny : n <= 65536 ? kT0Small : kT0Large; }
+
 static int vcycle(m2d_fields2d *f, const float *rhs, float *out, const int *stop)
 {
     Mg &mg = f->mg;
@@ -1166,13 +1182,19 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out, const int *stop
         return M2D_OK;
     }
     // ---- level 0 descent (LDS tiles: smoothing, residual, restriction)
-    const bool small = A0.n <= 65536;
-    const int T0 = small ? 16 : 32;
+    const int T0 = l0_tile(A0.n);
     const dim3 gt((A0.nz + T0 - 1) / T0, (A0.nx + T0 - 1) / T0);
     {
         MgLevel &C = mg.lv[1];
-        if (small) k_l0_down<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz, stop);
-        else k_l0_down<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz, stop);
+        auto down = [&](auto kern) {
+            kern<<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, A0.x, A0.nx, A0.nz, C.b, C.dinv, C.nx, C.nz, stop);
+        };
+        switch (T0) {
+        case 4: down(k_l0_down<4>); break;
+        case 8: down(k_l0_down<8>); break;
+        case 16: down(k_l0_down<16>); break;
+        default: down(k_l0_down<32>); break;
+        }
     }
     // ---- middle levels [1, lc) in LDS tiles
     for (int l = 1; l < mg.lc; l++) {
@@ -1202,19 +1224,28 @@ static int vcycle(m2d_fields2d *f, const float *rhs, float *out, const int *stop
         else
             k_vcycle_coarse<<<1, kCoarseThreads, sizeof(float) * mg.coarse_n * (1 + kDenseParts), s>>>(C);
     }
-    // ---- middle levels ascent
+    // ---- middle levels ascent: the descent's x in lv[l].x, the ascent's in
+    // lv[l].x2 (level lc's from the one-workgroup kernel in lv[lc].x)
+    auto ascent_x = [&](int l) -> const float * { return l >= mg.lc ? mg.lv[l].x : mg.lv[l].x2; };
     for (int l = mg.lc - 1; l >= 1; l--) {
         MgLevel &A = mg.lv[l];
         MgLevel &Cc = mg.lv[l + 1];
         const dim3 g((A.nz + kTm - 1) / kTm, (A.nx + kTm - 1) / kTm);
-        k_lm_up<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.dir, A.b, A.x, A.nx, A.nz, Cc.x, Cc.nx, Cc.nz, kOmega,
-                                         stop);
+        k_lm_up<<<g, kT0Threads, 0, s>>>(A.st, A.dinv, A.dir, A.b, A.x, A.x2, A.nx, A.nz, ascent_x(l + 1), Cc.nx,
+                                         Cc.nz, kOmega, stop);
     }
     // ---- level 0 ascent (LDS tiles: prolongation, smoothing)
     {
         MgLevel &C = mg.lv[1];
-        if (small) k_l0_up<16><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz, stop);
-        else k_l0_up<32><<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, out, A0.nx, A0.nz, C.x, C.nx, C.nz, stop);
+        auto up = [&](auto kern) {
+            kern<<<gt, kT0Threads, 0, s>>>(A0.dir, rhs, A0.x, out, A0.nx, A0.nz, ascent_x(1), C.nx, C.nz, stop);
+        };
+        switch (T0) {
+        case 4: up(k_l0_up<4>); break;
+        case 8: up(k_l0_up<8>); break;
+        case 16: up(k_l0_up<16>); break;
+        default: up(k_l0_up<32>); break;
+        }
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
