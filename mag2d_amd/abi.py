@@ -16,6 +16,8 @@ import numpy as np
 _LIB_PATH = Path(__file__).resolve().parent / "libmag2d_amd.so"
 
 M2D_OK = 0
+ABI_VERSION = 2  # M2D_ABI_VERSION of include/mag2d_amd.h
+
 STATUS = {-1: "BAD_ARG", -2: "OUT_OF_RANGE", -3: "NAN", -4: "HIP_ERROR", -5: "NOT_CONVERGED",
           -6: "UNSUPPORTED", -7: "NO_MEMORY"}
 FLAG_OUT_OF_RANGE, FLAG_NAN, FLAG_PARTNER, FLAG_EVENTS = 1, 2, 4, 8
@@ -245,6 +247,11 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        # the structures below follow this layout version (m2d_step_stats,
+        # m2d_adjoint_cfg): a library of another one would read them wrongly
+        if L.m2d_abi_version() != ABI_VERSION:
+            raise ImportError(f"{_LIB_PATH} has ABI version {L.m2d_abi_version()}, this binding "
+                              f"{ABI_VERSION}: rebuild it with `python -m mag2d_amd.build`")
         _lib = L
     return _lib
 

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert abi.lib().m2d_abi_version() == 2
+    assert abi.lib().m2d_abi_version() == abi.ABI_VERSION == 2
 
 
 @pytest.mark.parametrize("geo", range(10))
