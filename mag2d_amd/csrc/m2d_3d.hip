@@ -103,7 +103,7 @@ __global__ void k3_add_correction(const uint8_t *mask, double *u, const float *e
 __global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *mask, const double *b, const double *u,
                                                               int nx, int ny, int nz, double *r, float *d,
                                                               double *partial, unsigned int *ticket, double *out,
-                                                              int fin, const int *stop)
+                                                              int fin, const int *stop, CtlEval post)
 {
     if (stopped(stop)) return;
     const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kRedBlock) k3_residual64_fin(const uint8_t *ma
         v[0] += rv * rv;
         v[2] += (double)dm * dm;
     }
-    finish_sums<3>(v, partial, ticket, out, fin);
+    finish_sums<3>(v, partial, ticket, out, fin, post);
 }
 
 // Neighbour masks of the 7-point levels: one byte per node replaces the
@@ -1403,9 +1403,9 @@ static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
         }, stop);
         reduce_launch<1>(rb, f->partial, f->dots + 2, s, [&](int fin) {
             k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots + 4 * cur, f->dots + 1, n,
-                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin, stop);
-        }, stop);
-        k_ctl_check<<<1, 64, 0, s>>>(ctl, f->dots);
+                                                     it == 1, f->partial, f->ticket, f->dots + 2, fin, stop,
+                                                     CtlCheck{ctl});
+        }, stop, CtlCheck{ctl});
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -1418,11 +1418,11 @@ static void queue_residual3(m2d_fields3d *f, double tol)
     const unsigned rb = red_blocks3(f->n);
     SolveCtl *ctl = solve_ctl3(f);
     const int *stop = &ctl->stop_outer;
+    const CtlEval post{ctl, tol, kMaxOuter};
     reduce_launch<3>(rb, f->partial, f->dots + 8, s, [&](int fin) {
         k3_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, f->u, f->nx, f->ny, f->nz, f->r64, f->pr,
-                                                   f->partial, f->ticket, f->dots + 8, fin, stop);
-    }, stop);
-    k_ctl_eval<<<1, 64, 0, s>>>(ctl, f->dots, tol, kMaxOuter);
+                                                   f->partial, f->ticket, f->dots + 8, fin, stop, post);
+    }, stop, post);
 }
 
 static int build_tables3(m2d_fields3d *f)

@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, co
 __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r, const float *p, const float *q,
                                                             const double *rz, const double *pq, size_t n, int first,
                                                             double *partial, unsigned int *ticket, double *out,
-                                                            int fin, const int *stop)
+                                                            int fin, const int *stop, CtlCheck post)
 {
     if (stopped(stop)) return;
     double v[1] = {0};
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r,
         r[k] = rk;
         v[0] += (double)rk * rk;
     }
-    finish_sums<1>(v, partial, ticket, out, fin);
+    finish_sums<1>(v, partial, ticket, out, fin, post);
 }
 
 // the FP64 residual with its norms finished in the same launch: out[0] =
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update_dot(float *e, float *r,
 __global__ void __launch_bounds__(kRedBlock) k_residual64_fin(const uint8_t *mask, const double *b, const double *u,
                                                              size_t n, int nz, double *r, float *d, double *partial,
                                                              unsigned int *ticket, double *out, int fin,
-                                                             const int *stop)
+                                                             const int *stop, CtlEval post)
 {
     if (stopped(stop)) return;
     double v[3] = {0, 0, 0};
@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(kRedBlock) k_residual64_fin(const uint8_t *mas
         node_residual64(mask, b, u, nz, r, d, k, v[0], v[1]);
         v[2] = (double)d[k] * d[k];
     }
-    finish_sums<3>(v, partial, ticket, out, fin);
+    finish_sums<3>(v, partial, ticket, out, fin, post);
 }
 
 // prolongation value at fine node (i, j) (the increment of node_prolong)
@@ -1264,22 +1264,6 @@ __global__ void k_ctl_init(SolveCtl *c)
     *c = z;
 }
 
-// after k_residual64_fin (dots[8] = |r|^2, dots[9] = |b_eff|^2): the
-// relative residual and whether the refinement is done
-__global__ void k_ctl_eval(SolveCtl *c, const double *dots, double tol, int max_outer)
-{
-    if (threadIdx.x || c->stop_outer) return;
-    const double rn = sqrt(dots[8]), bn = sqrt(dots[9]);
-    const double rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
-    c->rel = rel;
-    if (!isfinite(rn)) {
-        c->nan = 1;
-        c->stop_outer = 1;
-    } else if (rel <= tol || c->outer >= max_outer) {
-        c->stop_outer = 1;
-    }
-}
-
 // opens a CG pass on the FP32 residual d (|d|^2 = dots[10])
 __global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol)
 {
@@ -1295,17 +1279,6 @@ __global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol)
     c->thr = inner_rtol(c->rel, tol) * r0;
     c->stop_pcg = 0;
     c->no_add = 0;
-}
-
-// after k_cg_update_dot of an iteration (dots[2] = |r|^2)
-__global__ void k_ctl_check(SolveCtl *c, const double *dots)
-{
-    if (threadIdx.x || c->stop_pcg) return;
-    c->iters++;
-    c->pass_iters++;
-    if (c->pass_iters > c->max_pass_iters) c->max_pass_iters = c->pass_iters;
-    if (c->outer < kMaxPasses) c->pass_count[c->outer] = c->pass_iters;
-    if (sqrt(dots[2]) <= c->thr) c->stop_pcg = 1;
 }
 
 // after the last iteration queued for a pass: a pass that has neither
@@ -1379,9 +1352,9 @@ static int queue_pcg(m2d_fields2d *f, int from, int to, int &cur)
         }, stop);
         reduce_launch<1>(rb, mg.partial, mg.dots + 2, s, [&](int fin) {
             k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
-                                                     it == 1, mg.partial, mg.ticket, mg.dots + 2, fin, stop);
-        }, stop);
-        k_ctl_check<<<1, 64, 0, s>>>(ctl, mg.dots);
+                                                     it == 1, mg.partial, mg.ticket, mg.dots + 2, fin, stop,
+                                                     CtlCheck{ctl});
+        }, stop, CtlCheck{ctl});
     }
     M2D_HIP(hipGetLastError());
     return M2D_OK;
@@ -1396,11 +1369,11 @@ static void queue_residual(m2d_fields2d *f, double *u, double tol, int max_outer
     const unsigned rb = red_blocks(n);
     SolveCtl *ctl = solve_ctl(mg);
     const int *stop = &ctl->stop_outer;
+    const CtlEval post{ctl, tol, max_outer};
     reduce_launch<3>(rb, mg.partial, mg.dots + 8, s, [&](int fin) {
         k_residual64_fin<<<rb, kRedBlock, 0, s>>>(f->mask, f->rho, u, n, f->nz, mg.r64, mg.pr, mg.partial, mg.ticket,
-                                                  mg.dots + 8, fin, stop);
-    }, stop);
-    k_ctl_eval<<<1, 64, 0, s>>>(ctl, mg.dots, tol, max_outer);
+                                                  mg.dots + 8, fin, stop, post);
+    }, stop, post);
 }
 
 // FP64 iterative refinement of u over FP32 multigrid-preconditioned CG

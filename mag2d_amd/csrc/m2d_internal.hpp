@@ -194,9 +194,6 @@ __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const 
 int noalloc_init_device(int device);
 __global__ void k_dot_fin(const float *a, const float *b, size_t n, double *partial, unsigned int *ticket,
                           double *out, int fin, const int *stop);
-__global__ void k_cg_update_dot(float *e, float *r, const float *p, const float *q, const double *rz,
-                                const double *pq, size_t n, int first, double *partial, unsigned int *ticket,
-                                double *out, int fin, const int *stop);
 constexpr int kCoarseThreads = 1024;
 constexpr int kDenseParts = 4;  // threads per row of the dense coarsest solves
 __global__ void k_dense_solve(const float *invT, const float *b, float *x, int n, const int *stop);
@@ -240,6 +237,51 @@ __host__ __device__ inline double inner_rtol(double rel, double tol)
 }
 
 
+// The control updates that read a reduction's sums run in the thread that
+// writes them, at the end of the reduction (finish_sums / k_sum_partials):
+// - after the FP64 residual's norms (|r|^2, |b_eff|^2): the relative
+//   residual and whether the refinement is done;
+// - after a CG iteration's |r|^2: the iteration counts, and whether the
+//   open pass has converged.
+// Both reductions are launches gated by the flag they may set, so a stopped
+// solve skips them with the rest.
+struct CtlEval {
+    SolveCtl *c;
+    double tol;
+    int max_outer;
+    __device__ void operator()(const double *out) const
+    {
+        const double rn = sqrt(out[0]), bn = sqrt(out[1]);
+        const double rel = bn > 0 ? rn / bn : (rn > 0 ? INFINITY : 0.0);
+        c->rel = rel;
+        if (!isfinite(rn)) {
+            c->nan = 1;
+            c->stop_outer = 1;
+        } else if (rel <= tol || c->outer >= max_outer) {
+            c->stop_outer = 1;
+        }
+    }
+};
+struct CtlCheck {
+    SolveCtl *c;
+    __device__ void operator()(const double *out) const
+    {
+        c->iters++;
+        c->pass_iters++;
+        if (c->pass_iters > c->max_pass_iters) c->max_pass_iters = c->pass_iters;
+        if (c->outer < kMaxPasses) c->pass_count[c->outer] = c->pass_iters;
+        if (sqrt(out[0]) <= c->thr) c->stop_pcg = 1;
+    }
+};
+// a reduction with no follow-up
+struct NoPost {
+    __device__ void operator()(const double *) const {}
+};
+// e += alpha p, r -= alpha q and |r|^2, then post (m2d_fields2d.hip)
+__global__ void k_cg_update_dot(float *e, float *r, const float *p, const float *q, const double *rz,
+                                const double *pq, size_t n, int first, double *partial, unsigned int *ticket,
+                                double *out, int fin, const int *stop, CtlCheck post);
+
 // the last solve's shape on a set of fields: refinement passes, and the CG
 // iterations each pass took (the next solve queues as many, plus one)
 struct SolveEst {
@@ -249,16 +291,14 @@ struct SolveEst {
 static_assert(kMaxPasses == 13, "SolveEst::iters initialiser");
 
 __global__ void k_ctl_init(SolveCtl *c);
-__global__ void k_ctl_eval(SolveCtl *c, const double *dots, double tol, int max_outer);
 __global__ void k_ctl_open(SolveCtl *c, const double *dots, double tol);
-__global__ void k_ctl_check(SolveCtl *c, const double *dots);
 __global__ void k_ctl_end(SolveCtl *c, int max_iters);
 __global__ void k_ctl_resume(SolveCtl *c);
 __global__ void k_ctl_close(SolveCtl *c);
 
 // FP64 iterative refinement over FP32 CG passes, queued under SolveCtl.  The
 // caller has queued the right-hand side, k_ctl_init and resid() (the FP64
-// residual, its norms and k_ctl_eval).  Passes are queued as many as the
+// residual, its norms and CtlEval).  Passes are queued as many as the
 // last solve took, each with one CG iteration more than its counterpart
 // took then (the last pass of a refinement needs far fewer than the first);
 // the control block is read once after them.  A pass that needs more
@@ -518,17 +558,19 @@ __device__ __forceinline__ void sum_partials(const double *partial, unsigned nb,
     }
 }
 
-template <int NV>
+// post(out): what the thread that wrote the sums does with them next
+template <int NV, class Post = NoPost>
 __global__ void __launch_bounds__(kRedBlock) k_sum_partials(const double *partial, unsigned nb, double *out,
-                                                           const int *stop)
+                                                           const int *stop, Post post = Post{})
 {
     if (stopped(stop)) return;
     sum_partials<NV>(partial, nb, out);
+    if (threadIdx.x == 0) post(out);
 }
 
-template <int NV>
+template <int NV, class Post = NoPost>
 __device__ __forceinline__ void finish_sums(double (&v)[NV], double *partial, unsigned int *ticket, double *out,
-                                            int fin)
+                                            int fin, Post post = Post{})
 {
     __shared__ double sh[NV][kRedBlock / 64];
     __shared__ bool last;
@@ -556,18 +598,22 @@ __device__ __forceinline__ void finish_sums(double (&v)[NV], double *partial, un
     if (!last) return;  // uniform over the block
     __threadfence();
     sum_partials<NV>(partial, gridDim.x, out);
-    if (threadIdx.x == 0) *ticket = 0u;
+    if (threadIdx.x == 0) {
+        *ticket = 0u;
+        post(out);
+    }
 }
 
 // launch a reduction kernel over nb blocks and finish its sums: in the
-// kernel (fin) when nb <= kFinBlocks, else with k_sum_partials
-template <int NV, class Launch>
+// kernel (fin) when nb <= kFinBlocks, else with k_sum_partials (the kernel
+// then leaves post to it: pass the same post to both)
+template <int NV, class Launch, class Post = NoPost>
 inline void reduce_launch(unsigned nb, double *partial, double *out, hipStream_t s, Launch launch,
-                          const int *stop = nullptr)
+                          const int *stop = nullptr, Post post = Post{})
 {
     const int fin = nb <= kFinBlocks;
     launch(fin);
-    if (!fin) k_sum_partials<NV><<<1, kRedBlock, 0, s>>>(partial, nb, out, stop);
+    if (!fin) k_sum_partials<NV, Post><<<1, kRedBlock, 0, s>>>(partial, nb, out, stop, post);
 }
 
 }  // namespace m2d
