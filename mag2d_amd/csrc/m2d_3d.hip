@@ -45,7 +45,7 @@ struct m2d_fields3d {
     int coarse_n = 0;
     bool coarse_dense = true;
     int lc3 = 1;                   // first level of the one-workgroup coarse V-cycle
-    float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
+    float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pp2 = nullptr, *pq = nullptr, *pe = nullptr;
     double *r64 = nullptr, *dots = nullptr, *h_dots = nullptr, *partial = nullptr;
     int npartial = 0;
     m2d::SolveEst est;  // the last solve's passes and iterations: what the next queues
@@ -149,19 +149,38 @@ __device__ __forceinline__ float apply7(uint8_t nb, const float *x, size_t m, si
     return s;
 }
 
-// y = A' x and out = x.y in one launch
-__global__ void __launch_bounds__(kRedBlock) k3_apply_dot(const uint8_t *nbm, const float *x, float *y, int nx, int ny,
-                                                         int nz, double *partial, unsigned int *ticket, double *out,
-                                                         int fin, const int *stop)
+// The CG direction and the operator in one launch: p' = z + beta p (first:
+// p' = z), q = A' p' and out = p'.q.  Each node's p' is recomputed at its
+// stencil neighbours from z and p (the same operations, so the same bits) and
+// written to pn, not p, which other blocks are still reading.
+__global__ void __launch_bounds__(kRedBlock) k3_dir_apply_dot(const uint8_t *nbm, const float *z, const float *p,
+                                                             float *pn, float *q, const double *rz_new,
+                                                             const double *rz_old, int first, int nx, int ny, int nz,
+                                                             double *partial, unsigned int *ticket, double *out,
+                                                             int fin, const int *stop)
 {
     if (stopped(stop)) return;
-    const size_t n = (size_t)nx * ny * nz;
+    const size_t n = (size_t)nx * ny * nz, sx = (size_t)ny * nz;
+    const float beta = first ? 0.f : cg_beta(rz_new, rz_old);
     double v[1] = {0};
     for (size_t m = (size_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (size_t)gridDim.x * kRedBlock) {
         const uint8_t nb = nbm[m];
-        const float ym = (nb & kNbSelf) ? 0.f : apply7(nb, x, m, (size_t)ny * nz, nz);
-        y[m] = ym;
-        v[0] += (double)x[m] * ym;
+        const float pk = cg_dir_at(z, p, m, beta, first);
+        pn[m] = pk;
+        float ym = 0.f;
+        if (!(nb & kNbSelf)) {
+            // apply7's order
+            float t = 6.f * pk;
+            if (nb & 1) t -= cg_dir_at(z, p, m - sx, beta, first);
+            if (nb & 2) t -= cg_dir_at(z, p, m - nz, beta, first);
+            if (nb & 4) t -= cg_dir_at(z, p, m - 1, beta, first);
+            if (nb & 8) t -= cg_dir_at(z, p, m + 1, beta, first);
+            if (nb & 16) t -= cg_dir_at(z, p, m + nz, beta, first);
+            if (nb & 32) t -= cg_dir_at(z, p, m + sx, beta, first);
+            ym = t;
+        }
+        q[m] = ym;
+        v[0] += (double)pk * ym;
     }
     finish_sums<1>(v, partial, ticket, out, fin);
 }
@@ -1249,7 +1268,7 @@ static int mg3_setup(m2d_fields3d *f, const std::vector<uint8_t> &mask)
         M2D_HIP(hipMemcpy(f->coarse_invT, invT.data(), sizeof(float) * invT.size(), hipMemcpyHostToDevice));
     }
     const size_t n = f->n;
-    for (float **q : {&f->pr, &f->pz, &f->pp, &f->pq, &f->pe}) M2D_TRY(dmalloc3((void **)q, sizeof(float) * n));
+    for (float **q : {&f->pr, &f->pz, &f->pp, &f->pp2, &f->pq, &f->pe}) M2D_TRY(dmalloc3((void **)q, sizeof(float) * n));
     M2D_TRY(dmalloc3((void **)&f->r64, sizeof(double) * n));
     // 16 scalars, then the solve's control block
     M2D_TRY(dmalloc3((void **)&f->dots, sizeof(double) * 32));
@@ -1375,16 +1394,17 @@ static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
 {
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
-    const unsigned gb = blocks_for(n), rb = red_blocks3(n);
+    const unsigned rb = red_blocks3(n);
     const Mg3Level &V = f->lv[0];
     SolveCtl *ctl = solve_ctl3(f);
     const int *stop = &ctl->stop_pcg;
+    // iteration it's direction goes to P[it & 1], the previous one's is read from the other
+    float *const P[2] = {f->pp, f->pp2};
     if (from == 0) {
         M2D_TRY(vcycle3(f, f->pr, f->pz, stop));
         reduce_launch<1>(rb, f->partial, f->dots, s, [&](int fin) {
             k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots, fin, stop);
         }, stop);
-        k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots, f->dots + 4, n, 1, stop);
         cur = 0;
     }
     for (int it = from + 1; it <= to; it++) {
@@ -1394,15 +1414,17 @@ static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
                 k_dot_fin<<<rb, kRedBlock, 0, s>>>(f->pr, f->pz, n, f->partial, f->ticket, f->dots + 4 * (cur ^ 1),
                                                    fin, stop);
             }, stop);
-            k_cg_dir<<<gb, 256, 0, s>>>(f->pp, f->pz, f->dots + 4 * (cur ^ 1), f->dots + 4 * cur, n, 0, stop);
             cur ^= 1;
         }
+        const float *pold = P[(it - 1) & 1];
+        float *pnew = P[it & 1];
         reduce_launch<1>(rb, f->partial, f->dots + 1, s, [&](int fin) {
-            k3_apply_dot<<<rb, kRedBlock, 0, s>>>(V.nbm, f->pp, f->pq, V.nx, V.ny, V.nz, f->partial, f->ticket,
-                                                  f->dots + 1, fin, stop);
+            k3_dir_apply_dot<<<rb, kRedBlock, 0, s>>>(V.nbm, f->pz, pold, pnew, f->pq, f->dots + 4 * cur,
+                                                      f->dots + 4 * (cur ^ 1), it == 1, V.nx, V.ny, V.nz, f->partial,
+                                                      f->ticket, f->dots + 1, fin, stop);
         }, stop);
         reduce_launch<1>(rb, f->partial, f->dots + 2, s, [&](int fin) {
-            k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(f->pe, f->pr, f->pp, f->pq, f->dots + 4 * cur, f->dots + 1, n,
+            k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(f->pe, f->pr, pnew, f->pq, f->dots + 4 * cur, f->dots + 1, n,
                                                      it == 1, f->partial, f->ticket, f->dots + 2, fin, stop,
                                                      CtlCheck{ctl});
         }, stop, CtlCheck{ctl});
@@ -1490,7 +1512,7 @@ static void free3(m2d_fields3d *f)
 {
     for (void *q : {(void *)f->mask, (void *)f->cellfree, (void *)f->voltage, (void *)f->u, (void *)f->rho,
                     (void *)f->tx, (void *)f->ty, (void *)f->tz, (void *)f->coarse_inv, (void *)f->pr,
-                    (void *)f->pz, (void *)f->pp, (void *)f->pq, (void *)f->pe, (void *)f->r64,
+                    (void *)f->pz, (void *)f->pp, (void *)f->pp2, (void *)f->pq, (void *)f->pe, (void *)f->r64,
                     (void *)f->dots, (void *)f->partial, (void *)f->coarse_invT, (void *)f->ticket})
         if (q) (void)hipFree(q);
     for (Mg3Level &V : f->lv)

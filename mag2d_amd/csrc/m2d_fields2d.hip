@@ -304,18 +304,6 @@ __global__ void __launch_bounds__(256) k_coarse_solve(const float *inv, const fl
     }
 }
 
-// p = z + beta p, beta = rz_new / rz_old
-__global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old,
-                         size_t n, int first, const int *stop)
-{
-    if (stopped(stop)) return;
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    if (first) { p[k] = z[k]; return; }
-    double o = *rz_old;
-    float beta = o != 0.0 ? (float)(*rz_new / o) : 0.f;
-    p[k] = fmaf(beta, p[k], z[k]);
-}
 
 // ------------------------------------------------------------------ fused solver kernels
 // The solve is launch-bound on small grids, so its kernels carry their own
@@ -336,17 +324,36 @@ __global__ void __launch_bounds__(kRedBlock) k_dot_fin(const float *a, const flo
 }
 
 // q = A' p on the finest level, out = p.q
-__global__ void __launch_bounds__(kRedBlock) k_apply5_dot(const uint8_t *dir, const float *p, float *q, size_t n,
-                                                         int nz, double *partial, unsigned int *ticket, double *out,
-                                                         int fin, const int *stop)
+// The CG direction and the operator in one launch: p' = z + beta p (first:
+// p' = z), q = A' p' and out = p'.q.  Each node's p' is recomputed at its
+// stencil neighbours from z and p -- the same operations as at the node
+// itself, so the same bits -- and written to pn, not p, which other blocks
+// are still reading.
+__global__ void __launch_bounds__(kRedBlock) k_dir_apply5_dot(const uint8_t *dir, const float *z, const float *p,
+                                                             float *pn, float *q, const double *rz_new,
+                                                             const double *rz_old, int first, size_t n, int nz,
+                                                             double *partial, unsigned int *ticket, double *out,
+                                                             int fin, const int *stop)
 {
     if (stopped(stop)) return;
     double v[1] = {0};
+    const float beta = first ? 0.f : cg_beta(rz_new, rz_old);
     const size_t k = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
     if (k < n) {
-        const float y = dir[k] ? 0.f : apply5(dir, p, k, nz);
+        const float pk = cg_dir_at(z, p, k, beta, first);
+        pn[k] = pk;
+        float y = 0.f;
+        if (!dir[k]) {
+            // apply5's order
+            float t = 4.f * pk;
+            if (!dir[k - nz]) t -= cg_dir_at(z, p, k - nz, beta, first);
+            if (!dir[k - 1]) t -= cg_dir_at(z, p, k - 1, beta, first);
+            if (!dir[k + 1]) t -= cg_dir_at(z, p, k + 1, beta, first);
+            if (!dir[k + nz]) t -= cg_dir_at(z, p, k + nz, beta, first);
+            y = t;
+        }
         q[k] = y;
-        v[0] = (double)p[k] * y;
+        v[0] = (double)pk * y;
     }
     finish_sums<1>(v, partial, ticket, out, fin);
 }
@@ -1124,6 +1131,7 @@ static int mg_setup(m2d_fields2d *f)
     M2D_TRY(dmalloc((void **)&mg.pr, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.pz, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.pp, sizeof(float) * n));
+    M2D_TRY(dmalloc((void **)&mg.pp2, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.pq, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.pe, sizeof(float) * n));
     M2D_TRY(dmalloc((void **)&mg.r64, sizeof(double) * n));
@@ -1152,7 +1160,7 @@ static void mg_free(Mg &mg)
                         (void *)L.r, (void *)L.b})
             if (p) (void)hipFree(p);
     mg.lv.clear();
-    for (void *p : {(void *)mg.coarse_inv, (void *)mg.pr, (void *)mg.pz, (void *)mg.pp, (void *)mg.pq,
+    for (void *p : {(void *)mg.coarse_inv, (void *)mg.pr, (void *)mg.pz, (void *)mg.pp, (void *)mg.pp2, (void *)mg.pq,
                     (void *)mg.pe, (void *)mg.r64, (void *)mg.dots, (void *)mg.partial, (void *)mg.ticket, (void *)mg.coarse_invT})
         if (p) (void)hipFree(p);
     if (mg.h_dots) (void)hipHostFree(mg.h_dots);
@@ -1324,16 +1332,17 @@ static int queue_pcg(m2d_fields2d *f, int from, int to, int &cur)
     Mg &mg = f->mg;
     hipStream_t s = f->ctx->stream;
     const size_t n = f->n;
-    const unsigned gb = (unsigned)((n + 255) / 256), rb = red_blocks(n);
+    const unsigned rb = red_blocks(n);
     const MgLevel &L0 = mg.lv[0];
     SolveCtl *ctl = solve_ctl(mg);
     const int *stop = &ctl->stop_pcg;
+    // iteration it's direction goes to P[it & 1], the previous one's is read from the other
+    float *const P[2] = {mg.pp, mg.pp2};
     if (from == 0) {
         M2D_TRY(vcycle(f, mg.pr, mg.pz, stop));
         reduce_launch<1>(rb, mg.partial, mg.dots, s, [&](int fin) {
             k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots, fin, stop);
         }, stop);
-        k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots, mg.dots + 4, n, 1, stop);
         cur = 0;
     }
     for (int it = from + 1; it <= to; it++) {
@@ -1343,15 +1352,17 @@ static int queue_pcg(m2d_fields2d *f, int from, int to, int &cur)
                 k_dot_fin<<<rb, kRedBlock, 0, s>>>(mg.pr, mg.pz, n, mg.partial, mg.ticket, mg.dots + 4 * (cur ^ 1),
                                                    fin, stop);
             }, stop);
-            k_cg_dir<<<gb, 256, 0, s>>>(mg.pp, mg.pz, mg.dots + 4 * (cur ^ 1), mg.dots + 4 * cur, n, 0, stop);
             cur ^= 1;
         }
+        const float *pold = P[(it - 1) & 1];
+        float *pnew = P[it & 1];
         reduce_launch<1>(rb, mg.partial, mg.dots + 1, s, [&](int fin) {
-            k_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pp, mg.pq, n, L0.nz, mg.partial, mg.ticket, mg.dots + 1,
-                                                  fin, stop);
+            k_dir_apply5_dot<<<rb, kRedBlock, 0, s>>>(L0.dir, mg.pz, pold, pnew, mg.pq, mg.dots + 4 * cur,
+                                                      mg.dots + 4 * (cur ^ 1), it == 1, n, L0.nz, mg.partial,
+                                                      mg.ticket, mg.dots + 1, fin, stop);
         }, stop);
         reduce_launch<1>(rb, mg.partial, mg.dots + 2, s, [&](int fin) {
-            k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, mg.pp, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
+            k_cg_update_dot<<<rb, kRedBlock, 0, s>>>(mg.pe, mg.pr, pnew, mg.pq, mg.dots + 4 * cur, mg.dots + 1, n,
                                                      it == 1, mg.partial, mg.ticket, mg.dots + 2, fin, stop,
                                                      CtlCheck{ctl});
         }, stop, CtlCheck{ctl});

@@ -181,8 +181,16 @@ int ensure(m2d_ctx *ctx, Scratch &s, size_t bytes);
 // returns at once (the flag is set before the launch starts and read the same
 // by every thread, so a whole launch runs or none of it).  Null: always run.
 __device__ __forceinline__ bool stopped(const int *stop) { return stop && *stop; }
-__global__ void k_cg_dir(float *p, const float *z, const double *rz_new, const double *rz_old, size_t n,
-                         int first, const int *stop);
+// the CG direction at node k: p' = z + beta p (first: p' = z), beta = rz_new / rz_old
+__device__ __forceinline__ float cg_beta(const double *rz_new, const double *rz_old)
+{
+    const double o = *rz_old;
+    return o != 0.0 ? (float)(*rz_new / o) : 0.f;
+}
+__device__ __forceinline__ float cg_dir_at(const float *z, const float *p, size_t k, float beta, int first)
+{
+    return first ? z[k] : fmaf(beta, p[k], z[k]);
+}
 __global__ void k_flush(unsigned long long *acc, double *rho, double charge, size_t n);
 // Species::accumulate into a fixed-point grid (m2d_particles2d.hip)
 __global__ void __launch_bounds__(kBlock) k_accumulate2d(const float *xs, const float *zs, const float *xls,

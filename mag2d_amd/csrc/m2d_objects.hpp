@@ -61,7 +61,7 @@ struct Mg {
     float *coarse_invT = nullptr; // the same, transposed
     int coarse_n = 0;
     // FP32 PCG vectors on the finest level
-    float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *pe = nullptr;
+    float *pr = nullptr, *pz = nullptr, *pp = nullptr, *pp2 = nullptr, *pq = nullptr, *pe = nullptr;
     // FP64 vectors
     double *r64 = nullptr;
     double *dots = nullptr;       // device scalars
