@@ -36,6 +36,7 @@ constexpr int kHalo = 8;               // deposit window halo (cells) around a t
 constexpr int kWin = kTile + 1 + 2 * kHalo;  // window side in nodes
 constexpr int kChunkCap = 1984;        // particles per block of the tiled 2D pass: with its
                                        // start rounded down to 64 slots, two staged rounds of 512
+constexpr int kRound2 = 1024;          // slots per round of the tiled 2D pass (256 threads x 4)
 constexpr int kChunkCap3 = 1984;       // particles per block of the tiled 3D pass
 
 // ---------------------------------------------------------------- errors
@@ -108,8 +109,8 @@ struct m2d_tiling {
     int64_t n = 0;
     int ntx = 0, nty = 1, ntz = 0;  // sort tiles along x, (y,) z
     int max_chunks = 0;        // launch bound
-    m2d::Scratch buf;          // device: tile_start, tile_end, chunk_first, chunk_count, sched, n_chunks
-    int *tile_start = nullptr, *tile_end = nullptr, *chunk_first = nullptr, *chunk_count = nullptr;
+    m2d::Scratch buf;          // device: sched, tile_start, tile_end, chunk_count, n_chunks
+    int *tile_start = nullptr, *tile_end = nullptr, *chunk_count = nullptr;
     int4 *sched = nullptr;
     unsigned int *n_chunks = nullptr;
 };
@@ -147,7 +148,7 @@ namespace m2d {
 m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n);
 int sort_keys_u32(m2d_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n);
 int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
-                 int nty, int ntz, int cap);
+                 int nty, int ntz, int cap, int round);
 void forget_tiling(m2d_ctx *ctx, const float *x);
 void free_tilings(m2d_ctx *ctx);
 // objects that hold a context keep it alive until they are destroyed

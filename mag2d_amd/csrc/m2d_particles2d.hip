@@ -176,6 +176,7 @@ __device__ __forceinline__ void deposit_global(unsigned long long *acc, int nz, 
 // a few thousand chunks is mostly resident at once.
 constexpr int kPass = 256;
 constexpr int kRound = kPass * 4;
+static_assert(kRound == kRound2, "the tile schedule orders chunks by rounds of the pass");
 constexpr int kGather = 2;                  // particles whose gathers are in flight together
 constexpr int kChunk = 2048;                // slots per block of the untiled pass
 constexpr int kPPT = kChunk / kBlock;       // slots per thread of the helper kernels

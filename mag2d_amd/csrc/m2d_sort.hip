@@ -94,32 +94,52 @@ __global__ void k_tile_chunks(const int *tstart, const int *tend, int ntiles, in
     count[t] = c > 0 ? (c + cap - 1) / cap : 0;
 }
 
-// exclusive scan of the per-tile chunk counts (one block) and the schedule
+// a chunk [a, e) that the pass finishes in one round of `round` slots (its
+// rounds start at a rounded down to 64); round = 0: none is short
+__device__ __forceinline__ bool short_chunk(int a, int e, int round)
+{
+    return round > 0 && e - (a & ~63) <= round;
+}
+
+// exclusive scans of the per-tile chunk counts (one block) and the
+// schedule: chunks that take more than one round come first, the short ones
+// last, each group in tile order.  Blocks start in index order as earlier
+// ones retire, so the launch ends on short blocks (longest first).
 __global__ void __launch_bounds__(1024) k_tile_schedule(const int *tstart, const int *tend,
-                                                        const int *count, int ntiles, int cap, int *first,
+                                                        const int *count, int ntiles, int cap, int round,
                                                         int4 *sched, unsigned int *n_chunks)
 {
-    __shared__ int s[1024];
+    __shared__ int s[1024], s1[1024];
     const int per = (ntiles + 1023) / 1024;
     const int lo = threadIdx.x * per, hi = min(ntiles, lo + per);
-    int loc = 0;
-    for (int t = lo; t < hi; t++) loc += count[t];
+    int loc = 0, loc1 = 0;
+    for (int t = lo; t < hi; t++) {
+        const int c = count[t];
+        loc += c;
+        for (int k = 0; k < c; k++) {
+            const int a = tstart[t] + k * cap;
+            loc1 += short_chunk(a, min(tend[t], a + cap), round);
+        }
+    }
     s[threadIdx.x] = loc;
+    s1[threadIdx.x] = loc1;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        int v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+        const bool in = (int)threadIdx.x >= o;
+        const int v = in ? s[threadIdx.x - o] : 0, v1 = in ? s1[threadIdx.x - o] : 0;
         __syncthreads();
         s[threadIdx.x] += v;
+        s1[threadIdx.x] += v1;
         __syncthreads();
     }
-    int run = s[threadIdx.x] - loc;
+    const int before = s[threadIdx.x] - loc, before1 = s1[threadIdx.x] - loc1;
+    int run = before - before1;                    // long chunks
+    int run1 = (s[1023] - s1[1023]) + before1;     // short chunks, after every long one
     for (int t = lo; t < hi; t++) {
-        first[t] = run;
-        for (int c = 0; c < count[t]; c++) {
-            int a = tstart[t] + c * cap;
-            sched[run + c] = make_int4(t, a, min(tend[t], a + cap), 0);
+        for (int k = 0; k < count[t]; k++) {
+            const int a = tstart[t] + k * cap, e = min(tend[t], a + cap);
+            sched[short_chunk(a, e, round) ? run1++ : run++] = make_int4(t, a, e, 0);
         }
-        run += count[t];
     }
     if (threadIdx.x == 1023) *n_chunks = (unsigned int)s[1023];
 }
@@ -148,9 +168,10 @@ void free_tilings(m2d_ctx *ctx)
 
 // build the schedule of the store at x (n slots) from its sorted keys; tiles
 // number ntx * nty * ntz (nty = 1 in 2D), 64 cells each; a tile's slots are
-// cut into chunks of at most cap
+// cut into chunks of at most cap, and chunks the pass finishes in one round
+// of `round` slots go last (round = 0: tile order only)
 int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
-                 int nty, int ntz, int cap)
+                 int nty, int ntz, int cap, int round)
 {
     forget_tiling(ctx, x);
     m2d_tiling *T = nullptr;
@@ -169,22 +190,21 @@ int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, 
     T->x = nullptr;
     const int ntiles = ntx * nty * ntz;
     const int max_chunks = ntiles + (int)(n / cap) + 1;
-    const size_t words = 4 * (size_t)ntiles + 4 * (size_t)max_chunks + 4;
+    const size_t words = 3 * (size_t)ntiles + 4 * (size_t)max_chunks + 4;
     M2D_TRY(ensure(ctx, T->buf, words * sizeof(int)));
     int *b = (int *)T->buf.ptr;
-    T->tile_start = b;
-    T->tile_end = b + ntiles;
-    T->chunk_first = b + 2 * ntiles;
-    T->chunk_count = b + 3 * ntiles;
-    T->sched = (int4 *)(b + 4 * ntiles);
-    T->n_chunks = (unsigned int *)(b + 4 * ntiles + 4 * max_chunks);
+    T->sched = (int4 *)b;  // first: 16-byte aligned
+    T->tile_start = b + 4 * (size_t)max_chunks;
+    T->tile_end = T->tile_start + ntiles;
+    T->chunk_count = T->tile_end + ntiles;
+    T->n_chunks = (unsigned int *)(T->chunk_count + ntiles);
     M2D_TRY(fill_bytes(ctx, T->tile_start, 0, sizeof(int) * 2 * (size_t)ntiles));
     k_tile_bounds<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(keys, n, dead_key, T->tile_start,
                                                                         T->tile_end);
     k_tile_chunks<<<(unsigned)((ntiles + 255) / 256), 256, 0, ctx->stream>>>(T->tile_start, T->tile_end,
                                                                             ntiles, cap, T->chunk_count);
-    k_tile_schedule<<<1, 1024, 0, ctx->stream>>>(T->tile_start, T->tile_end, T->chunk_count, ntiles, cap,
-                                                 T->chunk_first, T->sched, T->n_chunks);
+    k_tile_schedule<<<1, 1024, 0, ctx->stream>>>(T->tile_start, T->tile_end, T->chunk_count, ntiles, cap, round,
+                                                 T->sched, T->n_chunks);
     M2D_HIP(hipGetLastError());
     T->ntx = ntx;
     T->nty = nty;
@@ -269,7 +289,7 @@ int m2d_sort_2d(m2d_ctx *ctx, m2d_fields2d *f, const m2d_particles2d *src, const
     k_sort_gather<<<xcd_grid((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, g.dead_key,
                                                                               live);
     M2D_HIP(hipGetLastError());
-    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, g.dead_key, ntx, 1, g.ntz, kChunkCap));
+    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, g.dead_key, ntx, 1, g.ntz, kChunkCap, kRound2));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];
