@@ -1831,7 +1831,7 @@ int m2d_sort_3d(m2d_ctx *ctx, m2d_fields3d *f, const m2d_particles3d *src, const
     M2D_CHECK(e == hipSuccess, M2D_ERR_HIP, "radix_sort_pairs: %s", hipGetErrorString(e));
     k_sort_gather3d<<<xcd_grid(blocks_for(n)), 256, 0, ctx->stream>>>(*src, *dst, k1, v1, dead_key, live);
     M2D_HIP(hipGetLastError());
-    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, dead_key, ntx3, nty3, ntz3, kChunkCap3, 0));
+    M2D_TRY(build_tiling(ctx, dst->x, dst->n, k1, dead_key, ntx3, nty3, ntz3, kChunkCap3, 256));
     if (n_live) {
         M2D_TRY(read_counters(ctx));
         *n_live = (int64_t)ctx->h_counters[CNT_LIVE];

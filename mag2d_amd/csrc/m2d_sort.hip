@@ -94,54 +94,68 @@ __global__ void k_tile_chunks(const int *tstart, const int *tend, int ntiles, in
     count[t] = c > 0 ? (c + cap - 1) / cap : 0;
 }
 
-// a chunk [a, e) that the pass finishes in one round of `round` slots (its
-// rounds start at a rounded down to 64); round = 0: none is short
-__device__ __forceinline__ bool short_chunk(int a, int e, int round)
+// The schedule's order class of a chunk [a, e): the rounds of `round`
+// slots the pass takes over it (its rounds start at a rounded down to 64),
+// less one, at most kSchedClasses - 1; round = 0: class 0 for all
+constexpr int kSchedClasses = 8;
+__device__ __forceinline__ int chunk_class(int a, int e, int round)
 {
-    return round > 0 && e - (a & ~63) <= round;
+    if (round <= 0) return 0;
+    const int r = (e - (a & ~63) + round - 1) / round - 1;
+    return min(max(r, 0), kSchedClasses - 1);
 }
 
 // exclusive scans of the per-tile chunk counts (one block) and the
-// schedule: chunks that take more than one round come first, the short ones
-// last, each group in tile order.  Blocks start in index order as earlier
-// ones retire, so the launch ends on short blocks (longest first).
+// schedule: chunks in descending class (the longest first), each class in
+// tile order.  Blocks start in index order as earlier ones retire, so the
+// launch ends on short blocks.
 __global__ void __launch_bounds__(1024) k_tile_schedule(const int *tstart, const int *tend,
                                                         const int *count, int ntiles, int cap, int round,
                                                         int4 *sched, unsigned int *n_chunks)
 {
-    __shared__ int s[1024], s1[1024];
+    __shared__ int s[kSchedClasses][1024];  // per class: this thread's chunks, then their first index
+    const int tid = threadIdx.x;
     const int per = (ntiles + 1023) / 1024;
-    const int lo = threadIdx.x * per, hi = min(ntiles, lo + per);
-    int loc = 0, loc1 = 0;
+    const int lo = tid * per, hi = min(ntiles, lo + per);
+#pragma unroll
+    for (int c = 0; c < kSchedClasses; c++) s[c][tid] = 0;
     for (int t = lo; t < hi; t++) {
-        const int c = count[t];
-        loc += c;
-        for (int k = 0; k < c; k++) {
+        for (int k = 0; k < count[t]; k++) {
             const int a = tstart[t] + k * cap;
-            loc1 += short_chunk(a, min(tend[t], a + cap), round);
+            s[chunk_class(a, min(tend[t], a + cap), round)][tid]++;
         }
     }
-    s[threadIdx.x] = loc;
-    s1[threadIdx.x] = loc1;
+    int own[kSchedClasses];
+#pragma unroll
+    for (int c = 0; c < kSchedClasses; c++) own[c] = s[c][tid];
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        const bool in = (int)threadIdx.x >= o;
-        const int v = in ? s[threadIdx.x - o] : 0, v1 = in ? s1[threadIdx.x - o] : 0;
+        int v[kSchedClasses];
+#pragma unroll
+        for (int c = 0; c < kSchedClasses; c++) v[c] = tid >= o ? s[c][tid - o] : 0;
         __syncthreads();
-        s[threadIdx.x] += v;
-        s1[threadIdx.x] += v1;
+#pragma unroll
+        for (int c = 0; c < kSchedClasses; c++) s[c][tid] += v[c];
         __syncthreads();
     }
-    const int before = s[threadIdx.x] - loc, before1 = s1[threadIdx.x] - loc1;
-    int run = before - before1;                    // long chunks
-    int run1 = (s[1023] - s1[1023]) + before1;     // short chunks, after every long one
+    // class c starts after every chunk of the classes above it
+    int base = 0;
+    int first[kSchedClasses];
+#pragma unroll
+    for (int c = kSchedClasses - 1; c >= 0; c--) {
+        first[c] = base + s[c][tid] - own[c];
+        base += s[c][1023];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kSchedClasses; c++) s[c][tid] = first[c];
     for (int t = lo; t < hi; t++) {
         for (int k = 0; k < count[t]; k++) {
             const int a = tstart[t] + k * cap, e = min(tend[t], a + cap);
-            sched[short_chunk(a, e, round) ? run1++ : run++] = make_int4(t, a, e, 0);
+            sched[s[chunk_class(a, e, round)][tid]++] = make_int4(t, a, e, 0);
         }
     }
-    if (threadIdx.x == 1023) *n_chunks = (unsigned int)s[1023];
+    if (tid == 1023) *n_chunks = (unsigned int)base;
 }
 
 m2d_tiling *find_tiling(m2d_ctx *ctx, const float *x, int64_t n)
@@ -168,8 +182,8 @@ void free_tilings(m2d_ctx *ctx)
 
 // build the schedule of the store at x (n slots) from its sorted keys; tiles
 // number ntx * nty * ntz (nty = 1 in 2D), 64 cells each; a tile's slots are
-// cut into chunks of at most cap, and chunks the pass finishes in one round
-// of `round` slots go last (round = 0: tile order only)
+// cut into chunks of at most cap, ordered longest first by the rounds of
+// `round` slots the pass takes over them (round = 0: tile order only)
 int build_tiling(m2d_ctx *ctx, const float *x, int64_t n, const uint32_t *keys, uint32_t dead_key, int ntx,
                  int nty, int ntz, int cap, int round)
 {
