@@ -1430,8 +1430,9 @@ int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_
     const m2d_grid2d &g = p->grid;
     M2D_CHECK(g.nx >= 3 && g.nz >= 3 && g.dx > 0 && g.dz > 0, M2D_ERR_BAD_ARG,
               "m2d_fields2d_create: grid %dx%d spacing %g,%g", g.nx, g.nz, g.dx, g.dz);
-    // node indices are 32-bit and the gathers multiply in 24 bits
-    M2D_CHECK(g.nx < (1 << 24) && g.nz < (1 << 24) && (size_t)(g.nx + 1) * (g.nz + 1) < (1ull << 31),
+    // node indices are 32-bit, the gathers multiply in 24 bits and address
+    // the FP32 edge tables ((nx+1) (nz+1) floats at most) by 32-bit byte offsets
+    M2D_CHECK(g.nx < (1 << 24) && g.nz < (1 << 24) && (size_t)(g.nx + 1) * (g.nz + 1) < (1ull << 30),
               M2D_ERR_BAD_ARG, "m2d_fields2d_create: grid %dx%d too large", g.nx, g.nz);
     M2D_CHECK(p->dV > 0, M2D_ERR_BAD_ARG, "m2d_fields2d_create: dV must be > 0");
     const size_t n = (size_t)g.nx * g.nz;

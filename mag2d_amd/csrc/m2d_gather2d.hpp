@@ -146,9 +146,16 @@ __device__ __forceinline__ void gather_fetch(const A &a, const GatherAt &g, floa
     const int ox = __umul24(g.ih, nz) + g.jx, oz = __umul24(g.i0, nz + 1) + g.jh;
     float g00, g01, g10, g11, h00, h01, h10, h11;
     if constexpr (PHT) {
-        const float *r0 = a.phx + ox, *r1 = r0 + nz;
+        // 32-bit byte offsets from the table bases (m2d_fields2d_create
+        // keeps (nx+1) (nz+1) below 2^30, so the tables' (nx+1) nz and
+        // nx (nz+1) floats span less than 2^32 bytes): the loads take the
+        // base from scalar registers and one 32-bit offset each
+        const unsigned bx = (unsigned)ox * 4u, bz = (unsigned)oz * 4u;
+        const unsigned bx1 = bx + (unsigned)nz * 4u, bz1 = bz + (unsigned)(nz + 1) * 4u;
+        const char *tx = (const char *)a.phx, *tz = (const char *)a.phz;
+        const float *r0 = (const float *)(tx + bx), *r1 = (const float *)(tx + bx1);
+        const float *q0 = (const float *)(tz + bz), *q1 = (const float *)(tz + bz1);
         g00 = r0[0]; g01 = r0[1]; g10 = r1[0]; g11 = r1[1];
-        const float *q0 = a.phz + oz, *q1 = q0 + (nz + 1);
         h00 = q0[0]; h01 = q0[1]; h10 = q1[0]; h11 = q1[1];
     } else {
         const float2 *r0 = a.gx + ox, *r1 = r0 + nz;
