@@ -190,7 +190,8 @@ typedef struct {
 enum { M2D_FIELD_U = 0, M2D_FIELD_URF = 1, M2D_FIELD_RHO = 2, M2D_FIELD_VOLTAGE = 3 };
 
 /* Fields::Fields (fields.cpp:115-276): uploads mask and voltage, builds the
- * multigrid hierarchy that replaces the UMFPACK factorisation. */
+ * multigrid hierarchy that replaces the UMFPACK factorisation.  Grids need
+ * nx, nz >= 3 and (nx+1) (nz+1) < 2^30 (M2D_ERR_BAD_ARG otherwise). */
 int m2d_fields2d_create(m2d_ctx *ctx, const m2d_fields2d_params *p, const uint8_t *mask,
                         const double *voltage, m2d_fields2d **out);
 int m2d_fields2d_destroy(m2d_fields2d *f);
