@@ -576,7 +576,7 @@ __device__ __forceinline__ void scatter_queue(const PushArgs &a, unsigned long l
         float vx = w.x, vy = w.y, vz = w.z;
         Rng rg;
         rg.begin(a.seed, a.stream, p, a.step, kPurposeMcc);
-        scatter(*a.mcc, rg, vx, vy, vz, a.counters);
+        scatter_last(*a.mcc, rg, vx, vy, vz, a.counters);
         if (a.cq_defer) a.cqv[k] = make_float4(vx, vy, vz, 0.f);
         else { a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz; }
     }

@@ -83,9 +83,12 @@ __host__ __device__ inline double coulomb_sigma(const McInter &it, double E)
     return M_PI * L * L * log(it.lambda_D / L);
 }
 
+// E1: the caller knows the interaction is ELASTIC with a constant rate
+template <bool E1 = false>
 __device__ __forceinline__ double sigma_v(const McDev &m, const McInter &it, double v)
 {
     // Interaction::sigma_v (particles.hpp:61-70)
+    if (E1) return it.rate;
     if (it.type == M2D_COULOMB) return coulomb_sigma(it, 0.5 * it.mu * v * v) * v;
     if (it.tab_len > 0)
         return vec_interp(m.tabE + it.tab_off, m.tabS + it.tab_off, it.tab_len,
@@ -93,17 +96,34 @@ __device__ __forceinline__ double sigma_v(const McDev &m, const McInter &it, dou
     return it.rate;
 }
 
+// t_random::rot(len, x, y, z) (random.cpp:103-115) in two parts: the draws
+// (the polar cosine c, its sine s, the azimuth's sine and cosine) and the
+// vector of length len they give
+struct RotDraw {
+    double c, s, sp, cp;
+};
+__device__ __forceinline__ RotDraw rot_draw(Rng &r)
+{
+    RotDraw d;
+    d.c = 1.0 - 2.0 * r.uni();
+    d.s = sqrt(1.0 - d.c * d.c);
+    sincos(2.0 * M_PI * r.uni(), &d.sp, &d.cp);
+    return d;
+}
+
+template <class V>
+__device__ __forceinline__ void rot_apply(V len, const RotDraw &d, V &x, V &y, V &z)
+{
+    x = len * d.c;
+    y = len * d.s * d.sp;
+    z = len * d.s * d.cp;
+}
+
 template <class V>
 __device__ __forceinline__ void rot_len(Rng &r, V len, V &x, V &y, V &z)
 {
-    // t_random::rot(len, x, y, z) (random.cpp:103-115)
-    double cs = 1.0 - 2.0 * r.uni();
-    x = len * cs;
-    cs = sqrt(1.0 - cs * cs);
-    double sp, cp;
-    sincos(2.0 * M_PI * r.uni(), &sp, &cp);
-    y = len * cs * sp;
-    z = len * cs * cp;
+    const RotDraw d = rot_draw(r);
+    rot_apply(len, d, x, y, z);
 }
 
 __device__ __forceinline__ double ellint_k(double k)
@@ -144,20 +164,30 @@ __device__ inline void deflect(Rng &r, double angle, V &x, V &y, V &z)
 // the reference: species pick, partner, interaction pick, angles.  The
 // velocity changes only when an interaction fires.  V is double, or Dual3
 // for the Jacobian (the picks compare values only).
-template <class V>
+// E1: the caller has checked elastic1(m).  The same code then takes no
+// runtime branch before the null-collision test, so the word each draw
+// reads is known when it is compiled: the stream's Philox blocks, the three
+// normals' functions and the rotation's angles become independent chains
+// whose latencies overlap (one thread's chain of dependent FP64 operations
+// is what a scatter costs).  The angles are drawn before that test, from
+// the words the reference draws them from after it; a null collision thus
+// leaves the stream two words further on, so E1 is for callers that draw
+// nothing from the stream after the scatter.
+template <class V, bool E1 = false>
 __device__ inline void scatter_d(const McDev &m, Rng &r, V &vx, V &vy, V &vz, unsigned long long *counters)
 {
     double gamma = r.uni() / m.lifetime;
     double tmp = 0.0;
     int sid;
-    for (sid = 0; sid < m.n_targets - 1; sid++) {
+    const int n_targets = E1 ? 1 : m.n_targets;
+    for (sid = 0; sid < n_targets - 1; sid++) {
         tmp += m.t[sid].rate;
         if (tmp > gamma) break;
     }
     const McTarget &T = m.t[sid];
     double vr2, vz2, vt2;
     int64_t partner = -1;  // slot of the live partner (random_particle, particles.hpp:201-211)
-    if (T.px == nullptr || T.pn == 0) {
+    if (E1 || T.px == nullptr || T.pn == 0) {
         vr2 = r.normal() * T.vth;
         vz2 = r.normal() * T.vth;
         vt2 = r.normal() * T.vth;
@@ -174,16 +204,19 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, V &vx, V &vy, V &vz, un
     double m2 = T.mass, mass = m.mass;
 
     gamma = r.uni() * T.rate;
+    RotDraw rd{};
+    if constexpr (E1) rd = rot_draw(r);
     tmp = 0.0;
     int k;
-    for (k = 0; k < T.n; k++) {
-        tmp += sigma_v(m, m.in[T.first + k], val(v_rel)) * T.density;
+    const int n_inter = E1 ? 1 : T.n;
+    for (k = 0; k < n_inter; k++) {
+        tmp += sigma_v<E1>(m, m.in[T.first + k], val(v_rel)) * T.density;
         if (tmp > gamma) break;
     }
-    if (k == T.n) return;  // null collision
+    if (k == n_inter) return;  // null collision
     const McInter &it = m.in[T.first + k];
     tmp = 1.0 / (mass + m2);
-    switch (it.type) {
+    switch (E1 ? (int)M2D_ELASTIC : it.type) {
     case M2D_SUPERELASTIC: {
         V E = 0.5 * it.mu * v_rel * v_rel + it.DE;
         V vr = vsqrt(2 * E / it.mu);
@@ -202,7 +235,8 @@ __device__ inline void scatter_d(const McDev &m, Rng &r, V &vx, V &vy, V &vz, un
         V cz = (vz * mass + vz2 * m2) * tmp;
         V cy = (vy * mass + vt2 * m2) * tmp;
         V rx, ry, rz;
-        rot_len(r, v_rel, rx, ry, rz);
+        if constexpr (E1) rot_apply(v_rel, rd, rx, ry, rz);
+        else rot_len(r, v_rel, rx, ry, rz);
         vx = rx * m2 * tmp + cx;
         vz = rz * m2 * tmp + cz;
         vy = ry * m2 * tmp + cy;
@@ -248,6 +282,27 @@ __device__ inline void scatter(const McDev &m, Rng &r, float &vxf, float &vyf, f
 {
     double vx = vxf, vy = vyf, vz = vzf;
     scatter_d(m, r, vx, vy, vz, counters);
+    vxf = (float)vx; vyf = (float)vy; vzf = (float)vz;
+}
+
+// one target, a continuum partner and one ELASTIC interaction of constant
+// rate: the collision set scatter_d<V, true> is compiled for
+__device__ __forceinline__ bool elastic1(const McDev &m)
+{
+    const McTarget &T = m.t[0];
+    return m.n_targets == 1 && (T.px == nullptr || T.pn == 0) && T.n == 1 && m.in[T.first].type == M2D_ELASTIC &&
+           m.in[T.first].tab_len == 0;
+}
+
+// scatter() of a slot whose stream nothing draws from afterwards (the
+// collision queue): the same velocities, through scatter_d<double, true>
+// where elastic1(m) holds
+__device__ inline void scatter_last(const McDev &m, Rng &r, float &vxf, float &vyf, float &vzf,
+                                    unsigned long long *counters)
+{
+    double vx = vxf, vy = vyf, vz = vzf;
+    if (elastic1(m)) scatter_d<double, true>(m, r, vx, vy, vz, counters);
+    else scatter_d(m, r, vx, vy, vz, counters);
     vxf = (float)vx; vyf = (float)vy; vzf = (float)vz;
 }
 
