@@ -1433,7 +1433,7 @@ static int queue_pcg3(m2d_fields3d *f, int from, int to, int &cur)
     return M2D_OK;
 }
 
-// the FP64 residual of u, its norms, and the refinement's verdict on them
+// the FP64 residual of u, its norms, and the refinement's decision on them
 static void queue_residual3(m2d_fields3d *f, double tol)
 {
     hipStream_t s = f->ctx->stream;

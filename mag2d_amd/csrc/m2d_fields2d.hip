@@ -1371,7 +1371,7 @@ static int queue_pcg(m2d_fields2d *f, int from, int to, int &cur)
     return M2D_OK;
 }
 
-// the FP64 residual of u, its norms, and the refinement's verdict on them
+// the FP64 residual of u, its norms, and the refinement's decision on them
 static void queue_residual(m2d_fields2d *f, double *u, double tol, int max_outer)
 {
     Mg &mg = f->mg;

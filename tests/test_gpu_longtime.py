@@ -91,7 +91,7 @@ def test_rf22pt_2000_steps_long_time_positions_track_fp64(ctx):
     """The headline scene (512 x 512, He at 1e20 m^-3, deposit on, sorted
     store: the tiled specialised pass with phase-applied tables) for 2000
     steps.  What remains of the error of long-time positions is the FP32
-    velocities' (the push is FP32, as in the north-star design): measured
+    velocities' (the push stays FP32 for long-time positions): measured
     median / 99% / max 2.8e-9 / 1.6e-8 / 3.4e-8 m against 1.7e-8 / 1.3e-7 /
     3.4e-7 m for FP32 positions.  Bounds with margin: median within 1e-8 m
     and at least 3 times below the FP32 positions', maximum within 1e-7 m
