@@ -135,3 +135,17 @@ def test_poisson_large_grid_is_deterministic_and_converges_fast(ctx):
     scale = -(dx * dx) / abi.EPS0 / dV
     rn, bn = free_residual(us[0], mask, volt, rho, nx, nz, scale)
     assert rn <= 1.01e-10 * bn
+
+
+def test_grids_past_32bit_table_offsets_are_rejected(ctx):
+    """The 2D particle gathers address the FP32 field tables by 32-bit byte
+    offsets, so a 2D grid needs (nx+1) (nz+1) < 2^30: creation refuses a
+    larger grid before it reads mask or voltage (tiny arrays stand in for
+    them here)."""
+    mask, volt = np.zeros(16, np.uint8), np.zeros(16)
+    h = 1e-6
+    g = abi.grid2d(32767, 32767, h, h)  # (nx+1) (nz+1) = 2^30
+    params = abi.FieldsParams(g, h * 32766, h * 32766, 0, 0, 0, abi.BC_FREE, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                              0.0, 1e-12)
+    with pytest.raises(abi.M2DError, match="too large"):
+        abi.Fields2D(ctx, params, mask, volt)
